@@ -1,0 +1,205 @@
+/*
+ * serenx.h -- C ABI of the MI355X (gfx950) force-evaluation library for
+ * SEREN-style self-gravitating grad-h SPH.
+ *
+ * The library replaces SEREN's per-step force path: the Barnes-Hut tree
+ * build + stock, the grad-h smoothing-length/density solve with neighbour
+ * lists, the hydro pair forces and the tree-gravity walk.  Two ways to use
+ * it share the same kernels:
+ *
+ *  1. Host-state (drop-in) mode.  The caller binds its own host arrays, laid
+ *     out as SEREN's module arrays (column-major, 1-based ids in pplist and in
+ *     the reorder list), and calls serenx_tree_update / serenx_hguess /
+ *     serenx_sph_update / serenx_hydro_forces / serenx_grav_forces in place of
+ *     the reference driver subroutines.  Each call uploads what the reference
+ *     routine reads and downloads what it writes.
+ *
+ *  2. Device-resident mode.  serenx_load copies a cloud to the GPU once;
+ *     serenx_force_eval runs one whole force evaluation on the device with no
+ *     host transfer; serenx_fetch copies results back.
+ *
+ * All functions return SERENX_OK (0) or an error code; serenx_last_error()
+ * gives the message of the last failure on the calling thread.  One context
+ * must be driven from one host thread at a time.
+ *
+ * Citations are to the SEREN reference sources (src/...).
+ */
+#ifndef SERENX_H
+#define SERENX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct serenx_ctx serenx_ctx;
+
+/* status codes */
+enum {
+    SERENX_OK = 0,
+    SERENX_EINVAL = 1,   /* bad argument / option */
+    SERENX_EHIP = 2,     /* HIP runtime error (no device, launch failure, ...) */
+    SERENX_ELMAX = 3,    /* 'Reached maximum level' (BHgrav_build.F90:289) */
+    SERENX_ESTATE = 4,   /* call made out of order (e.g. hydro before sph) */
+    SERENX_ENOMEM = 5,   /* device allocation failed */
+    SERENX_EHNEG = 6,    /* negative smoothing length (all_sph_gradh.F90:486) */
+    SERENX_EFEW = 7      /* fewer particles than pp_gather: h solve cannot close */
+};
+
+/* MULTIPOLE option (Makefile: MULTIPOLE = QUADRUPOLE | OCTUPOLE) */
+enum { SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3 };
+/* MAC option (Makefile: MAC = GEOMETRIC | GADGET | GADGET2) */
+enum { SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, SERENX_MAC_GADGET2 = 2 };
+/* arithmetic of the gravity walk: FP64 throughout, or FP32 terms on
+ * cell-relative coordinates with compensated FP32 accumulation */
+enum { SERENX_FP64 = 0, SERENX_MIXED = 1 };
+
+/* serenx_force_eval flags */
+enum {
+    SERENX_EVAL_HGUESS = 1,   /* run BHhydro_hguess before the h solve (setup) */
+    SERENX_EVAL_NO_GRAV = 2,  /* skip the gravity walk */
+    SERENX_EVAL_NO_HYDRO = 4  /* skip the hydro pair forces */
+};
+
+typedef struct serenx_options {
+    int32_t multipole;   /* SERENX_QUADRUPOLE or SERENX_OCTUPOLE */
+    int32_t mac;         /* SERENX_MAC_* */
+    int32_t precision;   /* SERENX_FP64 or SERENX_MIXED */
+    int32_t reserved;
+    double h_fac;        /* grad-h h = h_fac (m/rho)^(1/3); params 'h_fac' (1.2) */
+    double thetamaxsqd;  /* geometric MAC theta^2; params 'thetamaxsqd' */
+    double abserror;     /* multipole MAC tolerance; params 'abserror' */
+    double alpha;        /* AB viscosity alpha (beta := 2 alpha); params 'alpha' */
+    double sound_speed;  /* isothermal c_s used by serenx_force_eval */
+} serenx_options;
+
+/* Fills SEREN's parameter defaults (default_parameters.F90:62-117):
+ * quadrupole, geometric MAC, h_fac 1.2, thetamaxsqd 0.5, abserror 0.1,
+ * alpha 1, FP64, sound speed 1 (DIMENSIONLESS isothermal, thermal.F90:52-60). */
+int serenx_default_options(serenx_options *opt);
+
+int serenx_create(const serenx_options *opt, int device, serenx_ctx **out);
+int serenx_destroy(serenx_ctx *ctx);
+const char *serenx_last_error(void);
+
+/* pp_gather = int((4 pi/3)(2 h_fac)^3), pp_limit = 2 pp_gather
+ * (initialize_seren_variables_1.F90:30-39) */
+int serenx_neighbour_params(const serenx_ctx *ctx, int32_t *pp_gather, int32_t *pp_limit);
+
+/* ---------------------------------------------------------------------------
+ * Host-state (drop-in) mode
+ * ------------------------------------------------------------------------- */
+
+/* Pointers to the caller's arrays, shaped as SEREN's module arrays
+ * (particle_module, hydro_module, neighbour_module; modules.F90:292-542).
+ * parray(6,ptot) = [x y z m h zeta/omega]; v(3,ptot); a(3,ptot);
+ * pplist(pp_limit,ptot) holds 1-based ids.  agravmag may be NULL under the
+ * geometric MAC.  accdo (Fortran default logical, 4 bytes) may be NULL: all
+ * particles active. */
+typedef struct serenx_host_state {
+    int32_t ptot;
+    int32_t pp_limit;
+    double *parray;
+    double *v;
+    double *a;
+    double *gpot;
+    double *agravmag;
+    double *rho;
+    double *omega;
+    double *press;
+    double *sound;
+    double *div_v;
+    double *drhodt;
+    int32_t *pptot;
+    int32_t *pplist;
+    int32_t *accdo;
+    double *rextent;     /* scalar out: max extent (bounding_box.F90:39) */
+} serenx_host_state;
+
+int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st);
+
+/* replaces tree_update (src/main/tree_update.F90:9-92): BHgrav_build,
+ * BH_reorder_particles' traversal, BHgrav_stock, copy_BHgrav_to_BHhydro,
+ * BHhydro_stock.  Writes the new tree order to dummylist(1:ptot) (1-based old
+ * ids, the list BH_reorder_particles.F90:123 passes to reorder_particle_arrays)
+ * and the domain extent to *rextent.  Host arrays are not touched: the caller
+ * then permutes all its particle arrays with that list (reorder_particle_arrays,
+ * reorder_array.F90:9-270), exactly where the reference does. */
+int serenx_tree_update(serenx_ctx *ctx, int32_t *dummylist);
+
+/* replaces BHhydro_hguess (src/BHtree/BHhydro_hguess.F90:11-163) */
+int serenx_hguess(serenx_ctx *ctx);
+
+/* replaces sph_update (src/sph_sim/sph_update.F90:10-158): all_sph_gradh over
+ * active particles, BHhydro_update_hmax, get_neib over active particles */
+int serenx_sph_update(serenx_ctx *ctx);
+
+/* replaces sph_hydro_forces (src/sph_sim/sph_hydro_forces.F90:9-120):
+ * a = a_hydro for active particles (hydro_gradh.F90:9-339) */
+int serenx_hydro_forces(serenx_ctx *ctx);
+
+/* replaces sph_grav_forces (src/sph_sim/sph_grav_forces.F90:10-199):
+ * a += a_grav, gpot = pot, agravmag = |a_grav| for active particles
+ * (BHgrav_accel.F90:21-348 with gravity_gradh.F90:11-95) */
+int serenx_grav_forces(serenx_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident mode
+ * ------------------------------------------------------------------------- */
+
+/* Copies one cloud (row-major pos[n][3], vel[n][3], mass[n], h[n]) to the
+ * device in its input order (porig = 1..n) and resets the grad-h state as
+ * initialize_sph_variables_1.F90:33-110 does.  h may be NULL (zeros). */
+int serenx_load(serenx_ctx *ctx, int32_t n, const double *pos, const double *vel,
+                const double *mass, const double *h);
+
+/* Replaces |a| from a previous evaluation (used by the GADGET/GADGET2 MACs,
+ * BHgrav_accel.F90:97-116); values are in the current tree order. */
+int serenx_set_agravmag(serenx_ctx *ctx, const double *agravmag);
+
+/* One force evaluation in the sph_setup.F90:42-66 order: tree_update,
+ * [hguess], sph_update, isothermal thermal (P = c_s^2 rho, c = c_s),
+ * sph_hydro_forces, sph_grav_forces.  Enqueued on the context's stream;
+ * returns after the work is queued (serenx_sync waits). */
+int serenx_force_eval(serenx_ctx *ctx, int32_t flags);
+int serenx_sync(serenx_ctx *ctx);
+
+/* Copies the current state to the host, in tree order.  Any pointer may be
+ * NULL.  Vectors are row-major [n][3]; pplist is [n][pp_limit], 1-based ids,
+ * rows valid where pptot <= pp_limit (get_neib.F90:115-118). */
+int serenx_fetch(serenx_ctx *ctx, int32_t *porig, double *pos, double *vel,
+                 double *mass, double *h, double *zeta, double *rho, double *omega,
+                 double *div_v, double *drhodt, double *press, double *sound,
+                 double *acc, double *gpot, double *agravmag, int32_t *pptot,
+                 int32_t *pplist);
+
+/* FP64 kernel-softened direct sum over all particles for nt targets (0-based
+ * indices in tree order): the semantics of direct_sph_gravity.F90:10-107. */
+int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets,
+                          double *acc, double *pot);
+
+/* Tree sizes: number of cells and the deepest level (ctot_grav + 1, ltot_grav). */
+int serenx_tree_sizes(const serenx_ctx *ctx, int32_t *ncells, int32_t *ltot);
+
+/* Exports the tree in the reference's form, indexed by BFS cell id
+ * (BHgrav_build.F90:149-157): leaf, plist[8] (1-based tree-order ids),
+ * nextcell (sentinel 2n+1), ifopen (branches), r[3] (COM), m, dminsqd, mac,
+ * q[5], s[10], hydro r[3] (bbox centre), rmax, hrangemax, and the first/last
+ * cell id of each level (31 entries each).  Any pointer may be NULL. */
+int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist,
+                       int32_t *nextcell, int32_t *ifopen, double *r, double *m,
+                       double *dminsqd, double *mac, double *q, double *s,
+                       double *hr, double *rmax, double *hrangemax,
+                       int32_t *first_cell, int32_t *last_cell);
+
+/* Per-phase device time of the last serenx_force_eval in milliseconds
+ * (tree, sph, hydro, grav) and interaction counts of the last gravity walk
+ * (cell terms, particle-particle terms) summed over particles. */
+int serenx_last_timings(const serenx_ctx *ctx, double *ms4);
+int serenx_interaction_counts(serenx_ctx *ctx, int64_t *ncell, int64_t *npp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SERENX_H */

@@ -1,0 +1,695 @@
+// capi.hip -- the C ABI of include/serenx.h: contexts, the host-state
+// (drop-in) mode that mirrors SEREN's driver subroutines, the device-resident
+// mode, tree export and the kernel tables.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "serenx_internal.h"
+
+namespace serenx {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s failed: %s", what, hipGetErrorString(e));
+    return set_error(SERENX_EHIP, buf);
+}
+
+// x**n for a small integer constant n as the reference compiler expands it,
+// a left-to-right product ((x*x)*x)*...
+static double ipow_seq(double x, int n) {
+    double y = x;
+    for (int i = 1; i < n; i++) y = y * x;
+    return y;
+}
+
+// kernel.F90:14-127 (M4, GRAD_H_SPH, NDIM = 3); tab[j*KT + i]
+void make_kernel_tables(double *tab) {
+    const double invkerntot = 1.0 / (double)KERNTOT;
+    const int kern1 = (int)HALFKERNTOT;
+    const double nd = 3.0;
+    double *w0 = tab + T_W0 * KT, *w1 = tab + T_W1 * KT, *w2 = tab + T_W2 * KT,
+           *w3 = tab + T_W3 * KT, *w4 = tab + T_W4 * KT, *w5 = tab + T_W5 * KT,
+           *w6 = tab + T_W6 * KT, *wh = tab + T_WH * KT, *wg = tab + T_WG * KT;
+    for (int i = 0; i <= KERNTOT; i++) {
+        const double s = (KERNRANGE * (double)i + 1.0) * invkerntot;
+        const double saux = KERNRANGE - s;
+        if (i < kern1) {
+            w1[i] = -(3.0 * s) + 2.25 * s * s;
+            w0[i] = 1.0 - 1.5 * s * s + 0.75 * ipow_seq(s, 3);
+            w2[i] = (4.0 / 3.0) * ipow_seq(s, 3) - 1.2 * ipow_seq(s, 5) + 0.5 * ipow_seq(s, 6);
+            w3[i] = (7.0 / 5.0) * s - (2.0 / 3.0) * ipow_seq(s, 3) + 0.3 * ipow_seq(s, 5) -
+                    0.1 * ipow_seq(s, 6);
+            w4[i] = -(3.0 * s) + 2.25 * s * s;
+            w5[i] = (4.0 / 3.0) * s - 1.2 * ipow_seq(s, 3) + 0.5 * ipow_seq(s, 4);
+            w6[i] = 7.0 / 5.0 - (2.0 / 3.0) * ipow_seq(s, 2) + 0.3 * ipow_seq(s, 4) -
+                    0.1 * ipow_seq(s, 5);
+            wh[i] = -nd + 1.5 * (nd + 2.0) * s * s - 0.75 * (nd + 3.0) * ipow_seq(s, 3);
+            wg[i] = 1.4 - 2.0 * s * s + 1.5 * ipow_seq(s, 4) - 0.6 * ipow_seq(s, 5);
+        } else {
+            w0[i] = 0.25 * ipow_seq(saux, 3);
+            w1[i] = -(0.75 * saux * saux);
+            w2[i] = (8.0 / 3.0) * ipow_seq(s, 3) - 3.0 * ipow_seq(s, 4) + 1.2 * ipow_seq(s, 5) -
+                    (1.0 / 6.0) * ipow_seq(s, 6) - (1.0 / 15.0);
+            w3[i] = -(1.0 / 15.0) + (8.0 / 5.0) * s - (4.0 / 3.0) * ipow_seq(s, 3) +
+                    ipow_seq(s, 4) - 0.3 * ipow_seq(s, 5) + (1.0 / 30.0) * ipow_seq(s, 6);
+            w4[i] = -(0.75 * saux * saux);
+            w5[i] = (8.0 / 3.0) * s - 3.0 * ipow_seq(s, 2) + 1.2 * ipow_seq(s, 3) -
+                    (1.0 / 6.0) * ipow_seq(s, 4) - 1.0 / (15.0 * ipow_seq(s, 2));
+            w6[i] = -(1.0 / (15.0 * s)) + 8.0 / 5.0 - (4.0 / 3.0) * ipow_seq(s, 2) +
+                    ipow_seq(s, 3) - 0.3 * ipow_seq(s, 4) + (1.0 / 30.0) * ipow_seq(s, 5);
+            wh[i] = -2.0 * nd + 3.0 * (nd + 1.0) * s - 1.50 * (nd + 2.0) * s * s +
+                    0.25 * (nd + 3.0) * ipow_seq(s, 3);
+            wg[i] = 1.6 - 4.0 * s * s + 4.0 * ipow_seq(s, 3) - 1.5 * ipow_seq(s, 4) +
+                    0.2 * ipow_seq(s, 5);
+        }
+        w0[i] = w0[i] * INVPI;
+        w1[i] = w1[i] * INVPI;
+        w4[i] = w4[i] * INVPI;
+        wh[i] = wh[i] * INVPI;
+    }
+}
+
+namespace {
+
+constexpr int TB = 256;
+inline int nblk(long n) { return (int)((n + TB - 1) / TB); }
+
+__global__ void k_init_state(int n, int *porig, uint8_t *active, double *rho, double *omega,
+                             double *zeta, double *ax, double *ay, double *az, double *gpot,
+                             double *agm, double *divv, double *press, double *sound,
+                             double *drhodt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    porig[i] = i + 1;
+    active[i] = 1;
+    rho[i] = 1.0;     // initialize_sph_variables_1.F90:68-72
+    omega[i] = 1.0;
+    zeta[i] = 0.0;
+    ax[i] = ay[i] = az[i] = 0.0;
+    gpot[i] = 0.0;
+    agm[i] = 0.0;
+    divv[i] = 0.0;
+    press[i] = sound[i] = drhodt[i] = 0.0;
+}
+
+__global__ void k_iota1(int n, int *porig) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) porig[i] = i + 1;
+}
+
+__global__ void k_zero_active(int n, const uint8_t *active, double *ax, double *ay, double *az) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && active[i]) ax[i] = ay[i] = az[i] = 0.0;
+}
+
+template <class T>
+int dalloc(T **p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess)
+        return set_error(SERENX_ENOMEM, "hipMalloc failed for particle arrays");
+    return SERENX_OK;
+}
+
+int ensure_capacity(Ctx &c, int n) {
+    if (n <= c.cap && c.x) return SERENX_OK;
+    c.cap = n;
+    const size_t nn = (size_t)n + 1;
+    double **arrs[] = {&c.x, &c.y, &c.z, &c.m, &c.h, &c.zeta, &c.vx, &c.vy, &c.vz,
+                       &c.rho, &c.omega, &c.press, &c.sound, &c.divv, &c.drhodt,
+                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm, &c.tmpd};
+    for (double **a : arrs) SX_TRY(dalloc(a, nn));
+    SX_TRY(dalloc(&c.porig, nn));
+    SX_TRY(dalloc(&c.tmpi, nn));
+    SX_TRY(dalloc(&c.active, nn));
+    SX_TRY(dalloc(&c.pptot, nn));
+    SX_TRY(dalloc(&c.pplist, nn * (size_t)c.pp_limit));
+    SX_TRY(dalloc(&c.ovf_off, nn));
+    SX_TRY(ensure_tree_buffers(c));
+    return SERENX_OK;
+}
+
+void free_all(Ctx &c) {
+    void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
+                    c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
+                    c.tmpd, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
+                    c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
+                    c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
+                    c.root, c.bbpart, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
+                    c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
+                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c.hostpin) (void)hipHostFree(c.hostpin);
+    if (c.hostpin_d) (void)hipHostFree(c.hostpin_d);
+    for (auto &e : c.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+int check_options(const serenx_options &o) {
+    if (o.multipole != SERENX_QUADRUPOLE && o.multipole != SERENX_OCTUPOLE)
+        return set_error(SERENX_EINVAL, "multipole must be SERENX_QUADRUPOLE or SERENX_OCTUPOLE");
+    if (o.mac < SERENX_MAC_GEOMETRIC || o.mac > SERENX_MAC_GADGET2)
+        return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET or GADGET2");
+    if (o.precision != SERENX_FP64 && o.precision != SERENX_MIXED)
+        return set_error(SERENX_EINVAL, "precision must be SERENX_FP64 or SERENX_MIXED");
+    if (!(o.h_fac > 0.0)) return set_error(SERENX_EINVAL, "h_fac must be positive");
+    if (o.mac != SERENX_MAC_GEOMETRIC && !(o.abserror > 0.0))
+        return set_error(SERENX_EINVAL, "abserror must be positive for multipole MACs");
+    return SERENX_OK;
+}
+
+Ctx *as_ctx(serenx_ctx *p) { return reinterpret_cast<Ctx *>(p); }
+const Ctx *as_ctx(const serenx_ctx *p) { return reinterpret_cast<const Ctx *>(p); }
+
+int run_tree_update(Ctx &c) {
+    SX_TRY(build_tree(c));
+    SX_TRY(stock_tree(c));
+    return SERENX_OK;
+}
+
+int run_sph_update(Ctx &c) {
+    if (!c.tree_ok) return set_error(SERENX_ESTATE, "sph_update called before tree_update");
+    SX_TRY(sph_hsolve(c));
+    SX_TRY(update_hmax(c));
+    SX_TRY(sph_neighbours(c));
+    c.sph_ok = true;
+    return SERENX_OK;
+}
+
+int run_hydro(Ctx &c) {
+    if (!c.sph_ok) return set_error(SERENX_ESTATE, "hydro forces need sph_update first");
+    hipLaunchKernelGGL(k_zero_active, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active, c.ax,
+                       c.ay, c.az);
+    SX_TRY(hydro_forces(c));
+    return SERENX_OK;
+}
+
+int run_grav(Ctx &c) {
+    if (!c.tree_ok) return set_error(SERENX_ESTATE, "gravity needs tree_update first");
+    SX_TRY(grav_forces(c));
+    return SERENX_OK;
+}
+
+// ---- host-state transfers -------------------------------------------------
+// parray(6,ptot) column-major: element (k, p) at parray[6 p + k]
+int up_parray(Ctx &c, const int *cols, int ncols) {
+    const int n = c.n;
+    double *dst[6] = {c.x, c.y, c.z, c.m, c.h, c.zeta};
+    c.hbuf.resize(n);
+    for (int t = 0; t < ncols; t++) {
+        const int k = cols[t];
+        for (int p = 0; p < n; p++) c.hbuf[p] = c.hs.parray[6 * (size_t)p + k];
+        SX_HIP(hipMemcpy(dst[k], c.hbuf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    return SERENX_OK;
+}
+
+int down_parray(Ctx &c, const int *cols, int ncols) {
+    const int n = c.n;
+    double *src[6] = {c.x, c.y, c.z, c.m, c.h, c.zeta};
+    c.hbuf.resize(n);
+    for (int t = 0; t < ncols; t++) {
+        const int k = cols[t];
+        SX_HIP(hipMemcpy(c.hbuf.data(), src[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < n; p++) c.hs.parray[6 * (size_t)p + k] = c.hbuf[p];
+    }
+    return SERENX_OK;
+}
+
+int up_vec3(Ctx &c, const double *h3, double *dx, double *dy, double *dz) {
+    const int n = c.n;
+    double *d[3] = {dx, dy, dz};
+    c.hbuf.resize(n);
+    for (int k = 0; k < 3; k++) {
+        for (int p = 0; p < n; p++) c.hbuf[p] = h3[3 * (size_t)p + k];
+        SX_HIP(hipMemcpy(d[k], c.hbuf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    return SERENX_OK;
+}
+
+int down_vec3(Ctx &c, double *h3, const double *dx, const double *dy, const double *dz) {
+    const int n = c.n;
+    const double *d[3] = {dx, dy, dz};
+    c.hbuf.resize(n);
+    for (int k = 0; k < 3; k++) {
+        SX_HIP(hipMemcpy(c.hbuf.data(), d[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < n; p++) h3[3 * (size_t)p + k] = c.hbuf[p];
+    }
+    return SERENX_OK;
+}
+
+int up_scalar(Ctx &c, const double *hsrc, double *d) {
+    if (!hsrc) return SERENX_OK;
+    SX_HIP(hipMemcpy(d, hsrc, sizeof(double) * c.n, hipMemcpyHostToDevice));
+    return SERENX_OK;
+}
+
+int down_scalar(Ctx &c, double *hdst, const double *d) {
+    if (!hdst) return SERENX_OK;
+    SX_HIP(hipMemcpy(hdst, d, sizeof(double) * c.n, hipMemcpyDeviceToHost));
+    return SERENX_OK;
+}
+
+int up_accdo(Ctx &c) {
+    std::vector<uint8_t> a(c.n, 1);
+    if (c.hs.accdo)
+        for (int p = 0; p < c.n; p++) a[p] = c.hs.accdo[p] != 0;
+    SX_HIP(hipMemcpy(c.active, a.data(), c.n, hipMemcpyHostToDevice));
+    return SERENX_OK;
+}
+
+int need_bound(Ctx &c) {
+    if (!c.bound) return set_error(SERENX_ESTATE, "no host state bound");
+    return SERENX_OK;
+}
+
+}  // namespace
+}  // namespace serenx
+
+using namespace serenx;
+
+extern "C" {
+
+const char *serenx_last_error(void) { return g_last_error.c_str(); }
+
+int serenx_default_options(serenx_options *opt) {
+    if (!opt) return set_error(SERENX_EINVAL, "null options");
+    std::memset(opt, 0, sizeof(*opt));
+    opt->multipole = SERENX_QUADRUPOLE;
+    opt->mac = SERENX_MAC_GEOMETRIC;
+    opt->precision = SERENX_FP64;
+    opt->h_fac = 1.2;
+    opt->thetamaxsqd = 0.5;
+    opt->abserror = 0.1;
+    opt->alpha = 1.0;
+    opt->sound_speed = 1.0;
+    return SERENX_OK;
+}
+
+int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
+    if (!opt || !out) return set_error(SERENX_EINVAL, "null argument");
+    *out = nullptr;
+    SX_TRY(check_options(*opt));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(SERENX_EHIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_error(SERENX_EINVAL, "bad device index");
+    SX_HIP(hipSetDevice(device));
+    Ctx *c = new Ctx();
+    c->opt = *opt;
+    c->device = device;
+    c->pp_gather = (int)((4.0 * PI_REF / 3.0) * ipow_seq(KERNRANGE * opt->h_fac, 3));
+    c->pp_limit = 2 * c->pp_gather;
+    c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (auto &ev : c->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * NTAB * KT * 3 / 2 + 64);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->icount, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->ovf_count, sizeof(int));
+    if (e != hipSuccess) {
+        free_all(*c);
+        delete c;
+        return hip_check(e, "serenx_create");
+    }
+    make_kernel_tables(c->host_ktab);
+    std::vector<float> tf(NTAB * KT);
+    for (int i = 0; i < NTAB * KT; i++) tf[i] = (float)c->host_ktab[i];
+    e = hipMemcpy(c->ktab, c->host_ktab, sizeof(double) * NTAB * KT, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->ktab + NTAB * KT, tf.data(), sizeof(float) * NTAB * KT,
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_all(*c);
+        delete c;
+        return hip_check(e, "serenx_create tables");
+    }
+    *out = reinterpret_cast<serenx_ctx *>(c);
+    return SERENX_OK;
+}
+
+int serenx_destroy(serenx_ctx *ctx) {
+    if (!ctx) return SERENX_OK;
+    Ctx *c = as_ctx(ctx);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(*c);
+    delete c;
+    return SERENX_OK;
+}
+
+int serenx_neighbour_params(const serenx_ctx *ctx, int32_t *pp_gather, int32_t *pp_limit) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    const Ctx *c = as_ctx(ctx);
+    if (pp_gather) *pp_gather = c->pp_gather;
+    if (pp_limit) *pp_limit = c->pp_limit;
+    return SERENX_OK;
+}
+
+// ---- host-state mode ------------------------------------------------------
+int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st) {
+    if (!ctx || !st) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    if (st->ptot <= 0 || !st->parray || !st->v || !st->a)
+        return set_error(SERENX_EINVAL, "host state needs ptot > 0, parray, v and a");
+    if (st->pp_limit != c.pp_limit)
+        return set_error(SERENX_EINVAL, "pp_limit of the host state differs from 2*pp_gather");
+    SX_HIP(hipSetDevice(c.device));
+    SX_TRY(ensure_capacity(c, st->ptot));
+    c.hs = *st;
+    c.n = st->ptot;
+    c.bound = true;
+    c.tree_ok = c.sph_ok = false;
+    hipLaunchKernelGGL(k_init_state, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.porig,
+                       c.active, c.rho, c.omega, c.zeta, c.ax, c.ay, c.az, c.gpot, c.agm, c.divv,
+                       c.press, c.sound, c.drhodt);
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return SERENX_OK;
+}
+
+int serenx_tree_update(serenx_ctx *ctx, int32_t *dummylist) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    SX_HIP(hipSetDevice(c.device));
+    const int cols[] = {0, 1, 2, 3, 4};
+    SX_TRY(up_parray(c, cols, 5));
+    hipLaunchKernelGGL(k_iota1, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.porig);
+    SX_TRY(run_tree_update(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    if (dummylist)
+        SX_HIP(hipMemcpy(dummylist, c.porig, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+    if (c.hs.rextent) *c.hs.rextent = c.rextent;
+    return SERENX_OK;
+}
+
+int serenx_hguess(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    if (!c.tree_ok) return set_error(SERENX_ESTATE, "hguess called before tree_update");
+    SX_HIP(hipSetDevice(c.device));
+    SX_TRY(hguess(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    const int cols[] = {4};
+    SX_TRY(down_parray(c, cols, 1));
+    return SERENX_OK;
+}
+
+int serenx_sph_update(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    SX_HIP(hipSetDevice(c.device));
+    const int cols[] = {0, 1, 2, 3, 4};
+    SX_TRY(up_parray(c, cols, 5));
+    SX_TRY(up_vec3(c, c.hs.v, c.vx, c.vy, c.vz));
+    SX_TRY(up_accdo(c));
+    SX_TRY(run_sph_update(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    const int outc[] = {4, 5};
+    SX_TRY(down_parray(c, outc, 2));
+    SX_TRY(down_scalar(c, c.hs.rho, c.rho));
+    SX_TRY(down_scalar(c, c.hs.omega, c.omega));
+    SX_TRY(down_scalar(c, c.hs.div_v, c.divv));
+    SX_TRY(down_scalar(c, c.hs.drhodt, c.drhodt));
+    if (c.hs.pptot) {
+        std::vector<int> tot(c.n);
+        SX_HIP(hipMemcpy(tot.data(), c.pptot, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+        std::vector<int> lst((size_t)c.n * c.pp_limit);
+        SX_HIP(hipMemcpy(lst.data(), c.pplist, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+        std::vector<uint8_t> act(c.n);
+        SX_HIP(hipMemcpy(act.data(), c.active, c.n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < c.n; p++) {
+            if (!act[p]) continue;
+            c.hs.pptot[p] = tot[p];
+            if (c.hs.pplist && tot[p] <= c.pp_limit)
+                for (int i = 0; i < tot[p]; i++)
+                    c.hs.pplist[(size_t)p * c.pp_limit + i] = lst[(size_t)p * c.pp_limit + i] + 1;
+        }
+    }
+    return SERENX_OK;
+}
+
+int serenx_hydro_forces(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    SX_HIP(hipSetDevice(c.device));
+    const int cols[] = {0, 1, 2, 3, 4};
+    SX_TRY(up_parray(c, cols, 5));
+    SX_TRY(up_vec3(c, c.hs.v, c.vx, c.vy, c.vz));
+    SX_TRY(up_vec3(c, c.hs.a, c.ax, c.ay, c.az));
+    SX_TRY(up_scalar(c, c.hs.rho, c.rho));
+    SX_TRY(up_scalar(c, c.hs.omega, c.omega));
+    SX_TRY(up_scalar(c, c.hs.press, c.press));
+    SX_TRY(up_scalar(c, c.hs.sound, c.sound));
+    SX_TRY(up_accdo(c));
+    SX_TRY(run_hydro(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    SX_TRY(down_vec3(c, c.hs.a, c.ax, c.ay, c.az));
+    return SERENX_OK;
+}
+
+int serenx_grav_forces(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    SX_HIP(hipSetDevice(c.device));
+    const int cols[] = {0, 1, 2, 3, 4, 5};
+    SX_TRY(up_parray(c, cols, 6));
+    SX_TRY(up_vec3(c, c.hs.a, c.ax, c.ay, c.az));
+    if (c.opt.mac != SERENX_MAC_GEOMETRIC && c.hs.agravmag) SX_TRY(up_scalar(c, c.hs.agravmag, c.agm));
+    SX_TRY(up_accdo(c));
+    SX_TRY(run_grav(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    SX_TRY(down_vec3(c, c.hs.a, c.ax, c.ay, c.az));
+    SX_TRY(down_scalar(c, c.hs.gpot, c.gpot));
+    if (c.opt.mac != SERENX_MAC_GEOMETRIC) SX_TRY(down_scalar(c, c.hs.agravmag, c.agm));
+    return SERENX_OK;
+}
+
+// ---- device-resident mode --------------------------------------------------
+int serenx_load(serenx_ctx *ctx, int32_t n, const double *pos, const double *vel,
+                const double *mass, const double *h) {
+    if (!ctx || !pos || !vel || !mass || n <= 0) return set_error(SERENX_EINVAL, "bad load args");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    SX_TRY(ensure_capacity(c, n));
+    c.n = n;
+    c.bound = false;
+    c.tree_ok = c.sph_ok = false;
+    SX_TRY(up_vec3(c, pos, c.x, c.y, c.z));
+    SX_TRY(up_vec3(c, vel, c.vx, c.vy, c.vz));
+    SX_HIP(hipMemcpy(c.m, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (h)
+        SX_HIP(hipMemcpy(c.h, h, sizeof(double) * n, hipMemcpyHostToDevice));
+    else
+        SX_HIP(hipMemset(c.h, 0, sizeof(double) * n));
+    hipLaunchKernelGGL(k_init_state, dim3(nblk(n)), dim3(TB), 0, c.stream, n, c.porig, c.active,
+                       c.rho, c.omega, c.zeta, c.ax, c.ay, c.az, c.gpot, c.agm, c.divv, c.press,
+                       c.sound, c.drhodt);
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return SERENX_OK;
+}
+
+int serenx_set_agravmag(serenx_ctx *ctx, const double *agravmag) {
+    if (!ctx || !agravmag) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipMemcpy(c.agm, agravmag, sizeof(double) * c.n, hipMemcpyHostToDevice));
+    return SERENX_OK;
+}
+
+int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipEventRecord(c.ev[0], c.stream));
+    SX_TRY(run_tree_update(c));
+    if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
+    SX_HIP(hipEventRecord(c.ev[1], c.stream));
+    SX_TRY(run_sph_update(c));
+    SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    SX_HIP(hipEventRecord(c.ev[2], c.stream));
+    if (flags & SERENX_EVAL_NO_HYDRO) {
+        hipLaunchKernelGGL(k_zero_active, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active,
+                           c.ax, c.ay, c.az);
+    } else {
+        SX_TRY(run_hydro(c));
+    }
+    SX_HIP(hipEventRecord(c.ev[3], c.stream));
+    if (!(flags & SERENX_EVAL_NO_GRAV)) SX_TRY(run_grav(c));
+    SX_HIP(hipEventRecord(c.ev[4], c.stream));
+    return SERENX_OK;
+}
+
+int serenx_sync(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return SERENX_OK;
+}
+
+int serenx_last_timings(const serenx_ctx *ctx, double *ms4) {
+    if (!ctx || !ms4) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipEventSynchronize(c.ev[4]));
+    for (int k = 0; k < 4; k++) {
+        float ms = 0.0f;
+        SX_HIP(hipEventElapsedTime(&ms, c.ev[k], c.ev[k + 1]));
+        ms4[k] = ms;
+    }
+    return SERENX_OK;
+}
+
+int serenx_interaction_counts(serenx_ctx *ctx, int64_t *ncell, int64_t *npp) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    unsigned long long v[2] = {0, 0};
+    SX_HIP(hipMemcpy(v, c.icount, sizeof(v), hipMemcpyDeviceToHost));
+    if (ncell) *ncell = (int64_t)v[0];
+    if (npp) *npp = (int64_t)v[1];
+    return SERENX_OK;
+}
+
+int serenx_fetch(serenx_ctx *ctx, int32_t *porig, double *pos, double *vel, double *mass,
+                 double *h, double *zeta, double *rho, double *omega, double *div_v,
+                 double *drhodt, double *press, double *sound, double *acc, double *gpot,
+                 double *agravmag, int32_t *pptot, int32_t *pplist) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    const size_t nb = sizeof(double) * c.n;
+    if (porig) SX_HIP(hipMemcpy(porig, c.porig, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+    if (pos) SX_TRY(down_vec3(c, pos, c.x, c.y, c.z));
+    if (vel) SX_TRY(down_vec3(c, vel, c.vx, c.vy, c.vz));
+    if (acc) SX_TRY(down_vec3(c, acc, c.ax, c.ay, c.az));
+    struct {
+        double *hp;
+        const double *dp;
+    } sc[] = {{mass, c.m},     {h, c.h},       {zeta, c.zeta},   {rho, c.rho},
+              {omega, c.omega}, {div_v, c.divv}, {drhodt, c.drhodt}, {press, c.press},
+              {sound, c.sound}, {gpot, c.gpot}, {agravmag, c.agm}};
+    for (auto &e : sc)
+        if (e.hp) SX_HIP(hipMemcpy(e.hp, e.dp, nb, hipMemcpyDeviceToHost));
+    if (pptot) SX_HIP(hipMemcpy(pptot, c.pptot, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+    if (pplist) {
+        const size_t cnt = (size_t)c.n * c.pp_limit;
+        SX_HIP(hipMemcpy(pplist, c.pplist, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+        std::vector<int> tot(c.n);
+        SX_HIP(hipMemcpy(tot.data(), c.pptot, sizeof(int) * c.n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < c.n; p++) {
+            int *row = pplist + (size_t)p * c.pp_limit;
+            const int k = tot[p] <= c.pp_limit ? tot[p] : 0;
+            for (int i = 0; i < c.pp_limit; i++) row[i] = (i < k) ? row[i] + 1 : 0;
+        }
+    }
+    return SERENX_OK;
+}
+
+int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets, double *acc,
+                          double *pot) {
+    if (!ctx || !targets || !acc || !pot || nt <= 0) return set_error(SERENX_EINVAL, "bad args");
+    Ctx &c = *as_ctx(ctx);
+    if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
+    for (int i = 0; i < nt; i++)
+        if (targets[i] < 0 || targets[i] >= c.n)
+            return set_error(SERENX_EINVAL, "direct-gravity target out of range");
+    SX_HIP(hipSetDevice(c.device));
+    int *dt = nullptr;
+    double *da = nullptr, *dp = nullptr;
+    SX_HIP(hipMalloc(&dt, sizeof(int) * nt));
+    SX_HIP(hipMalloc(&da, sizeof(double) * 3 * nt));
+    SX_HIP(hipMalloc(&dp, sizeof(double) * nt));
+    int rc = SERENX_OK;
+    if (hipMemcpy(dt, targets, sizeof(int) * nt, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "copy of targets failed");
+    if (rc == SERENX_OK) rc = direct_gravity(c, nt, dt, da, dp);
+    if (rc == SERENX_OK && hipStreamSynchronize(c.stream) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "direct gravity failed");
+    if (rc == SERENX_OK) {
+        (void)hipMemcpy(acc, da, sizeof(double) * 3 * nt, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(pot, dp, sizeof(double) * nt, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dt);
+    (void)hipFree(da);
+    (void)hipFree(dp);
+    return rc;
+}
+
+int serenx_tree_sizes(const serenx_ctx *ctx, int32_t *ncells, int32_t *ltot) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    const Ctx &c = *as_ctx(ctx);
+    if (ncells) *ncells = c.nc;
+    if (ltot) *ltot = c.ltot;
+    return SERENX_OK;
+}
+
+int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int32_t *nextcell,
+                       int32_t *ifopen, double *r, double *m, double *dminsqd, double *mac,
+                       double *q, double *s, double *hr, double *rmax, double *hrangemax,
+                       int32_t *first_cell, int32_t *last_cell) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *const_cast<Ctx *>(as_ctx(ctx));
+    if (!c.tree_ok) return set_error(SERENX_ESTATE, "no tree built");
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    const int nc = c.nc;
+    std::vector<int> b2d(nc), d2b(nc), cleaf(nc);
+    std::vector<GNode> g(nc);
+    std::vector<HNode> hh(nc);
+    SX_HIP(hipMemcpy(b2d.data(), c.bfs2dfs, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(d2b.data(), c.dfs2bfs, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(cleaf.data(), c.c_leaf, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(g.data(), c.gn, sizeof(GNode) * nc, hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(hh.data(), c.hn, sizeof(HNode) * nc, hipMemcpyDeviceToHost));
+    const int sentinel = 2 * c.n + 1;   // cmax_grav + 1 (BHgrav_build.F90:75)
+    for (int b = 0; b < nc; b++) {
+        const int d = b2d[b];
+        const GNode &gg = g[d];
+        if (leaf) leaf[b] = cleaf[b];
+        if (plist)
+            for (int i = 0; i < LEAFMAX; i++)
+                plist[LEAFMAX * b + i] = (i < cleaf[b]) ? gg.first + i + 1 : 0;
+        if (nextcell) nextcell[b] = (gg.next >= nc) ? sentinel : d2b[gg.next];
+        if (ifopen) ifopen[b] = (cleaf[b] == 0 && d + 1 < nc) ? d2b[d + 1] : 0;
+        for (int k = 0; k < 3; k++) {
+            if (r) r[3 * b + k] = gg.r[k];
+            if (hr) hr[3 * b + k] = hh[d].r[k];
+        }
+        if (m) m[b] = gg.m;
+        if (dminsqd) dminsqd[b] = gg.dminsqd;
+        if (mac) mac[b] = gg.mac;
+        if (q)
+            for (int k = 0; k < 5; k++) q[5 * b + k] = gg.q[k];
+        if (s)
+            for (int k = 0; k < 10; k++) s[10 * b + k] = gg.s[k];
+        if (rmax) rmax[b] = hh[d].rmax;
+        if (hrangemax) hrangemax[b] = hh[d].hrangemax;
+    }
+    for (int L = 0; L <= LMAX; L++) {
+        const bool has = L <= c.ltot && c.level_cnt[L] > 0;
+        if (first_cell) first_cell[L] = has ? c.level_off[L] : 0;
+        if (last_cell) last_cell[L] = has ? c.level_off[L] + c.level_cnt[L] - 1 : 0;
+    }
+    return SERENX_OK;
+}
+
+}  // extern "C"

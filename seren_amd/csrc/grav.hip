@@ -1,0 +1,472 @@
+// grav.hip -- Barnes-Hut gravity walk with monopole/quadrupole/octupole cell
+// terms and the grad-h kernel-softened near field, plus the FP64
+// kernel-softened direct sum, on gfx950.
+//
+// A wave walks the gravity tree for 64 consecutive (tree-ordered) particles
+// with group_walk.  At each cell every active lane applies its own MAC
+// (BHgrav_accel.F90:154-158): accepted cells add a multipole term, opened
+// leaves add kernel-softened pair terms (gravity_gradh.F90), opened branches
+// descend.  Each lane therefore sees exactly the reference particle's
+// interaction set; MAC tests use the reference's arithmetic (no contraction)
+// so that accept/open decisions are identical.  The reference sums all cell
+// terms first and the near-field pairs at the end (BHgrav_accel.F90:272-290);
+// here cells and pairs go to two accumulators added at the end, so results
+// agree with the reference to rounding.
+//
+// Mixed precision (SERENX_MIXED): the cell offset rp - rc is formed in FP64
+// and the multipole and pair terms are evaluated in FP32 on that
+// cell-relative offset, accumulated with Neumaier-compensated FP32 sums.
+#include "walk.h"
+
+namespace serenx {
+
+namespace {
+
+constexpr int TB = 256;
+inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
+
+// gravity_gradh.F90:42-86
+__device__ __forceinline__ void pair_fp64(const double *__restrict__ w4,
+                                          const double *__restrict__ w5,
+                                          const double *__restrict__ w6, double invhp,
+                                          double zo_p, double rpx, double rpy, double rpz,
+                                          double xj, double yj, double zj, double mj, double hj,
+                                          double zj_zeta, double &ax, double &ay, double &az,
+                                          double &pot) {
+    const double dx = xj - rpx, dy = yj - rpy, dz = zj - rpz;
+    const double drsqd = dx * dx + dy * dy + dz * dz;
+    const double drmag = sqrt(drsqd) + SMALL_NUMBER;
+    const double invdrmag = 1.0 / drmag;
+    double grav, dpot;
+    if (invdrmag > INVKERNRANGE * invhp) {
+        int kern = (int)(HALFKERNTOT * drmag * invhp);
+        kern = min(kern, KERNTOT);
+        grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
+        dpot = invhp * w6[kern];
+    } else {
+        grav = invdrmag * invdrmag;
+        dpot = invdrmag;
+    }
+    if (drmag < KERNRANGE * hj) {
+        const double invhpp = 1.0 / hj;
+        int kern = (int)(HALFKERNTOT * drmag * invhpp);
+        kern = min(kern, KERNTOT);
+        grav = grav + invhpp * invhpp * (w5[kern] + zj_zeta * w4[kern] * invhpp * invhpp);
+        dpot = dpot + invhpp * w6[kern];
+    } else {
+        grav = grav + invdrmag * invdrmag;
+        dpot = dpot + invdrmag;
+    }
+    const double f = 0.5 * mj * invdrmag * grav;
+    ax = ax + f * dx;
+    ay = ay + f * dy;
+    az = az + f * dz;
+    pot = pot + 0.5 * mj * dpot;
+}
+
+// BHgrav_accel.F90:159-222 for one accepted cell
+template <int OCT>
+__device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], double drsqd,
+                                          double &ax, double &ay, double &az, double &pot) {
+    const double *q = g.q;
+    const double invdrsqd = 1.0 / (drsqd + SMALL_NUMBER);
+    const double invdrmag = sqrt(invdrsqd);
+    const double mf = g.m * invdrsqd * invdrmag;
+    double at0 = -mf * dr[0], at1 = -mf * dr[1], at2 = -mf * dr[2];
+    double dpot = g.m * invdrmag;
+    const double invdr5 = invdrsqd * invdrsqd * invdrmag;
+    const double qscalar = q[0] * dr[0] * dr[0] + q[2] * dr[1] * dr[1] -
+                           (q[0] + q[2]) * dr[2] * dr[2] +
+                           2.0 * (q[1] * dr[0] * dr[1] + q[3] * dr[0] * dr[2] +
+                                  q[4] * dr[1] * dr[2]);
+    at0 = at0 + (q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2]) * invdr5 -
+          2.5 * qscalar * dr[0] * invdr5 * invdrsqd;
+    at1 = at1 + (q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2]) * invdr5 -
+          2.5 * qscalar * dr[1] * invdr5 * invdrsqd;
+    at2 = at2 + (q[3] * dr[0] + q[4] * dr[1] - (q[0] + q[2]) * dr[2]) * invdr5 -
+          2.5 * qscalar * dr[2] * invdr5 * invdrsqd;
+    dpot = dpot + 0.5 * qscalar * invdr5;
+    if (OCT) {
+        const double *s = g.s;
+        const double sscalar =
+            s[0] * dr[0] * dr[0] * dr[0] + s[1] * dr[0] * dr[0] * dr[1] +
+            s[2] * dr[1] * dr[1] * dr[0] + s[3] * dr[1] * dr[1] * dr[1] +
+            s[4] * dr[2] * dr[2] * dr[0] + s[5] * dr[2] * dr[2] * dr[1] +
+            s[6] * dr[2] * dr[2] * dr[2] + s[7] * dr[0] * dr[0] * dr[2] +
+            s[8] * dr[1] * dr[1] * dr[2] + s[9] * dr[0] * dr[1] * dr[2];
+        at0 = at0 +
+              0.5 * (3.0 * s[0] * dr[0] * dr[0] + 2.0 * s[1] * dr[0] * dr[1] +
+                     2.0 * s[7] * dr[0] * dr[2] + s[2] * dr[1] * dr[1] +
+                     s[4] * dr[2] * dr[2] + s[9] * dr[1] * dr[2]) *
+                  invdr5 * invdrsqd -
+              3.5 * sscalar * dr[0] * invdr5 * invdrsqd * invdrsqd;
+        at1 = at1 +
+              0.5 * (3.0 * s[3] * dr[1] * dr[1] + 2.0 * s[2] * dr[0] * dr[1] +
+                     2.0 * s[8] * dr[1] * dr[2] + s[1] * dr[0] * dr[0] +
+                     s[5] * dr[2] * dr[2] + s[9] * dr[0] * dr[2]) *
+                  invdr5 * invdrsqd -
+              3.5 * sscalar * dr[1] * invdr5 * invdrsqd * invdrsqd;
+        at2 = at2 +
+              0.5 * (3.0 * s[6] * dr[2] * dr[2] + 2.0 * s[4] * dr[0] * dr[2] +
+                     2.0 * s[5] * dr[1] * dr[2] + s[7] * dr[0] * dr[0] +
+                     s[8] * dr[1] * dr[1] + s[9] * dr[0] * dr[1]) *
+                  invdr5 * invdrsqd -
+              3.5 * sscalar * dr[2] * invdr5 * invdrsqd * invdrsqd;
+        dpot = dpot + 0.5 * sscalar * invdr5 * invdrsqd;
+    }
+    ax = ax + at0;
+    ay = ay + at1;
+    az = az + at2;
+    pot = pot + dpot;
+}
+
+template <int OCT, int MACT>
+__global__ void __launch_bounds__(256) k_grav_fp64(
+    int n, int nc, const GNode *__restrict__ gn, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const double *__restrict__ h, const double *__restrict__ zeta,
+    const uint8_t *__restrict__ active, const double *__restrict__ ktab,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    double *__restrict__ gpot, double *__restrict__ agm,
+    unsigned long long *__restrict__ icount) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n && active[p];
+    const double *w4 = ktab + T_W4 * KT;
+    const double *w5 = ktab + T_W5 * KT;
+    const double *w6 = ktab + T_W6 * KT;
+    double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
+    if (valid) {
+        rpx = x[p]; rpy = y[p]; rpz = z[p];
+        invhp = 1.0 / h[p];
+        zo_p = zeta[p];
+        if (MACT != SERENX_MAC_GEOMETRIC) {   // BHgrav_accel.F90:97-116
+            const double a = agm[p];
+            if (a > SMALL_NUMBER)
+                afactor = (MACT == SERENX_MAC_GADGET) ? pow(a, -ONETHIRD) : 1.0 / sqrt(a);
+        }
+    }
+    double acx = 0, acy = 0, acz = 0, pc = 0;   // cell terms
+    double apx = 0, apy = 0, apz = 0, pp = 0;   // particle-particle terms
+    unsigned ncell = 0, npp = 0;
+    group_walk(gn, nc, valid, [&](const GNode &g, int c) -> bool {
+        const int leaf = g.leaf;
+        if (leaf == 1) {                                  // :128-136
+            const int j = g.first;
+            if (j != p) {
+                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
+                          zeta[j], apx, apy, apz, pp);
+                npp++;
+            }
+            return false;
+        }
+        double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
+        const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        bool accept;
+        if (MACT == SERENX_MAC_GEOMETRIC)
+            accept = drsqd > g.dminsqd;
+        else
+            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
+        if (accept) {
+            cell_fp64<OCT>(g, dr, drsqd, acx, acy, acz, pc);
+            ncell++;
+            return false;
+        }
+        if (leaf > 0) {                                   // :242-253
+            for (int i = 0; i < leaf; i++) {
+                const int j = g.first + i;
+                if (j == p) continue;
+                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
+                          zeta[j], apx, apy, apz, pp);
+                npp++;
+            }
+            return false;
+        }
+        return true;
+    });
+    if (valid) {
+        const double gx = acx + apx, gy = acy + apy, gz = acz + apz;
+        ax[p] = ax[p] + gx;
+        ay[p] = ay[p] + gy;
+        az[p] = az[p] + gz;
+        gpot[p] = pc + pp;
+        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+    }
+    // per-wave interaction counts
+    unsigned long long a = ncell, b = npp;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(icount, a);
+        atomicAdd(icount + 1, b);
+    }
+}
+
+// ---- mixed precision ------------------------------------------------------
+// Neumaier compensated FP32 sum
+struct CSum {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v) {
+        const float t = s + v;
+        c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+__device__ __forceinline__ void pair_fp32(const float *__restrict__ w4, const float *__restrict__ w5,
+                                          const float *__restrict__ w6, float invhp, float zo_p,
+                                          float dx, float dy, float dz, float mj, float hj,
+                                          float zj, CSum &ax, CSum &ay, CSum &az, CSum &pot) {
+    const float drsqd = dx * dx + dy * dy + dz * dz;
+    const float drmag = sqrtf(drsqd) + 1.0e-20f;
+    const float invdrmag = 1.0f / drmag;
+    float grav, dpot;
+    if (invdrmag > 0.5f * invhp) {
+        int kern = (int)(5000.0f * drmag * invhp);
+        kern = min(kern, KERNTOT);
+        grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
+        dpot = invhp * w6[kern];
+    } else {
+        grav = invdrmag * invdrmag;
+        dpot = invdrmag;
+    }
+    if (drmag < 2.0f * hj) {
+        const float invhpp = 1.0f / hj;
+        int kern = (int)(5000.0f * drmag * invhpp);
+        kern = min(kern, KERNTOT);
+        grav = grav + invhpp * invhpp * (w5[kern] + zj * w4[kern] * invhpp * invhpp);
+        dpot = dpot + invhpp * w6[kern];
+    } else {
+        grav = grav + invdrmag * invdrmag;
+        dpot = dpot + invdrmag;
+    }
+    const float f = 0.5f * mj * invdrmag * grav;
+    ax.add(f * dx);
+    ay.add(f * dy);
+    az.add(f * dz);
+    pot.add(0.5f * mj * dpot);
+}
+
+template <int OCT>
+__device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], CSum &ax, CSum &ay,
+                                          CSum &az, CSum &pot) {
+    const float *q = g.q;
+    const float drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    const float invdrmag = rsqrtf(drsqd + 1.0e-30f);
+    const float invdrsqd = invdrmag * invdrmag;
+    const float mf = g.m * invdrsqd * invdrmag;
+    float at0 = -mf * dr[0], at1 = -mf * dr[1], at2 = -mf * dr[2];
+    float dpot = g.m * invdrmag;
+    const float invdr5 = invdrsqd * invdrsqd * invdrmag;
+    const float qscalar = q[0] * dr[0] * dr[0] + q[2] * dr[1] * dr[1] -
+                          (q[0] + q[2]) * dr[2] * dr[2] +
+                          2.0f * (q[1] * dr[0] * dr[1] + q[3] * dr[0] * dr[2] +
+                                  q[4] * dr[1] * dr[2]);
+    const float q7 = 2.5f * qscalar * invdr5 * invdrsqd;
+    at0 += (q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2]) * invdr5 - q7 * dr[0];
+    at1 += (q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2]) * invdr5 - q7 * dr[1];
+    at2 += (q[3] * dr[0] + q[4] * dr[1] - (q[0] + q[2]) * dr[2]) * invdr5 - q7 * dr[2];
+    dpot += 0.5f * qscalar * invdr5;
+    if (OCT) {
+        const float *s = g.s;
+        const float xx = dr[0] * dr[0], yy = dr[1] * dr[1], zz = dr[2] * dr[2];
+        const float sscalar = s[0] * xx * dr[0] + s[1] * xx * dr[1] + s[2] * yy * dr[0] +
+                              s[3] * yy * dr[1] + s[4] * zz * dr[0] + s[5] * zz * dr[1] +
+                              s[6] * zz * dr[2] + s[7] * xx * dr[2] + s[8] * yy * dr[2] +
+                              s[9] * dr[0] * dr[1] * dr[2];
+        const float i7 = invdr5 * invdrsqd;
+        const float s9 = 3.5f * sscalar * i7 * invdrsqd;
+        at0 += 0.5f * (3.0f * s[0] * xx + 2.0f * s[1] * dr[0] * dr[1] +
+                       2.0f * s[7] * dr[0] * dr[2] + s[2] * yy + s[4] * zz +
+                       s[9] * dr[1] * dr[2]) * i7 - s9 * dr[0];
+        at1 += 0.5f * (3.0f * s[3] * yy + 2.0f * s[2] * dr[0] * dr[1] +
+                       2.0f * s[8] * dr[1] * dr[2] + s[1] * xx + s[5] * zz +
+                       s[9] * dr[0] * dr[2]) * i7 - s9 * dr[1];
+        at2 += 0.5f * (3.0f * s[6] * zz + 2.0f * s[4] * dr[0] * dr[2] +
+                       2.0f * s[5] * dr[1] * dr[2] + s[7] * xx + s[8] * yy +
+                       s[9] * dr[0] * dr[1]) * i7 - s9 * dr[2];
+        dpot += 0.5f * sscalar * i7;
+    }
+    ax.add(at0);
+    ay.add(at1);
+    az.add(at2);
+    pot.add(dpot);
+}
+
+template <int OCT, int MACT>
+__global__ void __launch_bounds__(256) k_grav_mixed(
+    int n, int nc, const GNodeF *__restrict__ gn, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const double *__restrict__ h, const double *__restrict__ zeta,
+    const uint8_t *__restrict__ active, const float *__restrict__ ktabf,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    double *__restrict__ gpot, double *__restrict__ agm,
+    unsigned long long *__restrict__ icount) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n && active[p];
+    const float *w4 = ktabf + T_W4 * KT;
+    const float *w5 = ktabf + T_W5 * KT;
+    const float *w6 = ktabf + T_W6 * KT;
+    double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
+    float invhp = 1.0f, zo_p = 0.0f;
+    if (valid) {
+        rpx = x[p]; rpy = y[p]; rpz = z[p];
+        invhp = (float)(1.0 / h[p]);
+        zo_p = (float)zeta[p];
+        if (MACT != SERENX_MAC_GEOMETRIC) {
+            const double a = agm[p];
+            if (a > SMALL_NUMBER)
+                afactor = (MACT == SERENX_MAC_GADGET) ? pow(a, -ONETHIRD) : 1.0 / sqrt(a);
+        }
+    }
+    CSum cx, cy, cz, cp;
+    unsigned ncell = 0, npp = 0;
+    group_walk(gn, nc, valid, [&](const GNodeF &g, int c) -> bool {
+        const int leaf = g.leaf;
+        if (leaf == 1) {
+            const int j = g.first;
+            if (j != p) {
+                pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
+                          (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
+                          cz, cp);
+                npp++;
+            }
+            return false;
+        }
+        const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
+        const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
+        bool accept;
+        if (MACT == SERENX_MAC_GEOMETRIC)
+            accept = drsqd > g.dminsqd;
+        else
+            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
+        if (accept) {
+            const float dr[3] = {(float)d0, (float)d1, (float)d2};
+            cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
+            ncell++;
+            return false;
+        }
+        if (leaf > 0) {
+            for (int i = 0; i < leaf; i++) {
+                const int j = g.first + i;
+                if (j == p) continue;
+                pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
+                          (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
+                          cz, cp);
+                npp++;
+            }
+            return false;
+        }
+        return true;
+    });
+    if (valid) {
+        const double gx = cx.value(), gy = cy.value(), gz = cz.value();
+        ax[p] = ax[p] + gx;
+        ay[p] = ay[p] + gy;
+        az[p] = az[p] + gz;
+        gpot[p] = cp.value();
+        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+    }
+    unsigned long long a = ncell, b = npp;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(icount, a);
+        atomicAdd(icount + 1, b);
+    }
+}
+
+// direct_sph_gravity.F90:63-79: sum of gravity_gradh over all other
+// particles, FP64, sources staged through LDS
+__global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__restrict__ targets,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ z,
+                                                const double *__restrict__ m,
+                                                const double *__restrict__ h,
+                                                const double *__restrict__ zeta,
+                                                const double *__restrict__ ktab,
+                                                double *__restrict__ acc, double *__restrict__ pot) {
+    __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256], szt[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < nt;
+    const int p = valid ? targets[t] : 0;
+    const double *w4 = ktab + T_W4 * KT;
+    const double *w5 = ktab + T_W5 * KT;
+    const double *w6 = ktab + T_W6 * KT;
+    const double rpx = x[p], rpy = y[p], rpz = z[p], invhp = 1.0 / h[p], zo_p = zeta[p];
+    double gx = 0, gy = 0, gz = 0, gp = 0;
+    for (int base = 0; base < n; base += 256) {
+        const int j = base + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = x[j];
+            sy[threadIdx.x] = y[j];
+            sz[threadIdx.x] = z[j];
+            sm[threadIdx.x] = m[j];
+            sh[threadIdx.x] = h[j];
+            szt[threadIdx.x] = zeta[j];
+        }
+        __syncthreads();
+        const int lim = min(256, n - base);
+        if (valid) {
+            for (int k = 0; k < lim; k++) {
+                if (base + k == p) continue;
+                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, sx[k], sy[k], sz[k], sm[k],
+                          sh[k], szt[k], gx, gy, gz, gp);
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        acc[3 * t] = gx;
+        acc[3 * t + 1] = gy;
+        acc[3 * t + 2] = gz;
+        pot[t] = gp;
+    }
+}
+
+template <int OCT, int MACT>
+void launch_grav(Ctx &c) {
+    if (c.opt.precision == SERENX_MIXED)
+        hipLaunchKernelGGL((k_grav_mixed<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
+                           c.nc, c.gnf, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active,
+                           (const float *)(c.ktab + NTAB * KT), c.ax, c.ay, c.az, c.gpot, c.agm,
+                           c.icount);
+    else
+        hipLaunchKernelGGL((k_grav_fp64<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
+                           c.nc, c.gn, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active, c.ktab, c.ax,
+                           c.ay, c.az, c.gpot, c.agm, c.icount);
+}
+
+}  // namespace
+
+int grav_forces(Ctx &c) {
+    SX_HIP(hipMemsetAsync(c.icount, 0, 2 * sizeof(unsigned long long), c.stream));
+    const bool oct = c.opt.multipole == SERENX_OCTUPOLE;
+    switch (c.opt.mac) {
+    case SERENX_MAC_GEOMETRIC:
+        oct ? launch_grav<1, SERENX_MAC_GEOMETRIC>(c) : launch_grav<0, SERENX_MAC_GEOMETRIC>(c);
+        break;
+    case SERENX_MAC_GADGET:
+        oct ? launch_grav<1, SERENX_MAC_GADGET>(c) : launch_grav<0, SERENX_MAC_GADGET>(c);
+        break;
+    default:
+        oct ? launch_grav<1, SERENX_MAC_GADGET2>(c) : launch_grav<0, SERENX_MAC_GADGET2>(c);
+        break;
+    }
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot) {
+    hipLaunchKernelGGL(k_direct, dim3(nblk(nt)), dim3(256), 0, c.stream, c.n, nt, dtargets, c.x,
+                       c.y, c.z, c.m, c.h, c.zeta, c.ktab, dacc, dpot);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+}  // namespace serenx

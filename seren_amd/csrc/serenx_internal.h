@@ -1,0 +1,193 @@
+// serenx_internal.h -- shared definitions of the gfx950 force-evaluation
+// library: constants of the reference option set, device node layouts, the
+// context, and the launch functions each translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/serenx.h"
+
+namespace serenx {
+
+// Constants of the option set (src/headers/macros.h:12-191)
+constexpr int LEAFMAX = 8;
+constexpr int LMAX = 30;
+constexpr int KERNTOT = 10000;
+constexpr double HALFKERNTOT = 5000.0;
+constexpr double KERNRANGE = 2.0;
+constexpr double KERNRANGESQD = 4.0;
+constexpr double INVKERNRANGE = 0.5;
+constexpr double SMALL_NUMBER = 1.0e-20;
+constexpr double BIG_NUMBER = 9.9e20;
+constexpr double HMULT = 1.1;
+constexpr double H_CON = 0.02;
+constexpr int GRADH_ITERATION_MAX = 20;
+constexpr double ETA_SQD = 0.01;
+constexpr double ONETHIRD = 0.333333333;
+constexpr double PI_REF = 3.1415926536;
+constexpr double INVPI = 0.318309886;
+constexpr int NTAB = 9;            // w0 w1 w2 w3 w4 w5 w6 wh wg
+constexpr int KT = KERNTOT + 1;    // entries per table
+enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
+
+constexpr int WAVE = 64;
+
+// Gravity node, DFS pre-order.  Read with wave-uniform indices in the walk, so
+// a node is a run of scalar loads.  next = DFS index of the cell that follows
+// this subtree (nc for the last); leaf = particle count of a leaf, 0 for a
+// branch; first = first particle (tree order) of the cell.
+struct __attribute__((aligned(64))) GNode {
+    double r[3];      // centre of mass
+    double m;
+    double dminsqd;   // opening distance^2 (BHgrav_stock.F90:317-348)
+    double mac;       // multipole MAC term (GADGET/GADGET2)
+    double q[5];      // quadrupole [xx xy yy zx zy]
+    double s[10];     // octupole
+    int next;
+    int leaf;
+    int first;
+    int count;        // particles in the subtree
+};
+
+// Hydro node (BHhydro_node, modules.F90:929-937), DFS order.
+struct __attribute__((aligned(64))) HNode {
+    double r[3];      // bounding-box centre
+    double rmax;      // half-diagonal of the bounding box
+    double hrangemax; // 2 max(h) over the subtree
+    int next;
+    int leaf;
+    int first;
+    int count;
+};
+
+// Mixed-precision gravity node: FP64 COM (the cell offset is subtracted in
+// FP64), FP32 moments.
+struct __attribute__((aligned(64))) GNodeF {
+    double r[3];
+    double dminsqd;
+    double mac;
+    float m;
+    float q[5];
+    float s[10];
+    int next;
+    int leaf;
+    int first;
+    int pad;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    serenx_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    double last_ms[4] = {0, 0, 0, 0};
+
+    int n = 0;          // particles loaded
+    int cap = 0;        // particle capacity of the device arrays
+    int pp_gather = 0, pp_limit = 0;
+    double daux = 0.0;  // 10 huge^-0.2 (BHgrav_stock.F90:347)
+
+    // particle SoA, tree order
+    double *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr, *h = nullptr,
+           *zeta = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr,
+           *rho = nullptr, *omega = nullptr, *press = nullptr, *sound = nullptr,
+           *divv = nullptr, *drhodt = nullptr, *ax = nullptr, *ay = nullptr,
+           *az = nullptr, *gpot = nullptr, *agm = nullptr;
+    int *porig = nullptr;
+    uint8_t *active = nullptr;   // accdo
+    double *tmpd = nullptr;      // permutation scratch (cap doubles)
+    int *tmpi = nullptr;
+
+    // tree build scratch
+    uint64_t *khi = nullptr, *klo = nullptr, *khi2 = nullptr, *klo2 = nullptr;
+    int *idx = nullptr, *idx2 = nullptr;
+    uint8_t *cpl = nullptr, *pact = nullptr;
+    int *flag = nullptr, *incl = nullptr, *cellof = nullptr;
+    int *ncs = nullptr, *off = nullptr;
+    int8_t *firstlev = nullptr;
+    double *root = nullptr;      // [rc x y z, cellsize0, rextent, rmin3, rmax3]
+    double *bbpart = nullptr;    // bounding-box block partials
+    void *sorttmp = nullptr;
+    size_t sorttmp_bytes = 0;
+    int *hostpin = nullptr;      // pinned readback slots
+    double *hostpin_d = nullptr;
+
+    // cells by BFS id (capacity 2 cap + 2)
+    int ccap = 0;
+    int *c_start = nullptr, *c_end = nullptr, *c_parent = nullptr, *c_leaf = nullptr,
+        *c_level = nullptr, *bfs2dfs = nullptr, *dfs2bfs = nullptr, *dfs_parent = nullptr;
+    double *bbmin = nullptr, *bbmax = nullptr, *hmaxs = nullptr;  // per DFS cell
+    GNode *gn = nullptr;
+    GNodeF *gnf = nullptr;
+    HNode *hn = nullptr;
+    int nc = 0, ltot = 0;
+    int level_off[LMAX + 2] = {0};
+    int level_cnt[LMAX + 2] = {0};
+    double rextent = 0.0;
+    bool tree_ok = false, sph_ok = false;
+
+    // neighbour lists: rows of pp_limit ids (tree order, 0-based) + overflow CSR
+    int *pptot = nullptr, *pplist = nullptr;
+    int *ovf_off = nullptr, *ovf_list = nullptr;  // rows with pptot > pp_limit
+    size_t ovf_cap = 0;
+    int *ovf_count = nullptr;     // device scalar: number of overflow particles
+
+    // kernel tables [NTAB][KT] (kernel.F90)
+    double *ktab = nullptr;
+    double host_ktab[NTAB * KT];
+
+    // interaction counters of the gravity walk
+    unsigned long long *icount = nullptr;
+
+    // host-state binding
+    serenx_host_state hs{};
+    bool bound = false;
+    std::vector<double> hbuf;   // staging for host-state transfers
+    std::vector<int> hibuf;
+};
+
+// error reporting (capi.hip)
+int set_error(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+#define SX_HIP(call)                                                     \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) return ::serenx::hip_check(e_, #call);     \
+    } while (0)
+#define SX_TRY(call)                                                     \
+    do {                                                                 \
+        int r_ = (call);                                                 \
+        if (r_ != SERENX_OK) return r_;                                  \
+    } while (0)
+
+// host-side kernel tables (capi.hip): kernel.F90:14-127, M4, NDIM=3
+void make_kernel_tables(double *tab);
+
+// tree.hip
+int ensure_tree_buffers(Ctx &c);
+int build_tree(Ctx &c);          // bbox, keys, sort, cells, DFS, permutation
+int stock_tree(Ctx &c);          // BHgrav_stock + BHhydro_stock (fused)
+int hguess(Ctx &c);              // BHhydro_hguess
+int update_hmax(Ctx &c);         // BHhydro_update_hmax
+int permute_particles(Ctx &c, const int *perm);
+
+// sph.hip
+int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles
+int sph_neighbours(Ctx &c);      // get_neib over active particles
+int sph_thermal_iso(Ctx &c, double cs);
+int hydro_forces(Ctx &c);        // hydro_gradh over active particles
+
+// grav.hip
+int grav_forces(Ctx &c);         // BHgrav_accel over active particles
+int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot);
+
+}  // namespace serenx

@@ -1,0 +1,378 @@
+// sph.hip -- grad-h smoothing-length / density solve, neighbour lists,
+// isothermal thermal update and grad-h hydro pair forces on gfx950.
+//
+// Each wave owns 64 consecutive particles in tree order (a compact leaf
+// group) and walks the hydro tree once per pass for all of them with
+// group_walk; every lane keeps its own state machine of all_sph_gradh.
+// Sums run over the walk's leaves in DFS order, which is the order of the
+// reference's potential-neighbour list, and use the reference's operation
+// order (built without FMA contraction), so h, rho, Omega, zeta and the
+// neighbour lists come out identical to the reference.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "walk.h"
+
+namespace serenx {
+
+namespace {
+
+constexpr int TB = 256;
+inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
+
+// lane modes of the h solve
+enum { M_DONE = 0, M_ITER = 1, M_FINAL = 2 };
+
+// all_sph_gradh.F90:103-489 for every active particle
+__global__ void __launch_bounds__(256) k_hsolve(
+    int n, int nc, const HNode *__restrict__ hn, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
+    const uint8_t *__restrict__ active, const double *__restrict__ ktab, double h_fac,
+    double rextent, int pp_gather, double *__restrict__ h, double *__restrict__ zeta,
+    double *__restrict__ rho, double *__restrict__ omega, double *__restrict__ divv,
+    double *__restrict__ drhodt, int *__restrict__ err) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n && active[p];
+    const double *w0 = ktab + T_W0 * KT;
+    const double *w4 = ktab + T_W4 * KT;
+    const double *wh = ktab + T_WH * KT;
+    const double *wg = ktab + T_WG * KT;
+    double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1, vpx = 0, vpy = 0, vpz = 0;
+    if (valid) {
+        rpx = x[p]; rpy = y[p]; rpz = z[p];
+        mp = m[p]; hp = h[p];
+        vpx = vx[p]; vpy = vy[p]; vpz = vz[p];
+    }
+    double h_high = 0.0, h_old = hp, h_lower = 0.0, h_upper = rextent;
+    double rhotemp = 0.0;
+    int iteration = 0;
+    int mode = valid ? M_ITER : M_DONE;
+
+    while (__any(mode != M_DONE)) {
+        bool newlist = false;
+        if (mode == M_ITER && hp > h_high) {
+            h_high = hp * HMULT;
+            newlist = true;
+        }
+        if (mode == M_FINAL) hp = h_fac * ref_pow_third(mp / rhotemp);  // :317
+        const double R = KERNRANGE * h_high;
+        const double invhp = 1.0 / hp;
+        const double hfactor_p = cr_pow3(invhp);
+        const double hrangesqd = KERNRANGESQD * hp * hp;
+        // self contributions (:209-210, :323-346)
+        double srho = mp * w0[0] * hfactor_p;
+        double som = mp * hfactor_p * invhp * wh[0];
+        double sdiv = 0.0;
+        double szeta = mp * invhp * invhp * wg[0];
+        int cnt = 0;
+        const int lmode = mode;
+        group_walk(hn, nc, mode != M_DONE, [&](const HNode &nd, int c) -> bool {
+            const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+            const double drsqd = dx * dx + dy * dy + dz * dz;
+            const double lim = (R + nd.rmax) * (R + nd.rmax);
+            if (!(drsqd <= lim)) return false;
+            const int k = nd.leaf;
+            if (k == 0) return true;
+            const int f0 = nd.first;
+            cnt += k;
+            for (int i = 0; i < k; i++) {
+                const int j = f0 + i;
+                if (j == p) continue;
+                const double ex = x[j] - rpx, ey = y[j] - rpy, ez = z[j] - rpz;
+                double dsq = ex * ex + ey * ey + ez * ez;
+                if (lmode == M_ITER) {
+                    if (dsq >= hrangesqd) continue;                 // :223
+                    const double drmag = sqrt(dsq) + SMALL_NUMBER;
+                    const int kern = (int)(HALFKERNTOT * drmag * invhp);
+                    if (kern >= KERNTOT) continue;                  // :227
+                    const double mpp = m[j];
+                    srho = srho + mpp * hfactor_p * w0[kern];
+                    som = som + mpp * hfactor_p * invhp * wh[kern];
+                } else {
+                    if (dsq > hrangesqd) continue;                  // :364
+                    const double drmag = sqrt(dsq) + SMALL_NUMBER;
+                    const int kern = (int)(HALFKERNTOT * drmag * invhp);
+                    if (kern > KERNTOT) continue;                   // :368
+                    const double mpp = m[j];
+                    const double dvx = vx[j] - vpx, dvy = vy[j] - vpy, dvz = vz[j] - vpz;
+                    const double dvdr = dvx * ex + dvy * ey + dvz * ez;
+                    srho = srho + mpp * hfactor_p * w0[kern];
+                    som = som + mpp * hfactor_p * invhp * wh[kern];
+                    sdiv = sdiv - mpp * dvdr * w4[kern] * hfactor_p * invhp / drmag;
+                    szeta = szeta + mpp * invhp * invhp * wg[kern];
+                }
+            }
+            return false;
+        });
+
+        if (mode == M_ITER && newlist && cnt < pp_gather) {  // :185-189
+            h_high = 0.0;
+            hp = hp * HMULT;
+        } else if (mode == M_ITER) {
+            rhotemp = srho;
+            const double hnew = h_fac * ref_pow_third(mp / rhotemp);
+            if (fabs(hp - hnew) / hp < H_CON && iteration >= 1) {  // :246
+                mode = M_FINAL;
+            } else if (iteration < GRADH_ITERATION_MAX) {           // :252
+                hp = hnew;
+                iteration++;
+            } else if (iteration == GRADH_ITERATION_MAX) {          // :256
+                h_upper = rextent;
+                h_lower = 0.0;
+                hp = 0.5 * (h_lower + h_upper);
+                iteration++;
+            } else if (iteration < 2 * GRADH_ITERATION_MAX) {       // :262
+                double fb;
+                if (rhotemp > SMALL_NUMBER)
+                    fb = cr_pow3(hp) - cr_pow3(h_fac) * mp / rhotemp;
+                else
+                    fb = 1.0;
+                if (fb > 0.0)
+                    h_upper = hp;
+                else
+                    h_lower = hp;
+                hp = 0.5 * (h_lower + h_upper);
+                iteration++;
+            } else {                                                // :276-279
+                hp = h_old;
+                mode = M_FINAL;
+            }
+        } else if (mode == M_FINAL) {
+            const double om = 1.0 + ((hp * som) / (3.0 * srho));
+            const double dv = sdiv / srho;
+            rho[p] = srho;
+            omega[p] = om;
+            divv[p] = dv;
+            drhodt[p] = -srho * dv / om;
+            h[p] = hp;
+            zeta[p] = -(hp * szeta) / (3.0 * srho * om);
+            if (hp < 0.0) atomicOr(err, 1);
+            mode = M_DONE;
+        }
+    }
+}
+
+// get_neib.F90:46-118: gather-or-scatter neighbours in walk (DFS) order.
+// Rows of pp_limit ids; particles with more neighbours only get their count
+// here (the reference does not store their list either, :115-118) and are
+// listed completely by the OVERFLOW_PASS instance.
+template <bool OVERFLOW_PASS>
+__global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__restrict__ hn,
+                                              const double *__restrict__ x,
+                                              const double *__restrict__ y,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ h,
+                                              const uint8_t *__restrict__ active, int pp_limit,
+                                              int *__restrict__ pptot, int *__restrict__ pplist,
+                                              const int *__restrict__ ovf_off,
+                                              int *__restrict__ ovf_list,
+                                              int *__restrict__ ovf_count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = p < n && active[p];
+    if (OVERFLOW_PASS && valid) valid = pptot[p] > pp_limit;
+    double rpx = 0, rpy = 0, rpz = 0, hp = 0;
+    if (valid) {
+        rpx = x[p]; rpy = y[p]; rpz = z[p];
+        hp = h[p];
+    }
+    const double hrangesqd_p = KERNRANGESQD * hp * hp;
+    const double R = KERNRANGE * hp;
+    int tot = 0;
+    int *row = nullptr;
+    int lim = pp_limit;   // entries this lane may store
+    if (valid) {
+        row = OVERFLOW_PASS ? ovf_list + ovf_off[p] : pplist + (size_t)p * pp_limit;
+        if (OVERFLOW_PASS) lim = pptot[p];
+    }
+    group_walk(hn, nc, valid, [&](const HNode &nd, int c) -> bool {
+        const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+        const double drsqd = dx * dx + dy * dy + dz * dz;
+        const double m1 = (R + nd.rmax) * (R + nd.rmax);
+        const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
+        if (!(drsqd <= m1 || drsqd <= m2)) return false;   // BHhydro_walk.F90:43
+        const int k = nd.leaf;
+        if (k == 0) return true;
+        const int f0 = nd.first;
+        for (int i = 0; i < k; i++) {
+            const int j = f0 + i;
+            if (j == p) continue;
+            const double hj = h[j];
+            const double hrangesqd_pp = KERNRANGESQD * hj * hj;
+            const double ex = x[j] - rpx, ey = y[j] - rpy, ez = z[j] - rpz;
+            const double dsq = ex * ex + ey * ey + ez * ez;
+            if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
+                if (tot < lim) row[tot] = j;
+                tot++;
+            }
+        }
+        return false;
+    });
+    if (valid && !OVERFLOW_PASS) {
+        pptot[p] = tot;
+        if (tot > pp_limit) atomicAdd(ovf_count, 1);
+    }
+}
+
+__global__ void k_ovf_sizes(int n, int pp_limit, const int *__restrict__ pptot,
+                            const uint8_t *__restrict__ active, int *__restrict__ sz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sz[i] = (active[i] && pptot[i] > pp_limit) ? pptot[i] : 0;
+}
+
+// thermal.F90:52-60 with the sound speed given: P = cs^2 rho, c = cs
+__global__ void k_thermal_iso(int n, double cs, const uint8_t *__restrict__ active,
+                              const double *__restrict__ rho, double *__restrict__ press,
+                              double *__restrict__ sound) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !active[i]) return;
+    press[i] = cs * cs * rho[i];
+    sound[i] = cs;
+}
+
+// hydro_gradh.F90:87-297 (AB viscosity, beta := 2 alpha, isothermal)
+__global__ void __launch_bounds__(256) k_hydro(
+    int n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ h,
+    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
+    const double *__restrict__ rho, const double *__restrict__ omega,
+    const double *__restrict__ press, const double *__restrict__ sound,
+    const uint8_t *__restrict__ active, const int *__restrict__ pptot,
+    const int *__restrict__ pplist, const int *__restrict__ ovf_off,
+    const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab,
+    double alpha, double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double *w1 = ktab + T_W1 * KT;
+    const double rpx = x[p], rpy = y[p], rpz = z[p];
+    const double hp = h[p], invhp = 1.0 / hp, hfactor_p = cr_pow4(invhp);
+    const double vpx = vx[p], vpy = vy[p], vpz = vz[p];
+    const double sound_p = sound[p], rho_p = rho[p];
+    const double pfactor_p = press[p] / rho_p / rho_p / omega[p];
+    const double beta = 2.0 * alpha;
+    double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0;
+    const int nn = pptot[p];
+    const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
+    for (int i = 0; i < nn; i++) {
+        const int pp = lst[i];
+        double wmean = 0.0;
+        const double rho_pp = rho[pp], mpp = m[pp], hpp = h[pp];
+        double ux = x[pp] - rpx, uy = y[pp] - rpy, uz = z[pp] - rpz;
+        double drmag = ux * ux + uy * uy + uz * uz;
+        if (drmag >= KERNRANGESQD * hp * hp && drmag >= KERNRANGESQD * hpp * hpp) continue;
+        drmag = sqrt(drmag) + SMALL_NUMBER;
+        const double dvx = vx[pp] - vpx, dvy = vy[pp] - vpy, dvz = vz[pp] - vpz;
+        const double dvdr = dvx * ux + dvy * uy + dvz * uz;
+        const double invdrmag = 1.0 / drmag;
+        ux = ux * invdrmag;
+        uy = uy * invdrmag;
+        uz = uz * invdrmag;
+        const double pfactor_pp = press[pp] / rho_pp / rho_pp / omega[pp];
+        if (drmag < KERNRANGE * hp) {
+            int kern = (int)(HALFKERNTOT * drmag * invhp);
+            kern = min(kern, KERNTOT);
+            const double wk = w1[kern];
+            wmean = 0.5 * hfactor_p * wk;
+            const double f = mpp * pfactor_p * hfactor_p * wk;
+            ahx = ahx + f * ux;
+            ahy = ahy + f * uy;
+            ahz = ahz + f * uz;
+        }
+        if (drmag < KERNRANGE * hpp) {
+            const double invhpp = 1.0 / hpp, hfactor_pp = cr_pow4(invhpp);
+            int kern = (int)(HALFKERNTOT * drmag * invhpp);
+            kern = min(kern, KERNTOT);
+            const double wk = w1[kern];
+            wmean = wmean + 0.5 * hfactor_pp * wk;
+            const double f = mpp * pfactor_pp * hfactor_pp * wk;
+            ahx = ahx + f * ux;
+            ahy = ahy + f * uy;
+            ahz = ahz + f * uz;
+        }
+        const double rhomean = 0.5 * (rho_p + rho_pp);
+        if (dvdr < 0.0) {
+            const double vsignal = 0.5 * (sound_p + sound[pp]);
+            const double mu = 0.5 * (hp + hpp) * dvdr /
+                              (drmag * drmag + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
+            const double visc = (-alpha * vsignal * mu + beta * mu * mu) / rhomean;
+            const double f = mpp * visc * wmean;
+            adx = adx + f * ux;
+            ady = ady + f * uy;
+            adz = adz + f * uz;
+        }
+    }
+    ax[p] = ahx + adx;
+    ay[p] = ahy + ady;
+    az[p] = ahz + adz;
+}
+
+}  // namespace
+
+int sph_hsolve(Ctx &c) {
+    if (c.n < c.pp_gather)
+        return set_error(SERENX_EFEW, "fewer particles than pp_gather: the grad-h solve "
+                                      "cannot reach its neighbour number");
+    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn, c.x,
+                       c.y, c.z, c.m, c.vx, c.vy, c.vz, c.active, c.ktab, c.opt.h_fac, c.rextent,
+                       c.pp_gather, c.h, c.zeta, c.rho, c.omega, c.divv, c.drhodt, c.flag);
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    return SERENX_OK;
+}
+
+int sph_neighbours(Ctx &c) {
+    hipStream_t s = c.stream;
+    SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x, c.y,
+                       c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
+                       c.ovf_count);
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    if (c.hostpin[8] != 0)
+        return set_error(SERENX_EHNEG, "Negative smoothing lengths (all_sph_gradh)");
+    if (c.hostpin[9] > 0) {
+        // complete lists for particles above pp_limit, packed by offset
+        hipLaunchKernelGGL(k_ovf_sizes, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.pp_limit, c.pptot,
+                           c.active, c.flag);
+        SX_HIP(hipMemsetAsync(c.flag + c.n, 0, sizeof(int), s));
+        size_t tb = c.sorttmp_bytes;
+        SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.flag, c.ovf_off, 0, (size_t)c.n + 1,
+                                       rocprim::plus<int>(), s));
+        SX_HIP(hipMemcpyAsync(c.hostpin + 10, c.ovf_off + c.n, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        SX_HIP(hipStreamSynchronize(s));
+        const size_t need = (size_t)c.hostpin[10];
+        if (need > c.ovf_cap) {
+            if (c.ovf_list) (void)hipFree(c.ovf_list);
+            c.ovf_list = nullptr;
+            c.ovf_cap = need + need / 4 + 1024;
+            if (hipMalloc(&c.ovf_list, sizeof(int) * c.ovf_cap) != hipSuccess)
+                return set_error(SERENX_ENOMEM, "hipMalloc failed for overflow neighbour lists");
+        }
+        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x,
+                           c.y, c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off,
+                           c.ovf_list, c.ovf_count);
+        SX_HIP(hipGetLastError());
+    }
+    return SERENX_OK;
+}
+
+int sph_thermal_iso(Ctx &c, double cs) {
+    hipLaunchKernelGGL(k_thermal_iso, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, cs, c.active,
+                       c.rho, c.press, c.sound);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int hydro_forces(Ctx &c) {
+    hipLaunchKernelGGL(k_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z, c.m,
+                       c.h, c.vx, c.vy, c.vz, c.rho, c.omega, c.press, c.sound, c.active, c.pptot,
+                       c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab, c.opt.alpha, c.ax,
+                       c.ay, c.az);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+}  // namespace serenx

@@ -1,0 +1,721 @@
+// tree.hip -- Barnes-Hut octree build, particle reordering and stocking on
+// gfx950.
+//
+// The reference builds its octree level by level, comparing each particle
+// with the centre of its cell at every level (BHgrav_build.F90:94-291), and
+// numbers new cells serially in (parent, octant) order.  Here the same
+// comparisons are done per particle up front: following the FP64 centre
+// recurrence rc(l+1) = rc(l) +/- cs(l+1), cs(l+1) = cs(l)/2 of
+// BHgrav_build.F90:95,121-125,212-228, each particle gets its octant path to
+// LMAX = 30 levels as a 90-bit key (levels 1-21 in khi, 22-30 in klo).  One
+// radix sort of the keys groups every cell's particles contiguously, in the
+// reference's (parent, octant) = BFS order at every level.  Cells are then
+// emitted level by level as runs of equal key prefix inside non-leaf parents;
+// a run with <= LEAFMAX particles is a leaf.  Within a leaf the particles are
+// put back in their previous array order, which is the reference's plist
+// order, and the concatenation of the leaves is the reference's tree order
+// (BH_reorder_particles.F90:51-78).
+//
+// Cells are stored in DFS pre-order for the walks (ifopen = c + 1, nextcell =
+// first cell after the subtree) and keep their BFS id for export.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "serenx_internal.h"
+
+namespace serenx {
+
+namespace {
+
+constexpr int TB = 256;
+
+inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
+
+template <class T>
+int dalloc(T **p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) return set_error(SERENX_ENOMEM, "hipMalloc failed in tree buffers");
+    return SERENX_OK;
+}
+
+// ---- bounding box (bounding_box.F90:11-43) ------------------------------
+__global__ void __launch_bounds__(256) k_bbox_partial(int n, const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      double *__restrict__ part) {
+    __shared__ double s[6][256];
+    double mn0 = BIG_NUMBER, mn1 = BIG_NUMBER, mn2 = BIG_NUMBER;
+    double mx0 = -BIG_NUMBER, mx1 = -BIG_NUMBER, mx2 = -BIG_NUMBER;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = x[i], b = y[i], c = z[i];
+        mn0 = fmin(mn0, a); mx0 = fmax(mx0, a);
+        mn1 = fmin(mn1, b); mx1 = fmax(mx1, b);
+        mn2 = fmin(mn2, c); mx2 = fmax(mx2, c);
+    }
+    int t = threadIdx.x;
+    s[0][t] = mn0; s[1][t] = mn1; s[2][t] = mn2;
+    s[3][t] = mx0; s[4][t] = mx1; s[5][t] = mx2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 3; k++) s[k][t] = fmin(s[k][t], s[k][t + w]);
+            for (int k = 3; k < 6; k++) s[k][t] = fmax(s[k][t], s[k][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t < 6) part[blockIdx.x * 6 + t] = s[t][0];
+}
+
+// root cell: centre 0.5 (rmin + rmax), half-size 0.5 max extent
+// (BHgrav_build.F90:65-67); rextent = max extent (bounding_box.F90:39)
+__global__ void k_bbox_final(int nb, const double *__restrict__ part, double *__restrict__ root) {
+    if (threadIdx.x != 0) return;
+    double mn[3] = {BIG_NUMBER, BIG_NUMBER, BIG_NUMBER};
+    double mx[3] = {-BIG_NUMBER, -BIG_NUMBER, -BIG_NUMBER};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(mn[k], part[6 * b + k]);
+            mx[k] = fmax(mx[k], part[6 * b + 3 + k]);
+        }
+    double e0 = mx[0] - mn[0], e1 = mx[1] - mn[1], e2 = mx[2] - mn[2];
+    double ext = e0;
+    if (e1 > ext) ext = e1;
+    if (e2 > ext) ext = e2;
+    for (int k = 0; k < 3; k++) root[k] = 0.5 * (mn[k] + mx[k]);
+    root[3] = 0.5 * ext;
+    root[4] = ext;
+    for (int k = 0; k < 3; k++) {
+        root[5 + k] = mn[k];
+        root[8 + k] = mx[k];
+    }
+}
+
+// ---- octant keys ---------------------------------------------------------
+__global__ void __launch_bounds__(256) k_keys(int n, const double *__restrict__ x,
+                                              const double *__restrict__ y,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ root,
+                                              uint64_t *__restrict__ khi,
+                                              uint64_t *__restrict__ klo,
+                                              int *__restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = x[i], py = y[i], pz = z[i];
+    double cx = root[0], cy = root[1], cz = root[2], cs = root[3];
+    uint64_t hi = 0, lo = 0;
+#pragma unroll
+    for (int l = 1; l <= LMAX; l++) {
+        cs = 0.5 * cs;
+        const bool bx = px > cx, by = py > cy, bz = pz > cz;
+        const uint64_t d = (uint64_t)bx | ((uint64_t)by << 1) | ((uint64_t)bz << 2);
+        cx = bx ? cx + cs : cx - cs;
+        cy = by ? cy + cs : cy - cs;
+        cz = bz ? cz + cs : cz - cs;
+        if (l <= 21)
+            hi |= d << (3 * (21 - l));
+        else
+            lo |= d << (3 * (30 - l));
+    }
+    khi[i] = hi;
+    klo[i] = lo;
+    idx[i] = i;
+}
+
+__global__ void k_gather_u64(int n, const int *__restrict__ idx, const uint64_t *__restrict__ src,
+                             uint64_t *__restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// any run of > LEAFMAX equal 21-level keys needs the deeper 9 levels sorted
+__global__ void k_deep_check(int n, const uint64_t *__restrict__ khi, int *__restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + LEAFMAX < n && khi[i] == khi[i + LEAFMAX]) *flag = 1;
+}
+
+// common key prefix of sorted neighbours i-1, i in levels (0..30)
+__global__ void k_cpl(int n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
+                      uint8_t *__restrict__ cpl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0) {
+        cpl[0] = 0;
+        return;
+    }
+    uint64_t a = khi[i - 1] ^ khi[i];
+    int lv;
+    if (a) {
+        lv = (__clzll(a) - 1) / 3;       // bits 62..0 hold levels 1..21
+    } else {
+        uint64_t b = klo[i - 1] ^ klo[i];
+        lv = b ? 21 + (__clzll(b) - 37) / 3 : 30;  // bits 26..0 hold levels 22..30
+    }
+    cpl[i] = (uint8_t)lv;
+}
+
+__global__ void k_init_levels(int n, uint8_t *pact, int *cellof, int *ncs, int8_t *firstlev,
+                              int *c_start, int *c_end, int *c_parent, int *c_leaf, int *c_level) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) {
+        ncs[i] = (i == 0) ? 1 : 0;
+        firstlev[i] = (i == 0) ? 0 : -1;
+    }
+    if (i < n) {
+        pact[i] = 1;
+        cellof[i] = 0;
+    }
+    if (i == 0) {
+        c_start[0] = 0;
+        c_end[0] = n;
+        c_parent[0] = -1;
+        c_leaf[0] = 0;  // the root is always divided (BHgrav_build.F90:74)
+        c_level[0] = 0;
+    }
+}
+
+// child-start flags at level L: an active particle starts a cell when its
+// L-prefix differs from its predecessor's
+__global__ void k_mark(int n, int L, const uint8_t *__restrict__ pact,
+                       const uint8_t *__restrict__ cpl, int *__restrict__ f) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f[i] = (pact[i] && (i == 0 || cpl[i] < L)) ? 1 : 0;
+}
+
+__global__ void k_emit(int n, int L, int base, const uint8_t *__restrict__ pact,
+                       const int *__restrict__ f, const int *__restrict__ incl,
+                       int *__restrict__ cellof, int *__restrict__ c_start,
+                       int *__restrict__ c_end, int *__restrict__ c_parent,
+                       int *__restrict__ c_level, int *__restrict__ ncs,
+                       int8_t *__restrict__ firstlev) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !pact[i]) return;
+    const int c = base + incl[i] - 1;
+    if (f[i]) {
+        c_start[c] = i;
+        c_parent[c] = cellof[i];
+        c_level[c] = L;
+        if (ncs[i] == 0) firstlev[i] = (int8_t)L;
+        ncs[i] += 1;
+    }
+    cellof[i] = c;
+    atomicMax(&c_end[c], i + 1);
+}
+
+__global__ void k_leafcells(int base, int cnt, const int *__restrict__ c_start,
+                            const int *__restrict__ c_end, int *__restrict__ c_leaf) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    const int c = base + j;
+    const int k = c_end[c] - c_start[c];
+    c_leaf[c] = (k <= LEAFMAX) ? k : 0;
+}
+
+__global__ void k_deactivate(int n, uint8_t *__restrict__ pact, const int *__restrict__ cellof,
+                             const int *__restrict__ c_leaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && pact[i] && c_leaf[cellof[i]] > 0) pact[i] = 0;
+}
+
+// DFS pre-order index of every cell: cells starting at sorted position s form
+// a chain of consecutive levels, so dfs = off[s] + (L - firstlev[s]); the
+// cell after a subtree is the first cell starting at its end.
+__global__ void k_dfs(int n, int nc, const int *__restrict__ c_start, const int *__restrict__ c_end,
+                      const int *__restrict__ c_level, const int *__restrict__ c_leaf,
+                      const int *__restrict__ off, const int8_t *__restrict__ firstlev,
+                      int *__restrict__ bfs2dfs, int *__restrict__ dfs2bfs, GNode *__restrict__ gn,
+                      HNode *__restrict__ hn) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int s = c_start[c], e = c_end[c], L = c_level[c];
+    const int d = off[s] + (L - firstlev[s]);
+    bfs2dfs[c] = d;
+    dfs2bfs[d] = c;
+    const int nxt = (e < n) ? off[e] : nc;
+    gn[d].next = nxt;
+    gn[d].leaf = c_leaf[c];
+    gn[d].first = s;
+    gn[d].count = e - s;
+    hn[d].next = nxt;
+    hn[d].leaf = c_leaf[c];
+    hn[d].first = s;
+    hn[d].count = e - s;
+}
+
+__global__ void k_dfs_parent(int nc, const int *__restrict__ c_parent, const int *__restrict__ bfs2dfs,
+                             int *__restrict__ dfs_parent) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int p = c_parent[c];
+    dfs_parent[bfs2dfs[c]] = (p >= 0) ? bfs2dfs[p] : -1;
+}
+
+// leaf particles back into their previous array order (the plist order of
+// BHgrav_build.F90:195-204, which follows the linked list in array order)
+__global__ void k_leafsort(int nc, const int *__restrict__ c_start, const int *__restrict__ c_leaf,
+                           int *__restrict__ idx) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int k = c_leaf[c];
+    if (k <= 1) return;
+    int v[LEAFMAX];
+    const int s = c_start[c];
+    for (int i = 0; i < k; i++) v[i] = idx[s + i];
+    for (int i = 1; i < k; i++) {
+        int t = v[i], j = i - 1;
+        while (j >= 0 && v[j] > t) {
+            v[j + 1] = v[j];
+            j--;
+        }
+        v[j + 1] = t;
+    }
+    for (int i = 0; i < k; i++) idx[s + i] = v[i];
+}
+
+template <class T>
+__global__ void k_gather(int n, const int *__restrict__ perm, const T *__restrict__ src,
+                         T *__restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+// ---- stocking: BHgrav_stock.F90:53-354 and BHhydro_stock.F90:40-159 -------
+// One thread per cell of one level, deepest level first.  Arithmetic follows
+// the reference's operation order (built without FMA contraction) so that
+// moments, COM and opening distances are bit-identical.
+__global__ void __launch_bounds__(256) k_stock(
+    int nc, int base, int cnt, const int *__restrict__ bfs2dfs, GNode *__restrict__ gn,
+    HNode *__restrict__ hn, double *__restrict__ bbmin, double *__restrict__ bbmax,
+    double *__restrict__ hmaxs, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ h,
+    int octupole, int mac, double thetamaxsqd, double abserror, double daux) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    const int c = bfs2dfs[base + j];
+    GNode &g = gn[c];
+    const int leaf = g.leaf, first = g.first, next = g.next;
+    double mc = 0.0, rc[3] = {0.0, 0.0, 0.0}, dr[3], drsqd;
+    double bmn[3] = {BIG_NUMBER, BIG_NUMBER, BIG_NUMBER};
+    double bmx[3] = {-BIG_NUMBER, -BIG_NUMBER, -BIG_NUMBER};
+    double hmax = 0.0;
+    double qc[5] = {0, 0, 0, 0, 0};
+    double sc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double gm, gr[3];
+    if (leaf == 1) {
+        const int p = first;
+        rc[0] = x[p]; rc[1] = y[p]; rc[2] = z[p];
+        gr[0] = rc[0]; gr[1] = rc[1]; gr[2] = rc[2];
+        gm = m[p];
+        hmax = h[p];
+        for (int k = 0; k < 3; k++) bmn[k] = bmx[k] = rc[k];
+    } else if (leaf > 0) {
+        for (int i = 0; i < leaf; i++) {
+            const int p = first + i;
+            const double pr[3] = {x[p], y[p], z[p]};
+            const double mp = m[p];
+            mc = mc + mp;
+            hmax = fmax(hmax, h[p]);
+            for (int k = 0; k < 3; k++) rc[k] = rc[k] + mp * pr[k];
+            for (int k = 0; k < 3; k++) {
+                bmx[k] = fmax(bmx[k], pr[k]);
+                bmn[k] = fmin(bmn[k], pr[k]);
+            }
+        }
+        for (int k = 0; k < 3; k++) rc[k] = rc[k] / mc;
+        for (int i = 0; i < leaf; i++) {
+            const int p = first + i;
+            dr[0] = x[p] - rc[0]; dr[1] = y[p] - rc[1]; dr[2] = z[p] - rc[2];
+            const double mp = m[p];
+            drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            qc[0] = qc[0] + mp * (3.0 * dr[0] * dr[0] - drsqd);
+            qc[1] = qc[1] + mp * (3.0 * dr[0] * dr[1]);
+            qc[2] = qc[2] + mp * (3.0 * dr[1] * dr[1] - drsqd);
+            qc[3] = qc[3] + mp * (3.0 * dr[2] * dr[0]);
+            qc[4] = qc[4] + mp * (3.0 * dr[2] * dr[1]);
+            if (octupole) {
+                sc[0] = sc[0] + mp * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd);
+                sc[1] = sc[1] + mp * dr[1] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd);
+                sc[2] = sc[2] + mp * dr[0] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd);
+                sc[3] = sc[3] + mp * dr[1] * (5.0 * dr[1] * dr[1] - 3.0 * drsqd);
+                sc[4] = sc[4] + mp * dr[0] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd);
+                sc[5] = sc[5] + mp * dr[1] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd);
+                sc[6] = sc[6] + mp * dr[2] * (5.0 * dr[2] * dr[2] - 3.0 * drsqd);
+                sc[7] = sc[7] + mp * dr[2] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd);
+                sc[8] = sc[8] + mp * dr[2] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd);
+                sc[9] = sc[9] + 15.0 * mp * dr[0] * dr[1] * dr[2];
+            }
+        }
+        gm = mc;
+        gr[0] = rc[0]; gr[1] = rc[1]; gr[2] = rc[2];
+    } else {
+        // branch: children are c+1 and the chain of next pointers up to next
+        for (int cc = c + 1; cc != next && cc < nc; cc = gn[cc].next) {
+            const GNode &k = gn[cc];
+            mc = mc + k.m;
+            for (int d = 0; d < 3; d++) rc[d] = rc[d] + k.m * k.r[d];
+            hmax = fmax(hmaxs[cc], hmax);
+            for (int d = 0; d < 3; d++) {
+                bmx[d] = fmax(bbmax[3 * cc + d], bmx[d]);
+                bmn[d] = fmin(bbmin[3 * cc + d], bmn[d]);
+            }
+        }
+        if (mc > 0.0)
+            for (int d = 0; d < 3; d++) rc[d] = rc[d] / mc;
+        for (int cc = c + 1; cc != next && cc < nc; cc = gn[cc].next) {
+            const GNode &k = gn[cc];
+            const double mcc = k.m;
+            const double *q = k.q;
+            for (int d = 0; d < 5; d++) qc[d] = qc[d] + q[d];
+            for (int d = 0; d < 3; d++) dr[d] = k.r[d] - rc[d];
+            drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            qc[0] = qc[0] + mcc * (3.0 * dr[0] * dr[0] - drsqd);
+            qc[1] = qc[1] + mcc * (3.0 * dr[0] * dr[1]);
+            qc[2] = qc[2] + mcc * (3.0 * dr[1] * dr[1] - drsqd);
+            qc[3] = qc[3] + mcc * (3.0 * dr[2] * dr[0]);
+            qc[4] = qc[4] + mcc * (3.0 * dr[2] * dr[1]);
+            if (octupole) {
+                // parallel-axis shift exactly as BHgrav_stock.F90:263-297
+                for (int d = 0; d < 10; d++) sc[d] = sc[d] + k.s[d];
+                sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+                        1.5 * dr[0] * q[0] - dr[1] * q[1] - dr[2] * q[3];
+                sc[1] = sc[1] + mcc * dr[1] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+                        4.0 * dr[0] * q[1] + 2.5 * dr[1] * q[0] - dr[1] * q[2] - dr[2] * q[4];
+                sc[2] = sc[2] + mcc * dr[0] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+                        4.0 * dr[1] * q[1] + 2.5 * dr[0] * q[2] - dr[0] * q[0] - dr[2] * q[3];
+                sc[3] = sc[3] + mcc * dr[1] * (5.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+                        1.5 * dr[1] * q[2] - dr[0] * q[1] - dr[2] * q[4];
+                sc[4] = sc[4] + mcc * dr[0] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+                        4.0 * dr[2] * q[3] - 2.5 * dr[0] * (q[0] + q[2]) - dr[0] * q[0] -
+                        dr[1] * q[1];
+                sc[5] = sc[5] + mcc * dr[1] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+                        4.0 * dr[2] * q[4] - 2.5 * dr[1] * (q[0] + q[2]) - dr[0] * q[1] -
+                        dr[1] * q[2];
+                sc[6] = sc[6] + mcc * dr[2] * (5.0 * dr[2] * dr[2] - 3.0 * drsqd) -
+                        1.5 * dr[2] * (q[0] + q[2]) - dr[0] * q[3] - dr[1] * q[4];
+                sc[7] = sc[7] + mcc * dr[2] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+                        4.0 * dr[0] * q[3] + 2.5 * dr[2] * q[0] - dr[1] * q[4] +
+                        dr[2] * (q[0] + q[2]);
+                sc[8] = sc[8] + mcc * dr[2] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+                        4.0 * dr[1] * q[4] + 2.5 * dr[2] * q[2] - dr[0] * q[3] +
+                        dr[2] * (q[0] + q[2]);
+                sc[9] = sc[9] + 15.0 * mcc * dr[0] * dr[1] * dr[2] +
+                        25.0 * (dr[0] * q[4] + dr[1] * q[3] + dr[2] * q[1]);
+            }
+        }
+        gm = mc;
+        gr[0] = rc[0]; gr[1] = rc[1]; gr[2] = rc[2];
+    }
+    // MAC quantities (BHgrav_stock.F90:317-348); rc is the COM, or the
+    // particle position for a one-particle leaf
+    for (int k = 0; k < 3; k++) dr[k] = fmax(bmx[k] - rc[k], rc[k] - bmn[k]);
+    const double diagonalsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    const double rsize = sqrt(diagonalsqd) + KERNRANGE * hmax;
+    double dminsqd, macv = 0.0;
+    if (mac == SERENX_MAC_GEOMETRIC) {
+        dminsqd = (thetamaxsqd > 0.0) ? diagonalsqd / thetamaxsqd : BIG_NUMBER;
+    } else if (mac == SERENX_MAC_GADGET) {
+        dminsqd = diagonalsqd;
+        macv = pow(mc * diagonalsqd * diagonalsqd / abserror, ONETHIRD);
+    } else {
+        dminsqd = diagonalsqd;
+        macv = sqrt(mc * diagonalsqd / abserror);
+    }
+    dminsqd = fmax(dminsqd, rsize * rsize);
+    if (dminsqd < daux) dminsqd = daux;
+
+    for (int k = 0; k < 3; k++) g.r[k] = gr[k];
+    g.m = gm;
+    g.dminsqd = dminsqd;
+    g.mac = macv;
+    for (int k = 0; k < 5; k++) g.q[k] = qc[k];
+    for (int k = 0; k < 10; k++) g.s[k] = octupole ? sc[k] : 0.0;
+    for (int k = 0; k < 3; k++) {
+        bbmin[3 * c + k] = bmn[k];
+        bbmax[3 * c + k] = bmx[k];
+    }
+    hmaxs[c] = hmax;
+
+    // hydro node: bounding-box centre and half-diagonal
+    HNode &hh = hn[c];
+    if (leaf == 1) {
+        for (int k = 0; k < 3; k++) hh.r[k] = bmn[k];
+        hh.rmax = 0.0;
+    } else {
+        double hc[3], half[3], dsq = 0.0;
+        for (int k = 0; k < 3; k++) {
+            hc[k] = 0.5 * (bmx[k] + bmn[k]);
+            half[k] = bmx[k] - hc[k];
+            dsq = dsq + half[k] * half[k];
+        }
+        for (int k = 0; k < 3; k++) hh.r[k] = hc[k];
+        hh.rmax = sqrt(dsq);
+    }
+}
+
+__global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn, GNodeF *__restrict__ gf) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const GNode &g = gn[c];
+    GNodeF &f = gf[c];
+    for (int k = 0; k < 3; k++) f.r[k] = g.r[k];
+    f.dminsqd = g.dminsqd;
+    f.mac = g.mac;
+    f.m = (float)g.m;
+    for (int k = 0; k < 5; k++) f.q[k] = (float)g.q[k];
+    for (int k = 0; k < 10; k++) f.s[k] = (float)g.s[k];
+    f.next = g.next;
+    f.leaf = g.leaf;
+    f.first = g.first;
+    f.pad = 0;
+}
+
+// ---- BHhydro_hguess.F90:108-155 -----------------------------------------
+__global__ void k_hguess(int nc, int thresh, const int *__restrict__ c_leaf,
+                         const int *__restrict__ c_start, const int *__restrict__ c_end,
+                         const int *__restrict__ c_parent, const int *__restrict__ c_level,
+                         const int *__restrict__ bfs2dfs, const HNode *__restrict__ hn,
+                         double *__restrict__ h) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int k = c_leaf[c];
+    if (k <= 0) return;
+    int a = c, l = c_level[c];
+    for (;;) {
+        if ((c_end[a] - c_start[a]) > thresh || l == 0) break;
+        a = c_parent[a];
+        l--;
+    }
+    const double hg = INVKERNRANGE * hn[bfs2dfs[a]].rmax;
+    const int s = c_start[c];
+    for (int i = 0; i < k; i++) h[s + i] = hg;
+}
+
+// ---- BHhydro_update_hmax.F90:29-88 --------------------------------------
+__global__ void k_hmax(int nc, int base, int cnt, const int *__restrict__ bfs2dfs, HNode *__restrict__ hn,
+                       const double *__restrict__ h) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    const int c = bfs2dfs[base + j];
+    HNode &hh = hn[c];
+    double hr = 0.0;
+    if (hh.leaf == 1) {
+        hr = KERNRANGE * h[hh.first];
+    } else if (hh.leaf > 0) {
+        for (int i = 0; i < hh.leaf; i++) hr = fmax(hr, KERNRANGE * h[hh.first + i]);
+    } else {
+        for (int cc = c + 1; cc != hh.next && cc < nc; cc = hn[cc].next) hr = fmax(hr, hn[cc].hrangemax);
+    }
+    hh.hrangemax = hr;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+int ensure_tree_buffers(Ctx &c) {
+    const int n = c.cap;
+    const size_t nn = (size_t)n + 1;
+    SX_TRY(dalloc(&c.khi, nn));
+    SX_TRY(dalloc(&c.klo, nn));
+    SX_TRY(dalloc(&c.khi2, nn));
+    SX_TRY(dalloc(&c.klo2, nn));
+    SX_TRY(dalloc(&c.idx, nn));
+    SX_TRY(dalloc(&c.idx2, nn));
+    SX_TRY(dalloc(&c.cpl, nn));
+    SX_TRY(dalloc(&c.pact, nn));
+    SX_TRY(dalloc(&c.flag, nn));
+    SX_TRY(dalloc(&c.incl, nn));
+    SX_TRY(dalloc(&c.cellof, nn));
+    SX_TRY(dalloc(&c.ncs, nn));
+    SX_TRY(dalloc(&c.off, nn));
+    SX_TRY(dalloc(&c.firstlev, nn));
+    SX_TRY(dalloc(&c.root, 16));
+    SX_TRY(dalloc(&c.bbpart, 6 * 1024));
+    c.ccap = 2 * n + 2;
+    const size_t cc = (size_t)c.ccap;
+    SX_TRY(dalloc(&c.c_start, cc));
+    SX_TRY(dalloc(&c.c_end, cc));
+    SX_TRY(dalloc(&c.c_parent, cc));
+    SX_TRY(dalloc(&c.c_leaf, cc));
+    SX_TRY(dalloc(&c.c_level, cc));
+    SX_TRY(dalloc(&c.bfs2dfs, cc));
+    SX_TRY(dalloc(&c.dfs2bfs, cc));
+    SX_TRY(dalloc(&c.dfs_parent, cc));
+    SX_TRY(dalloc(&c.bbmin, 3 * cc));
+    SX_TRY(dalloc(&c.bbmax, 3 * cc));
+    SX_TRY(dalloc(&c.hmaxs, cc));
+    SX_TRY(dalloc(&c.gn, cc));
+    SX_TRY(dalloc(&c.gnf, cc));
+    SX_TRY(dalloc(&c.hn, cc));
+    // temporary storage for sort and scans
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    rocprim::radix_sort_pairs(nullptr, b1, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0, 63,
+                              c.stream);
+    rocprim::inclusive_scan(nullptr, b2, c.flag, c.incl, (size_t)n, rocprim::plus<int>(),
+                            c.stream);
+    rocprim::exclusive_scan(nullptr, b3, c.ncs, c.off, 0, (size_t)n + 1, rocprim::plus<int>(),
+                            c.stream);
+    size_t need = std::max(b1, std::max(b2, b3));
+    if (c.sorttmp) (void)hipFree(c.sorttmp);
+    c.sorttmp = nullptr;
+    if (hipMalloc(&c.sorttmp, need + 256) != hipSuccess)
+        return set_error(SERENX_ENOMEM, "hipMalloc failed for sort scratch");
+    c.sorttmp_bytes = need + 256;
+    if (!c.hostpin) SX_HIP(hipHostMalloc((void **)&c.hostpin, 64 * sizeof(int)));
+    if (!c.hostpin_d) SX_HIP(hipHostMalloc((void **)&c.hostpin_d, 16 * sizeof(double)));
+    return SERENX_OK;
+}
+
+int permute_particles(Ctx &c, const int *perm) {
+    const int n = c.n;
+    double **arrs[] = {&c.x, &c.y, &c.z, &c.m, &c.h, &c.zeta, &c.vx, &c.vy, &c.vz,
+                       &c.rho, &c.omega, &c.press, &c.sound, &c.divv, &c.drhodt,
+                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm};
+    for (double **a : arrs) {
+        hipLaunchKernelGGL(k_gather<double>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                           (const double *)*a, c.tmpd);
+        std::swap(*a, c.tmpd);
+    }
+    hipLaunchKernelGGL(k_gather<int>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                       (const int *)c.porig, c.tmpi);
+    std::swap(c.porig, c.tmpi);
+    hipLaunchKernelGGL(k_gather<uint8_t>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                       (const uint8_t *)c.active, c.pact);
+    std::swap(c.active, c.pact);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int build_tree(Ctx &c) {
+    const int n = c.n;
+    hipStream_t s = c.stream;
+    c.tree_ok = false;
+    c.sph_ok = false;
+    // bounding box and root cell
+    int nb = std::min(1024, nblk(n));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, n, c.x, c.y, c.z, c.bbpart);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, nb, c.bbpart, c.root);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(TB), 0, s, n, c.x, c.y, c.z, c.root, c.khi,
+                       c.klo, c.idx);
+    SX_HIP(hipGetLastError());
+    // sort by the 21-level key; redo with all 30 levels only when needed
+    size_t tb = c.sorttmp_bytes;
+    SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0,
+                                     63, s));
+    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_deep_check, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.flag);
+    SX_HIP(hipMemcpyAsync(c.hostpin, c.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    SX_HIP(hipMemcpyAsync(c.hostpin_d, c.root, 11 * sizeof(double), hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    c.rextent = c.hostpin_d[4];
+    const bool deep = c.hostpin[0] != 0;
+    if (deep) {
+        // stable LSD order: deep 9 levels first, then the top 21 levels
+        tb = c.sorttmp_bytes;
+        SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.klo, c.klo2, c.idx, c.idx2,
+                                         (unsigned)n, 0, 27, s));
+        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx2, c.khi, c.khi2);
+        tb = c.sorttmp_bytes;
+        SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.khi2, c.khi, c.idx2, c.idx,
+                                         (unsigned)n, 0, 63, s));
+        // khi = sorted keys, idx = order; gather the deep keys in that order
+        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx, c.klo, c.klo2);
+        std::swap(c.khi, c.khi2);  // khi2 <- sorted hi keys
+    } else {
+        std::swap(c.idx, c.idx2);  // idx <- sorted order
+        hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx, c.klo, c.klo2);
+    }
+    hipLaunchKernelGGL(k_cpl, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.klo2, c.cpl);
+    hipLaunchKernelGGL(k_init_levels, dim3(nblk(n + 1)), dim3(TB), 0, s, n, c.pact, c.cellof,
+                       c.ncs, c.firstlev, c.c_start, c.c_end, c.c_parent, c.c_leaf, c.c_level);
+    SX_HIP(hipGetLastError());
+    c.level_off[0] = 0;
+    c.level_cnt[0] = 1;
+    int base = 1, ltot = 0;
+    for (int L = 1; L <= LMAX + 1; L++) {
+        hipLaunchKernelGGL(k_mark, dim3(nblk(n)), dim3(TB), 0, s, n, L, c.pact, c.cpl, c.flag);
+        tb = c.sorttmp_bytes;
+        SX_HIP(rocprim::inclusive_scan(c.sorttmp, tb, c.flag, c.incl, (size_t)n,
+                                       rocprim::plus<int>(), s));
+        SX_HIP(hipMemcpyAsync(c.hostpin, c.incl + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        SX_HIP(hipStreamSynchronize(s));
+        const int cnt = c.hostpin[0];
+        if (cnt == 0) break;
+        if (L == LMAX + 1)
+            return set_error(SERENX_ELMAX, "Reached maximum level (more than 8 particles "
+                                           "share a level-30 cell)");
+        if (base + cnt > c.ccap) return set_error(SERENX_ENOMEM, "cell capacity exceeded");
+        SX_HIP(hipMemsetAsync(c.c_end + base, 0, sizeof(int) * cnt, s));
+        hipLaunchKernelGGL(k_emit, dim3(nblk(n)), dim3(TB), 0, s, n, L, base, c.pact, c.flag,
+                           c.incl, c.cellof, c.c_start, c.c_end, c.c_parent, c.c_level, c.ncs,
+                           c.firstlev);
+        hipLaunchKernelGGL(k_leafcells, dim3(nblk(cnt)), dim3(TB), 0, s, base, cnt, c.c_start,
+                           c.c_end, c.c_leaf);
+        hipLaunchKernelGGL(k_deactivate, dim3(nblk(n)), dim3(TB), 0, s, n, c.pact, c.cellof,
+                           c.c_leaf);
+        SX_HIP(hipGetLastError());
+        c.level_off[L] = base;
+        c.level_cnt[L] = cnt;
+        base += cnt;
+        ltot = L;
+    }
+    c.ltot = ltot;
+    c.nc = base;
+    for (int L = ltot + 1; L <= LMAX + 1; L++) c.level_off[L] = base, c.level_cnt[L] = 0;
+    // DFS layout
+    SX_HIP(hipMemsetAsync(c.ncs + n, 0, sizeof(int), s));
+    tb = c.sorttmp_bytes;
+    SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.ncs, c.off, 0, (size_t)n + 1,
+                                   rocprim::plus<int>(), s));
+    hipLaunchKernelGGL(k_dfs, dim3(nblk(c.nc)), dim3(TB), 0, s, n, c.nc, c.c_start, c.c_end,
+                       c.c_level, c.c_leaf, c.off, c.firstlev, c.bfs2dfs, c.dfs2bfs, c.gn, c.hn);
+    hipLaunchKernelGGL(k_dfs_parent, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.c_parent,
+                       c.bfs2dfs, c.dfs_parent);
+    hipLaunchKernelGGL(k_leafsort, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.c_start, c.c_leaf,
+                       c.idx);
+    SX_HIP(hipGetLastError());
+    SX_TRY(permute_particles(c, c.idx));
+    c.tree_ok = true;
+    return SERENX_OK;
+}
+
+int stock_tree(Ctx &c) {
+    hipStream_t s = c.stream;
+    const int oct = c.opt.multipole == SERENX_OCTUPOLE;
+    for (int L = c.ltot; L >= 0; L--) {
+        const int cnt = c.level_cnt[L];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_stock, dim3(nblk(cnt)), dim3(TB), 0, s, c.nc, c.level_off[L], cnt,
+                           c.bfs2dfs, c.gn, c.hn, c.bbmin, c.bbmax, c.hmaxs, c.x, c.y, c.z, c.m,
+                           c.h, oct, c.opt.mac, c.opt.thetamaxsqd, c.opt.abserror, c.daux);
+    }
+    if (c.opt.precision == SERENX_MIXED)
+        hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gnf);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int hguess(Ctx &c) {
+    const int thresh = (int)(0.5 * (double)c.pp_gather);
+    hipLaunchKernelGGL(k_hguess, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, thresh, c.c_leaf,
+                       c.c_start, c.c_end, c.c_parent, c.c_level, c.bfs2dfs, c.hn, c.h);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int update_hmax(Ctx &c) {
+    for (int L = c.ltot; L >= 0; L--) {
+        const int cnt = c.level_cnt[L];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_hmax, dim3(nblk(cnt)), dim3(TB), 0, c.stream, c.nc, c.level_off[L], cnt,
+                           c.bfs2dfs, c.hn, c.h);
+    }
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+}  // namespace serenx

@@ -1,0 +1,93 @@
+// walk.h -- device helpers shared by the tree walks: wave-wide reductions,
+// correctly rounded small powers, and the leaf-group tree walk.
+#pragma once
+
+#include "serenx_internal.h"
+
+namespace serenx {
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// x^3 and x^4 correctly rounded (double-double products with exact FMA
+// residuals, one final rounding).  The reference evaluates invh**3.0 and
+// invh**4.0 with libm pow, which is correctly rounded for these arguments
+// except in rare hard cases.
+__device__ __forceinline__ double cr_pow3(double x) {
+    const double p = x * x;
+    const double e1 = __fma_rn(x, x, -p);
+    const double q = p * x;
+    const double e2 = __fma_rn(p, x, -q);
+    const double t = __fma_rn(e1, x, e2);
+    return q + t;
+}
+
+__device__ __forceinline__ double cr_pow4(double x) {
+    const double p = x * x;
+    const double e1 = __fma_rn(x, x, -p);
+    const double q = p * p;
+    const double e2 = __fma_rn(p, p, -q);
+    const double t = __fma_rn(2.0 * p, e1, e2);
+    return q + t;
+}
+
+// x**0.333333333 (the reference's ONETHIRD, macros.h:18) correctly rounded,
+// as libm pow returns it on the host.  With y = 0.333333333 = 1/3 - delta,
+//   x^y = cbrt(x) * exp(-delta ln x).
+// cbrt(x) is taken to ~2^-100 relative by one Newton step whose residual
+// c^3 - x is formed exactly with FMA products; |delta ln x| < 2e-8, so its
+// exponential needs only three Taylor terms, and a double-precision ln x
+// leaves ~2^-79 relative error in the sum.  One final rounding follows.
+__device__ __forceinline__ double ref_pow_third(double x) {
+    if (!(x > 0.0) || __builtin_isinf(x)) return pow(x, ONETHIRD);
+    constexpr double THIRD_HI = 0.3333333333333333;       // nearest double to 1/3
+    constexpr double THIRD_LO = 1.850371707708594e-17;    // 1/3 - THIRD_HI
+    constexpr double DELTA = (THIRD_HI - ONETHIRD) + THIRD_LO;
+    const double c0 = cbrt(x);
+    const double p = c0 * c0;
+    const double e1 = __fma_rn(c0, c0, -p);
+    const double q = p * c0;
+    const double e2 = __fma_rn(p, c0, -q);
+    const double res = (q - x) + __fma_rn(e1, c0, e2);   // c0^3 - x
+    const double clo = -res / (3.0 * p);
+    const double d = -DELTA * log(x);
+    const double g = d + d * d * (0.5 + d * (1.0 / 6.0));
+    return c0 + __fma_rn(c0, g, clo);
+}
+
+// Leaf-group walk over a DFS-ordered tree.  Every lane of the wave owns one
+// target; the wave visits, in DFS order, the union of the cells its lanes
+// visit on their own stackless walks (nextcell / ifopen = c + 1).  A lane is
+// active at cell c when its own walk would be at c (resume == c); it then
+// tests c, and either descends (resume = c + 1) or moves past the subtree
+// (resume = next).  Each lane therefore sees exactly its own interaction set
+// in its own order, as a per-particle walk would.
+//
+//   Node  : node type with fields next, leaf (count or 0), first
+//   visit : bool visit(const Node&, int c, bool &descend) for an active lane;
+//           returns nothing useful, sets descend for an opened branch
+template <class Node, class Visit>
+__device__ __forceinline__ void group_walk(const Node *__restrict__ nodes, int nc, bool walking,
+                                           Visit &&visit) {
+    int resume = walking ? 0 : nc;
+    int c = uniform(wave_min_int(resume));
+    while (c < nc) {
+        const int next = nodes[c].next;
+        bool descend = false;
+        if (resume == c) {
+            descend = visit(nodes[c], c);
+            resume = descend ? c + 1 : next;
+        }
+        if (__any(descend))
+            c = c + 1;
+        else
+            c = uniform(wave_min_int(resume));
+    }
+}
+
+}  // namespace serenx

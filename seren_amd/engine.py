@@ -1,0 +1,330 @@
+"""Python binding of the gfx950 force-evaluation library (libserenx.so).
+
+Two layers over the C ABI of include/serenx.h:
+
+* ``ForceEngine`` -- device-resident use: ``load`` a cloud once, run
+  ``force_eval`` repeatedly with no host transfers, ``fetch`` results.
+* The drop-in drivers ``tree_update``, ``BHhydro_hguess``, ``sph_update``,
+  ``sph_hydro_forces`` and ``sph_grav_forces`` operate on a ``SerenState``
+  whose arrays have the memory layout of SEREN's module arrays (parray(6,N),
+  v(3,N), pplist(pp_limit,N), 1-based ids), the way SEREN's own drivers
+  (src/main/tree_update.F90, src/sph_sim/sph_update.F90,
+  src/sph_sim/sph_hydro_forces.F90, src/sph_sim/sph_grav_forces.F90) operate
+  on its module state.  ``tree_update`` permutes every state array with the
+  returned tree order, as reorder_particle_arrays (src/main/reorder_array.F90)
+  does.
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise ``SerenxError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libserenx.so")
+
+QUADRUPOLE, OCTUPOLE = 2, 3
+MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2 = 0, 1, 2
+FP64, MIXED = 0, 1
+EVAL_HGUESS, EVAL_NO_GRAV, EVAL_NO_HYDRO = 1, 2, 4
+
+_MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE}
+_MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2}
+_PREC = {"fp64": FP64, "mixed": MIXED}
+
+EXPORTED = [
+    "serenx_default_options", "serenx_create", "serenx_destroy", "serenx_last_error",
+    "serenx_neighbour_params", "serenx_bind_host_state", "serenx_tree_update",
+    "serenx_hguess", "serenx_sph_update", "serenx_hydro_forces", "serenx_grav_forces",
+    "serenx_load", "serenx_set_agravmag", "serenx_force_eval", "serenx_sync",
+    "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
+    "serenx_last_timings", "serenx_interaction_counts",
+]
+
+
+class SerenxError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    _fields_ = [("multipole", C.c_int32), ("mac", C.c_int32), ("precision", C.c_int32),
+                ("reserved", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
+                ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class HostState(C.Structure):
+    _fields_ = [("ptot", C.c_int32), ("pp_limit", C.c_int32), ("parray", _dp), ("v", _dp),
+                ("a", _dp), ("gpot", _dp), ("agravmag", _dp), ("rho", _dp), ("omega", _dp),
+                ("press", _dp), ("sound", _dp), ("div_v", _dp), ("drhodt", _dp),
+                ("pptot", _ip), ("pplist", _ip), ("accdo", _ip), ("rextent", _dp)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libserenx.so (built by seren_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SerenxError(f"{path} is not built; run python -m seren_amd.build")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.serenx_last_error.restype = C.c_char_p
+    lib.serenx_create.argtypes = [C.POINTER(Options), C.c_int, C.POINTER(vp)]
+    lib.serenx_default_options.argtypes = [C.POINTER(Options)]
+    lib.serenx_destroy.argtypes = [vp]
+    lib.serenx_neighbour_params.argtypes = [vp, _ip, _ip]
+    lib.serenx_bind_host_state.argtypes = [vp, C.POINTER(HostState)]
+    lib.serenx_tree_update.argtypes = [vp, _ip]
+    for f in ("serenx_hguess", "serenx_sph_update", "serenx_hydro_forces",
+              "serenx_grav_forces", "serenx_sync"):
+        getattr(lib, f).argtypes = [vp]
+    lib.serenx_load.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp]
+    lib.serenx_set_agravmag.argtypes = [vp, _dp]
+    lib.serenx_force_eval.argtypes = [vp, C.c_int32]
+    lib.serenx_fetch.argtypes = [vp, _ip] + [_dp] * 14 + [_ip, _ip]
+    lib.serenx_direct_gravity.argtypes = [vp, C.c_int32, _ip, _dp, _dp]
+    lib.serenx_tree_sizes.argtypes = [vp, _ip, _ip]
+    lib.serenx_export_tree.argtypes = [vp, _ip, _ip, _ip, _ip] + [_dp] * 9 + [_ip, _ip]
+    lib.serenx_last_timings.argtypes = [vp, _dp]
+    lib.serenx_interaction_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().serenx_last_error().decode(errors="replace")
+        raise SerenxError(f"serenx error {rc}: {msg}")
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+class ForceEngine:
+    """One device context holding one resident cloud."""
+
+    def __init__(self, multipole: str = "quadrupole", mac: str = "geometric",
+                 precision: str = "fp64", h_fac: float = 1.2, thetamaxsqd: float = 0.25,
+                 abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
+                 device: int = 0):
+        self.lib = load_library()
+        opt = Options()
+        _check(self.lib.serenx_default_options(C.byref(opt)))
+        opt.multipole = _MULTIPOLE[multipole]
+        opt.mac = _MAC[mac]
+        opt.precision = _PREC[precision]
+        opt.h_fac = h_fac
+        opt.thetamaxsqd = thetamaxsqd
+        opt.abserror = abserror
+        opt.alpha = alpha
+        opt.sound_speed = sound_speed
+        self.opt = opt
+        self.ctx = C.c_void_p()
+        _check(self.lib.serenx_create(C.byref(opt), device, C.byref(self.ctx)))
+        g, l = C.c_int32(), C.c_int32()
+        _check(self.lib.serenx_neighbour_params(self.ctx, C.byref(g), C.byref(l)))
+        self.pp_gather, self.pp_limit = g.value, l.value
+        self.n = 0
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.serenx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device-resident mode ---------------------------------------------
+    def load(self, pos, vel, mass, h=None) -> None:
+        pos = np.ascontiguousarray(pos, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        mass = np.ascontiguousarray(mass, np.float64)
+        h = None if h is None else np.ascontiguousarray(h, np.float64)
+        self.n = len(mass)
+        _check(self.lib.serenx_load(self.ctx, self.n, _dptr(pos), _dptr(vel), _dptr(mass),
+                                    _dptr(h)))
+
+    def set_agravmag(self, am) -> None:
+        am = np.ascontiguousarray(am, np.float64)
+        _check(self.lib.serenx_set_agravmag(self.ctx, _dptr(am)))
+
+    def force_eval(self, hguess: bool = False, grav: bool = True, hydro: bool = True) -> None:
+        flags = (EVAL_HGUESS if hguess else 0) | (0 if grav else EVAL_NO_GRAV) | \
+            (0 if hydro else EVAL_NO_HYDRO)
+        _check(self.lib.serenx_force_eval(self.ctx, flags))
+
+    def sync(self) -> None:
+        _check(self.lib.serenx_sync(self.ctx))
+
+    def fetch(self, neighbours: bool = True) -> Dict[str, np.ndarray]:
+        n = self.n
+        out = dict(porig=np.zeros(n, np.int32), pos=np.zeros((n, 3)), v=np.zeros((n, 3)),
+                   m=np.zeros(n), h=np.zeros(n), zeta=np.zeros(n), rho=np.zeros(n),
+                   omega=np.zeros(n), div_v=np.zeros(n), drhodt=np.zeros(n),
+                   press=np.zeros(n), sound=np.zeros(n), a=np.zeros((n, 3)), gpot=np.zeros(n),
+                   agravmag=np.zeros(n), pptot=np.zeros(n, np.int32))
+        pl = np.zeros((n, self.pp_limit), np.int32) if neighbours else None
+        _check(self.lib.serenx_fetch(
+            self.ctx, _iptr(out["porig"]), _dptr(out["pos"]), _dptr(out["v"]), _dptr(out["m"]),
+            _dptr(out["h"]), _dptr(out["zeta"]), _dptr(out["rho"]), _dptr(out["omega"]),
+            _dptr(out["div_v"]), _dptr(out["drhodt"]), _dptr(out["press"]), _dptr(out["sound"]),
+            _dptr(out["a"]), _dptr(out["gpot"]), _dptr(out["agravmag"]), _iptr(out["pptot"]),
+            _iptr(pl)))
+        if pl is not None:
+            out["pplist"] = pl
+        return out
+
+    def direct_gravity(self, targets):
+        t = np.ascontiguousarray(targets, np.int32)
+        acc = np.zeros((len(t), 3))
+        pot = np.zeros(len(t))
+        _check(self.lib.serenx_direct_gravity(self.ctx, len(t), _iptr(t), _dptr(acc),
+                                              _dptr(pot)))
+        return acc, pot
+
+    def tree_sizes(self):
+        nc, lt = C.c_int32(), C.c_int32()
+        _check(self.lib.serenx_tree_sizes(self.ctx, C.byref(nc), C.byref(lt)))
+        return nc.value, lt.value
+
+    def export_tree(self) -> Dict[str, np.ndarray]:
+        nc, _ = self.tree_sizes()
+        t = dict(leaf=np.zeros(nc, np.int32), plist=np.zeros((nc, 8), np.int32),
+                 nextcell=np.zeros(nc, np.int32), ifopen=np.zeros(nc, np.int32),
+                 r=np.zeros((nc, 3)), m=np.zeros(nc), dminsqd=np.zeros(nc), mac=np.zeros(nc),
+                 q=np.zeros((nc, 5)), s=np.zeros((nc, 10)), hr=np.zeros((nc, 3)),
+                 rmax=np.zeros(nc), hrangemax=np.zeros(nc),
+                 first_cell=np.zeros(31, np.int32), last_cell=np.zeros(31, np.int32))
+        _check(self.lib.serenx_export_tree(
+            self.ctx, _iptr(t["leaf"]), _iptr(t["plist"]), _iptr(t["nextcell"]),
+            _iptr(t["ifopen"]), _dptr(t["r"]), _dptr(t["m"]), _dptr(t["dminsqd"]),
+            _dptr(t["mac"]), _dptr(t["q"]), _dptr(t["s"]), _dptr(t["hr"]), _dptr(t["rmax"]),
+            _dptr(t["hrangemax"]), _iptr(t["first_cell"]), _iptr(t["last_cell"])))
+        return t
+
+    def timings(self):
+        ms = np.zeros(4)
+        _check(self.lib.serenx_last_timings(self.ctx, _dptr(ms)))
+        return dict(tree=ms[0], sph=ms[1], hydro=ms[2], grav=ms[3])
+
+    def interaction_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        _check(self.lib.serenx_interaction_counts(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+# ---------------------------------------------------------------------------
+# Drop-in drivers over SEREN-shaped host state
+# ---------------------------------------------------------------------------
+class SerenState:
+    """Host arrays with the memory layout of SEREN's module arrays.
+
+    parray is (N, 6) C-order == parray(6, N) Fortran order: [x y z m h zeta];
+    v, a are (N, 3); pplist is (N, pp_limit) with 1-based ids; porig tracks
+    the original 1-based index as SEREN's porig array does.
+    """
+
+    def __init__(self, pos, vel, mass, h, pp_limit: int):
+        n = len(mass)
+        self.n = n
+        self.pp_limit = pp_limit
+        self.parray = np.zeros((n, 6))
+        self.parray[:, 0:3] = pos
+        self.parray[:, 3] = mass
+        self.parray[:, 4] = h
+        self.v = np.ascontiguousarray(vel, np.float64).copy()
+        self.a = np.zeros((n, 3))
+        self.gpot = np.zeros(n)
+        self.agravmag = np.zeros(n)
+        self.rho = np.ones(n)       # initialize_sph_variables_1.F90:68-72
+        self.omega = np.ones(n)
+        self.press = np.zeros(n)
+        self.sound = np.zeros(n)
+        self.div_v = np.zeros(n)
+        self.drhodt = np.zeros(n)
+        self.pptot = np.zeros(n, np.int32)
+        self.pplist = np.zeros((n, pp_limit), np.int32)
+        self.accdo = np.ones(n, np.int32)
+        self.porig = np.arange(1, n + 1, dtype=np.int32)
+        self.rextent = np.zeros(1)
+
+    def reorder(self, dummylist: np.ndarray) -> None:
+        """reorder_particle_arrays (reorder_array.F90:9-270): new p <- old dummylist(p)."""
+        idx = dummylist - 1
+        for name in ("parray", "v", "a", "gpot", "agravmag", "rho", "omega", "press", "sound",
+                     "div_v", "drhodt", "accdo", "porig"):
+            arr = getattr(self, name)
+            arr[...] = arr[idx]   # in place: the bound pointers stay valid
+
+
+def bind(engine: ForceEngine, st: SerenState) -> None:
+    hs = HostState()
+    hs.ptot = st.n
+    hs.pp_limit = st.pp_limit
+    hs.parray, hs.v, hs.a = _dptr(st.parray), _dptr(st.v), _dptr(st.a)
+    hs.gpot, hs.agravmag = _dptr(st.gpot), _dptr(st.agravmag)
+    hs.rho, hs.omega, hs.press, hs.sound = (_dptr(st.rho), _dptr(st.omega), _dptr(st.press),
+                                            _dptr(st.sound))
+    hs.div_v, hs.drhodt = _dptr(st.div_v), _dptr(st.drhodt)
+    hs.pptot, hs.pplist, hs.accdo = _iptr(st.pptot), _iptr(st.pplist), _iptr(st.accdo)
+    hs.rextent = _dptr(st.rextent)
+    engine._hs = hs   # keep the struct alive with the engine
+    engine._hs_state = st
+    _check(engine.lib.serenx_bind_host_state(engine.ctx, C.byref(hs)))
+
+
+def tree_update(engine: ForceEngine, st: SerenState) -> None:
+    """tree_update.F90:9-92 incl. BH_reorder_particles' permutation of all arrays."""
+    if getattr(engine, "_hs_state", None) is not st:
+        bind(engine, st)
+    dl = np.zeros(st.n, np.int32)
+    _check(engine.lib.serenx_tree_update(engine.ctx, _iptr(dl)))
+    st.reorder(dl)
+
+
+def BHhydro_hguess(engine: ForceEngine, st: SerenState) -> None:
+    _check(engine.lib.serenx_hguess(engine.ctx))
+
+
+def sph_update(engine: ForceEngine, st: SerenState) -> None:
+    _check(engine.lib.serenx_sph_update(engine.ctx))
+
+
+def update_thermal_properties_isothermal(st: SerenState, cs: float) -> None:
+    """thermal.F90:52-60 with the sound speed given (P = cs^2 rho, c = cs)."""
+    st.press[:] = cs * cs * st.rho
+    st.sound[:] = cs
+
+
+def sph_hydro_forces(engine: ForceEngine, st: SerenState) -> None:
+    _check(engine.lib.serenx_hydro_forces(engine.ctx))
+
+
+def sph_grav_forces(engine: ForceEngine, st: SerenState) -> None:
+    _check(engine.lib.serenx_grav_forces(engine.ctx))
