@@ -1,0 +1,151 @@
+"""Benchmark: particles/s of one SEREN force evaluation on one MI355X.
+
+Workload (one "step"): one complete force evaluation -- tree build + stock,
+grad-h h/rho solve with neighbour lists, isothermal thermal update, grad-h
+hydro pair forces, Barnes-Hut gravity -- of a resident cloud of
+N = 1,048,576 equal-mass particles, rho ~ r^-2 truncated at r = 1, with a
+divergence-free P(k) ~ k^-4 velocity field on a 128^3 grid at rms Mach 2
+(c_s = 0.2), octupole moments and the GADGET2 multipole MAC (abserror 1e-4),
+FP64 throughout.  Inputs are resident in HBM; the timed region is K force
+evaluations.  Two untimed evaluations come first (the setup evaluation with
+the tree h guess, then one that provides |a| for the multipole MAC).
+
+Prints one JSON line.  --cpu-baseline times the reference Fortran/OpenMP path
+(the flang build of the reference sources in oracle/_ref) on the same cloud.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/s (tree grav+grad-h hydro), FP32-mixed vs FP64 @1e-4 rms; p99 lat"
+
+# Algorithmic flop counts per interaction (counted from the reference source,
+# + - * / sqrt = 1, compares/table loads = 0):
+#   octupole cell term, BHgrav_accel.F90:146-226        190
+#   quadrupole cell term                                  69
+#   kernel-softened pair, gravity_gradh.F90:42-86        29 + 8 [r < 2h_p] + 9 [r < 2h_j]
+F_CELL = {"octupole": 190.0, "quadrupole": 69.0}
+F_PAIR = 29.0
+MI355X_FP64_TFLOPS = 78.6
+MI355X_FP32_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--profile", default="r2")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--multipole", default="octupole")
+    ap.add_argument("--mac", default="gadget2")
+    ap.add_argument("--abserror", type=float, default=1e-4)
+    ap.add_argument("--mach", type=float, default=2.0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    return ap.parse_args()
+
+
+def cpu_baseline(pos, vel, mass, h, args):
+    """Time the reference Fortran/OpenMP force evaluation on the same cloud."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        from oracle_lib import RefSeren
+        ref = RefSeren(args.multipole == "octupole", args.mac)
+    except Exception as e:  # reference build absent
+        return {"value": None, "unit": "particles/s", "cores": 0, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    n = len(mass)
+    ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=args.abserror)
+    ref.force_eval(0.2, hguess=True)          # setup evaluation (untimed)
+    t0 = time.perf_counter()
+    ref.tree_update()
+    ref.sph_update()
+    ref.thermal_iso(0.2)
+    ref.hydro_forces()
+    ref.grav_forces()
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n / dt, "unit": "particles/s", "cores": cores, "kind": "reference",
+            "sample": f"one force evaluation of the full {n}-particle cloud (after one "
+                      f"untimed setup evaluation), {dt:.2f} s wall"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
+                                   "design-only (see DESIGN.md), use --gpus 1"}))
+        sys.exit(2)
+    from seren_amd import ics
+    from seren_amd.engine import ForceEngine
+    print(f"# generating {args.n} particles ({args.profile}, Mach {args.mach})", file=sys.stderr,
+          flush=True)
+    pos, vel, mass, h = ics.make_cloud(args.n, args.profile, seed=args.seed, mach=args.mach,
+                                       cs=0.2, ngrid=128)
+    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=args.precision,
+                      thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
+    eng.load(pos, vel, mass, h)
+    eng.force_eval(hguess=True)
+    eng.sync()
+    for _ in range(max(0, args.warmup - 1)):
+        eng.force_eval()
+    eng.sync()
+    print("# warm", file=sys.stderr, flush=True)
+    grav_ms = []
+    phases = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.force_eval()
+        t = eng.timings()          # waits for this evaluation (events on its stream)
+        grav_ms.append(t["grav"])
+        phases += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
+    eng.sync()
+    dt = time.perf_counter() - t0
+    ncell, npp, npp_in_p, npp_in_j = eng.interaction_counts()
+    n = args.n
+    value = n * args.steps / dt
+    ms_step = 1e3 * dt / args.steps
+    # gravity-walk roofline: algorithmic flops per launch / measured launch time
+    flops = ncell * F_CELL[args.multipole] + npp * F_PAIR + 8.0 * npp_in_p + 9.0 * npp_in_j
+    gms = float(np.mean(grav_ms))
+    achieved = flops / (gms * 1e-3) / 1e12
+    peak = MI355X_FP64_TFLOPS if args.precision == "fp64" else MI355X_FP32_TFLOPS
+    out = {
+        "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
+        "data": "synthetic (seeded r^-2 sphere, P(k)~k^-4 turbulent velocities)",
+        "config": {"workload": f"force evaluation, N={n} {args.profile} sphere, Mach "
+                               f"{args.mach}, {args.multipole} + {args.mac} MAC "
+                               f"(abserror {args.abserror}), {args.precision}",
+                   "n": n, "profile": args.profile, "multipole": args.multipole,
+                   "mac": args.mac, "abserror": args.abserror, "precision": args.precision},
+        "phases_ms": {k: float(v / args.steps) for k, v in
+                      zip(("tree", "sph", "hydro", "grav"), phases)},
+        "interactions_per_particle": {"cell": ncell / n, "pair": npp / n},
+        "roofline": {"kernel": "gravity walk", "bound": "valu", "achieved": achieved,
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None},
+    }
+    eng.close()
+    if args.cpu:
+        print("# cpu baseline (reference Fortran/OpenMP)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(pos, vel, mass, h, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -194,10 +194,11 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist,
                        int32_t *first_cell, int32_t *last_cell);
 
 /* Per-phase device time of the last serenx_force_eval in milliseconds
- * (tree, sph, hydro, grav) and interaction counts of the last gravity walk
- * (cell terms, particle-particle terms) summed over particles. */
+ * (tree, sph, hydro, grav), and interaction counts of the last gravity walk
+ * summed over particles: counts[0] cell terms, [1] particle-particle terms,
+ * [2] pairs inside 2 h_p, [3] pairs inside 2 h_j (gravity_gradh.F90:61,72). */
 int serenx_last_timings(const serenx_ctx *ctx, double *ms4);
-int serenx_interaction_counts(serenx_ctx *ctx, int64_t *ncell, int64_t *npp);
+int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4);
 
 #ifdef __cplusplus
 }

@@ -558,13 +558,13 @@ int serenx_last_timings(const serenx_ctx *ctx, double *ms4) {
     return SERENX_OK;
 }
 
-int serenx_interaction_counts(serenx_ctx *ctx, int64_t *ncell, int64_t *npp) {
-    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4) {
+    if (!ctx || !counts4) return set_error(SERENX_EINVAL, "null argument");
     Ctx &c = *as_ctx(ctx);
-    unsigned long long v[2] = {0, 0};
+    unsigned long long v[4] = {0, 0, 0, 0};
+    SX_HIP(hipStreamSynchronize(c.stream));
     SX_HIP(hipMemcpy(v, c.icount, sizeof(v), hipMemcpyDeviceToHost));
-    if (ncell) *ncell = (int64_t)v[0];
-    if (npp) *npp = (int64_t)v[1];
+    for (int k = 0; k < 4; k++) counts4[k] = (int64_t)v[k];
     return SERENX_OK;
 }
 

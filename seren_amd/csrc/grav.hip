@@ -25,8 +25,18 @@ namespace {
 constexpr int TB = 256;
 inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
 
+// interaction counters: cell terms, pair terms, pairs inside 2h_p, inside 2h_j
+__device__ __forceinline__ void wave_count4(unsigned long long *icount, unsigned a0, unsigned a1,
+                                            unsigned a2, unsigned a3) {
+    unsigned long long v[4] = {a0, a1, a2, a3};
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 4; k++) v[k] += __shfl_xor(v[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; k++) atomicAdd(icount + k, v[k]);
+}
+
 // gravity_gradh.F90:42-86
-__device__ __forceinline__ void pair_fp64(const double *__restrict__ w4,
+__device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
                                           const double *__restrict__ w5,
                                           const double *__restrict__ w6, double invhp,
                                           double zo_p, double rpx, double rpy, double rpz,
@@ -38,7 +48,9 @@ __device__ __forceinline__ void pair_fp64(const double *__restrict__ w4,
     const double drmag = sqrt(drsqd) + SMALL_NUMBER;
     const double invdrmag = 1.0 / drmag;
     double grav, dpot;
+    int inside = 0;
     if (invdrmag > INVKERNRANGE * invhp) {
+        inside = 1;
         int kern = (int)(HALFKERNTOT * drmag * invhp);
         kern = min(kern, KERNTOT);
         grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
@@ -48,6 +60,7 @@ __device__ __forceinline__ void pair_fp64(const double *__restrict__ w4,
         dpot = invdrmag;
     }
     if (drmag < KERNRANGE * hj) {
+        inside |= 2;
         const double invhpp = 1.0 / hj;
         int kern = (int)(HALFKERNTOT * drmag * invhpp);
         kern = min(kern, KERNTOT);
@@ -62,6 +75,7 @@ __device__ __forceinline__ void pair_fp64(const double *__restrict__ w4,
     ay = ay + f * dy;
     az = az + f * dz;
     pot = pot + 0.5 * mj * dpot;
+    return inside;
 }
 
 // BHgrav_accel.F90:159-222 for one accepted cell
@@ -147,14 +161,14 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     }
     double acx = 0, acy = 0, acz = 0, pc = 0;   // cell terms
     double apx = 0, apy = 0, apz = 0, pp = 0;   // particle-particle terms
-    unsigned ncell = 0, npp = 0;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0;
     group_walk(gn, nc, valid, [&](const GNode &g, int c) -> bool {
         const int leaf = g.leaf;
         if (leaf == 1) {                                  // :128-136
             const int j = g.first;
             if (j != p) {
-                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
-                          zeta[j], apx, apy, apz, pp);
+                { const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
+                          zeta[j], apx, apy, apz, pp); nin_p += in_ & 1; nin_j += in_ >> 1; }
                 npp++;
             }
             return false;
@@ -175,8 +189,8 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
             for (int i = 0; i < leaf; i++) {
                 const int j = g.first + i;
                 if (j == p) continue;
-                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
-                          zeta[j], apx, apy, apz, pp);
+                { const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
+                          zeta[j], apx, apy, apz, pp); nin_p += in_ & 1; nin_j += in_ >> 1; }
                 npp++;
             }
             return false;
@@ -192,15 +206,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
     // per-wave interaction counts
-    unsigned long long a = ncell, b = npp;
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(icount, a);
-        atomicAdd(icount + 1, b);
-    }
+    wave_count4(icount, ncell, npp, nin_p, nin_j);
 }
 
 // ---- mixed precision ------------------------------------------------------
@@ -215,7 +221,7 @@ struct CSum {
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
 };
 
-__device__ __forceinline__ void pair_fp32(const float *__restrict__ w4, const float *__restrict__ w5,
+__device__ __forceinline__ int pair_fp32(const float *__restrict__ w4, const float *__restrict__ w5,
                                           const float *__restrict__ w6, float invhp, float zo_p,
                                           float dx, float dy, float dz, float mj, float hj,
                                           float zj, CSum &ax, CSum &ay, CSum &az, CSum &pot) {
@@ -223,7 +229,9 @@ __device__ __forceinline__ void pair_fp32(const float *__restrict__ w4, const fl
     const float drmag = sqrtf(drsqd) + 1.0e-20f;
     const float invdrmag = 1.0f / drmag;
     float grav, dpot;
+    int inside = 0;
     if (invdrmag > 0.5f * invhp) {
+        inside = 1;
         int kern = (int)(5000.0f * drmag * invhp);
         kern = min(kern, KERNTOT);
         grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
@@ -233,6 +241,7 @@ __device__ __forceinline__ void pair_fp32(const float *__restrict__ w4, const fl
         dpot = invdrmag;
     }
     if (drmag < 2.0f * hj) {
+        inside |= 2;
         const float invhpp = 1.0f / hj;
         int kern = (int)(5000.0f * drmag * invhpp);
         kern = min(kern, KERNTOT);
@@ -247,6 +256,7 @@ __device__ __forceinline__ void pair_fp32(const float *__restrict__ w4, const fl
     ay.add(f * dy);
     az.add(f * dz);
     pot.add(0.5f * mj * dpot);
+    return inside;
 }
 
 template <int OCT>
@@ -322,15 +332,15 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         }
     }
     CSum cx, cy, cz, cp;
-    unsigned ncell = 0, npp = 0;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0;
     group_walk(gn, nc, valid, [&](const GNodeF &g, int c) -> bool {
         const int leaf = g.leaf;
         if (leaf == 1) {
             const int j = g.first;
             if (j != p) {
-                pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
+                { const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
                           (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
-                          cz, cp);
+                          cz, cp); nin_p += in_ & 1; nin_j += in_ >> 1; }
                 npp++;
             }
             return false;
@@ -352,9 +362,9 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
             for (int i = 0; i < leaf; i++) {
                 const int j = g.first + i;
                 if (j == p) continue;
-                pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
+                { const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
                           (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
-                          cz, cp);
+                          cz, cp); nin_p += in_ & 1; nin_j += in_ >> 1; }
                 npp++;
             }
             return false;
@@ -369,15 +379,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         gpot[p] = cp.value();
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
-    unsigned long long a = ncell, b = npp;
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(icount, a);
-        atomicAdd(icount + 1, b);
-    }
+    wave_count4(icount, ncell, npp, nin_p, nin_j);
 }
 
 // direct_sph_gravity.F90:63-79: sum of gravity_gradh over all other
@@ -415,7 +417,7 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
         if (valid) {
             for (int k = 0; k < lim; k++) {
                 if (base + k == p) continue;
-                pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, sx[k], sy[k], sz[k], sm[k],
+                (void)pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, sx[k], sy[k], sz[k], sm[k],
                           sh[k], szt[k], gx, gy, gz, gp);
             }
         }
@@ -445,7 +447,7 @@ void launch_grav(Ctx &c) {
 }  // namespace
 
 int grav_forces(Ctx &c) {
-    SX_HIP(hipMemsetAsync(c.icount, 0, 2 * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.icount, 0, 4 * sizeof(unsigned long long), c.stream));
     const bool oct = c.opt.multipole == SERENX_OCTUPOLE;
     switch (c.opt.mac) {
     case SERENX_MAC_GEOMETRIC:

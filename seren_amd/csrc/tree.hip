@@ -204,7 +204,8 @@ __global__ void k_emit(int n, int L, int base, const uint8_t *__restrict__ pact,
         ncs[i] += 1;
     }
     cellof[i] = c;
-    atomicMax(&c_end[c], i + 1);
+    // the last particle of a run closes the cell (single writer)
+    if (i + 1 == n || !pact[i + 1] || f[i + 1]) c_end[c] = i + 1;
 }
 
 __global__ void k_leafcells(int base, int cnt, const int *__restrict__ c_start,
@@ -649,7 +650,6 @@ int build_tree(Ctx &c) {
             return set_error(SERENX_ELMAX, "Reached maximum level (more than 8 particles "
                                            "share a level-30 cell)");
         if (base + cnt > c.ccap) return set_error(SERENX_ENOMEM, "cell capacity exceeded");
-        SX_HIP(hipMemsetAsync(c.c_end + base, 0, sizeof(int) * cnt, s));
         hipLaunchKernelGGL(k_emit, dim3(nblk(n)), dim3(TB), 0, s, n, L, base, c.pact, c.flag,
                            c.incl, c.cellof, c.c_start, c.c_end, c.c_parent, c.c_level, c.ncs,
                            c.firstlev);

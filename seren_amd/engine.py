@@ -98,7 +98,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_tree_sizes.argtypes = [vp, _ip, _ip]
     lib.serenx_export_tree.argtypes = [vp, _ip, _ip, _ip, _ip] + [_dp] * 9 + [_ip, _ip]
     lib.serenx_last_timings.argtypes = [vp, _dp]
-    lib.serenx_interaction_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.serenx_interaction_counts.argtypes = [vp, C.POINTER(C.c_int64)]
     _lib = lib
     return lib
 
@@ -234,9 +234,10 @@ class ForceEngine:
         return dict(tree=ms[0], sph=ms[1], hydro=ms[2], grav=ms[3])
 
     def interaction_counts(self):
-        a, b = C.c_int64(), C.c_int64()
-        _check(self.lib.serenx_interaction_counts(self.ctx, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""
+        v = (C.c_int64 * 4)()
+        _check(self.lib.serenx_interaction_counts(self.ctx, v))
+        return tuple(int(x) for x in v)
 
 
 # ---------------------------------------------------------------------------
