@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--abserror", type=float, default=1e-4)
     ap.add_argument("--mach", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--direct-targets", type=int, default=16384)
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     return ap.parse_args()
@@ -114,6 +115,15 @@ def main():
     eng.sync()
     dt = time.perf_counter() - t0
     ncell, npp, npp_in_p, npp_in_j = eng.interaction_counts()
+    wstats = eng.walk_stats()
+    # accuracy of the tree gravity against the FP64 direct sum on a seeded
+    # subsample of targets (one more, gravity-only, evaluation; untimed)
+    eng.force_eval(hydro=False)
+    a_tree = eng.fetch(neighbours=False)["a"]
+    rng = np.random.default_rng(args.seed + 1)
+    tg = np.sort(rng.choice(args.n, min(args.n, args.direct_targets), replace=False))
+    a_dir, _ = eng.direct_gravity(tg.astype(np.int32))
+    err = np.linalg.norm(a_tree[tg] - a_dir, axis=1) / np.linalg.norm(a_dir, axis=1)
     n = args.n
     value = n * args.steps / dt
     ms_step = 1e3 * dt / args.steps
@@ -136,6 +146,10 @@ def main():
         "phases_ms": {k: float(v / args.steps) for k, v in
                       zip(("tree", "sph", "hydro", "grav"), phases)},
         "interactions_per_particle": {"cell": ncell / n, "pair": npp / n},
+        "walk_stats": wstats,
+        "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
+                                         "rms_rel": float(np.sqrt(np.mean(err * err))),
+                                         "max_rel": float(err.max())},
         "roofline": {"kernel": "gravity walk", "bound": "valu", "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None},

@@ -200,6 +200,14 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist,
 int serenx_last_timings(const serenx_ctx *ctx, double *ms4);
 int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4);
 
+/* Walk statistics of the last serenx_force_eval (performance diagnostics):
+ * [0] gravity cells visited by waves, [1] gravity lane visits,
+ * [2] h-solve cells visited, [3] h-solve lane visits, [4] h-solve passes
+ * summed over waves, [5] most passes of one wave, [6] h iterations summed over
+ * particles, [7] h-solve waves, [8] neighbour-walk cells visited,
+ * [9] neighbour-walk lane visits; [10..15] zero. */
+int serenx_walk_stats(serenx_ctx *ctx, int64_t *stats16);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,7 +149,7 @@ void free_all(Ctx &c) {
                     c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
                     c.root, c.bbpart, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
-                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount};
+                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount, c.wstat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
@@ -320,6 +320,8 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * NTAB * KT * 3 / 2 + 64);
     if (e == hipSuccess) e = hipMalloc((void **)&c->icount, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->wstat, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->wstat, 0, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->ovf_count, sizeof(int));
     if (e != hipSuccess) {
         free_all(*c);
@@ -520,6 +522,7 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     Ctx &c = *as_ctx(ctx);
     if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
     SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipMemsetAsync(c.wstat, 0, 16 * sizeof(unsigned long long), c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     SX_TRY(run_tree_update(c));
     if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
@@ -565,6 +568,16 @@ int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4) {
     SX_HIP(hipStreamSynchronize(c.stream));
     SX_HIP(hipMemcpy(v, c.icount, sizeof(v), hipMemcpyDeviceToHost));
     for (int k = 0; k < 4; k++) counts4[k] = (int64_t)v[k];
+    return SERENX_OK;
+}
+
+int serenx_walk_stats(serenx_ctx *ctx, int64_t *stats16) {
+    if (!ctx || !stats16) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    unsigned long long v[16];
+    SX_HIP(hipStreamSynchronize(c.stream));
+    SX_HIP(hipMemcpy(v, c.wstat, sizeof(v), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 16; k++) stats16[k] = (int64_t)v[k];
     return SERENX_OK;
 }
 

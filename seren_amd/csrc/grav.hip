@@ -43,6 +43,7 @@ __device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
                                           double xj, double yj, double zj, double mj, double hj,
                                           double zj_zeta, double &ax, double &ay, double &az,
                                           double &pot) {
+#pragma clang fp contract(fast)
     const double dx = xj - rpx, dy = yj - rpy, dz = zj - rpz;
     const double drsqd = dx * dx + dy * dy + dz * dz;
     const double drmag = sqrt(drsqd) + SMALL_NUMBER;
@@ -82,6 +83,7 @@ __device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
 template <int OCT>
 __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], double drsqd,
                                           double &ax, double &ay, double &az, double &pot) {
+#pragma clang fp contract(fast)
     const double *q = g.q;
     const double invdrsqd = 1.0 / (drsqd + SMALL_NUMBER);
     const double invdrmag = sqrt(invdrsqd);
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     const uint8_t *__restrict__ active, const double *__restrict__ ktab,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
-    unsigned long long *__restrict__ icount) {
+    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const double *w4 = ktab + T_W4 * KT;
@@ -161,42 +163,39 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     }
     double acx = 0, acy = 0, acz = 0, pc = 0;   // cell terms
     double apx = 0, apy = 0, apz = 0, pp = 0;   // particle-particle terms
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0;
-    group_walk(gn, nc, valid, [&](const GNode &g, int c) -> bool {
-        const int leaf = g.leaf;
-        if (leaf == 1) {                                  // :128-136
-            const int j = g.first;
-            if (j != p) {
-                { const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
-                          zeta[j], apx, apy, apz, pp); nin_p += in_ & 1; nin_j += in_ >> 1; }
-                npp++;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
+    const unsigned vis = group_walk(
+        gn, nc, valid, lvis,
+        [&](const GNode &g) -> int {
+            if (g.leaf == 1) return WALK_LEAF;                // :128-136, no MAC test
+            const double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
+            const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            bool accept;
+            if (MACT == SERENX_MAC_GEOMETRIC)
+                accept = drsqd > g.dminsqd;
+            else
+                accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
+            if (accept) {
+                cell_fp64<OCT>(g, dr, drsqd, acx, acy, acz, pc);
+                ncell++;
+                return WALK_SKIP;
             }
-            return false;
-        }
-        double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
-        const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-        bool accept;
-        if (MACT == SERENX_MAC_GEOMETRIC)
-            accept = drsqd > g.dminsqd;
-        else
-            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-        if (accept) {
-            cell_fp64<OCT>(g, dr, drsqd, acx, acy, acz, pc);
-            ncell++;
-            return false;
-        }
-        if (leaf > 0) {                                   // :242-253
-            for (int i = 0; i < leaf; i++) {
-                const int j = g.first + i;
-                if (j == p) continue;
-                { const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, x[j], y[j], z[j], m[j], h[j],
-                          zeta[j], apx, apy, apz, pp); nin_p += in_ & 1; nin_j += in_ >> 1; }
+            return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
+        },
+        [&](const GNode &g, bool mine) {
+            const int k = g.leaf, f0 = g.first;
+            for (int i = 0; i < k; i++) {
+                const int j = f0 + i;
+                const double xj = x[j], yj = y[j], zj = z[j], mj = m[j], hj = h[j],
+                             zetaj = zeta[j];
+                if (!mine || j == p) continue;
+                const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, xj, yj, zj, mj,
+                                          hj, zetaj, apx, apy, apz, pp);
                 npp++;
+                nin_p += in_ & 1;
+                nin_j += in_ >> 1;
             }
-            return false;
-        }
-        return true;
-    });
+        });
     if (valid) {
         const double gx = acx + apx, gy = acy + apy, gz = acz + apz;
         ax[p] = ax[p] + gx;
@@ -207,6 +206,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     }
     // per-wave interaction counts
     wave_count4(icount, ncell, npp, nin_p, nin_j);
+    wave_stats(wstat, 0, vis, lvis);
 }
 
 // ---- mixed precision ------------------------------------------------------
@@ -225,6 +225,7 @@ __device__ __forceinline__ int pair_fp32(const float *__restrict__ w4, const flo
                                           const float *__restrict__ w6, float invhp, float zo_p,
                                           float dx, float dy, float dz, float mj, float hj,
                                           float zj, CSum &ax, CSum &ay, CSum &az, CSum &pot) {
+#pragma clang fp contract(fast)
     const float drsqd = dx * dx + dy * dy + dz * dz;
     const float drmag = sqrtf(drsqd) + 1.0e-20f;
     const float invdrmag = 1.0f / drmag;
@@ -262,6 +263,7 @@ __device__ __forceinline__ int pair_fp32(const float *__restrict__ w4, const flo
 template <int OCT>
 __device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], CSum &ax, CSum &ay,
                                           CSum &az, CSum &pot) {
+#pragma clang fp contract(fast)
     const float *q = g.q;
     const float drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     const float invdrmag = rsqrtf(drsqd + 1.0e-30f);
@@ -313,7 +315,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     const uint8_t *__restrict__ active, const float *__restrict__ ktabf,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
-    unsigned long long *__restrict__ icount) {
+    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const float *w4 = ktabf + T_W4 * KT;
@@ -332,45 +334,41 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         }
     }
     CSum cx, cy, cz, cp;
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0;
-    group_walk(gn, nc, valid, [&](const GNodeF &g, int c) -> bool {
-        const int leaf = g.leaf;
-        if (leaf == 1) {
-            const int j = g.first;
-            if (j != p) {
-                { const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
-                          (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
-                          cz, cp); nin_p += in_ & 1; nin_j += in_ >> 1; }
-                npp++;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
+    const unsigned vis = group_walk(
+        gn, nc, valid, lvis,
+        [&](const GNodeF &g) -> int {
+            if (g.leaf == 1) return WALK_LEAF;
+            const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
+            const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
+            bool accept;
+            if (MACT == SERENX_MAC_GEOMETRIC)
+                accept = drsqd > g.dminsqd;
+            else
+                accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
+            if (accept) {
+                const float dr[3] = {(float)d0, (float)d1, (float)d2};
+                cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
+                ncell++;
+                return WALK_SKIP;
             }
-            return false;
-        }
-        const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
-        const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
-        bool accept;
-        if (MACT == SERENX_MAC_GEOMETRIC)
-            accept = drsqd > g.dminsqd;
-        else
-            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-        if (accept) {
-            const float dr[3] = {(float)d0, (float)d1, (float)d2};
-            cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
-            ncell++;
-            return false;
-        }
-        if (leaf > 0) {
-            for (int i = 0; i < leaf; i++) {
-                const int j = g.first + i;
-                if (j == p) continue;
-                { const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(x[j] - rpx), (float)(y[j] - rpy),
-                          (float)(z[j] - rpz), (float)m[j], (float)h[j], (float)zeta[j], cx, cy,
-                          cz, cp); nin_p += in_ & 1; nin_j += in_ >> 1; }
+            return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
+        },
+        [&](const GNodeF &g, bool mine) {
+            const int k = g.leaf, f0 = g.first;
+            for (int i = 0; i < k; i++) {
+                const int j = f0 + i;
+                const double xj = x[j], yj = y[j], zj = z[j];
+                const float mj = (float)m[j], hj = (float)h[j], zetaj = (float)zeta[j];
+                if (!mine || j == p) continue;
+                const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(xj - rpx),
+                                          (float)(yj - rpy), (float)(zj - rpz), mj, hj, zetaj,
+                                          cx, cy, cz, cp);
                 npp++;
+                nin_p += in_ & 1;
+                nin_j += in_ >> 1;
             }
-            return false;
-        }
-        return true;
-    });
+        });
     if (valid) {
         const double gx = cx.value(), gy = cy.value(), gz = cz.value();
         ax[p] = ax[p] + gx;
@@ -380,6 +378,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
     wave_count4(icount, ncell, npp, nin_p, nin_j);
+    wave_stats(wstat, 0, vis, lvis);
 }
 
 // direct_sph_gravity.F90:63-79: sum of gravity_gradh over all other
@@ -437,11 +436,11 @@ void launch_grav(Ctx &c) {
         hipLaunchKernelGGL((k_grav_mixed<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
                            c.nc, c.gnf, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active,
                            (const float *)(c.ktab + NTAB * KT), c.ax, c.ay, c.az, c.gpot, c.agm,
-                           c.icount);
+                           c.icount, c.wstat);
     else
         hipLaunchKernelGGL((k_grav_fp64<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
                            c.nc, c.gn, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active, c.ktab, c.ax,
-                           c.ay, c.az, c.gpot, c.agm, c.icount);
+                           c.ay, c.az, c.gpot, c.agm, c.icount, c.wstat);
 }
 
 }  // namespace

@@ -147,6 +147,8 @@ struct Ctx {
 
     // interaction counters of the gravity walk
     unsigned long long *icount = nullptr;
+    // walk statistics of the last force evaluation (serenx_walk_stats)
+    unsigned long long *wstat = nullptr;
 
     // host-state binding
     serenx_host_state hs{};

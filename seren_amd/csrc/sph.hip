@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) k_hsolve(
     const uint8_t *__restrict__ active, const double *__restrict__ ktab, double h_fac,
     double rextent, int pp_gather, double *__restrict__ h, double *__restrict__ zeta,
     double *__restrict__ rho, double *__restrict__ omega, double *__restrict__ divv,
-    double *__restrict__ drhodt, int *__restrict__ err) {
+    double *__restrict__ drhodt, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const double *w0 = ktab + T_W0 * KT;
@@ -48,8 +48,10 @@ __global__ void __launch_bounds__(256) k_hsolve(
     double rhotemp = 0.0;
     int iteration = 0;
     int mode = valid ? M_ITER : M_DONE;
+    unsigned passes = 0, vis = 0, lvis = 0;
 
     while (__any(mode != M_DONE)) {
+        passes++;
         bool newlist = false;
         if (mode == M_ITER && hp > h_high) {
             h_high = hp * HMULT;
@@ -67,44 +69,46 @@ __global__ void __launch_bounds__(256) k_hsolve(
         double szeta = mp * invhp * invhp * wg[0];
         int cnt = 0;
         const int lmode = mode;
-        group_walk(hn, nc, mode != M_DONE, [&](const HNode &nd, int c) -> bool {
-            const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
-            const double drsqd = dx * dx + dy * dy + dz * dz;
-            const double lim = (R + nd.rmax) * (R + nd.rmax);
-            if (!(drsqd <= lim)) return false;
-            const int k = nd.leaf;
-            if (k == 0) return true;
-            const int f0 = nd.first;
-            cnt += k;
-            for (int i = 0; i < k; i++) {
-                const int j = f0 + i;
-                if (j == p) continue;
-                const double ex = x[j] - rpx, ey = y[j] - rpy, ez = z[j] - rpz;
-                double dsq = ex * ex + ey * ey + ez * ez;
-                if (lmode == M_ITER) {
-                    if (dsq >= hrangesqd) continue;                 // :223
-                    const double drmag = sqrt(dsq) + SMALL_NUMBER;
-                    const int kern = (int)(HALFKERNTOT * drmag * invhp);
-                    if (kern >= KERNTOT) continue;                  // :227
-                    const double mpp = m[j];
-                    srho = srho + mpp * hfactor_p * w0[kern];
-                    som = som + mpp * hfactor_p * invhp * wh[kern];
-                } else {
-                    if (dsq > hrangesqd) continue;                  // :364
-                    const double drmag = sqrt(dsq) + SMALL_NUMBER;
-                    const int kern = (int)(HALFKERNTOT * drmag * invhp);
-                    if (kern > KERNTOT) continue;                   // :368
-                    const double mpp = m[j];
-                    const double dvx = vx[j] - vpx, dvy = vy[j] - vpy, dvz = vz[j] - vpz;
-                    const double dvdr = dvx * ex + dvy * ey + dvz * ez;
-                    srho = srho + mpp * hfactor_p * w0[kern];
-                    som = som + mpp * hfactor_p * invhp * wh[kern];
-                    sdiv = sdiv - mpp * dvdr * w4[kern] * hfactor_p * invhp / drmag;
-                    szeta = szeta + mpp * invhp * invhp * wg[kern];
+        vis += group_walk(
+            hn, nc, mode != M_DONE, lvis,
+            [&](const HNode &nd) -> int {
+                const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+                const double drsqd = dx * dx + dy * dy + dz * dz;
+                const double lim = (R + nd.rmax) * (R + nd.rmax);
+                if (!(drsqd <= lim)) return WALK_SKIP;       // BHhydrowalk_hgather.F90:40
+                return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
+            },
+            [&](const HNode &nd, bool mine) {
+                const int k = nd.leaf, f0 = nd.first;
+                if (mine) cnt += k;
+                for (int i = 0; i < k; i++) {
+                    const int j = f0 + i;
+                    const double xj = x[j], yj = y[j], zj = z[j], mj = m[j];
+                    const double vxj = vx[j], vyj = vy[j], vzj = vz[j];
+                    if (!mine || j == p) continue;
+                    const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
+                    const double dsq = ex * ex + ey * ey + ez * ez;
+                    if (lmode == M_ITER) {
+                        if (dsq >= hrangesqd) continue;                 // :223
+                        const double drmag = sqrt(dsq) + SMALL_NUMBER;
+                        const int kern = (int)(HALFKERNTOT * drmag * invhp);
+                        if (kern >= KERNTOT) continue;                  // :227
+                        srho = srho + mj * hfactor_p * w0[kern];
+                        som = som + mj * hfactor_p * invhp * wh[kern];
+                    } else {
+                        if (dsq > hrangesqd) continue;                  // :364
+                        const double drmag = sqrt(dsq) + SMALL_NUMBER;
+                        const int kern = (int)(HALFKERNTOT * drmag * invhp);
+                        if (kern > KERNTOT) continue;                   // :368
+                        const double dvx = vxj - vpx, dvy = vyj - vpy, dvz = vzj - vpz;
+                        const double dvdr = dvx * ex + dvy * ey + dvz * ez;
+                        srho = srho + mj * hfactor_p * w0[kern];
+                        som = som + mj * hfactor_p * invhp * wh[kern];
+                        sdiv = sdiv - mj * dvdr * w4[kern] * hfactor_p * invhp / drmag;
+                        szeta = szeta + mj * invhp * invhp * wg[kern];
+                    }
                 }
-            }
-            return false;
-        });
+            });
 
         if (mode == M_ITER && newlist && cnt < pp_gather) {  // :185-189
             h_high = 0.0;
@@ -151,6 +155,17 @@ __global__ void __launch_bounds__(256) k_hsolve(
             mode = M_DONE;
         }
     }
+    // walk statistics: cells visited / lane visits, passes, lane iterations
+    wave_stats(wstat, 2, vis, lvis);
+    unsigned long long it = valid ? (unsigned long long)iteration : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) it += __shfl_xor(it, o, 64);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(wstat + 4, (unsigned long long)passes);
+        atomicMax(wstat + 5, (unsigned long long)passes);
+        atomicAdd(wstat + 6, it);
+        atomicAdd(wstat + 7, 1ull);
+    }
 }
 
 // get_neib.F90:46-118: gather-or-scatter neighbours in walk (DFS) order.
@@ -167,7 +182,8 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
                                               int *__restrict__ pptot, int *__restrict__ pplist,
                                               const int *__restrict__ ovf_off,
                                               int *__restrict__ ovf_list,
-                                              int *__restrict__ ovf_count) {
+                                              int *__restrict__ ovf_count,
+                                              unsigned long long *__restrict__ wstat) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = p < n && active[p];
     if (OVERFLOW_PASS && valid) valid = pptot[p] > pp_limit;
@@ -185,33 +201,37 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
         row = OVERFLOW_PASS ? ovf_list + ovf_off[p] : pplist + (size_t)p * pp_limit;
         if (OVERFLOW_PASS) lim = pptot[p];
     }
-    group_walk(hn, nc, valid, [&](const HNode &nd, int c) -> bool {
-        const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
-        const double drsqd = dx * dx + dy * dy + dz * dz;
-        const double m1 = (R + nd.rmax) * (R + nd.rmax);
-        const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
-        if (!(drsqd <= m1 || drsqd <= m2)) return false;   // BHhydro_walk.F90:43
-        const int k = nd.leaf;
-        if (k == 0) return true;
-        const int f0 = nd.first;
-        for (int i = 0; i < k; i++) {
-            const int j = f0 + i;
-            if (j == p) continue;
-            const double hj = h[j];
-            const double hrangesqd_pp = KERNRANGESQD * hj * hj;
-            const double ex = x[j] - rpx, ey = y[j] - rpy, ez = z[j] - rpz;
-            const double dsq = ex * ex + ey * ey + ez * ez;
-            if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
-                if (tot < lim) row[tot] = j;
-                tot++;
+    unsigned lvis = 0;
+    const unsigned vis = group_walk(
+        hn, nc, valid, lvis,
+        [&](const HNode &nd) -> int {
+            const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+            const double drsqd = dx * dx + dy * dy + dz * dz;
+            const double m1 = (R + nd.rmax) * (R + nd.rmax);
+            const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
+            if (!(drsqd <= m1 || drsqd <= m2)) return WALK_SKIP;   // BHhydro_walk.F90:43
+            return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
+        },
+        [&](const HNode &nd, bool mine) {
+            const int k = nd.leaf, f0 = nd.first;
+            for (int i = 0; i < k; i++) {
+                const int j = f0 + i;
+                const double xj = x[j], yj = y[j], zj = z[j], hj = h[j];
+                if (!mine || j == p) continue;
+                const double hrangesqd_pp = KERNRANGESQD * hj * hj;
+                const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
+                const double dsq = ex * ex + ey * ey + ez * ez;
+                if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
+                    if (tot < lim) row[tot] = j;
+                    tot++;
+                }
             }
-        }
-        return false;
-    });
+        });
     if (valid && !OVERFLOW_PASS) {
         pptot[p] = tot;
         if (tot > pp_limit) atomicAdd(ovf_count, 1);
     }
+    if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
 }
 
 __global__ void k_ovf_sizes(int n, int pp_limit, const int *__restrict__ pptot,
@@ -315,7 +335,7 @@ int sph_hsolve(Ctx &c) {
     SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
     hipLaunchKernelGGL(k_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn, c.x,
                        c.y, c.z, c.m, c.vx, c.vy, c.vz, c.active, c.ktab, c.opt.h_fac, c.rextent,
-                       c.pp_gather, c.h, c.zeta, c.rho, c.omega, c.divv, c.drhodt, c.flag);
+                       c.pp_gather, c.h, c.zeta, c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     return SERENX_OK;
@@ -326,7 +346,7 @@ int sph_neighbours(Ctx &c) {
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x, c.y,
                        c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
-                       c.ovf_count);
+                       c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
@@ -353,7 +373,7 @@ int sph_neighbours(Ctx &c) {
         }
         hipLaunchKernelGGL(k_neib<true>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x,
                            c.y, c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off,
-                           c.ovf_list, c.ovf_count);
+                           c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());
     }
     return SERENX_OK;

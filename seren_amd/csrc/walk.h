@@ -6,11 +6,10 @@
 
 namespace serenx {
 
-__device__ __forceinline__ int wave_min_int(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+
+// minimum over the 64 lanes of the wave (DPP reduction in the device library)
+__device__ __forceinline__ int wave_min_int(int v) { return __ockl_wfred_min_i32(v); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -64,29 +63,57 @@ __device__ __forceinline__ double ref_pow_third(double x) {
 // target; the wave visits, in DFS order, the union of the cells its lanes
 // visit on their own stackless walks (nextcell / ifopen = c + 1).  A lane is
 // active at cell c when its own walk would be at c (resume == c); it then
-// tests c, and either descends (resume = c + 1) or moves past the subtree
-// (resume = next).  Each lane therefore sees exactly its own interaction set
-// in its own order, as a per-particle walk would.
+// tests c and returns WALK_DESCEND (open a branch: resume = c + 1),
+// WALK_LEAF (open a leaf) or WALK_SKIP (cell used or rejected), the latter two
+// moving past the subtree (resume = next).  Each lane therefore sees exactly
+// its own interaction set in its own order, as a per-particle walk would.
 //
-//   Node  : node type with fields next, leaf (count or 0), first
-//   visit : bool visit(const Node&, int c, bool &descend) for an active lane;
-//           returns nothing useful, sets descend for an opened branch
-template <class Node, class Visit>
-__device__ __forceinline__ void group_walk(const Node *__restrict__ nodes, int nc, bool walking,
-                                           Visit &&visit) {
+// Opened leaves are handled in wave-uniform control flow: leaf(node, mine) is
+// called once per leaf any lane opened, with mine set on the lanes that did,
+// so the leaf's particles (contiguous in tree order) can be read once through
+// the scalar cache for the whole wave.
+//
+// Returns the number of cells the wave visited; lane_visits counts the cells
+// at which this lane was active (their ratio is the walk's SIMD utilisation).
+enum { WALK_SKIP = 0, WALK_DESCEND = 1, WALK_LEAF = 2 };
+
+template <class Node, class Visit, class Leaf>
+__device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, int nc,
+                                               bool walking, unsigned &lane_visits,
+                                               Visit &&visit, Leaf &&leaf) {
     int resume = walking ? 0 : nc;
     int c = uniform(wave_min_int(resume));
+    unsigned visits = 0;
     while (c < nc) {
-        const int next = nodes[c].next;
-        bool descend = false;
+        // read the cell in wave-uniform control flow from a uniform address so
+        // that it comes through the scalar cache into SGPRs
+        const Node nd = nodes[c];
+        int act = WALK_SKIP;
+        visits++;
         if (resume == c) {
-            descend = visit(nodes[c], c);
-            resume = descend ? c + 1 : next;
+            lane_visits++;
+            act = visit(nd);
+            resume = (act == WALK_DESCEND) ? c + 1 : nd.next;
         }
-        if (__any(descend))
+        if (__any(act == WALK_LEAF)) leaf(nd, act == WALK_LEAF);
+        if (__any(act == WALK_DESCEND))
             c = c + 1;
         else
             c = uniform(wave_min_int(resume));
+    }
+    return visits;
+}
+
+// Adds per-wave walk statistics into stats[k0], stats[k0 + 1] from lane 0:
+// the wave's visit count and the wave's sum of lane visits.
+__device__ __forceinline__ void wave_stats(unsigned long long *stats, int k0, unsigned visits,
+                                           unsigned lane_visits) {
+    unsigned long long v = lane_visits;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(stats + k0, (unsigned long long)visits);
+        atomicAdd(stats + k0 + 1, v);
     }
 }
 

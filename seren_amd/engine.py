@@ -43,7 +43,7 @@ EXPORTED = [
     "serenx_hguess", "serenx_sph_update", "serenx_hydro_forces", "serenx_grav_forces",
     "serenx_load", "serenx_set_agravmag", "serenx_force_eval", "serenx_sync",
     "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
-    "serenx_last_timings", "serenx_interaction_counts",
+    "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
 ]
 
 
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_export_tree.argtypes = [vp, _ip, _ip, _ip, _ip] + [_dp] * 9 + [_ip, _ip]
     lib.serenx_last_timings.argtypes = [vp, _dp]
     lib.serenx_interaction_counts.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.serenx_walk_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     _lib = lib
     return lib
 
@@ -232,6 +233,18 @@ class ForceEngine:
         ms = np.zeros(4)
         _check(self.lib.serenx_last_timings(self.ctx, _dptr(ms)))
         return dict(tree=ms[0], sph=ms[1], hydro=ms[2], grav=ms[3])
+
+    def walk_stats(self) -> Dict[str, float]:
+        """Per-walk SIMD utilisation and h-solve pass counts of the last evaluation."""
+        v = (C.c_int64 * 16)()
+        _check(self.lib.serenx_walk_stats(self.ctx, v))
+        v = [int(x) for x in v]
+        util = lambda a, b: (b / (64.0 * a)) if a else 0.0  # noqa: E731
+        return {"grav_cells_per_wave_visit_util": util(v[0], v[1]), "grav_wave_visits": v[0],
+                "hsolve_util": util(v[2], v[3]), "hsolve_wave_visits": v[2],
+                "hsolve_mean_passes": v[4] / max(v[7], 1), "hsolve_max_passes": v[5],
+                "hsolve_mean_iterations": v[6] / max(self.n, 1),
+                "neib_util": util(v[8], v[9]), "neib_wave_visits": v[8]}
 
     def interaction_counts(self):
         """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""
