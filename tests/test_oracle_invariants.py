@@ -1,0 +1,216 @@
+"""Closed forms and invariants that pin the oracle independently of the
+reference build (each check is chosen so that a dropped term, wrong sign or
+index, or transposed operand in the oracle fails it):
+
+* kernel tables (kernel.F90): normalisation of W, w1 = dW/ds, w5 = enclosed
+  mass / s^2, w6' = -w5, wg = w6 - s w5, wh = -3 w0 - s w4;
+* gravity_gradh: point-mass law outside both kernels, pairwise antisymmetry;
+* neighbour lists (get_neib): equal to the brute-force O(N^2) gather-or-
+  scatter sets, in ascending tree order;
+* density (all_sph_gradh final pass): brute-force SPH sum at the stored h;
+* tree (BHgrav_build/BHgrav_stock): leaf sizes, every particle in one leaf,
+  DFS closure, monopole and quadrupole equal to direct sums over each cell's
+  particles, multipole expansion converging to the direct potential;
+* hydro forces: total momentum conserved (pairwise antisymmetry).
+"""
+import numpy as np
+import pytest
+
+from seren_amd import ics
+
+KT = 10001
+S = (2.0 * np.arange(KT) + 1.0) / 10000.0
+DS = 2.0e-4
+
+
+@pytest.fixture(scope="module")
+def tables():
+    from oracle_lib import CSeren
+    pos, vel, mass, h = ics.make_cloud(64, "uniform", seed=3)
+    o = CSeren(False, "geometric")
+    o.setup(pos, mass, h, vel)
+    return o.kernel()
+
+
+def test_kernel_normalisation(tables):
+    w0 = tables[0]
+    mass = 4.0 * np.pi * np.sum(w0 * S * S) * DS   # midpoint rule over [0, 2]
+    assert abs(mass - 1.0) < 1e-6
+
+
+def test_kernel_gradient_tables(tables):
+    w0, w1, w4 = tables[0], tables[1], tables[4]
+    dw0 = (w0[2:] - w0[:-2]) / (2 * DS)
+    assert np.max(np.abs(dw0 - w1[1:-1])) < 1e-6
+    assert np.array_equal(w1, w4)              # plain M4 (no Thomas-Couchman)
+
+
+def test_kernel_gravity_tables(tables):
+    w0, w5, w6, wg = tables[0], tables[5], tables[6], tables[8]
+    # w5(s) = M(<s)/s^2 with M(<s) = 4 pi int_0^s W s'^2 ds' (unit mass)
+    menc = 4.0 * np.pi * (np.cumsum(w0 * S * S) - 0.5 * w0 * S * S) * DS
+    assert np.max(np.abs(menc - w5 * S * S)) < 1e-7
+    # potential: d w6/ds = -w5, and w6(2) = 1/2 (point mass at the kernel edge)
+    dw6 = (w6[2:] - w6[:-2]) / (2 * DS)
+    assert np.max(np.abs(dw6 + w5[1:-1])) < 1e-6
+    assert abs(w6[-1] - 1.0 / S[-1]) < 1e-3
+    # d(phi)/dh table: wg = w6 - s w5 (zero outside the kernel)
+    assert np.max(np.abs(wg - (w6 - S * w5))) < 1e-12
+
+
+def test_kernel_dh_table(tables):
+    w0, w4, wh = tables[0], tables[4], tables[7]
+    assert np.max(np.abs(wh - (-3.0 * w0 - S * w4))) < 1e-12
+
+
+def _oracle_cloud(n=2000, profile="uniform", seed=17, octupole=False, mac="geometric"):
+    from oracle_lib import CSeren
+    pos, vel, mass, h = ics.make_cloud(n, profile, seed=seed, mach=1.0, ngrid=16)
+    o = CSeren(octupole, mac)
+    o.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+    o.tree_update()
+    o.hguess()
+    o.sph_update()
+    o.thermal_iso(0.2)
+    o.hydro_forces()
+    a_h = o.particles()["a"].copy()
+    o.grav_forces()
+    return o, a_h
+
+
+def test_gravity_pair_point_mass_and_antisymmetry():
+    from oracle_lib import CSeren
+    # two particles far outside each other's kernels, then overlapping ones
+    for sep, hh in ((3.0, (0.5, 0.7)), (0.3, (0.4, 0.25))):
+        pos = np.array([[0.0, 0.0, 0.0], [sep, 0.0, 0.0]])
+        mass = np.array([0.3, 0.7])
+        o = CSeren(False, "geometric")
+        o.setup(pos, mass, np.array(hh), np.zeros((2, 3)))
+        o.set_h(np.array(hh))
+        a, pot = o.direct_gravity(np.array([1, 2], np.int32))
+        if sep > 2 * max(hh):
+            assert abs(a[0, 0] - mass[1] / sep ** 2) < 1e-15
+            assert abs(a[1, 0] + mass[0] / sep ** 2) < 1e-15
+            assert abs(pot[0] - mass[1] / sep) < 1e-15
+        # Newton's third law of the symmetrised softened force
+        assert abs(mass[0] * a[0, 0] + mass[1] * a[1, 0]) < 1e-14 * abs(mass[0] * a[0, 0])
+
+
+def test_neighbour_lists_equal_brute_force_sets():
+    o, _ = _oracle_cloud()
+    p = o.particles()
+    pptot, pplist = o.neib()
+    x, h = p["pos"], p["h"]
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    near = (d2 <= (4.0 * h * h)[:, None]) | (d2 <= (4.0 * h * h)[None, :])
+    np.fill_diagonal(near, False)
+    for i in range(len(h)):
+        expect = np.nonzero(near[i])[0] + 1       # 1-based ids, ascending tree order
+        assert pptot[i] == len(expect)
+        if pptot[i] <= pplist.shape[1]:
+            assert np.array_equal(pplist[i, :pptot[i]], expect)
+
+
+def test_density_equals_brute_force_sum():
+    o, _ = _oracle_cloud()
+    p = o.particles()
+    w0 = o.kernel()[0]
+    x, h, m = p["pos"], p["h"], p["m"]
+    r = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)) + 1e-20
+    s = r / h[:, None]
+    k = (5000.0 * s).astype(np.int64)
+    inside = (r * r <= (4.0 * h * h)[:, None] + 0.0) & (k <= 10000)
+    np.fill_diagonal(inside, False)
+    w = np.where(inside, w0[np.minimum(k, 10000)], 0.0)
+    rho = (m[None, :] * w).sum(1) / h ** 3 + m * w0[0] / h ** 3
+    assert np.max(np.abs(rho - p["rho"]) / p["rho"]) < 1e-12
+    # grad-h closure: h within H_CON of h_fac (m/rho)^(1/3)
+    assert np.max(np.abs(h - 1.2 * (m / p["rho"]) ** 0.333333333) / h) < 0.05
+
+
+def _cell_ranges(t):
+    """Particle range [first, last) of every cell from the exported topology."""
+    nc = len(t["leaf"])
+    rng = np.zeros((nc, 2), np.int64)
+    order = []
+    c = 0
+    while c < nc:                                 # DFS over ifopen / nextcell
+        order.append(c)
+        c = t["ifopen"][c] if t["leaf"][c] == 0 else t["nextcell"][c]
+    for c in order:
+        if t["leaf"][c] > 0:
+            rng[c] = (t["plist"][c, 0] - 1, t["plist"][c, t["leaf"][c] - 1] - 1)
+    for c in reversed(order):
+        if t["leaf"][c] == 0:
+            kids = []
+            k = t["ifopen"][c]
+            while k < nc and k != t["nextcell"][c]:
+                kids.append(k)
+                k = t["nextcell"][k]
+            rng[c] = (rng[kids[0], 0], rng[kids[-1], 1])
+    return order, rng
+
+
+def test_tree_structure_and_moments():
+    o, _ = _oracle_cloud(octupole=True, mac="gadget2")
+    p = o.particles()
+    t = o.grav_tree()
+    n = len(p["m"])
+    leaf = t["leaf"]
+    assert leaf.max() <= 8 and leaf[0] == 0
+    order, rng = _cell_ranges(t)
+    assert sorted(order) == list(range(len(leaf)))      # DFS visits every cell once
+    ids = np.concatenate([t["plist"][c, :leaf[c]] for c in order if leaf[c] > 0])
+    assert np.array_equal(ids, np.arange(1, n + 1))     # tree order = DFS leaf order
+    x, m = p["pos"], p["m"]
+    for c in order:
+        a, b = rng[c]
+        xs, ms = x[a:b + 1], m[a:b + 1]
+        if leaf[c] == 1:
+            assert np.array_equal(t["r"][c], xs[0])
+            continue
+        mc = ms.sum()
+        com = (ms[:, None] * xs).sum(0) / mc
+        assert abs(t["m"][c] - mc) <= 1e-12 * mc
+        assert np.max(np.abs(t["r"][c] - com)) <= 1e-12
+        d = xs - com
+        d2 = (d * d).sum(1)
+        q = np.array([(ms * (3 * d[:, 0] * d[:, 0] - d2)).sum(), (ms * 3 * d[:, 0] * d[:, 1]).sum(),
+                      (ms * (3 * d[:, 1] * d[:, 1] - d2)).sum(), (ms * 3 * d[:, 2] * d[:, 0]).sum(),
+                      (ms * 3 * d[:, 2] * d[:, 1]).sum()])
+        scale = (ms * d2).sum() + 1e-300
+        assert np.max(np.abs(t["q"][c] - q)) <= 1e-10 * scale
+
+
+def test_multipole_expansion_converges_to_direct_potential():
+    """Monopole+quadrupole potential of the root cell at distance R: error
+    falls as R^-4 relative to the R^-1 potential (no dropped or mis-signed
+    quadrupole term)."""
+    o, _ = _oracle_cloud(n=1500, profile="plummer")
+    p = o.particles()
+    t = o.grav_tree()
+    x, m = p["pos"], p["m"]
+    q = t["q"][0]
+    Q = np.array([[q[0], q[1], q[3]], [q[1], q[2], q[4]], [q[3], q[4], -(q[0] + q[2])]])
+    errs, mono = [], []
+    for R in (64.0, 128.0):
+        pt = t["r"][0] + R * np.array([0.48, -0.6, 0.64])
+        direct = np.sum(m / np.linalg.norm(x - pt, axis=1))
+        dr = pt - t["r"][0]
+        r = np.linalg.norm(dr)
+        approx = t["m"][0] / r + 0.5 * dr @ Q @ dr / r ** 5
+        errs.append(abs(approx - direct) / direct)
+        mono.append(abs(t["m"][0] / r - direct) / direct)
+    # about the COM the dipole vanishes: monopole error ~ R^-2 (x1/4 per
+    # doubling); with the quadrupole it is ~ R^-3 (x1/8) and much smaller
+    assert errs[1] < errs[0] / 6.0
+    assert mono[1] > mono[0] / 6.0
+    assert errs[1] < mono[1] / 5.0
+
+
+def test_hydro_momentum_conservation():
+    o, a_h = _oracle_cloud(n=3000, profile="r2")
+    m = o.particles()["m"]
+    tot = np.abs((m[:, None] * a_h).sum(0)).max()
+    scale = (m[:, None] * np.abs(a_h)).sum(0).max()
+    assert tot <= 1e-12 * scale

@@ -1,0 +1,52 @@
+"""Oracle against the live reference build on fresh seeded inputs.
+
+Runs only where oracle/_ref has been built from the reference sources
+(make -f oracle/Makefile.ref); elsewhere the committed golden vectors pin the
+oracle (test_oracle_golden.py).
+"""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(os.path.dirname(HERE), "oracle", "_ref")
+pytestmark = pytest.mark.skipif(
+    not os.path.exists(os.path.join(REF_DIR, "libseren_ref_quad_geo.so")),
+    reason="reference not built (oracle/_ref)")
+
+CASES = [
+    (1500, "uniform", 11, False, "geometric", 1),
+    (2500, "plummer", 12, False, "geometric", 1),
+    (3000, "r2", 13, True, "gadget2", 2),
+    (700, "uniform", 14, True, "gadget2", 3),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_bit_exact_with_reference(case):
+    from oracle_lib import CSeren, RefSeren
+    from seren_amd import ics
+    n, prof, seed, octu, mac, nev = case
+    pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=1.5, ngrid=16)
+    outs = []
+    for cls in (RefSeren, CSeren):
+        o = cls(octu, mac)
+        o.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+        for e in range(nev):
+            o.tree_update()
+            if e == 0:
+                o.hguess()
+            o.sph_update()
+            o.thermal_iso(0.2)
+            o.hydro_forces()
+            o.grav_forces()
+        outs.append((o.particles(), o.neib(), o.grav_tree()))
+    (pr, nr, tr), (pc, nc, tc) = outs
+    for k in ("porig", "h", "rho", "omega", "zeta", "div_v", "a", "gpot", "agravmag"):
+        assert np.array_equal(pr[k], pc[k]), k
+    assert np.array_equal(nr[0], nc[0])
+    ok = (nr[0] <= nr[1].shape[1])[:, None] & (np.arange(nr[1].shape[1])[None, :] < nr[0][:, None])
+    assert np.array_equal(np.where(ok, nr[1], 0), np.where(ok, nc[1], 0))
+    for k in ("leaf", "nextcell", "m", "r", "q", "s", "dminsqd", "mac"):
+        assert np.array_equal(tr[k], tc[k]), k
