@@ -55,7 +55,37 @@ def parse():
     ap.add_argument("--direct-targets", type=int, default=16384)
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB device buffer once (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """One device-to-device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of rocprofv3 to bytes."""
+    import torch
+    n = 32 * 1024 * 1024                      # 256 MiB of float64
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per gravity-walk dispatch from the committed rocprofv3 counter
+    record of this workload (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel
+    calibration), or None when no record matches."""
+    import glob
+    for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
+        try:
+            rec = json.load(open(f))
+        except (OSError, ValueError):
+            continue
+        if rec.get("workload") == workload:
+            return rec.get("traffic_bytes_per_dispatch")
+    return None
 
 
 def cpu_baseline(pos, vel, mass, h, args):
@@ -85,6 +115,9 @@ def cpu_baseline(pos, vel, mass, h, args):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
                                    "design-only (see DESIGN.md), use --gpus 1"}))
@@ -132,15 +165,16 @@ def main():
     gms = float(np.mean(grav_ms))
     achieved = flops / (gms * 1e-3) / 1e12
     peak = MI355X_FP64_TFLOPS if args.precision == "fp64" else MI355X_FP32_TFLOPS
+    workload = (f"force evaluation, N={n} {args.profile} sphere, Mach {args.mach}, "
+                f"{args.multipole} + {args.mac} MAC (abserror {args.abserror}), {args.precision}")
+    traffic = measured_traffic(workload)
     out = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
         "data": "synthetic (seeded r^-2 sphere, P(k)~k^-4 turbulent velocities)",
-        "config": {"workload": f"force evaluation, N={n} {args.profile} sphere, Mach "
-                               f"{args.mach}, {args.multipole} + {args.mac} MAC "
-                               f"(abserror {args.abserror}), {args.precision}",
+        "config": {"workload": workload,
                    "n": n, "profile": args.profile, "multipole": args.multipole,
                    "mac": args.mac, "abserror": args.abserror, "precision": args.precision},
         "phases_ms": {k: float(v / args.steps) for k, v in
@@ -152,7 +186,7 @@ def main():
                                          "max_rel": float(err.max())},
         "roofline": {"kernel": "gravity walk", "bound": "valu", "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None},
+                     "traffic": traffic},
     }
     eng.close()
     if args.cpu:
