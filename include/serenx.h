@@ -117,6 +117,8 @@ typedef struct serenx_host_state {
     double *rextent;     /* scalar out: max extent (bounding_box.F90:39) */
 } serenx_host_state;
 
+/* Binding a state with a different ptot resets the device state; re-binding
+ * the same ptot (arrays moved) keeps the tree and neighbour lists. */
 int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st);
 
 /* replaces tree_update (src/main/tree_update.F90:9-92): BHgrav_build,
@@ -127,6 +129,12 @@ int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st);
  * then permutes all its particle arrays with that list (reorder_particle_arrays,
  * reorder_array.F90:9-270), exactly where the reference does. */
 int serenx_tree_update(serenx_ctx *ctx, int32_t *dummylist);
+
+/* replaces the stock-only branch of tree_update (tree_update.F90:35,60): the
+ * tree keeps its topology and particle order, cells are restocked from the
+ * bound parray (BHgrav_stock + BHhydro_stock).  Requires a previous
+ * serenx_tree_update on the same particles. */
+int serenx_tree_stock(serenx_ctx *ctx);
 
 /* replaces BHhydro_hguess (src/BHtree/BHhydro_hguess.F90:11-163) */
 int serenx_hguess(serenx_ctx *ctx);

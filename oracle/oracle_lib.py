@@ -140,11 +140,16 @@ class RefSeren(_Common):
 
     _libs: Dict[str, C.CDLL] = {}
 
-    def __init__(self, octupole: bool, mac: str):
+    def __init__(self, octupole: bool, mac: str, dropin: bool = False):
+        """dropin=True loads the reference force path whose five drivers are
+        replaced by the Fortran binding of the GPU library (make -f
+        oracle/Makefile.ref dropin): the reference code as the host."""
         variant = VARIANTS[(bool(octupole), mac)]
-        path = os.path.join(HERE, "_ref", f"libseren_ref_{variant}.so")
+        kind = "dropin" if dropin else "ref"
+        path = os.path.join(HERE, "_ref", f"libseren_{kind}_{variant}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} not built (make -f oracle/Makefile.ref)")
+        variant = kind + "_" + variant
         if variant not in self._libs:
             lib = C.CDLL(path)
             lib.ref_setup.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,

@@ -371,10 +371,14 @@ int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st) {
     if (st->pp_limit != c.pp_limit)
         return set_error(SERENX_EINVAL, "pp_limit of the host state differs from 2*pp_gather");
     SX_HIP(hipSetDevice(c.device));
+    // re-binding the same particle count keeps the device tree and lists: a
+    // host may re-bind before every call (its arrays can move between steps)
+    const bool same = c.bound && st->ptot == c.n;
     SX_TRY(ensure_capacity(c, st->ptot));
     c.hs = *st;
     c.n = st->ptot;
     c.bound = true;
+    if (same) return SERENX_OK;
     c.tree_ok = c.sph_ok = false;
     hipLaunchKernelGGL(k_init_state, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.porig,
                        c.active, c.rho, c.omega, c.zeta, c.ax, c.ay, c.az, c.gpot, c.agm, c.divv,
@@ -396,6 +400,19 @@ int serenx_tree_update(serenx_ctx *ctx, int32_t *dummylist) {
     if (dummylist)
         SX_HIP(hipMemcpy(dummylist, c.porig, sizeof(int) * c.n, hipMemcpyDeviceToHost));
     if (c.hs.rextent) *c.hs.rextent = c.rextent;
+    return SERENX_OK;
+}
+
+int serenx_tree_stock(serenx_ctx *ctx) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_TRY(need_bound(c));
+    if (!c.tree_ok) return set_error(SERENX_ESTATE, "tree_stock called before tree_update");
+    SX_HIP(hipSetDevice(c.device));
+    const int cols[] = {0, 1, 2, 3, 4};
+    SX_TRY(up_parray(c, cols, 5));
+    SX_TRY(stock_tree(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
     return SERENX_OK;
 }
 

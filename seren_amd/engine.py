@@ -40,6 +40,7 @@ _PREC = {"fp64": FP64, "mixed": MIXED}
 EXPORTED = [
     "serenx_default_options", "serenx_create", "serenx_destroy", "serenx_last_error",
     "serenx_neighbour_params", "serenx_bind_host_state", "serenx_tree_update",
+    "serenx_tree_stock",
     "serenx_hguess", "serenx_sph_update", "serenx_hydro_forces", "serenx_grav_forces",
     "serenx_load", "serenx_set_agravmag", "serenx_force_eval", "serenx_sync",
     "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
@@ -87,7 +88,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_neighbour_params.argtypes = [vp, _ip, _ip]
     lib.serenx_bind_host_state.argtypes = [vp, C.POINTER(HostState)]
     lib.serenx_tree_update.argtypes = [vp, _ip]
-    for f in ("serenx_hguess", "serenx_sph_update", "serenx_hydro_forces",
+    for f in ("serenx_tree_stock", "serenx_hguess", "serenx_sph_update", "serenx_hydro_forces",
               "serenx_grav_forces", "serenx_sync"):
         getattr(lib, f).argtypes = [vp]
     lib.serenx_load.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp]
@@ -320,6 +321,11 @@ def tree_update(engine: ForceEngine, st: SerenState) -> None:
     dl = np.zeros(st.n, np.int32)
     _check(engine.lib.serenx_tree_update(engine.ctx, _iptr(dl)))
     st.reorder(dl)
+
+
+def tree_stock(engine: ForceEngine, st: SerenState) -> None:
+    """The stock-only branch of tree_update.F90 (nstock == nsteps, no rebuild)."""
+    _check(engine.lib.serenx_tree_stock(engine.ctx))
 
 
 def BHhydro_hguess(engine: ForceEngine, st: SerenState) -> None:
