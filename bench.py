@@ -119,8 +119,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
-                                   "design-only (see DESIGN.md), use --gpus 1"}))
+        print(json.dumps({"error": "this force path runs on one GPU; sharding over "
+                                   "several GPUs is not built, use --gpus 1"}))
         sys.exit(2)
     from seren_amd import ics
     from seren_amd.engine import ForceEngine

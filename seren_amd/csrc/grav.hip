@@ -79,61 +79,74 @@ __device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
     return inside;
 }
 
-// BHgrav_accel.F90:159-222 for one accepted cell
+// 1/sqrt(a) for a > 0 to a few ulp: the hardware estimate refined by two
+// Newton steps y <- y + y (1 - a y^2) / 2 (each squares the relative error).
+__device__ __forceinline__ double rsqrt_fp64(double a) {
+#pragma clang fp contract(fast)
+    double y = __builtin_amdgcn_rsq(a);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double e = __fma_rn(-a * y, y, 1.0);
+        y = __fma_rn(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// Multipole terms of one accepted cell at offset (x, y, z) = r_p - r_cell:
+// the monopole, quadrupole and octupole terms of BHgrav_accel.F90:159-222,
+// evaluated in factored form.  With Q the traceless quadrupole,
+// S(r) = sum_k s_k m_k(r) the octupole cubic and G = grad S,
+//   a   = -m r / r^3 + Q r / r^5 - 5/2 (r.Q.r) r / r^7
+//         + G / (2 r^7) - 7/2 S r / r^9,
+//   phi = m / r + (r.Q.r) / (2 r^5) + S / (2 r^7),
+// term for term the reference's expressions (its octupole force terms are
+// the components of G / 2); S is formed as (r.G) / 3 (Euler's identity for a
+// homogeneous cubic).  Coefficient arrays (t = scalar type):
+//   q = [xx xy yy zx zy], zz = -(xx + yy)
+//   s = [x^3 x^2y y^2x y^3 z^2x z^2y z^3 x^2z y^2z xyz]
+template <int OCT, class T>
+__device__ __forceinline__ void cell_terms(T x, T y, T z, T ir, T m, const T *q, const T *s,
+                                           T &ax, T &ay, T &az, T &pot) {
+#pragma clang fp contract(fast)
+    const T ir2 = ir * ir;
+    const T ir3 = ir2 * ir;
+    const T ir5 = ir3 * ir2;
+    const T ir7 = ir5 * ir2;
+    const T qzz = -(q[0] + q[2]);
+    const T Qx = q[0] * x + q[1] * y + q[3] * z;
+    const T Qy = q[1] * x + q[2] * y + q[4] * z;
+    const T Qz = q[3] * x + q[4] * y + qzz * z;
+    const T qs = x * Qx + y * Qy + z * Qz;
+    T crad = -(m * ir3) - T(2.5) * qs * ir7;
+    T phi = m * ir + T(0.5) * qs * ir5;
+    T fx = Qx * ir5, fy = Qy * ir5, fz = Qz * ir5;
+    if (OCT) {
+        const T xx = x * x, yy = y * y, zz = z * z;
+        const T xy = x * y, xz = x * z, yz = y * z;
+        const T xx3 = T(3) * xx, yy3 = T(3) * yy, zz3 = T(3) * zz;
+        const T xy2 = xy + xy, xz2 = xz + xz, yz2 = yz + yz;
+        const T Gx = s[0] * xx3 + s[1] * xy2 + s[7] * xz2 + s[2] * yy + s[4] * zz + s[9] * yz;
+        const T Gy = s[1] * xx + s[2] * xy2 + s[3] * yy3 + s[5] * zz + s[9] * xz + s[8] * yz2;
+        const T Gz = s[7] * xx + s[8] * yy + s[6] * zz3 + s[4] * xz2 + s[5] * yz2 + s[9] * xy;
+        const T rg = x * Gx + y * Gy + z * Gz;   // 3 S
+        const T h7 = T(0.5) * ir7;
+        crad = crad - T(3.5 / 3.0) * rg * ir7 * ir2;
+        phi = phi + T(0.5 / 3.0) * rg * ir7;
+        fx = fx + h7 * Gx;
+        fy = fy + h7 * Gy;
+        fz = fz + h7 * Gz;
+    }
+    ax = ax + (fx + crad * x);
+    ay = ay + (fy + crad * y);
+    az = az + (fz + crad * z);
+    pot = pot + phi;
+}
+
 template <int OCT>
 __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], double drsqd,
                                           double &ax, double &ay, double &az, double &pot) {
-#pragma clang fp contract(fast)
-    const double *q = g.q;
-    const double invdrsqd = 1.0 / (drsqd + SMALL_NUMBER);
-    const double invdrmag = sqrt(invdrsqd);
-    const double mf = g.m * invdrsqd * invdrmag;
-    double at0 = -mf * dr[0], at1 = -mf * dr[1], at2 = -mf * dr[2];
-    double dpot = g.m * invdrmag;
-    const double invdr5 = invdrsqd * invdrsqd * invdrmag;
-    const double qscalar = q[0] * dr[0] * dr[0] + q[2] * dr[1] * dr[1] -
-                           (q[0] + q[2]) * dr[2] * dr[2] +
-                           2.0 * (q[1] * dr[0] * dr[1] + q[3] * dr[0] * dr[2] +
-                                  q[4] * dr[1] * dr[2]);
-    at0 = at0 + (q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2]) * invdr5 -
-          2.5 * qscalar * dr[0] * invdr5 * invdrsqd;
-    at1 = at1 + (q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2]) * invdr5 -
-          2.5 * qscalar * dr[1] * invdr5 * invdrsqd;
-    at2 = at2 + (q[3] * dr[0] + q[4] * dr[1] - (q[0] + q[2]) * dr[2]) * invdr5 -
-          2.5 * qscalar * dr[2] * invdr5 * invdrsqd;
-    dpot = dpot + 0.5 * qscalar * invdr5;
-    if (OCT) {
-        const double *s = g.s;
-        const double sscalar =
-            s[0] * dr[0] * dr[0] * dr[0] + s[1] * dr[0] * dr[0] * dr[1] +
-            s[2] * dr[1] * dr[1] * dr[0] + s[3] * dr[1] * dr[1] * dr[1] +
-            s[4] * dr[2] * dr[2] * dr[0] + s[5] * dr[2] * dr[2] * dr[1] +
-            s[6] * dr[2] * dr[2] * dr[2] + s[7] * dr[0] * dr[0] * dr[2] +
-            s[8] * dr[1] * dr[1] * dr[2] + s[9] * dr[0] * dr[1] * dr[2];
-        at0 = at0 +
-              0.5 * (3.0 * s[0] * dr[0] * dr[0] + 2.0 * s[1] * dr[0] * dr[1] +
-                     2.0 * s[7] * dr[0] * dr[2] + s[2] * dr[1] * dr[1] +
-                     s[4] * dr[2] * dr[2] + s[9] * dr[1] * dr[2]) *
-                  invdr5 * invdrsqd -
-              3.5 * sscalar * dr[0] * invdr5 * invdrsqd * invdrsqd;
-        at1 = at1 +
-              0.5 * (3.0 * s[3] * dr[1] * dr[1] + 2.0 * s[2] * dr[0] * dr[1] +
-                     2.0 * s[8] * dr[1] * dr[2] + s[1] * dr[0] * dr[0] +
-                     s[5] * dr[2] * dr[2] + s[9] * dr[0] * dr[2]) *
-                  invdr5 * invdrsqd -
-              3.5 * sscalar * dr[1] * invdr5 * invdrsqd * invdrsqd;
-        at2 = at2 +
-              0.5 * (3.0 * s[6] * dr[2] * dr[2] + 2.0 * s[4] * dr[0] * dr[2] +
-                     2.0 * s[5] * dr[1] * dr[2] + s[7] * dr[0] * dr[0] +
-                     s[8] * dr[1] * dr[1] + s[9] * dr[0] * dr[1]) *
-                  invdr5 * invdrsqd -
-              3.5 * sscalar * dr[2] * invdr5 * invdrsqd * invdrsqd;
-        dpot = dpot + 0.5 * sscalar * invdr5 * invdrsqd;
-    }
-    ax = ax + at0;
-    ay = ay + at1;
-    az = az + at2;
-    pot = pot + dpot;
+    const double ir = rsqrt_fp64(drsqd + SMALL_NUMBER);
+    cell_terms<OCT, double>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
 }
 
 template <int OCT, int MACT>
@@ -145,7 +158,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const double *w4 = ktab + T_W4 * KT;
     const double *w5 = ktab + T_W5 * KT;
@@ -166,8 +179,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
     const unsigned vis = group_walk(
         gn, nc, valid, lvis,
-        [&](const GNode &g) -> int {
-            if (g.leaf == 1) return WALK_LEAF;                // :128-136, no MAC test
+        [&](const GNode &g, bool here) -> int {
             const double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
             const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
             bool accept;
@@ -175,11 +187,12 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
                 accept = drsqd > g.dminsqd;
             else
                 accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-            if (accept) {
+            accept = accept && g.leaf != 1;                    // :128-136, no MAC test
+            if (here && accept) {
                 cell_fp64<OCT>(g, dr, drsqd, acx, acy, acz, pc);
                 ncell++;
-                return WALK_SKIP;
             }
+            if (accept) return WALK_SKIP;
             return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
         },
         [&](const GNode &g, bool mine) {
@@ -260,51 +273,19 @@ __device__ __forceinline__ int pair_fp32(const float *__restrict__ w4, const flo
     return inside;
 }
 
+// FP32 multipole terms on the FP32 cell offset, accumulated in FP64
 template <int OCT>
-__device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], CSum &ax, CSum &ay,
-                                          CSum &az, CSum &pot) {
+__device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], double &ax,
+                                          double &ay, double &az, double &pot) {
 #pragma clang fp contract(fast)
-    const float *q = g.q;
     const float drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-    const float invdrmag = rsqrtf(drsqd + 1.0e-30f);
-    const float invdrsqd = invdrmag * invdrmag;
-    const float mf = g.m * invdrsqd * invdrmag;
-    float at0 = -mf * dr[0], at1 = -mf * dr[1], at2 = -mf * dr[2];
-    float dpot = g.m * invdrmag;
-    const float invdr5 = invdrsqd * invdrsqd * invdrmag;
-    const float qscalar = q[0] * dr[0] * dr[0] + q[2] * dr[1] * dr[1] -
-                          (q[0] + q[2]) * dr[2] * dr[2] +
-                          2.0f * (q[1] * dr[0] * dr[1] + q[3] * dr[0] * dr[2] +
-                                  q[4] * dr[1] * dr[2]);
-    const float q7 = 2.5f * qscalar * invdr5 * invdrsqd;
-    at0 += (q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2]) * invdr5 - q7 * dr[0];
-    at1 += (q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2]) * invdr5 - q7 * dr[1];
-    at2 += (q[3] * dr[0] + q[4] * dr[1] - (q[0] + q[2]) * dr[2]) * invdr5 - q7 * dr[2];
-    dpot += 0.5f * qscalar * invdr5;
-    if (OCT) {
-        const float *s = g.s;
-        const float xx = dr[0] * dr[0], yy = dr[1] * dr[1], zz = dr[2] * dr[2];
-        const float sscalar = s[0] * xx * dr[0] + s[1] * xx * dr[1] + s[2] * yy * dr[0] +
-                              s[3] * yy * dr[1] + s[4] * zz * dr[0] + s[5] * zz * dr[1] +
-                              s[6] * zz * dr[2] + s[7] * xx * dr[2] + s[8] * yy * dr[2] +
-                              s[9] * dr[0] * dr[1] * dr[2];
-        const float i7 = invdr5 * invdrsqd;
-        const float s9 = 3.5f * sscalar * i7 * invdrsqd;
-        at0 += 0.5f * (3.0f * s[0] * xx + 2.0f * s[1] * dr[0] * dr[1] +
-                       2.0f * s[7] * dr[0] * dr[2] + s[2] * yy + s[4] * zz +
-                       s[9] * dr[1] * dr[2]) * i7 - s9 * dr[0];
-        at1 += 0.5f * (3.0f * s[3] * yy + 2.0f * s[2] * dr[0] * dr[1] +
-                       2.0f * s[8] * dr[1] * dr[2] + s[1] * xx + s[5] * zz +
-                       s[9] * dr[0] * dr[2]) * i7 - s9 * dr[1];
-        at2 += 0.5f * (3.0f * s[6] * zz + 2.0f * s[4] * dr[0] * dr[2] +
-                       2.0f * s[5] * dr[1] * dr[2] + s[7] * xx + s[8] * yy +
-                       s[9] * dr[0] * dr[1]) * i7 - s9 * dr[2];
-        dpot += 0.5f * sscalar * i7;
-    }
-    ax.add(at0);
-    ay.add(at1);
-    az.add(at2);
-    pot.add(dpot);
+    const float ir = __builtin_amdgcn_rsqf(drsqd + 1.0e-30f);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f, fp = 0.0f;
+    cell_terms<OCT, float>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, fx, fy, fz, fp);
+    ax += (double)fx;
+    ay += (double)fy;
+    az += (double)fz;
+    pot += (double)fp;
 }
 
 template <int OCT, int MACT>
@@ -316,7 +297,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const float *w4 = ktabf + T_W4 * KT;
     const float *w5 = ktabf + T_W5 * KT;
@@ -333,12 +314,12 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 afactor = (MACT == SERENX_MAC_GADGET) ? pow(a, -ONETHIRD) : 1.0 / sqrt(a);
         }
     }
-    CSum cx, cy, cz, cp;
+    double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
+    CSum px, py, pz, pq;                              // pair terms
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
     const unsigned vis = group_walk(
         gn, nc, valid, lvis,
-        [&](const GNodeF &g) -> int {
-            if (g.leaf == 1) return WALK_LEAF;
+        [&](const GNodeF &g, bool here) -> int {
             const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
             const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
             bool accept;
@@ -346,12 +327,13 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 accept = drsqd > g.dminsqd;
             else
                 accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-            if (accept) {
+            accept = accept && g.leaf != 1;
+            if (here && accept) {
                 const float dr[3] = {(float)d0, (float)d1, (float)d2};
                 cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
                 ncell++;
-                return WALK_SKIP;
             }
+            if (accept) return WALK_SKIP;
             return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
         },
         [&](const GNodeF &g, bool mine) {
@@ -363,18 +345,18 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 if (!mine || j == p) continue;
                 const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(xj - rpx),
                                           (float)(yj - rpy), (float)(zj - rpz), mj, hj, zetaj,
-                                          cx, cy, cz, cp);
+                                          px, py, pz, pq);
                 npp++;
                 nin_p += in_ & 1;
                 nin_j += in_ >> 1;
             }
         });
     if (valid) {
-        const double gx = cx.value(), gy = cy.value(), gz = cz.value();
+        const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
         ax[p] = ax[p] + gx;
         ay[p] = ay[p] + gy;
         az[p] = az[p] + gz;
-        gpot[p] = cp.value();
+        gpot[p] = cp + pq.value();
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
     wave_count4(icount, ncell, npp, nin_p, nin_j);

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -37,20 +38,22 @@ enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
 constexpr int WAVE = 64;
 
 // Gravity node, DFS pre-order.  Read with wave-uniform indices in the walk, so
-// a node is a run of scalar loads.  next = DFS index of the cell that follows
-// this subtree (nc for the last); leaf = particle count of a leaf, 0 for a
-// branch; first = first particle (tree order) of the cell.
+// a node is a run of scalar loads; everything a visit needs (MAC data and
+// links) sits in the first 64 bytes, one s_load_dwordx16, and the moments
+// follow.  next = DFS index of the cell that follows this subtree (nc for the
+// last); leaf = particle count of a leaf, 0 for a branch; first = first
+// particle (tree order) of the cell.
 struct __attribute__((aligned(64))) GNode {
     double r[3];      // centre of mass
-    double m;
     double dminsqd;   // opening distance^2 (BHgrav_stock.F90:317-348)
     double mac;       // multipole MAC term (GADGET/GADGET2)
-    double q[5];      // quadrupole [xx xy yy zx zy]
-    double s[10];     // octupole
+    double m;
     int next;
     int leaf;
     int first;
     int count;        // particles in the subtree
+    double q[5];      // quadrupole [xx xy yy zx zy]
+    double s[10];     // octupole
 };
 
 // Hydro node (BHhydro_node, modules.F90:929-937), DFS order.
@@ -65,19 +68,23 @@ struct __attribute__((aligned(64))) HNode {
 };
 
 // Mixed-precision gravity node: FP64 COM (the cell offset is subtracted in
-// FP64), FP32 moments.
+// FP64) and MAC data, FP32 moments; walk data in the first 64 bytes.
 struct __attribute__((aligned(64))) GNodeF {
     double r[3];
     double dminsqd;
     double mac;
     float m;
-    float q[5];
-    float s[10];
     int next;
     int leaf;
     int first;
     int pad;
+    float q[5];
+    float s[10];
 };
+
+static_assert(offsetof(GNode, q) == 64 && sizeof(GNode) == 192, "GNode layout");
+static_assert(offsetof(GNodeF, q) <= 64 && sizeof(GNodeF) == 128, "GNodeF layout");
+static_assert(sizeof(HNode) == 64, "HNode layout");
 
 struct DevBuf {
     void *p = nullptr;
@@ -104,6 +111,7 @@ struct Ctx {
            *az = nullptr, *gpot = nullptr, *agm = nullptr;
     int *porig = nullptr;
     uint8_t *active = nullptr;   // accdo
+    double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
     double *tmpd = nullptr;      // permutation scratch (cap doubles)
     int *tmpi = nullptr;
 

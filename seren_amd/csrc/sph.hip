@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_hsolve(
     double rextent, int pp_gather, double *__restrict__ h, double *__restrict__ zeta,
     double *__restrict__ rho, double *__restrict__ omega, double *__restrict__ divv,
     double *__restrict__ drhodt, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     const double *w0 = ktab + T_W0 * KT;
     const double *w4 = ktab + T_W4 * KT;
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_hsolve(
         const int lmode = mode;
         vis += group_walk(
             hn, nc, mode != M_DONE, lvis,
-            [&](const HNode &nd) -> int {
+            [&](const HNode &nd, bool) -> int {
                 const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
                 const double drsqd = dx * dx + dy * dy + dz * dz;
                 const double lim = (R + nd.rmax) * (R + nd.rmax);
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
                                               int *__restrict__ ovf_list,
                                               int *__restrict__ ovf_count,
                                               unsigned long long *__restrict__ wstat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     bool valid = p < n && active[p];
     if (OVERFLOW_PASS && valid) valid = pptot[p] > pp_limit;
     double rpx = 0, rpy = 0, rpz = 0, hp = 0;
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
     unsigned lvis = 0;
     const unsigned vis = group_walk(
         hn, nc, valid, lvis,
-        [&](const HNode &nd) -> int {
+        [&](const HNode &nd, bool) -> int {
             const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
             const double drsqd = dx * dx + dy * dy + dz * dz;
             const double m1 = (R + nd.rmax) * (R + nd.rmax);
@@ -250,44 +250,65 @@ __global__ void k_thermal_iso(int n, double cs, const uint8_t *__restrict__ acti
     sound[i] = cs;
 }
 
+// Per-particle record of everything a hydro pair reads from the neighbour,
+// 128 bytes (one cache line) per particle, so that each neighbour costs one
+// line instead of one line per field.  The per-particle factors are formed
+// with the reference's operations (hydro_gradh.F90:90-96, :178, :195-196),
+// so the pair terms use the same values bit for bit:
+//   [0] x y z h   [1] vx vy vz m   [2] rho pfactor sound invh   [3] hfactor
+__global__ void k_pack_hydro(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, const double *__restrict__ h,
+                             const double *__restrict__ vx, const double *__restrict__ vy,
+                             const double *__restrict__ vz, const double *__restrict__ m,
+                             const double *__restrict__ rho, const double *__restrict__ omega,
+                             const double *__restrict__ press, const double *__restrict__ sound,
+                             double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hi = h[i], rh = rho[i];
+    const double invh = 1.0 / hi;
+    const double pf = press[i] / rh / rh / omega[i];
+    double4 *r = rec + 4 * (size_t)i;
+    r[0] = make_double4(x[i], y[i], z[i], hi);
+    r[1] = make_double4(vx[i], vy[i], vz[i], m[i]);
+    r[2] = make_double4(rh, pf, sound[i], invh);
+    r[3] = make_double4(cr_pow4(invh), 0.0, 0.0, 0.0);
+}
+
 // hydro_gradh.F90:87-297 (AB viscosity, beta := 2 alpha, isothermal)
 __global__ void __launch_bounds__(256) k_hydro(
-    int n, const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ h,
-    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
-    const double *__restrict__ rho, const double *__restrict__ omega,
-    const double *__restrict__ press, const double *__restrict__ sound,
-    const uint8_t *__restrict__ active, const int *__restrict__ pptot,
-    const int *__restrict__ pplist, const int *__restrict__ ovf_off,
-    const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab,
-    double alpha, double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
+    const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
+    const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab, double alpha,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n || !active[p]) return;
     const double *w1 = ktab + T_W1 * KT;
-    const double rpx = x[p], rpy = y[p], rpz = z[p];
-    const double hp = h[p], invhp = 1.0 / hp, hfactor_p = cr_pow4(invhp);
-    const double vpx = vx[p], vpy = vy[p], vpz = vz[p];
-    const double sound_p = sound[p], rho_p = rho[p];
-    const double pfactor_p = press[p] / rho_p / rho_p / omega[p];
+    const double4 *rp = rec + 4 * (size_t)p;
+    const double4 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3];
+    const double rpx = r0.x, rpy = r0.y, rpz = r0.z, hp = r0.w;
+    const double vpx = r1.x, vpy = r1.y, vpz = r1.z;
+    const double rho_p = r2.x, pfactor_p = r2.y, sound_p = r2.z, invhp = r2.w;
+    const double hfactor_p = r3.x;
     const double beta = 2.0 * alpha;
     double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0;
     const int nn = pptot[p];
     const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
     for (int i = 0; i < nn; i++) {
-        const int pp = lst[i];
+        const double4 *rq = rec + 4 * (size_t)lst[i];
+        const double4 q0 = rq[0], q1 = rq[1], q2 = rq[2];
         double wmean = 0.0;
-        const double rho_pp = rho[pp], mpp = m[pp], hpp = h[pp];
-        double ux = x[pp] - rpx, uy = y[pp] - rpy, uz = z[pp] - rpz;
+        const double hpp = q0.w, mpp = q1.w, rho_pp = q2.x;
+        double ux = q0.x - rpx, uy = q0.y - rpy, uz = q0.z - rpz;
         double drmag = ux * ux + uy * uy + uz * uz;
         if (drmag >= KERNRANGESQD * hp * hp && drmag >= KERNRANGESQD * hpp * hpp) continue;
         drmag = sqrt(drmag) + SMALL_NUMBER;
-        const double dvx = vx[pp] - vpx, dvy = vy[pp] - vpy, dvz = vz[pp] - vpz;
+        const double dvx = q1.x - vpx, dvy = q1.y - vpy, dvz = q1.z - vpz;
         const double dvdr = dvx * ux + dvy * uy + dvz * uz;
         const double invdrmag = 1.0 / drmag;
         ux = ux * invdrmag;
         uy = uy * invdrmag;
         uz = uz * invdrmag;
-        const double pfactor_pp = press[pp] / rho_pp / rho_pp / omega[pp];
         if (drmag < KERNRANGE * hp) {
             int kern = (int)(HALFKERNTOT * drmag * invhp);
             kern = min(kern, KERNTOT);
@@ -299,7 +320,7 @@ __global__ void __launch_bounds__(256) k_hydro(
             ahz = ahz + f * uz;
         }
         if (drmag < KERNRANGE * hpp) {
-            const double invhpp = 1.0 / hpp, hfactor_pp = cr_pow4(invhpp);
+            const double invhpp = q2.w, hfactor_pp = rq[3].x, pfactor_pp = q2.y;
             int kern = (int)(HALFKERNTOT * drmag * invhpp);
             kern = min(kern, KERNTOT);
             const double wk = w1[kern];
@@ -311,7 +332,7 @@ __global__ void __launch_bounds__(256) k_hydro(
         }
         const double rhomean = 0.5 * (rho_p + rho_pp);
         if (dvdr < 0.0) {
-            const double vsignal = 0.5 * (sound_p + sound[pp]);
+            const double vsignal = 0.5 * (sound_p + q2.z);
             const double mu = 0.5 * (hp + hpp) * dvdr /
                               (drmag * drmag + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
             const double visc = (-alpha * vsignal * mu + beta * mu * mu) / rhomean;
@@ -387,10 +408,13 @@ int sph_thermal_iso(Ctx &c, double cs) {
 }
 
 int hydro_forces(Ctx &c) {
-    hipLaunchKernelGGL(k_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z, c.m,
-                       c.h, c.vx, c.vy, c.vz, c.rho, c.omega, c.press, c.sound, c.active, c.pptot,
-                       c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab, c.opt.alpha, c.ax,
-                       c.ay, c.az);
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.omega, c.press, c.sound, rec);
+    SX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec, c.active,
+                       c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab, c.opt.alpha,
+                       c.ax, c.ay, c.az);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }

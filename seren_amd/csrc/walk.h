@@ -13,6 +13,19 @@ __device__ __forceinline__ int wave_min_int(int v) { return __ockl_wfred_min_i32
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The dispatcher deals the workgroups of a launch to the 8 XCDs round-robin
+// (workgroup i runs on XCD i % 8), and each XCD has its own L2.  This returns
+// the logical block index under which XCD x receives the x-th contiguous
+// eighth of the blocks, so that spatially adjacent leaf groups -- which walk
+// the same cells and read the same particles -- share one L2.  A bijection of
+// [0, gridDim.x) for any grid size.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, i = blockIdx.x;
+    const int q = nb / NXCD, r = nb % NXCD, xcd = i % NXCD;
+    return xcd * q + min(xcd, r) + i / NXCD;
+}
+
 // x^3 and x^4 correctly rounded (double-double products with exact FMA
 // residuals, one final rounding).  The reference evaluates invh**3.0 and
 // invh**4.0 with libm pow, which is correctly rounded for these arguments
@@ -68,6 +81,11 @@ __device__ __forceinline__ double ref_pow_third(double x) {
 // moving past the subtree (resume = next).  Each lane therefore sees exactly
 // its own interaction set in its own order, as a per-particle walk would.
 //
+// visit(node, here) runs on every lane in wave-uniform control flow, so the
+// node's fields come in one batch of scalar loads at the top of the visit
+// rather than behind a divergent branch; it must act (accumulate) only where
+// here is set, and its return value is ignored on the other lanes.
+//
 // Opened leaves are handled in wave-uniform control flow: leaf(node, mine) is
 // called once per leaf any lane opened, with mine set on the lanes that did,
 // so the leaf's particles (contiguous in tree order) can be read once through
@@ -88,12 +106,14 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
         // read the cell in wave-uniform control flow from a uniform address so
         // that it comes through the scalar cache into SGPRs
         const Node nd = nodes[c];
-        int act = WALK_SKIP;
         visits++;
-        if (resume == c) {
+        const bool here = resume == c;
+        int act = visit(nd, here);
+        if (here) {
             lane_visits++;
-            act = visit(nd);
             resume = (act == WALK_DESCEND) ? c + 1 : nd.next;
+        } else {
+            act = WALK_SKIP;
         }
         if (__any(act == WALK_LEAF)) leaf(nd, act == WALK_LEAF);
         if (__any(act == WALK_DESCEND))
