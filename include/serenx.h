@@ -213,7 +213,10 @@ int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4);
  * [2] h-solve cells visited, [3] h-solve lane visits, [4] h-solve passes
  * summed over waves, [5] most passes of one wave, [6] h iterations summed over
  * particles, [7] h-solve waves, [8] neighbour-walk cells visited,
- * [9] neighbour-walk lane visits; [10..15] zero. */
+ * [9] neighbour-walk lane visits, [10] h-solve list-recording walks summed over
+ * waves, [11] h-solve passes summed over waves whose lists overflowed (summed
+ * over the tree walk instead), [12] longest potential-neighbour list in
+ * leaves; [13..15] zero. */
 int serenx_walk_stats(serenx_ctx *ctx, int64_t *stats16);
 
 #ifdef __cplusplus

@@ -124,6 +124,9 @@ int dalloc(T **p, size_t count) {
 }
 
 int ensure_capacity(Ctx &c, int n) {
+    // leaf descriptors of the h solve pack a particle index in 27 bits
+    if (n >= (1 << 27))
+        return set_error(SERENX_EINVAL, "more than 2^27 - 1 particles on one device");
     if (n <= c.cap && c.x) return SERENX_OK;
     c.cap = n;
     const size_t nn = (size_t)n + 1;
@@ -138,6 +141,7 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.pplist, nn * (size_t)c.pp_limit));
     SX_TRY(dalloc(&c.ovf_off, nn));
     SX_TRY(dalloc(&c.hrec, nn * 16));
+    SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     SX_TRY(ensure_tree_buffers(c));
     return SERENX_OK;
 }
@@ -145,7 +149,7 @@ int ensure_capacity(Ctx &c, int n) {
 void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
-                    c.tmpd, c.hrec, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
+                    c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
                     c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
                     c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
                     c.root, c.bbpart, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,

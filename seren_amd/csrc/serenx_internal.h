@@ -36,6 +36,8 @@ constexpr int KT = KERNTOT + 1;    // entries per table
 enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
 
 constexpr int WAVE = 64;
+// potential-neighbour list slots (opened leaves) per particle in the h solve
+constexpr int HL_SLOTS = 96;
 
 // Gravity node, DFS pre-order.  Read with wave-uniform indices in the walk, so
 // a node is a run of scalar loads; everything a visit needs (MAC data and
@@ -112,6 +114,7 @@ struct Ctx {
     int *porig = nullptr;
     uint8_t *active = nullptr;   // accdo
     double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
+    int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     double *tmpd = nullptr;      // permutation scratch (cap doubles)
     int *tmpi = nullptr;
 

@@ -23,101 +23,166 @@ inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
 // lane modes of the h solve
 enum { M_DONE = 0, M_ITER = 1, M_FINAL = 2 };
 
-// all_sph_gradh.F90:103-489 for every active particle
-__global__ void __launch_bounds__(256) k_hsolve(
-    int n, int nc, const HNode *__restrict__ hn, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const double *__restrict__ vx, const double *__restrict__ vy, const double *__restrict__ vz,
-    const uint8_t *__restrict__ active, const double *__restrict__ ktab, double h_fac,
-    double rextent, int pp_gather, double *__restrict__ h, double *__restrict__ zeta,
-    double *__restrict__ rho, double *__restrict__ omega, double *__restrict__ divv,
-    double *__restrict__ drhodt, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
+// Positions, velocities and masses as 64-byte records for the per-lane
+// candidate loops of the h solve: [0] x y z h  [1] vx vy vz m (the first half
+// of the hydro pair record, which the hydro pass rewrites in full).
+__global__ void k_pack_hsolve(int n, const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, const double *__restrict__ h,
+                              const double *__restrict__ vx, const double *__restrict__ vy,
+                              const double *__restrict__ vz, const double *__restrict__ m,
+                              double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 *r = rec + 4 * (size_t)i;
+    r[0] = make_double4(x[i], y[i], z[i], h[i]);
+    r[1] = make_double4(vx[i], vy[i], vz[i], m[i]);
+}
+
+// HNode visit of BHhydrowalk_hgather.F90:40 for a search radius R
+__device__ __forceinline__ int hgather_visit(const HNode &nd, double rpx, double rpy, double rpz,
+                                             double R) {
+    const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+    const double drsqd = dx * dx + dy * dy + dz * dz;
+    const double lim = (R + nd.rmax) * (R + nd.rmax);
+    if (!(drsqd <= lim)) return WALK_SKIP;
+    return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
+}
+
+// Loops every summing lane over its own potential-neighbour list, leaf by
+// leaf in DFS order, calling pair(j) for each listed particle j != p.
+template <class Pair>
+__device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot, bool sum, int p,
+                                          Pair &&pair) {
+    const int smax = uniform(wave_max_int(sum ? nslot : 0));
+    for (int sl = 0; sl < smax; sl++) {
+        if (sum && sl < nslot) {
+            const int d = wl[sl * WAVE];
+            const int f0 = d >> 4, k = d & 15;
+            for (int i = 0; i < k; i++) {
+                const int j = f0 + i;
+                if (j != p) pair(j);
+            }
+        }
+    }
+}
+
+// all_sph_gradh.F90:103-309, the h iteration, for every active particle.
+//
+// Potential-neighbour lists (BHhydrowalk_hgather) are recorded once per
+// change of search radius: a group walk stores, for every lane, the leaves
+// its own walk opens, in DFS order, as (first << 4) | count in HL_SLOTS
+// slots per lane (slot-major, so the lanes of a wave read one contiguous row
+// per step).  The iterations loop over each lane's own list, which is the
+// reference's pp_potlist in the reference's order, so every sum is formed
+// exactly as the reference forms it.  A wave with a lane whose list does not
+// fit sums over the tree walk itself for that radius (same leaves, same
+// order).  On exit every lane leaves rhotemp in rho, h_high in hhigh and its
+// list length in nlist (-1: list incomplete) for k_hsolve_final; its list
+// stays in hlist (a lane joins no walk after it has converged).
+__global__ void __launch_bounds__(256) k_hsolve_iter(
+    int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    int *__restrict__ hlist, const uint8_t *__restrict__ active, const double *__restrict__ ktab,
+    double h_fac, double rextent, int pp_gather, double *__restrict__ h,
+    double *__restrict__ rho, double *__restrict__ hhigh, int *__restrict__ nlist,
+    unsigned long long *__restrict__ wstat) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
+    int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + (threadIdx.x & (WAVE - 1));
     const double *w0 = ktab + T_W0 * KT;
-    const double *w4 = ktab + T_W4 * KT;
     const double *wh = ktab + T_WH * KT;
-    const double *wg = ktab + T_WG * KT;
-    double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1, vpx = 0, vpy = 0, vpz = 0;
+    double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1;
     if (valid) {
-        rpx = x[p]; rpy = y[p]; rpz = z[p];
-        mp = m[p]; hp = h[p];
-        vpx = vx[p]; vpy = vy[p]; vpz = vz[p];
+        const double4 a = rec[4 * (size_t)p], b = rec[4 * (size_t)p + 1];
+        rpx = a.x; rpy = a.y; rpz = a.z; hp = a.w;
+        mp = b.w;
     }
     double h_high = 0.0, h_old = hp, h_lower = 0.0, h_upper = rextent;
     double rhotemp = 0.0;
     int iteration = 0;
-    int mode = valid ? M_ITER : M_DONE;
-    unsigned passes = 0, vis = 0, lvis = 0;
+    bool iterating = valid;
+    unsigned passes = 0, vis = 0, lvis = 0, walks = 0, fallback = 0;
+    int nslot = 0, cnt = 0, slot_max = 0;
+    bool have_list = false, wave_ovf = false, my_ovf = false;
 
-    while (__any(mode != M_DONE)) {
+    while (__any(iterating)) {
         passes++;
         bool newlist = false;
-        if (mode == M_ITER && hp > h_high) {
+        if (iterating && hp > h_high) {
             h_high = hp * HMULT;
             newlist = true;
         }
-        if (mode == M_FINAL) hp = h_fac * ref_pow_third(mp / rhotemp);  // :317
         const double R = KERNRANGE * h_high;
         const double invhp = 1.0 / hp;
         const double hfactor_p = cr_pow3(invhp);
         const double hrangesqd = KERNRANGESQD * hp * hp;
-        // self contributions (:209-210, :323-346)
-        double srho = mp * w0[0] * hfactor_p;
+        double srho = mp * w0[0] * hfactor_p;                  // :209-210
         double som = mp * hfactor_p * invhp * wh[0];
-        double sdiv = 0.0;
-        double szeta = mp * invhp * invhp * wg[0];
-        int cnt = 0;
-        const int lmode = mode;
-        vis += group_walk(
-            hn, nc, mode != M_DONE, lvis,
-            [&](const HNode &nd, bool) -> int {
-                const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
-                const double drsqd = dx * dx + dy * dy + dz * dz;
-                const double lim = (R + nd.rmax) * (R + nd.rmax);
-                if (!(drsqd <= lim)) return WALK_SKIP;       // BHhydrowalk_hgather.F90:40
-                return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
-            },
-            [&](const HNode &nd, bool mine) {
-                const int k = nd.leaf, f0 = nd.first;
-                if (mine) cnt += k;
-                for (int i = 0; i < k; i++) {
-                    const int j = f0 + i;
-                    const double xj = x[j], yj = y[j], zj = z[j], mj = m[j];
-                    const double vxj = vx[j], vyj = vy[j], vzj = vz[j];
-                    if (!mine || j == p) continue;
-                    const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
-                    const double dsq = ex * ex + ey * ey + ez * ez;
-                    if (lmode == M_ITER) {
-                        if (dsq >= hrangesqd) continue;                 // :223
-                        const double drmag = sqrt(dsq) + SMALL_NUMBER;
-                        const int kern = (int)(HALFKERNTOT * drmag * invhp);
-                        if (kern >= KERNTOT) continue;                  // :227
-                        srho = srho + mj * hfactor_p * w0[kern];
-                        som = som + mj * hfactor_p * invhp * wh[kern];
-                    } else {
-                        if (dsq > hrangesqd) continue;                  // :364
-                        const double drmag = sqrt(dsq) + SMALL_NUMBER;
-                        const int kern = (int)(HALFKERNTOT * drmag * invhp);
-                        if (kern > KERNTOT) continue;                   // :368
-                        const double dvx = vxj - vpx, dvy = vyj - vpy, dvz = vzj - vpz;
-                        const double dvdr = dvx * ex + dvy * ey + dvz * ez;
-                        srho = srho + mj * hfactor_p * w0[kern];
-                        som = som + mj * hfactor_p * invhp * wh[kern];
-                        sdiv = sdiv - mj * dvdr * w4[kern] * hfactor_p * invhp / drmag;
-                        szeta = szeta + mj * invhp * invhp * wg[kern];
-                    }
+        auto visit = [&](const HNode &nd, bool) { return hgather_visit(nd, rpx, rpy, rpz, R); };
+        // :214-231 for one potential neighbour
+        auto pair = [&](double xj, double yj, double zj, double mj) {
+            const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
+            const double dsq = ex * ex + ey * ey + ez * ez;
+            if (dsq >= hrangesqd) return;                       // :223
+            const double drmag = sqrt(dsq) + SMALL_NUMBER;
+            const int kern = (int)(HALFKERNTOT * drmag * invhp);
+            if (kern >= KERNTOT) return;                        // :227
+            srho = srho + mj * hfactor_p * w0[kern];
+            som = som + mj * hfactor_p * invhp * wh[kern];
+        };
+
+        if (!have_list || __any(newlist)) {
+            // record every iterating lane's list (unchanged radii give the same
+            // lists); converged lanes keep theirs
+            if (iterating) {
+                nslot = 0;
+                cnt = 0;
+            }
+            bool lovf = false;
+            vis += group_walk(hn, nc, iterating, lvis, visit, [&](const HNode &nd, bool mine) {
+                if (mine) {
+                    cnt += nd.leaf;
+                    if (nslot < HL_SLOTS)
+                        wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
+                    else
+                        lovf = true;
+                    nslot++;
                 }
             });
+            if (iterating) {
+                slot_max = max(slot_max, nslot);
+                my_ovf = lovf;
+            }
+            have_list = true;
+            wave_ovf = __any(lovf);
+            walks++;
+        }
+        // too few potential neighbours: grow h and gather again (:185-189)
+        const bool grow = iterating && newlist && cnt < pp_gather;
+        const bool sum = iterating && !grow;
+        if (!wave_ovf) {
+            list_loop(wl, nslot, sum, p, [&](int j) {
+                const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
+                pair(a.x, a.y, a.z, b.w);
+            });
+        } else {
+            vis += group_walk(hn, nc, sum, lvis, visit, [&](const HNode &nd, bool mine) {
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = nd.first + i;
+                    const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
+                    if (mine && j != p) pair(a.x, a.y, a.z, b.w);
+                }
+            });
+            fallback++;
+        }
 
-        if (mode == M_ITER && newlist && cnt < pp_gather) {  // :185-189
+        if (grow) {
             h_high = 0.0;
             hp = hp * HMULT;
-        } else if (mode == M_ITER) {
+        } else if (iterating) {
             rhotemp = srho;
             const double hnew = h_fac * ref_pow_third(mp / rhotemp);
             if (fabs(hp - hnew) / hp < H_CON && iteration >= 1) {  // :246
-                mode = M_FINAL;
+                iterating = false;
             } else if (iteration < GRADH_ITERATION_MAX) {           // :252
                 hp = hnew;
                 iteration++;
@@ -139,32 +204,115 @@ __global__ void __launch_bounds__(256) k_hsolve(
                 hp = 0.5 * (h_lower + h_upper);
                 iteration++;
             } else {                                                // :276-279
-                hp = h_old;
-                mode = M_FINAL;
+                iterating = false;
             }
-        } else if (mode == M_FINAL) {
-            const double om = 1.0 + ((hp * som) / (3.0 * srho));
-            const double dv = sdiv / srho;
-            rho[p] = srho;
-            omega[p] = om;
-            divv[p] = dv;
-            drhodt[p] = -srho * dv / om;
-            h[p] = hp;
-            zeta[p] = -(hp * szeta) / (3.0 * srho * om);
-            if (hp < 0.0) atomicOr(err, 1);
-            mode = M_DONE;
         }
     }
-    // walk statistics: cells visited / lane visits, passes, lane iterations
+    (void)h_old;   // the final h is formed from rhotemp alone (:317)
+    if (valid) {
+        rho[p] = rhotemp;
+        hhigh[p] = h_high;
+        nlist[p] = my_ovf ? -1 : nslot;
+    }
+    // walk statistics: cells visited / lane visits, passes, lane iterations,
+    // list walks, fallback passes, longest list
     wave_stats(wstat, 2, vis, lvis);
     unsigned long long it = valid ? (unsigned long long)iteration : 0ull;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) it += __shfl_xor(it, o, 64);
+    const int smx = wave_max_int(slot_max);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(wstat + 4, (unsigned long long)passes);
         atomicMax(wstat + 5, (unsigned long long)passes);
         atomicAdd(wstat + 6, it);
         atomicAdd(wstat + 7, 1ull);
+        atomicAdd(wstat + 10, (unsigned long long)walks);
+        atomicAdd(wstat + 11, (unsigned long long)fallback);
+        atomicMax(wstat + 12, (unsigned long long)smx);
+    }
+}
+
+// all_sph_gradh.F90:310-489: with h from the converged density, the gather
+// sums rho, Omega, div v and zeta over the last potential-neighbour list.
+__global__ void __launch_bounds__(256) k_hsolve_final(
+    int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    const int *__restrict__ hlist, const uint8_t *__restrict__ active,
+    const double *__restrict__ ktab, double h_fac, const double *__restrict__ hhigh,
+    const int *__restrict__ nlist, double *__restrict__ h, double *__restrict__ zeta,
+    double *__restrict__ rho, double *__restrict__ omega, double *__restrict__ divv,
+    double *__restrict__ drhodt, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = p < n && active[p];
+    const int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + (threadIdx.x & (WAVE - 1));
+    const double *w0 = ktab + T_W0 * KT;
+    const double *w4 = ktab + T_W4 * KT;
+    const double *wh = ktab + T_WH * KT;
+    const double *wg = ktab + T_WG * KT;
+    double rpx = 0, rpy = 0, rpz = 0, mp = 0, vpx = 0, vpy = 0, vpz = 0, R = 0, hp = 1;
+    int nslot = 0;
+    if (valid) {
+        const double4 a = rec[4 * (size_t)p], b = rec[4 * (size_t)p + 1];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        vpx = b.x; vpy = b.y; vpz = b.z; mp = b.w;
+        hp = h_fac * ref_pow_third(mp / rho[p]);                 // :317
+        R = KERNRANGE * hhigh[p];
+        nslot = nlist[p];
+    }
+    const double invhp = 1.0 / hp;
+    const double hfactor_p = cr_pow3(invhp);
+    const double hrangesqd = KERNRANGESQD * hp * hp;
+    // self contributions (:323-346)
+    double srho = mp * w0[0] * hfactor_p;
+    double som = mp * hfactor_p * invhp * wh[0];
+    double sdiv = 0.0;
+    double szeta = mp * invhp * invhp * wg[0];
+    // :356-381 for one potential neighbour
+    auto pair = [&](double xj, double yj, double zj, double mj, double vxj, double vyj,
+                    double vzj) {
+        const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
+        const double dsq = ex * ex + ey * ey + ez * ez;
+        if (dsq > hrangesqd) return;                            // :364
+        const double drmag = sqrt(dsq) + SMALL_NUMBER;
+        const int kern = (int)(HALFKERNTOT * drmag * invhp);
+        if (kern > KERNTOT) return;                             // :368
+        const double dvx = vxj - vpx, dvy = vyj - vpy, dvz = vzj - vpz;
+        const double dvdr = dvx * ex + dvy * ey + dvz * ez;
+        srho = srho + mj * hfactor_p * w0[kern];
+        som = som + mj * hfactor_p * invhp * wh[kern];
+        sdiv = sdiv - mj * dvdr * w4[kern] * hfactor_p * invhp / drmag;
+        szeta = szeta + mj * invhp * invhp * wg[kern];
+    };
+    list_loop(wl, nslot, valid && nslot >= 0, p, [&](int j) {
+        const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
+        pair(a.x, a.y, a.z, b.w, b.x, b.y, b.z);
+    });
+    unsigned vis = 0, lvis = 0;
+    const bool walk = valid && nslot < 0;
+    if (__any(walk)) {
+        // lists that did not fit: gather over the tree walk at the same radius
+        vis = group_walk(
+            hn, nc, walk, lvis,
+            [&](const HNode &nd, bool) { return hgather_visit(nd, rpx, rpy, rpz, R); },
+            [&](const HNode &nd, bool mine) {
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = nd.first + i;
+                    const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
+                    if (mine && j != p) pair(a.x, a.y, a.z, b.w, b.x, b.y, b.z);
+                }
+            });
+        wave_stats(wstat, 2, vis, lvis);
+        if ((threadIdx.x & 63) == 0) atomicAdd(wstat + 11, 1ull);
+    }
+    if (valid) {
+        const double om = 1.0 + ((hp * som) / (3.0 * srho));
+        const double dv = sdiv / srho;
+        rho[p] = srho;
+        omega[p] = om;
+        divv[p] = dv;
+        drhodt[p] = -srho * dv / om;
+        h[p] = hp;
+        zeta[p] = -(hp * szeta) / (3.0 * srho * om);
+        if (hp < 0.0) atomicOr(err, 1);
     }
 }
 
@@ -354,9 +502,18 @@ int sph_hsolve(Ctx &c) {
         return set_error(SERENX_EFEW, "fewer particles than pp_gather: the grad-h solve "
                                       "cannot reach its neighbour number");
     SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
-    hipLaunchKernelGGL(k_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn, c.x,
-                       c.y, c.z, c.m, c.vx, c.vy, c.vz, c.active, c.ktab, c.opt.h_fac, c.rextent,
-                       c.pp_gather, c.h, c.zeta, c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.h, c.vx, c.vy, c.vz, c.m, rec);
+    SX_HIP(hipGetLastError());
+    // h_high per particle in the permutation scratch, list lengths in tmpi
+    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
+                       rec, c.hlist, c.active, c.ktab, c.opt.h_fac, c.rextent, c.pp_gather, c.h,
+                       c.rho, c.tmpd, c.tmpi, c.wstat);
+    SX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
+                       rec, c.hlist, c.active, c.ktab, c.opt.h_fac, c.tmpd, c.tmpi, c.h, c.zeta,
+                       c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     return SERENX_OK;

@@ -7,9 +7,12 @@
 namespace serenx {
 
 extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
 
-// minimum over the 64 lanes of the wave (DPP reduction in the device library)
+// minimum / maximum over the active lanes of the wave (DPP reductions in the
+// device library)
 __device__ __forceinline__ int wave_min_int(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max_int(int v) { return __ockl_wfred_max_i32(v); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -245,7 +245,9 @@ class ForceEngine:
                 "hsolve_util": util(v[2], v[3]), "hsolve_wave_visits": v[2],
                 "hsolve_mean_passes": v[4] / max(v[7], 1), "hsolve_max_passes": v[5],
                 "hsolve_mean_iterations": v[6] / max(self.n, 1),
-                "neib_util": util(v[8], v[9]), "neib_wave_visits": v[8]}
+                "neib_util": util(v[8], v[9]), "neib_wave_visits": v[8],
+                "hsolve_list_walks_per_wave": v[10] / max(v[7], 1),
+                "hsolve_overflow_passes": v[11], "hsolve_longest_list_leaves": v[12]}
 
     def interaction_counts(self):
         """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""
