@@ -26,14 +26,33 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles every csrc/*.hip in parallel, then links libserenx.so."""
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp", *srcs]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        cmd = [HIPCC, *FLAGS, "-c", "-I", os.path.join(ROOT, "include"), "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for obj, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed compiling the HIP sources of libserenx.so")
+    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *[o for o, _ in procs]]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libserenx.so")
+        raise RuntimeError("hipcc failed linking libserenx.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

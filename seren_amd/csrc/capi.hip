@@ -323,7 +323,7 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (auto &ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * NTAB * KT * 3 / 2 + 64);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * KT_TOTAL);
     if (e == hipSuccess) e = hipMalloc((void **)&c->icount, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->wstat, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->wstat, 0, 16 * sizeof(unsigned long long));
@@ -334,12 +334,23 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         return hip_check(e, "serenx_create");
     }
     make_kernel_tables(c->host_ktab);
-    std::vector<float> tf(NTAB * KT);
-    for (int i = 0; i < NTAB * KT; i++) tf[i] = (float)c->host_ktab[i];
-    e = hipMemcpy(c->ktab, c->host_ktab, sizeof(double) * NTAB * KT, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(c->ktab + NTAB * KT, tf.data(), sizeof(float) * NTAB * KT,
-                      hipMemcpyHostToDevice);
+    {
+        std::vector<double> dev(KT_TOTAL, 0.0);
+        const double *t = c->host_ktab;
+        std::copy(t, t + NTAB * KT, dev.begin());
+        float *tf = reinterpret_cast<float *>(dev.data() + KT_F32_OFF);
+        for (int i = 0; i < NTAB * KT; i++) tf[i] = (float)t[i];
+        double *g4 = dev.data() + KT_G4_OFF;
+        float *g4f = reinterpret_cast<float *>(dev.data() + KT_G4F_OFF);
+        for (int k = 0; k < KT; k++) {
+            const double w[3] = {t[T_W4 * KT + k], t[T_W5 * KT + k], t[T_W6 * KT + k]};
+            for (int q = 0; q < 3; q++) {
+                g4[4 * k + q] = w[q];
+                g4f[4 * k + q] = (float)w[q];
+            }
+        }
+        e = hipMemcpy(c->ktab, dev.data(), sizeof(double) * KT_TOTAL, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         free_all(*c);
         delete c;

@@ -25,50 +25,50 @@ namespace {
 constexpr int TB = 256;
 inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
 
-// interaction counters: cell terms, pair terms, pairs inside 2h_p, inside 2h_j
-__device__ __forceinline__ void wave_count4(unsigned long long *icount, unsigned a0, unsigned a1,
-                                            unsigned a2, unsigned a3) {
-    unsigned long long v[4] = {a0, a1, a2, a3};
-    for (int o = 32; o > 0; o >>= 1)
-        for (int k = 0; k < 4; k++) v[k] += __shfl_xor(v[k], o, 64);
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 4; k++) atomicAdd(icount + k, v[k]);
+// 1/d for d > 0 to a few ulp: the hardware estimate refined by two Newton
+// steps y <- y + y (1 - d y) (each squares the relative error).
+__device__ __forceinline__ double rcp_fp64(double d) {
+#pragma clang fp contract(fast)
+    double y = __builtin_amdgcn_rcp(d);
+#pragma unroll
+    for (int k = 0; k < 2; k++) y = __fma_rn(y, __fma_rn(-d, y, 1.0), y);
+    return y;
 }
 
-// gravity_gradh.F90:42-86
-__device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
-                                          const double *__restrict__ w5,
-                                          const double *__restrict__ w6, double invhp,
-                                          double zo_p, double rpx, double rpy, double rpz,
-                                          double xj, double yj, double zj, double mj, double hj,
-                                          double zj_zeta, double &ax, double &ay, double &az,
-                                          double &pot) {
+// gravity_gradh.F90:42-86 for one pair, (dx, dy, dz) = r_j - r_p; t4 holds
+// (w4, w5, w6, 0) per table index; invhj = 1/h_j.  The kernel index uses the
+// reference's drmag = sqrt(r^2) + 1e-20; 1/drmag comes from rcp_fp64.
+// Returns bit 0: r inside 2 h_p, bit 1: r inside 2 h_j.
+__device__ __forceinline__ unsigned pair_fp64(const double4 *__restrict__ t4, double invhp,
+                                               double zo_p, double dx, double dy, double dz,
+                                               double mj, double hj, double invhj, double zetaj,
+                                               double &ax, double &ay, double &az,
+                                               double &pot) {
 #pragma clang fp contract(fast)
-    const double dx = xj - rpx, dy = yj - rpy, dz = zj - rpz;
     const double drsqd = dx * dx + dy * dy + dz * dz;
     const double drmag = sqrt(drsqd) + SMALL_NUMBER;
-    const double invdrmag = 1.0 / drmag;
+    const double invdrmag = rcp_fp64(drmag);
+    const double id2 = invdrmag * invdrmag;
     double grav, dpot;
-    int inside = 0;
+    unsigned inside = 0;
     if (invdrmag > INVKERNRANGE * invhp) {
         inside = 1;
-        int kern = (int)(HALFKERNTOT * drmag * invhp);
-        kern = min(kern, KERNTOT);
-        grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
-        dpot = invhp * w6[kern];
+        const int kern = min((int)(HALFKERNTOT * drmag * invhp), KERNTOT);
+        const double4 w = t4[kern];
+        grav = invhp * invhp * (w.y + zo_p * w.x * invhp * invhp);
+        dpot = invhp * w.z;
     } else {
-        grav = invdrmag * invdrmag;
+        grav = id2;
         dpot = invdrmag;
     }
     if (drmag < KERNRANGE * hj) {
         inside |= 2;
-        const double invhpp = 1.0 / hj;
-        int kern = (int)(HALFKERNTOT * drmag * invhpp);
-        kern = min(kern, KERNTOT);
-        grav = grav + invhpp * invhpp * (w5[kern] + zj_zeta * w4[kern] * invhpp * invhpp);
-        dpot = dpot + invhpp * w6[kern];
+        const int kern = min((int)(HALFKERNTOT * drmag * invhj), KERNTOT);
+        const double4 w = t4[kern];
+        grav = grav + invhj * invhj * (w.y + zetaj * w.x * invhj * invhj);
+        dpot = dpot + invhj * w.z;
     } else {
-        grav = grav + invdrmag * invdrmag;
+        grav = grav + id2;
         dpot = dpot + invdrmag;
     }
     const double f = 0.5 * mj * invdrmag * grav;
@@ -77,6 +77,19 @@ __device__ __forceinline__ int pair_fp64(const double *__restrict__ w4,
     az = az + f * dz;
     pot = pot + 0.5 * mj * dpot;
     return inside;
+}
+
+// Gravity source record of particle j, 64 bytes in the second half of the
+// 128-byte particle record: [2] x y z m  [3] h 1/h zeta 0
+__global__ void k_pack_grav(int n, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const double *__restrict__ m,
+                            const double *__restrict__ h, const double *__restrict__ zeta,
+                            double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 *r = rec + 4 * (size_t)i;
+    r[2] = make_double4(x[i], y[i], z[i], m[i]);
+    r[3] = make_double4(h[i], 1.0 / h[i], zeta[i], 0.0);
 }
 
 // 1/sqrt(a) for a > 0 to a few ulp: the hardware estimate refined by two
@@ -151,32 +164,29 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
 
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_fp64(
-    int n, int nc, const GNode *__restrict__ gn, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const double *__restrict__ h, const double *__restrict__ zeta,
-    const uint8_t *__restrict__ active, const double *__restrict__ ktab,
+    int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
+    const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
-    const double *w4 = ktab + T_W4 * KT;
-    const double *w5 = ktab + T_W5 * KT;
-    const double *w6 = ktab + T_W6 * KT;
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
     if (valid) {
-        rpx = x[p]; rpy = y[p]; rpz = z[p];
-        invhp = 1.0 / h[p];
-        zo_p = zeta[p];
+        const double4 a = rec[4 * (size_t)p + 2], b = rec[4 * (size_t)p + 3];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        invhp = b.y;
+        zo_p = b.z;
         if (MACT != SERENX_MAC_GEOMETRIC) {   // BHgrav_accel.F90:97-116
-            const double a = agm[p];
-            if (a > SMALL_NUMBER)
-                afactor = (MACT == SERENX_MAC_GADGET) ? pow(a, -ONETHIRD) : 1.0 / sqrt(a);
+            const double am = agm[p];
+            if (am > SMALL_NUMBER)
+                afactor = (MACT == SERENX_MAC_GADGET) ? pow(am, -ONETHIRD) : 1.0 / sqrt(am);
         }
     }
-    double acx = 0, acy = 0, acz = 0, pc = 0;   // cell terms
-    double apx = 0, apy = 0, apz = 0, pp = 0;   // particle-particle terms
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
+    // one accumulator for cell and pair terms (the reference adds the pairs
+    // after the cells, :272-290; the order changes only the rounding)
+    double gx = 0, gy = 0, gz = 0, gp = 0;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
     const unsigned vis = group_walk(
         gn, nc, valid, lvis,
         [&](const GNode &g, bool here) -> int {
@@ -188,37 +198,38 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
             else
                 accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
             accept = accept && g.leaf != 1;                    // :128-136, no MAC test
-            if (here && accept) {
-                cell_fp64<OCT>(g, dr, drsqd, acx, acy, acz, pc);
-                ncell++;
-            }
+            ncell += lanes(here && accept);
+            if (here && accept) cell_fp64<OCT>(g, dr, drsqd, gx, gy, gz, gp);
             if (accept) return WALK_SKIP;
             return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
         },
         [&](const GNode &g, bool mine) {
-            const int k = g.leaf, f0 = g.first;
-            for (int i = 0; i < k; i++) {
-                const int j = f0 + i;
-                const double xj = x[j], yj = y[j], zj = z[j], mj = m[j], hj = h[j],
-                             zetaj = zeta[j];
-                if (!mine || j == p) continue;
-                const int in_ = pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, xj, yj, zj, mj,
-                                          hj, zetaj, apx, apy, apz, pp);
-                npp++;
-                nin_p += in_ & 1;
-                nin_j += in_ >> 1;
+            for (int i = 0; i < g.leaf; i++) {
+                const int j = g.first + i;
+                const double4 a = rec[4 * (size_t)j + 2], b = rec[4 * (size_t)j + 3];
+                const bool act = mine && j != p;
+                unsigned in_ = 0;
+                if (act)
+                    in_ = pair_fp64(t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w, b.x,
+                                    b.y, b.z, gx, gy, gz, gp);
+                npp += lanes(act);
+                nin_p += lanes(in_ & 1);
+                nin_j += lanes(in_ & 2);
             }
         });
     if (valid) {
-        const double gx = acx + apx, gy = acy + apy, gz = acz + apz;
         ax[p] = ax[p] + gx;
         ay[p] = ay[p] + gy;
         az[p] = az[p] + gz;
-        gpot[p] = pc + pp;
+        gpot[p] = gp;
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
-    // per-wave interaction counts
-    wave_count4(icount, ncell, npp, nin_p, nin_j);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(icount + 0, (unsigned long long)ncell);
+        atomicAdd(icount + 1, (unsigned long long)npp);
+        atomicAdd(icount + 2, (unsigned long long)nin_p);
+        atomicAdd(icount + 3, (unsigned long long)nin_j);
+    }
     wave_stats(wstat, 0, vis, lvis);
 }
 
@@ -234,33 +245,32 @@ struct CSum {
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
 };
 
-__device__ __forceinline__ int pair_fp32(const float *__restrict__ w4, const float *__restrict__ w5,
-                                          const float *__restrict__ w6, float invhp, float zo_p,
-                                          float dx, float dy, float dz, float mj, float hj,
-                                          float zj, CSum &ax, CSum &ay, CSum &az, CSum &pot) {
+__device__ __forceinline__ unsigned pair_fp32(const float4 *__restrict__ t4, float invhp,
+                                               float zo_p, float dx, float dy, float dz, float mj,
+                                               float hj, float invhj, float zj, CSum &ax,
+                                               CSum &ay, CSum &az, CSum &pot) {
 #pragma clang fp contract(fast)
     const float drsqd = dx * dx + dy * dy + dz * dz;
     const float drmag = sqrtf(drsqd) + 1.0e-20f;
-    const float invdrmag = 1.0f / drmag;
+    const float invdrmag = __builtin_amdgcn_rcpf(drmag);
     float grav, dpot;
-    int inside = 0;
+    unsigned inside = 0;
     if (invdrmag > 0.5f * invhp) {
         inside = 1;
-        int kern = (int)(5000.0f * drmag * invhp);
-        kern = min(kern, KERNTOT);
-        grav = invhp * invhp * (w5[kern] + zo_p * w4[kern] * invhp * invhp);
-        dpot = invhp * w6[kern];
+        const int kern = min((int)(5000.0f * drmag * invhp), KERNTOT);
+        const float4 w = t4[kern];
+        grav = invhp * invhp * (w.y + zo_p * w.x * invhp * invhp);
+        dpot = invhp * w.z;
     } else {
         grav = invdrmag * invdrmag;
         dpot = invdrmag;
     }
     if (drmag < 2.0f * hj) {
         inside |= 2;
-        const float invhpp = 1.0f / hj;
-        int kern = (int)(5000.0f * drmag * invhpp);
-        kern = min(kern, KERNTOT);
-        grav = grav + invhpp * invhpp * (w5[kern] + zj * w4[kern] * invhpp * invhpp);
-        dpot = dpot + invhpp * w6[kern];
+        const int kern = min((int)(5000.0f * drmag * invhj), KERNTOT);
+        const float4 w = t4[kern];
+        grav = grav + invhj * invhj * (w.y + zj * w.x * invhj * invhj);
+        dpot = dpot + invhj * w.z;
     } else {
         grav = grav + invdrmag * invdrmag;
         dpot = dpot + invdrmag;
@@ -290,33 +300,29 @@ __device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], do
 
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_mixed(
-    int n, int nc, const GNodeF *__restrict__ gn, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const double *__restrict__ h, const double *__restrict__ zeta,
-    const uint8_t *__restrict__ active, const float *__restrict__ ktabf,
+    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
-    const float *w4 = ktabf + T_W4 * KT;
-    const float *w5 = ktabf + T_W5 * KT;
-    const float *w6 = ktabf + T_W6 * KT;
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
     float invhp = 1.0f, zo_p = 0.0f;
     if (valid) {
-        rpx = x[p]; rpy = y[p]; rpz = z[p];
-        invhp = (float)(1.0 / h[p]);
-        zo_p = (float)zeta[p];
+        const double4 a = rec[4 * (size_t)p + 2], b = rec[4 * (size_t)p + 3];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        invhp = (float)b.y;
+        zo_p = (float)b.z;
         if (MACT != SERENX_MAC_GEOMETRIC) {
-            const double a = agm[p];
-            if (a > SMALL_NUMBER)
-                afactor = (MACT == SERENX_MAC_GADGET) ? pow(a, -ONETHIRD) : 1.0 / sqrt(a);
+            const double am = agm[p];
+            if (am > SMALL_NUMBER)
+                afactor = (MACT == SERENX_MAC_GADGET) ? pow(am, -ONETHIRD) : 1.0 / sqrt(am);
         }
     }
     double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
     CSum px, py, pz, pq;                              // pair terms
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
     const unsigned vis = group_walk(
         gn, nc, valid, lvis,
         [&](const GNodeF &g, bool here) -> int {
@@ -328,27 +334,27 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
             else
                 accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
             accept = accept && g.leaf != 1;
+            ncell += lanes(here && accept);
             if (here && accept) {
                 const float dr[3] = {(float)d0, (float)d1, (float)d2};
                 cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
-                ncell++;
             }
             if (accept) return WALK_SKIP;
             return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
         },
         [&](const GNodeF &g, bool mine) {
-            const int k = g.leaf, f0 = g.first;
-            for (int i = 0; i < k; i++) {
-                const int j = f0 + i;
-                const double xj = x[j], yj = y[j], zj = z[j];
-                const float mj = (float)m[j], hj = (float)h[j], zetaj = (float)zeta[j];
-                if (!mine || j == p) continue;
-                const int in_ = pair_fp32(w4, w5, w6, invhp, zo_p, (float)(xj - rpx),
-                                          (float)(yj - rpy), (float)(zj - rpz), mj, hj, zetaj,
-                                          px, py, pz, pq);
-                npp++;
-                nin_p += in_ & 1;
-                nin_j += in_ >> 1;
+            for (int i = 0; i < g.leaf; i++) {
+                const int j = g.first + i;
+                const double4 a = rec[4 * (size_t)j + 2], b = rec[4 * (size_t)j + 3];
+                const bool act = mine && j != p;
+                unsigned in_ = 0;
+                if (act)
+                    in_ = pair_fp32(t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
+                                    (float)(a.z - rpz), (float)a.w, (float)b.x, (float)b.y,
+                                    (float)b.z, px, py, pz, pq);
+                npp += lanes(act);
+                nin_p += lanes(in_ & 1);
+                nin_j += lanes(in_ & 2);
             }
         });
     if (valid) {
@@ -359,7 +365,12 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         gpot[p] = cp + pq.value();
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
-    wave_count4(icount, ncell, npp, nin_p, nin_j);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(icount + 0, (unsigned long long)ncell);
+        atomicAdd(icount + 1, (unsigned long long)npp);
+        atomicAdd(icount + 2, (unsigned long long)nin_p);
+        atomicAdd(icount + 3, (unsigned long long)nin_j);
+    }
     wave_stats(wstat, 0, vis, lvis);
 }
 
@@ -372,15 +383,12 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
                                                 const double *__restrict__ m,
                                                 const double *__restrict__ h,
                                                 const double *__restrict__ zeta,
-                                                const double *__restrict__ ktab,
+                                                const double4 *__restrict__ t4,
                                                 double *__restrict__ acc, double *__restrict__ pot) {
-    __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256], szt[256];
+    __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256], sih[256], szt[256];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < nt;
     const int p = valid ? targets[t] : 0;
-    const double *w4 = ktab + T_W4 * KT;
-    const double *w5 = ktab + T_W5 * KT;
-    const double *w6 = ktab + T_W6 * KT;
     const double rpx = x[p], rpy = y[p], rpz = z[p], invhp = 1.0 / h[p], zo_p = zeta[p];
     double gx = 0, gy = 0, gz = 0, gp = 0;
     for (int base = 0; base < n; base += 256) {
@@ -391,6 +399,7 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
             sz[threadIdx.x] = z[j];
             sm[threadIdx.x] = m[j];
             sh[threadIdx.x] = h[j];
+            sih[threadIdx.x] = 1.0 / h[j];
             szt[threadIdx.x] = zeta[j];
         }
         __syncthreads();
@@ -398,8 +407,8 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
         if (valid) {
             for (int k = 0; k < lim; k++) {
                 if (base + k == p) continue;
-                (void)pair_fp64(w4, w5, w6, invhp, zo_p, rpx, rpy, rpz, sx[k], sy[k], sz[k], sm[k],
-                          sh[k], szt[k], gx, gy, gz, gp);
+                (void)pair_fp64(t4, invhp, zo_p, sx[k] - rpx, sy[k] - rpy, sz[k] - rpz, sm[k],
+                                sh[k], sih[k], szt[k], gx, gy, gz, gp);
             }
         }
         __syncthreads();
@@ -414,15 +423,19 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
 
 template <int OCT, int MACT>
 void launch_grav(Ctx &c) {
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.m, c.h, c.zeta, rec);
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL((k_grav_mixed<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gnf, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active,
-                           (const float *)(c.ktab + NTAB * KT), c.ax, c.ay, c.az, c.gpot, c.agm,
-                           c.icount, c.wstat);
+                           c.nc, c.gnf, rec, c.active,
+                           reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.ax, c.ay,
+                           c.az, c.gpot, c.agm, c.icount, c.wstat);
     else
         hipLaunchKernelGGL((k_grav_fp64<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gn, c.x, c.y, c.z, c.m, c.h, c.zeta, c.active, c.ktab, c.ax,
-                           c.ay, c.az, c.gpot, c.agm, c.icount, c.wstat);
+                           c.nc, c.gn, rec, c.active,
+                           reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.ax, c.ay,
+                           c.az, c.gpot, c.agm, c.icount, c.wstat);
 }
 
 }  // namespace
@@ -447,7 +460,8 @@ int grav_forces(Ctx &c) {
 
 int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot) {
     hipLaunchKernelGGL(k_direct, dim3(nblk(nt)), dim3(256), 0, c.stream, c.n, nt, dtargets, c.x,
-                       c.y, c.z, c.m, c.h, c.zeta, c.ktab, dacc, dpot);
+                       c.y, c.z, c.m, c.h, c.zeta,
+                       reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), dacc, dpot);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }

@@ -34,6 +34,13 @@ constexpr double INVPI = 0.318309886;
 constexpr int NTAB = 9;            // w0 w1 w2 w3 w4 w5 w6 wh wg
 constexpr int KT = KERNTOT + 1;    // entries per table
 enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
+// Device table buffer (in doubles): the NTAB FP64 tables, their FP32 copy,
+// then the gravity tables interleaved per index as (w4, w5, w6, 0) in FP64
+// and in FP32, so that one softened pair term reads one 32- or 16-byte entry.
+constexpr size_t KT_F32_OFF = (size_t)NTAB * KT;
+constexpr size_t KT_G4_OFF = ((KT_F32_OFF + ((size_t)NTAB * KT + 1) / 2) + 3) / 4 * 4;
+constexpr size_t KT_G4F_OFF = KT_G4_OFF + 4 * (size_t)KT;
+constexpr size_t KT_TOTAL = KT_G4F_OFF + 2 * (size_t)KT;
 
 constexpr int WAVE = 64;
 // potential-neighbour list slots (opened leaves) per particle in the h solve

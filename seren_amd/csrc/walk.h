@@ -94,8 +94,9 @@ __device__ __forceinline__ double ref_pow_third(double x) {
 // so the leaf's particles (contiguous in tree order) can be read once through
 // the scalar cache for the whole wave.
 //
-// Returns the number of cells the wave visited; lane_visits counts the cells
-// at which this lane was active (their ratio is the walk's SIMD utilisation).
+// Returns the number of cells the wave visited; lane_visits (wave-uniform)
+// adds the number of lanes active at each visited cell (their ratio is the
+// walk's SIMD utilisation).
 enum { WALK_SKIP = 0, WALK_DESCEND = 1, WALK_LEAF = 2 };
 
 template <class Node, class Visit, class Leaf>
@@ -112,8 +113,8 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
         visits++;
         const bool here = resume == c;
         int act = visit(nd, here);
+        lane_visits += (unsigned)__popcll(__ballot(here));
         if (here) {
-            lane_visits++;
             resume = (act == WALK_DESCEND) ? c + 1 : nd.next;
         } else {
             act = WALK_SKIP;
@@ -128,16 +129,16 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
 }
 
 // Adds per-wave walk statistics into stats[k0], stats[k0 + 1] from lane 0:
-// the wave's visit count and the wave's sum of lane visits.
+// the wave's visit count and its (wave-uniform) sum of lane visits.
 __device__ __forceinline__ void wave_stats(unsigned long long *stats, int k0, unsigned visits,
                                            unsigned lane_visits) {
-    unsigned long long v = lane_visits;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(stats + k0, (unsigned long long)visits);
-        atomicAdd(stats + k0 + 1, v);
+        atomicAdd(stats + k0 + 1, (unsigned long long)lane_visits);
     }
 }
+
+// number of lanes of the wave for which pred holds (wave-uniform)
+__device__ __forceinline__ unsigned lanes(bool pred) { return (unsigned)__popcll(__ballot(pred)); }
 
 }  // namespace serenx
