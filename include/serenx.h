@@ -219,6 +219,63 @@ int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4);
  * leaves; [13..15] zero. */
 int serenx_walk_stats(serenx_ctx *ctx, int64_t *stats16);
 
+/* ---------------------------------------------------------------------------
+ * Device-resident integration steps (sph_integrate.F90:41-113)
+ * ------------------------------------------------------------------------- */
+
+enum {
+    SERENX_INT_RK = 0,    /* advance_runge_kutta.F90 (the reference default) */
+    SERENX_INT_LFKDK = 1  /* advance_leapfrog_kdk.F90, with the second-half
+                             kick taken over the time since the half step
+                             (the reference leaves dt_half unset there) */
+};
+
+typedef struct serenx_step_options {
+    int32_t integrator;   /* SERENX_INT_RK or SERENX_INT_LFKDK */
+    int32_t nlevels;      /* block time-step levels (parameter nlevels, default 1) */
+    int32_t nbuildstep;   /* integer steps between tree builds (default 8) */
+    int32_t reserved;
+    double accel_mult;    /* timestep_size acceleration factor (default 0.5) */
+    double courant_mult;  /* timestep_size Courant factor (default 0.2) */
+} serenx_step_options;
+
+typedef struct serenx_step_info {
+    int64_t n;            /* integer time (time_module n) */
+    int64_t nsteps;       /* steps taken */
+    int64_t nresync;      /* integer time of the next resynchronisation */
+    int64_t level_max;    /* deepest occupied time-step level */
+    int64_t level_step;   /* level of the smallest (half) step */
+    int64_t nbuild;       /* step of the next tree build */
+    int64_t nstock;       /* step of the next tree stock */
+    int64_t nactive;      /* particles flagged for a force evaluation next step */
+    double time;          /* physical time */
+    double timestep;      /* real time of the smallest integer step */
+    double dt_max;        /* largest step size */
+    double reserved;
+} serenx_step_info;
+
+void serenx_default_step_options(serenx_step_options *opt);
+
+/* Starts device-resident stepping after the setup evaluation
+ * (serenx_force_eval with SERENX_EVAL_HGUESS, as sph_setup.F90 does): the
+ * step state of initialize_sph_variables_2.F90:56-70,157-175 and
+ * initialize_seren_variables_2.F90:60-61 (r_old = r, v_old = v_half = v, no
+ * particle active, n = nresync = 0, next build and stock at step 1). */
+int serenx_step_begin(serenx_ctx *ctx, const serenx_step_options *opt);
+
+/* count x sph_integrate: tree build/stock on the reference's cadence
+ * (tree_update.F90:24-88), sph_update, thermal and forces for the active
+ * particles, sph_timesteps (sph_timesteps.F90:30-337 with timestep_size.F90),
+ * sph_advance.  Synchronous. */
+int serenx_step(serenx_ctx *ctx, int32_t count);
+
+int serenx_step_get_info(const serenx_ctx *ctx, serenx_step_info *info);
+
+/* Per-particle step state in tree order; any pointer may be NULL.  Vectors
+ * row-major [n][3]; active = accdo (1/0). */
+int serenx_fetch_step(serenx_ctx *ctx, int32_t *nlevel, int64_t *nlast, double *laststep,
+                      int32_t *active, double *r_old, double *v_old, double *v_half);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,7 @@ LMAX = 30
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 VARIANTS = {
     # (octupole, mac) -> reference library variant
@@ -165,6 +166,10 @@ class RefSeren(_Common):
             lib.ref_get_grav_tree.argtypes = [C.c_int, _ip, _ip, _ip, _ip] + [_dp] * 7
             lib.ref_get_hydro_tree.argtypes = [C.c_int, _dp, _dp, _dp]
             lib.ref_get_kernel.argtypes = [_dp]
+            lib.ref_step_setup.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+            lib.ref_integrate.argtypes = [C.c_double]
+            lib.ref_get_step.argtypes = [_lp, _dp]
+            lib.ref_get_step_particles.argtypes = [C.c_int, _lp, _lp, _dp, _ip, _dp, _dp, _dp]
             self._libs[variant] = lib
         self.lib = self._libs[variant]
         self.variant = variant
@@ -193,6 +198,34 @@ class RefSeren(_Common):
 
     def grav_forces(self):
         self.lib.ref_grav_forces()
+
+    # ---- integration steps (sph_integrate.F90 with RUNGE_KUTTA) ---------
+    def step_setup(self, nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2):
+        """Step state after the setup evaluation (initialize_sph_variables_2)."""
+        self.lib.ref_step_setup(nlevels, nbuildstep, accel_mult, courant_mult)
+
+    def integrate(self, cs):
+        """One sph_integrate: tree_update, sph_update, thermal, forces,
+        sph_timesteps, sph_advance."""
+        self.lib.ref_integrate(cs)
+
+    def step_info(self):
+        ist = np.zeros(8, np.int64)
+        rst = np.zeros(4)
+        self.lib.ref_get_step(ist, rst)
+        keys = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
+        out = {k: int(v) for k, v in zip(keys, ist)}
+        out.update(time=float(rst[0]), timestep=float(rst[1]), dt_max=float(rst[2]))
+        return out
+
+    def step_particles(self):
+        n = self.n
+        nlevel, nlast = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        laststep, accdo = np.zeros(n), np.zeros(n, np.int32)
+        vh, ro, vo = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self.lib.ref_get_step_particles(n, nlevel, nlast, laststep, accdo, vh, ro, vo)
+        return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=accdo,
+                    v_half=vh, r_old=ro, v_old=vo)
 
     def set_agravmag(self, am):
         self.lib.ref_set_agravmag(self.n, _f64(am))

@@ -18,6 +18,12 @@
 !   * ref_hydro_forces  ~ sph_hydro_forces.F90:9-120
 !   * ref_grav_forces   ~ sph_grav_forces.F90:10-199
 !   * ref_direct_gravity~ direct_sph_gravity.F90:10-107 for a target list
+!   * ref_step_setup    ~ initialize_sph_variables_2.F90:56-70,157-175 and
+!                         initialize_seren_variables_2.F90:60-61 (step state
+!                         after the setup force evaluation)
+!   * ref_integrate     ~ sph_integrate.F90:41-113 (tree_update, sph_update,
+!                         thermal, forces, sph_timesteps, sph_advance with
+!                         RUNGE_KUTTA) without sinks, output or extra physics
 ! plus getters that copy module state out through plain C pointers.
 ! ============================================================================
 
@@ -49,6 +55,7 @@ CONTAINS
     deallocate(pptot, pplist)
     deallocate(rho, temp, press, sound, div_v, rho_old, drhodt, omega)
     deallocate(accdo, nlast, nlevel, laststep)
+    deallocate(v_half)
     deallocate(first_cell_grav, last_cell_grav, BHgrav)
     deallocate(first_cell_hydro, last_cell_hydro, BHhydro)
     deallocate(cellof, BHnextptcl, whichchild, BHtemp, BHstock)
@@ -90,6 +97,7 @@ CONTAINS
     allocate(rho(1:pmax), temp(1:pmax), press(1:pmax), sound(1:pmax))
     allocate(div_v(1:pmax), rho_old(1:pmax), drhodt(1:pmax), omega(1:pmax))
     allocate(accdo(1:pmax), nlast(1:pmax), nlevel(1:pmax), laststep(1:pmax))
+    allocate(v_half(1:VDIM,1:pmax))
 
     cmax_grav  = 2*(pgas + pcdm)
     cmax_hydro = 2*pmax
@@ -342,5 +350,69 @@ CONTAINS
     tab(:,4) = w3(:) ; tab(:,5) = w4(:) ; tab(:,6) = w5(:)
     tab(:,7) = w6(:) ; tab(:,8) = wh(:) ; tab(:,9) = wg(:)
   end subroutine ref_get_kernel
+
+! ----------------------------------------------------------------------------
+  subroutine ref_step_setup(nlevels_in, nbuildstep_in, accel_in, courant_in) &
+       & bind(C, name="ref_step_setup")
+    integer(c_int), value :: nlevels_in, nbuildstep_in
+    real(c_double), value :: accel_in, courant_in
+    r_old(1:NDIM,1:ptot) = parray(1:NDIM,1:ptot)
+    v_old(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
+    v_half(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
+    rho_old(1:ptot) = rho(1:ptot)
+    accdo(1:ptot) = .false.
+    nlevels = nlevels_in
+    nbuildstep = nbuildstep_in
+    accel_mult = accel_in
+    courant_mult = courant_in
+    dt_fixed = 1.0_DP
+    n = 0
+    nresync = n
+    time = 0.0_DP
+    nbuild = nsteps + 1
+    nstock = nsteps + 1
+  end subroutine ref_step_setup
+
+  subroutine ref_integrate(cs) bind(C, name="ref_integrate")
+    real(c_double), value :: cs
+    integer :: p
+    call tree_update
+    call sph_update
+    call ref_thermal_iso(cs)
+    do p=1,ptot
+       if (accdo(p)) a(1:VDIM,p) = 0.0_PR
+    end do
+    call sph_hydro_forces
+    call sph_grav_forces
+    call sph_timesteps
+    call sph_advance
+  end subroutine ref_integrate
+
+  subroutine ref_get_step(istate, rstate) bind(C, name="ref_get_step")
+    integer(c_long_long), intent(out) :: istate(8)
+    real(c_double), intent(out) :: rstate(4)
+    istate(1) = n ; istate(2) = nsteps ; istate(3) = nresync
+    istate(4) = level_max ; istate(5) = level_step
+    istate(6) = nbuild ; istate(7) = nstock ; istate(8) = 0
+    rstate(1) = time ; rstate(2) = timestep ; rstate(3) = dt_max ; rstate(4) = 0.0_DP
+  end subroutine ref_get_step
+
+  subroutine ref_get_step_particles(np, nlev, nl, lstep, acc, vh, ro, vo) &
+       & bind(C, name="ref_get_step_particles")
+    integer(c_int), value :: np
+    integer(c_long_long), intent(out) :: nlev(np), nl(np)
+    real(c_double), intent(out) :: lstep(np), vh(3,np), ro(3,np), vo(3,np)
+    integer(c_int), intent(out) :: acc(np)
+    integer :: p
+    do p=1,np
+       nlev(p) = nlevel(p)
+       nl(p) = nlast(p)
+       lstep(p) = laststep(p)
+       acc(p) = merge(1, 0, accdo(p))
+       vh(1:3,p) = v_half(1:3,p)
+       ro(1:3,p) = r_old(1:3,p)
+       vo(1:3,p) = v_old(1:3,p)
+    end do
+  end subroutine ref_get_step_particles
 
 END MODULE ref_driver

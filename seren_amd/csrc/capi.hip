@@ -147,6 +147,7 @@ int ensure_capacity(Ctx &c, int n) {
 }
 
 void free_all(Ctx &c) {
+    step_free(c);
     void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
                     c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
@@ -180,6 +181,8 @@ int check_options(const serenx_options &o) {
 Ctx *as_ctx(serenx_ctx *p) { return reinterpret_cast<Ctx *>(p); }
 const Ctx *as_ctx(const serenx_ctx *p) { return reinterpret_cast<const Ctx *>(p); }
 
+}  // namespace
+
 int run_tree_update(Ctx &c) {
     SX_TRY(build_tree(c));
     SX_TRY(stock_tree(c));
@@ -208,6 +211,8 @@ int run_grav(Ctx &c) {
     SX_TRY(grav_forces(c));
     return SERENX_OK;
 }
+
+namespace {
 
 // ---- host-state transfers -------------------------------------------------
 // parray(6,ptot) column-major: element (k, p) at parray[6 p + k]
@@ -524,6 +529,7 @@ int serenx_load(serenx_ctx *ctx, int32_t n, const double *pos, const double *vel
     if (!ctx || !pos || !vel || !mass || n <= 0) return set_error(SERENX_EINVAL, "bad load args");
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
+    step_free(c);
     SX_TRY(ensure_capacity(c, n));
     c.n = n;
     c.bound = false;
@@ -736,6 +742,58 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int
         if (last_cell) last_cell[L] = has ? c.level_off[L] + c.level_cnt[L] - 1 : 0;
     }
     return SERENX_OK;
+}
+
+void serenx_default_step_options(serenx_step_options *opt) {
+    if (!opt) return;
+    *opt = serenx_step_options{};
+    opt->integrator = SERENX_INT_RK;     // Makefile:45 SPH_INTEGRATION = RK
+    opt->nlevels = 1;                    // default_parameters.F90:66
+    opt->nbuildstep = 8;                 // default_parameters.F90:117
+    opt->accel_mult = 0.5;               // default_parameters.F90:63
+    opt->courant_mult = 0.2;             // default_parameters.F90:62
+}
+
+int serenx_step_begin(serenx_ctx *ctx, const serenx_step_options *opt) {
+    if (!ctx || !opt) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return step_begin(c, *opt);
+}
+
+int serenx_step(serenx_ctx *ctx, int32_t count) {
+    if (!ctx || count < 0) return set_error(SERENX_EINVAL, "bad step arguments");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return step_run(c, count);
+}
+
+int serenx_step_get_info(const serenx_ctx *ctx, serenx_step_info *info) {
+    if (!ctx || !info) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    if (!c.st.on) return set_error(SERENX_ESTATE, "no step state (serenx_step_begin)");
+    const Ctx::Step &s = c.st;
+    *info = serenx_step_info{};
+    info->n = s.n;
+    info->nsteps = s.nsteps;
+    info->nresync = s.nresync;
+    info->level_max = s.level_max;
+    info->level_step = s.level_step;
+    info->nbuild = s.nbuild;
+    info->nstock = s.nstock;
+    info->nactive = s.nactive;
+    info->time = s.time;
+    info->timestep = s.timestep;
+    info->dt_max = s.dt_max;
+    return SERENX_OK;
+}
+
+int serenx_fetch_step(serenx_ctx *ctx, int32_t *nlevel, int64_t *nlast, double *laststep,
+                      int32_t *active, double *r_old, double *v_old, double *v_half) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return step_fetch(c, nlevel, nlast, laststep, active, r_old, v_old, v_half);
 }
 
 }  // extern "C"

@@ -168,6 +168,23 @@ struct Ctx {
     // walk statistics of the last force evaluation (serenx_walk_stats)
     unsigned long long *wstat = nullptr;
 
+    // device-resident stepping (sph_integrate): per-particle state in tree
+    // order, permuted with the particles, and the time_module scalars
+    struct Step {
+        bool on = false;
+        serenx_step_options opt{};
+        double *rox = nullptr, *roy = nullptr, *roz = nullptr;   // r_old
+        double *vox = nullptr, *voy = nullptr, *voz = nullptr;   // v_old
+        double *vhx = nullptr, *vhy = nullptr, *vhz = nullptr;   // v_half
+        double *laststep = nullptr, *dt = nullptr;
+        long long *nlast = nullptr;
+        int *nlevel = nullptr;
+        unsigned long long *red = nullptr;   // device reductions [dt_min bits, p_update, level max, nactive]
+        long long n = 0, nsteps = 0, nresync = 0, level_max = 0, level_step = 0;
+        long long nbuild = 0, nstock = 0, nactive = 0;
+        double time = 0.0, timestep = 0.0, dt_max = 0.0;
+    } st;
+
     // host-state binding
     serenx_host_state hs{};
     bool bound = false;
@@ -205,6 +222,20 @@ int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles
 int sph_neighbours(Ctx &c);      // get_neib over active particles
 int sph_thermal_iso(Ctx &c, double cs);
 int hydro_forces(Ctx &c);        // hydro_gradh over active particles
+
+// step.hip
+int step_begin(Ctx &c, const serenx_step_options &opt);
+int step_run(Ctx &c, int count);
+int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_t *active,
+               double *r_old, double *v_old, double *v_half);
+void step_free(Ctx &c);
+int permute_step(Ctx &c, const int *perm);
+
+// capi.hip: the force path pieces the step driver reuses
+int run_tree_update(Ctx &c);
+int run_sph_update(Ctx &c);
+int run_hydro(Ctx &c);
+int run_grav(Ctx &c);
 
 // grav.hip
 int grav_forces(Ctx &c);         // BHgrav_accel over active particles

@@ -1,0 +1,474 @@
+// step.hip -- device-resident integration steps on gfx950: the order of
+// sph_integrate.F90:41-113 around the force path, the hierarchical block
+// time steps of sph_timesteps.F90 with timestep_size.F90, and the particle
+// advance of advance_runge_kutta.F90 (default) or advance_leapfrog_kdk.F90.
+//
+// Per-particle step state (r_old, v_old, v_half, nlast, nlevel, laststep)
+// lives on the device in tree order and is permuted with the particles on
+// every tree build, as reorder_particle_arrays does.  The integer-time
+// scalars of time_module stay on the host; each step reads back three
+// reductions (minimum dt, number of updated particles, deepest level).
+//
+// All arithmetic follows the reference's operation order without FMA
+// contraction (the library is compiled with -ffp-contract=off).
+#include <cmath>
+#include <vector>
+
+#include "serenx_internal.h"
+
+namespace serenx {
+
+namespace {
+
+constexpr int TB = 256;
+inline int nblk(long n) { return (int)((n + TB - 1) / TB); }
+
+constexpr double INVLOGETWO = 1.442695041;   // macros.h:15 (1/ln 2 to 10 digits)
+constexpr double TVISC_FAC = 0.6;            // macros.h:133
+constexpr double SMALL_NUMBER_DP = 1.0e-20;  // macros.h:82
+
+enum { R_DTMIN = 0, R_PUPDATE = 1, R_LEVMAX = 2, R_NACTIVE = 3 };
+
+// timestep_size.F90:92-114: acceleration and Courant criteria with the AB
+// viscosity terms (beta = 2 alpha, hydro_gradh.F90:227)
+__device__ __forceinline__ double timestep_size(double ax, double ay, double az, double hp,
+                                                double divv, double vsignal, double alpha_p,
+                                                double beta_p, double accel_mult,
+                                                double courant_mult) {
+    const double div_v_p = fabs(divv);
+    const double amag = sqrt(ax * ax + ay * ay + az * az);
+    const double tacc = accel_mult * sqrt(hp / (amag + SMALL_NUMBER_DP));
+    double dt = tacc;
+    const double tcour = courant_mult * hp /
+                         (vsignal + hp * div_v_p +
+                          TVISC_FAC * (alpha_p * vsignal + beta_p * hp * div_v_p));
+    dt = (tcour < dt) ? tcour : dt;
+    return dt;
+}
+
+// INVLOGETWO*log(dt_max/dt) truncated, + 1 (sph_timesteps.F90:99, :159)
+__device__ __forceinline__ long long level_of(double dt_max, double dt) {
+    return (long long)(INVLOGETWO * log(dt_max / dt)) + 1;
+}
+
+__device__ __forceinline__ void wave_min_bits(unsigned long long *dst, double v) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(b, o, 64);
+        b = t < b ? t : b;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(dst, b);
+}
+
+// sph_timesteps.F90:59-66 and :94-102: every particle's step size, the
+// minimum, nlast = 0; then (second kernel) the level of each particle.
+__global__ void k_ts_all(int n, const double *__restrict__ ax, const double *__restrict__ ay,
+                         const double *__restrict__ az, const double *__restrict__ h,
+                         const double *__restrict__ divv, const double *__restrict__ sound,
+                         double alpha, double accel_mult, double courant_mult,
+                         double *__restrict__ dt, long long *__restrict__ nlast,
+                         unsigned long long *__restrict__ red) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 9.9e20;
+    if (p < n) {
+        nlast[p] = 0;
+        d = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha, 2.0 * alpha,
+                          accel_mult, courant_mult);
+        dt[p] = d;
+    }
+    wave_min_bits(red + R_DTMIN, d);
+}
+
+__global__ void k_ts_levels(int n, double dt_max, long long level_max,
+                            const double *__restrict__ dt, int *__restrict__ nlevel) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    long long level = level_of(dt_max, dt[p]);
+    level = level < level_max ? level : level_max;
+    level = level > 0 ? level : 0;
+    nlevel[p] = (int)level;
+}
+
+// sph_timesteps.F90:151-170: particles at the end of their step take a new
+// level (down by at most one, and only on a synchronised integer time);
+// counts the updates and the deepest level over all particles.
+__global__ void k_ts_update(int n, long long nint, long long level_step, double dt_max,
+                            const double *__restrict__ ax, const double *__restrict__ ay,
+                            const double *__restrict__ az, const double *__restrict__ h,
+                            const double *__restrict__ divv, const double *__restrict__ sound,
+                            double alpha, double accel_mult, double courant_mult,
+                            const long long *__restrict__ nlast, int *__restrict__ nlevel,
+                            unsigned long long *__restrict__ red) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned upd = 0;
+    int lev = 0;
+    if (p < n) {
+        lev = nlevel[p];
+        if (nint == nlast[p]) {
+            upd = 1;
+            const long long nlastlevel = lev;
+            const double dt = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha,
+                                            2.0 * alpha, accel_mult, courant_mult);
+            long long level = level_of(dt_max, dt);
+            level = level > 0 ? level : 0;
+            if (level < nlastlevel && nlastlevel > 1 &&
+                nint % (1LL << (level_step - (nlastlevel - 1))) == 0)
+                lev = (int)(nlastlevel - 1);
+            else if (level > nlastlevel)
+                lev = (int)level;
+            nlevel[p] = lev;
+        }
+    }
+    unsigned long long u = upd;
+    int lm = lev;
+    for (int o = 32; o > 0; o >>= 1) {
+        u += __shfl_xor(u, o, 64);
+        lm = max(lm, __shfl_xor(lm, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(red + R_PUPDATE, u);
+        atomicMax(red + R_LEVMAX, (unsigned long long)lm);
+    }
+}
+
+// sph_timesteps.F90:261-277: integer times follow a change of level_max
+__global__ void k_rescale_nlast(int n, int shift, long long *__restrict__ nlast) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (shift > 0)
+        nlast[p] = nlast[p] * (1LL << shift);
+    else
+        nlast[p] = nlast[p] / (1LL << (-shift));
+}
+
+// advance_runge_kutta.F90:30-112 (RK) and advance_leapfrog_kdk.F90:32-116
+// (KDK, second-half kick over the time since the half step)
+template <int KDK>
+__global__ void k_advance(int n, long long nint, long long level_step, double timestep,
+                          double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                          double *__restrict__ vx, double *__restrict__ vy,
+                          double *__restrict__ vz, const double *__restrict__ ax,
+                          const double *__restrict__ ay, const double *__restrict__ az,
+                          double *__restrict__ rox, double *__restrict__ roy,
+                          double *__restrict__ roz, double *__restrict__ vox,
+                          double *__restrict__ voy, double *__restrict__ voz,
+                          double *__restrict__ vhx, double *__restrict__ vhy,
+                          double *__restrict__ vhz, long long *__restrict__ nlast,
+                          const int *__restrict__ nlevel, double *__restrict__ laststep,
+                          uint8_t *__restrict__ active, unsigned long long *__restrict__ red) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned act = 0;
+    if (p < n) {
+        const double a[3] = {ax[p], ay[p], az[p]};
+        double vo[3] = {vox[p], voy[p], voz[p]};
+        double vh[3] = {vhx[p], vhy[p], vhz[p]};
+        if (KDK && active[p]) {   // :34-38
+            for (int k = 0; k < 3; k++) vo[k] = vh[k] + 0.5 * a[k] * laststep[p];
+            vox[p] = vo[0]; voy[p] = vo[1]; voz[p] = vo[2];
+        }
+        const double ro[3] = {rox[p], roy[p], roz[p]};
+        const long long dn = nint - nlast[p];
+        const long long nfull = 1LL << (level_step - nlevel[p]);
+        const long long nhalf = nfull / 2;
+        const double dt = timestep * (double)dn;
+        double rp[3], vp[3];
+        if (dn <= nhalf) {
+            for (int k = 0; k < 3; k++) {
+                rp[k] = ro[k] + vo[k] * dt;
+                vp[k] = vo[k] + a[k] * dt;
+            }
+            if (dn == nhalf) {
+                vhx[p] = vp[0]; vhy[p] = vp[1]; vhz[p] = vp[2];
+                if (!KDK) act = 1;
+            }
+        } else if (!KDK) {
+            for (int k = 0; k < 3; k++) {
+                rp[k] = ro[k] + vh[k] * dt;
+                vp[k] = vo[k] + a[k] * dt;
+            }
+        } else {
+            const double dt_half = timestep * (double)(dn - nhalf);
+            for (int k = 0; k < 3; k++) {
+                rp[k] = ro[k] + vh[k] * dt;
+                vp[k] = vh[k] + a[k] * dt_half;
+            }
+        }
+        x[p] = rp[0]; y[p] = rp[1]; z[p] = rp[2];
+        vx[p] = vp[0]; vy[p] = vp[1]; vz[p] = vp[2];
+        if (dn == nfull) {
+            act = 1;
+            nlast[p] = nint;
+            laststep[p] = timestep * (double)nfull;
+            rox[p] = rp[0]; roy[p] = rp[1]; roz[p] = rp[2];
+            vox[p] = vp[0]; voy[p] = vp[1]; voz[p] = vp[2];
+        }
+        active[p] = (uint8_t)act;
+    }
+    unsigned long long u = act;
+    for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(red + R_NACTIVE, u);
+}
+
+__global__ void k_step_init(int n, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const double *__restrict__ vx,
+                            const double *__restrict__ vy, const double *__restrict__ vz,
+                            double *__restrict__ rox, double *__restrict__ roy,
+                            double *__restrict__ roz, double *__restrict__ vox,
+                            double *__restrict__ voy, double *__restrict__ voz,
+                            double *__restrict__ vhx, double *__restrict__ vhy,
+                            double *__restrict__ vhz, long long *__restrict__ nlast,
+                            int *__restrict__ nlevel, double *__restrict__ laststep,
+                            uint8_t *__restrict__ active) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    rox[p] = x[p]; roy[p] = y[p]; roz[p] = z[p];
+    vox[p] = vx[p]; voy[p] = vy[p]; voz[p] = vz[p];
+    vhx[p] = vx[p]; vhy[p] = vy[p]; vhz[p] = vz[p];
+    nlast[p] = 0;
+    nlevel[p] = 0;
+    laststep[p] = 0.0;
+    active[p] = 0;
+}
+
+template <class T>
+__global__ void k_gather_step(int n, const int *__restrict__ perm, const T *__restrict__ src,
+                              T *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+template <class T>
+int dmalloc(T **p, size_t count) {
+    if (hipMalloc((void **)p, sizeof(T) * (count ? count : 1)) != hipSuccess)
+        return set_error(SERENX_ENOMEM, "hipMalloc failed for step state");
+    return SERENX_OK;
+}
+
+// sph_timesteps.F90:30-337 (no sinks, no neighbour-step checks)
+int step_timesteps(Ctx &c) {
+    Ctx::Step &s = c.st;
+    const int n = c.n;
+    hipStream_t st = c.stream;
+    unsigned long long init[4] = {(unsigned long long)__builtin_bit_cast(long long, 9.9e20), 0,
+                                  0, 0};
+    SX_HIP(hipMemcpyAsync(s.red, init, sizeof(init), hipMemcpyHostToDevice, st));
+    const double alpha = c.opt.alpha;
+    if (s.n == s.nresync) {
+        s.n = 0;
+        hipLaunchKernelGGL(k_ts_all, dim3(nblk(n)), dim3(TB), 0, st, n, c.ax, c.ay, c.az, c.h,
+                           c.divv, c.sound, alpha, s.opt.accel_mult, s.opt.courant_mult, s.dt,
+                           s.nlast, s.red);
+        SX_HIP(hipGetLastError());
+        unsigned long long r[4];
+        SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
+        SX_HIP(hipStreamSynchronize(st));
+        const double dt_min = __builtin_bit_cast(double, (long long)r[R_DTMIN]);
+        s.level_max = s.opt.nlevels - 1;
+        s.dt_max = dt_min * std::ldexp(1.0, (int)s.level_max);
+        hipLaunchKernelGGL(k_ts_levels, dim3(nblk(n)), dim3(TB), 0, st, n, s.dt_max,
+                           s.level_max, s.dt, s.nlevel);
+        SX_HIP(hipGetLastError());
+        s.level_step = s.level_max + 1;   // :131-135, RK and KDK take half steps
+        s.nresync = 1LL << s.level_step;
+        s.timestep = s.dt_max / (double)s.nresync;
+    } else {
+        const long long level_old = s.level_max;
+        hipLaunchKernelGGL(k_ts_update, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
+                           s.dt_max, c.ax, c.ay, c.az, c.h, c.divv, c.sound, alpha,
+                           s.opt.accel_mult, s.opt.courant_mult, s.nlast, s.nlevel, s.red);
+        SX_HIP(hipGetLastError());
+        unsigned long long r[4];
+        SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
+        SX_HIP(hipStreamSynchronize(st));
+        if (r[R_PUPDATE] > 0) {
+            long long level_max = (long long)r[R_LEVMAX];
+            if (level_max <= level_old - 1 && level_old > 1 &&
+                s.n % (1LL << (s.level_step - (level_old - 1))) == 0)
+                level_max = level_old - 1;
+            else if (level_max < level_old)
+                level_max = level_old;
+            const int shift = (int)(level_max - level_old);
+            if (shift != 0) {
+                hipLaunchKernelGGL(k_rescale_nlast, dim3(nblk(n)), dim3(TB), 0, st, n, shift,
+                                   s.nlast);
+                SX_HIP(hipGetLastError());
+                if (shift > 0) {
+                    s.n *= 1LL << shift;
+                    s.nresync *= 1LL << shift;
+                } else {
+                    s.n /= 1LL << (-shift);
+                    s.nresync /= 1LL << (-shift);
+                }
+            }
+            s.level_max = level_max;
+            s.level_step = s.level_max + 1;
+            s.timestep = s.dt_max / (double)s.nresync;
+        }
+    }
+    s.n = s.n + 1;
+    s.nsteps = s.nsteps + 1;
+    s.time = s.time + s.timestep;
+    return SERENX_OK;
+}
+
+int step_advance(Ctx &c) {
+    Ctx::Step &s = c.st;
+    const int n = c.n;
+    hipStream_t st = c.stream;
+    SX_HIP(hipMemsetAsync(s.red + R_NACTIVE, 0, sizeof(unsigned long long), st));
+    if (s.opt.integrator == SERENX_INT_LFKDK)
+        hipLaunchKernelGGL(k_advance<1>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
+                           s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
+                           s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz, s.nlast,
+                           s.nlevel, s.laststep, c.active, s.red);
+    else
+        hipLaunchKernelGGL(k_advance<0>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
+                           s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
+                           s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz, s.nlast,
+                           s.nlevel, s.laststep, c.active, s.red);
+    SX_HIP(hipGetLastError());
+    unsigned long long na = 0;
+    SX_HIP(hipMemcpyAsync(&na, s.red + R_NACTIVE, sizeof(na), hipMemcpyDeviceToHost, st));
+    SX_HIP(hipStreamSynchronize(st));
+    s.nactive = (long long)na;
+    return SERENX_OK;
+}
+
+// sph_integrate.F90:41-113 for one step
+int step_once(Ctx &c) {
+    Ctx::Step &s = c.st;
+    // tree_update.F90:24-88
+    const bool build = s.nbuild == s.nsteps;
+    if (build) {
+        SX_TRY(build_tree(c));   // build + reorder, permutes the step state too
+        s.nstock = s.nsteps;
+    }
+    if (s.nstock == s.nsteps) SX_TRY(stock_tree(c));
+    if (build) s.nbuild = s.nbuild + s.opt.nbuildstep;
+    if (s.nstock == s.nsteps) s.nstock = s.nstock + (s.opt.integrator == SERENX_INT_LFKDK ? 2 : 1);
+    // sph_update, thermal, forces for the active particles
+    SX_TRY(run_sph_update(c));
+    SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    SX_TRY(run_hydro(c));
+    SX_TRY(run_grav(c));
+    SX_TRY(step_timesteps(c));
+    SX_TRY(step_advance(c));
+    return SERENX_OK;
+}
+
+}  // namespace
+
+void step_free(Ctx &c) {
+    Ctx::Step &s = c.st;
+    void *ptrs[] = {s.rox, s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz,
+                    s.laststep, s.dt, s.nlast, s.nlevel, s.red};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    s = Ctx::Step{};
+}
+
+int permute_step(Ctx &c, const int *perm) {
+    Ctx::Step &s = c.st;
+    if (!s.on) return SERENX_OK;
+    const int n = c.n;
+    double **arrs[] = {&s.rox, &s.roy, &s.roz, &s.vox, &s.voy, &s.voz, &s.vhx, &s.vhy, &s.vhz,
+                       &s.laststep};
+    for (double **a : arrs) {
+        hipLaunchKernelGGL(k_gather_step<double>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                           (const double *)*a, s.dt);
+        std::swap(*a, s.dt);
+    }
+    // the integer arrays go through the double scratch (same element size
+    // for nlast; nlevel uses its lower half)
+    long long *tl = reinterpret_cast<long long *>(s.dt);
+    hipLaunchKernelGGL(k_gather_step<long long>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                       (const long long *)s.nlast, tl);
+    s.dt = reinterpret_cast<double *>(s.nlast);
+    s.nlast = tl;
+    int *ti = reinterpret_cast<int *>(s.dt);
+    hipLaunchKernelGGL(k_gather_step<int>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                       (const int *)s.nlevel, ti);
+    // nlevel's old buffer (n ints) is too small to serve as the double
+    // scratch, so copy back rather than swap
+    SX_HIP(hipMemcpyAsync(s.nlevel, ti, sizeof(int) * n, hipMemcpyDeviceToDevice, c.stream));
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int step_begin(Ctx &c, const serenx_step_options &opt) {
+    if (opt.integrator != SERENX_INT_RK && opt.integrator != SERENX_INT_LFKDK)
+        return set_error(SERENX_EINVAL, "integrator must be SERENX_INT_RK or SERENX_INT_LFKDK");
+    if (opt.nlevels < 1 || opt.nlevels > 40)
+        return set_error(SERENX_EINVAL, "nlevels must be in 1..40");
+    if (opt.nbuildstep < 1) return set_error(SERENX_EINVAL, "nbuildstep must be positive");
+    if (!(opt.accel_mult > 0.0) || !(opt.courant_mult > 0.0))
+        return set_error(SERENX_EINVAL, "accel_mult and courant_mult must be positive");
+    if (!c.tree_ok || !c.sph_ok)
+        return set_error(SERENX_ESTATE, "step_begin needs the setup force evaluation first");
+    step_free(c);
+    Ctx::Step &s = c.st;
+    const size_t n = (size_t)c.cap + 1;
+    double **arrs[] = {&s.rox, &s.roy, &s.roz, &s.vox, &s.voy, &s.voz, &s.vhx, &s.vhy, &s.vhz,
+                       &s.laststep, &s.dt};
+    for (double **a : arrs) SX_TRY(dmalloc(a, n));
+    SX_TRY(dmalloc(&s.nlast, n));
+    SX_TRY(dmalloc(&s.nlevel, n));
+    SX_TRY(dmalloc(&s.red, 4));
+    s.opt = opt;
+    s.on = true;
+    hipLaunchKernelGGL(k_step_init, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.vx, c.vy, c.vz, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy,
+                       s.vhz, s.nlast, s.nlevel, s.laststep, c.active);
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipStreamSynchronize(c.stream));
+    // initialize_seren_variables_2.F90:60-61, initialize_sph_variables_2.F90:165-175
+    s.n = 0;
+    s.nresync = s.n;
+    s.nsteps = 0;
+    s.time = 0.0;
+    s.timestep = 0.0;
+    s.dt_max = 0.0;
+    s.level_max = 0;
+    s.level_step = 0;
+    s.nactive = 0;
+    const long long ahead = (opt.integrator == SERENX_INT_LFKDK) ? 2 : 1;
+    s.nbuild = s.nsteps + ahead;
+    s.nstock = s.nsteps + ahead;
+    return SERENX_OK;
+}
+
+int step_run(Ctx &c, int count) {
+    if (!c.st.on) return set_error(SERENX_ESTATE, "serenx_step before serenx_step_begin");
+    for (int k = 0; k < count; k++) SX_TRY(step_once(c));
+    return SERENX_OK;
+}
+
+int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_t *active,
+               double *r_old, double *v_old, double *v_half) {
+    Ctx::Step &s = c.st;
+    if (!s.on) return set_error(SERENX_ESTATE, "no step state (serenx_step_begin)");
+    const int n = c.n;
+    SX_HIP(hipStreamSynchronize(c.stream));
+    if (nlevel) SX_HIP(hipMemcpy(nlevel, s.nlevel, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (nlast) SX_HIP(hipMemcpy(nlast, s.nlast, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    if (laststep)
+        SX_HIP(hipMemcpy(laststep, s.laststep, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (active) {
+        std::vector<uint8_t> a(n);
+        SX_HIP(hipMemcpy(a.data(), c.active, n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < n; p++) active[p] = a[p];
+    }
+    double *dst[3] = {r_old, v_old, v_half};
+    double *srcs[3][3] = {{s.rox, s.roy, s.roz}, {s.vox, s.voy, s.voz}, {s.vhx, s.vhy, s.vhz}};
+    std::vector<double> t(n);
+    for (int v = 0; v < 3; v++) {
+        if (!dst[v]) continue;
+        for (int k = 0; k < 3; k++) {
+            SX_HIP(hipMemcpy(t.data(), srcs[v][k], sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int p = 0; p < n; p++) dst[v][3 * (size_t)p + k] = t[p];
+        }
+    }
+    return SERENX_OK;
+}
+
+}  // namespace serenx

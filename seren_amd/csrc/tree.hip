@@ -588,7 +588,7 @@ int permute_particles(Ctx &c, const int *perm) {
                        (const uint8_t *)c.active, c.pact);
     std::swap(c.active, c.pact);
     SX_HIP(hipGetLastError());
-    return SERENX_OK;
+    return permute_step(c, perm);
 }
 
 int build_tree(Ctx &c) {

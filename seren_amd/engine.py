@@ -45,7 +45,11 @@ EXPORTED = [
     "serenx_load", "serenx_set_agravmag", "serenx_force_eval", "serenx_sync",
     "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
     "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
+    "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
+    "serenx_fetch_step",
 ]
+INT_RK, INT_LFKDK = 0, 1
+_INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
 
 
 class SerenxError(RuntimeError):
@@ -67,6 +71,18 @@ class HostState(C.Structure):
                 ("a", _dp), ("gpot", _dp), ("agravmag", _dp), ("rho", _dp), ("omega", _dp),
                 ("press", _dp), ("sound", _dp), ("div_v", _dp), ("drhodt", _dp),
                 ("pptot", _ip), ("pplist", _ip), ("accdo", _ip), ("rextent", _dp)]
+
+
+class StepOptions(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("nlevels", C.c_int32), ("nbuildstep", C.c_int32),
+                ("reserved", C.c_int32), ("accel_mult", C.c_double),
+                ("courant_mult", C.c_double)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("n", "nsteps", "nresync", "level_max", "level_step",
+                                         "nbuild", "nstock", "nactive")] + \
+               [(k, C.c_double) for k in ("time", "timestep", "dt_max", "reserved")]
 
 
 _lib: Optional[C.CDLL] = None
@@ -101,6 +117,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_last_timings.argtypes = [vp, _dp]
     lib.serenx_interaction_counts.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.serenx_walk_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.serenx_default_step_options.argtypes = [C.POINTER(StepOptions)]
+    lib.serenx_default_step_options.restype = None
+    lib.serenx_step_begin.argtypes = [vp, C.POINTER(StepOptions)]
+    lib.serenx_step.argtypes = [vp, C.c_int32]
+    lib.serenx_step_get_info.argtypes = [vp, C.POINTER(StepInfo)]
+    lib.serenx_fetch_step.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _ip, _dp, _dp, _dp]
     _lib = lib
     return lib
 
@@ -248,6 +270,40 @@ class ForceEngine:
                 "neib_util": util(v[8], v[9]), "neib_wave_visits": v[8],
                 "hsolve_list_walks_per_wave": v[10] / max(v[7], 1),
                 "hsolve_overflow_passes": v[11], "hsolve_longest_list_leaves": v[12]}
+
+    # ---- device-resident integration (sph_integrate.F90) ----------------
+    def step_begin(self, integrator: str = "rk", nlevels: int = 1, nbuildstep: int = 8,
+                   accel_mult: float = 0.5, courant_mult: float = 0.2) -> None:
+        """Start stepping after the setup evaluation (force_eval(hguess=True))."""
+        o = StepOptions()
+        self.lib.serenx_default_step_options(C.byref(o))
+        o.integrator = _INTEGRATOR[integrator]
+        o.nlevels, o.nbuildstep = nlevels, nbuildstep
+        o.accel_mult, o.courant_mult = accel_mult, courant_mult
+        _check(self.lib.serenx_step_begin(self.ctx, C.byref(o)))
+
+    def step(self, count: int = 1) -> None:
+        """count x sph_integrate on the device (synchronous)."""
+        _check(self.lib.serenx_step(self.ctx, count))
+
+    def step_info(self) -> Dict[str, float]:
+        i = StepInfo()
+        _check(self.lib.serenx_step_get_info(self.ctx, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in StepInfo._fields_ if k != "reserved"}
+
+    def fetch_step(self) -> Dict[str, np.ndarray]:
+        """Per-particle step state in tree order (see fetch()['porig'])."""
+        n = self.n
+        nlevel = np.zeros(n, np.int32)
+        nlast = np.zeros(n, np.int64)
+        laststep = np.zeros(n)
+        active = np.zeros(n, np.int32)
+        r_old, v_old, v_half = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        _check(self.lib.serenx_fetch_step(
+            self.ctx, _iptr(nlevel), nlast.ctypes.data_as(C.POINTER(C.c_int64)),
+            _dptr(laststep), _iptr(active), _dptr(r_old), _dptr(v_old), _dptr(v_half)))
+        return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=active,
+                    r_old=r_old, v_old=v_old, v_half=v_half)
 
     def interaction_counts(self):
         """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""

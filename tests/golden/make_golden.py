@@ -97,9 +97,53 @@ def run_case(name, spec):
     print(name, "ctot", par["ctot_grav"], "ltot", par["ltot_grav"], "pptot max", pptot.max())
 
 
+# Integration steps (sph_integrate.F90 with RUNGE_KUTTA) after the setup
+# evaluation: name: (n, profile, seed, mach, octupole, mac, nlevels, steps)
+STEP_CASES = {
+    "step_rk_uniform2k_quad_geo_nlev3": (2000, "uniform", 11, 1.0, False, "geometric", 3, 6),
+    "step_rk_r2_2k_oct_gadget2_nlev1": (2048, "r2", 12, 2.0, True, "gadget2", 1, 4),
+}
+STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
+
+
+def step_case_data(spec):
+    """Runs one step case on the reference and returns the fixture arrays."""
+    n, prof, seed, mach, octu, mac, nlevels, nsteps = spec
+    pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
+    ref = RefSeren(octu, mac)
+    ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+    ref.force_eval(CS, hguess=True)
+    ref.step_setup(nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
+    ints, reals, nact = [], [], []
+    for _ in range(nsteps):
+        ref.integrate(CS)
+        info = ref.step_info()
+        ints.append([info[k] for k in STEP_KEYS])
+        reals.append([info["time"], info["timestep"], info["dt_max"]])
+        nact.append(int(ref.step_particles()["active"].sum()))
+    p = ref.particles()
+    sp = ref.step_particles()
+    return dict(
+        spec=np.array([n, seed, int(octu), nlevels, nsteps], np.int64), profile=np.array(prof),
+        mac=np.array(mac), mach=np.array(mach), pos0=pos, vel0=vel, mass=mass,
+        step_ints=np.array(ints, np.int64), step_reals=np.array(reals),
+        step_nactive=np.array(nact, np.int64),
+        porig=p["porig"], pos=p["pos"], v=p["v"], a=p["a"], h=p["h"], rho=p["rho"],
+        nlevel=sp["nlevel"], nlast=sp["nlast"], laststep=sp["laststep"], active=sp["active"],
+        r_old=sp["r_old"], v_old=sp["v_old"], v_half=sp["v_half"])
+
+
+def run_step_case(name, spec):
+    d = step_case_data(spec)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+    print(name, "levels", np.bincount(d["nlevel"]), "time", d["step_reals"][-1][0])
+
+
 def main():
     for name, spec in CASES.items():
         run_case(name, spec)
+    for name, spec in STEP_CASES.items():
+        run_step_case(name, spec)
     # kernel tables (kernel.F90): full, they pin the tabulated M4 functions
     ref = RefSeren(False, "geometric")
     pos, vel, mass, h = ics.make_cloud(100, "uniform", seed=9, mach=0.0)

@@ -50,3 +50,16 @@ def test_oracle_bit_exact_with_reference(case):
     assert np.array_equal(np.where(ok, nr[1], 0), np.where(ok, nc[1], 0))
     for k in ("leaf", "nextcell", "m", "r", "q", "s", "dminsqd", "mac"):
         assert np.array_equal(tr[k], tc[k]), k
+
+
+def test_step_golden_reproduced_by_reference():
+    """The committed integration-step fixtures are what the reference's own
+    sph_timesteps / sph_advance (RUNGE_KUTTA) produce, bit for bit."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_golden
+    for name, spec in make_golden.STEP_CASES.items():
+        g = np.load(os.path.join(HERE, "golden", name + ".npz"))
+        d = make_golden.step_case_data(spec)
+        for key in g.files:
+            assert np.array_equal(np.asarray(d[key]), g[key]), (name, key)
