@@ -50,7 +50,12 @@ enum {
 /* MULTIPOLE option (Makefile: MULTIPOLE = QUADRUPOLE | OCTUPOLE) */
 enum { SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3 };
 /* MAC option (Makefile: MAC = GEOMETRIC | GADGET | GADGET2) */
-enum { SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, SERENX_MAC_GADGET2 = 2 };
+enum {
+    SERENX_MAC_GEOMETRIC = 0, /* GEOMETRIC_MAC: d^2 > diag^2 / theta^2 */
+    SERENX_MAC_GADGET = 1,    /* GADGET_MAC (BHgrav_stock.F90:329-331) */
+    SERENX_MAC_GADGET2 = 2,   /* GADGET2_MAC (:332-334) */
+    SERENX_MAC_EIGEN = 3      /* EIGEN_MAC (:335-337, eigenvalue_mac.F90), |phi| based */
+};
 /* arithmetic of the gravity walk: FP64 throughout, or FP32 terms on
  * cell-relative coordinates with compensated FP32 accumulation */
 enum { SERENX_FP64 = 0, SERENX_MIXED = 1 };

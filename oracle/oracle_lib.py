@@ -39,6 +39,8 @@ VARIANTS = {
     # (octupole, mac) -> reference library variant
     (False, "geometric"): "quad_geo",
     (True, "gadget2"): "oct_gadget2",
+    (True, "gadget"): "oct_gadget",
+    (True, "eigen"): "oct_eigen",
 }
 
 
@@ -297,7 +299,7 @@ class CSeren(_Common):
     def __init__(self, octupole: bool, mac: str):
         self.lib = self._load()
         self.octupole = bool(octupole)
-        self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2}[mac]
+        self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2, "eigen": 3}[mac]
         self.ctx = None
 
     def __del__(self):

@@ -9,7 +9,8 @@
  * order (compiled with -ffp-contract=off so no FMA contraction happens), the
  * option set GRAD_H_SPH + M4 kernel + AB viscosity + isothermal + KS gravity +
  * BH tree + REORDER_PARTICLES + NEIGHBOUR_LISTS, NDIM=3, DOUBLE precision,
- * with QUADRUPOLE or OCTUPOLE moments and GEOMETRIC or GADGET2 MAC.
+ * with QUADRUPOLE or OCTUPOLE moments and the GEOMETRIC, GADGET, GADGET2 or
+ * EIGEN MAC.
  *
  * Arrays are 1-based like the Fortran module arrays they mirror: index 0 of
  * every particle array is unused; tree cell ids start at 0 (root).
@@ -50,8 +51,9 @@
 #define PI_ 3.1415926536
 #define INVPI 0.318309886
 #define ONETHIRD 0.333333333
+#define TWOTHIRDS_DP 0.666666666666666666
 
-enum { MAC_GEOMETRIC = 0, MAC_GADGET = 1, MAC_GADGET2 = 2 };
+enum { MAC_GEOMETRIC = 0, MAC_GADGET = 1, MAC_GADGET2 = 2, MAC_EIGEN = 3 };
 
 typedef struct {
     int leaf, plist[LEAFMAX], nextcell, ifopen;
@@ -565,9 +567,21 @@ static void BHgrav_stock(oracle_ctx *c) {
             } else if (c->mac == MAC_GADGET) {
                 g->dminsqd = diagonalsqd;
                 g->mac = pow(mc * diagonalsqd * diagonalsqd / c->abserror, ONETHIRD);
-            } else {
+            } else if (c->mac == MAC_GADGET2) {
                 g->dminsqd = diagonalsqd;
                 g->mac = sqrt(mc * diagonalsqd / c->abserror);
+            } else {
+                /* eigenvalue_mac.F90:12-33 */
+                const double *q = g->q;
+                double pq = q[0] * q[2] - (q[0] + q[2]) * (q[0] + q[2]) - q[1] * q[1] -
+                            q[3] * q[3] - q[4] * q[4];
+                g->dminsqd = diagonalsqd;
+                if (pq >= 0.0) {
+                    g->mac = 0.0;
+                } else {
+                    double lambda = 2.0 * sqrt(-pq / 3.0);
+                    g->mac = pow(0.5 * lambda / c->abserror, TWOTHIRDS_DP);
+                }
             }
             g->dminsqd = fmax(g->dminsqd, rsize * rsize);
             if (g->dminsqd < daux) g->dminsqd = daux;
@@ -1067,8 +1081,10 @@ static void BHgrav_accel(const oracle_ctx *c, int p, double invhp, const double 
         if (afactor > SMALL_NUMBER) {
             if (c->mac == MAC_GADGET)
                 afactor = pow(afactor, -ONETHIRD);
-            else
+            else if (c->mac == MAC_GADGET2)
                 afactor = 1.0 / sqrt(afactor);
+            else /* EIGEN */
+                afactor = pow(1.0 / c->gpot[p], 2.0 * ONETHIRD);
         } else {
             afactor = 0.0;
         }

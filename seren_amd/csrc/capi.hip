@@ -168,8 +168,8 @@ void free_all(Ctx &c) {
 int check_options(const serenx_options &o) {
     if (o.multipole != SERENX_QUADRUPOLE && o.multipole != SERENX_OCTUPOLE)
         return set_error(SERENX_EINVAL, "multipole must be SERENX_QUADRUPOLE or SERENX_OCTUPOLE");
-    if (o.mac < SERENX_MAC_GEOMETRIC || o.mac > SERENX_MAC_GADGET2)
-        return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET or GADGET2");
+    if (o.mac < SERENX_MAC_GEOMETRIC || o.mac > SERENX_MAC_EIGEN)
+        return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET, GADGET2 or EIGEN");
     if (o.precision != SERENX_FP64 && o.precision != SERENX_MIXED)
         return set_error(SERENX_EINVAL, "precision must be SERENX_FP64 or SERENX_MIXED");
     if (!(o.h_fac > 0.0)) return set_error(SERENX_EINVAL, "h_fac must be positive");
@@ -514,6 +514,8 @@ int serenx_grav_forces(serenx_ctx *ctx) {
     SX_TRY(up_parray(c, cols, 6));
     SX_TRY(up_vec3(c, c.hs.a, c.ax, c.ay, c.az));
     if (c.opt.mac != SERENX_MAC_GEOMETRIC && c.hs.agravmag) SX_TRY(up_scalar(c, c.hs.agravmag, c.agm));
+    // the EIGEN MAC reads the previous potential (BHgrav_accel.F90:109-110)
+    if (c.opt.mac == SERENX_MAC_EIGEN && c.hs.gpot) SX_TRY(up_scalar(c, c.hs.gpot, c.gpot));
     SX_TRY(up_accdo(c));
     SX_TRY(run_grav(c));
     SX_HIP(hipStreamSynchronize(c.stream));

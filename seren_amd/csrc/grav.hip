@@ -25,6 +25,17 @@ namespace {
 constexpr int TB = 256;
 inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
 
+// BHgrav_accel.F90:97-116: the walk's MAC factor from the previous
+// evaluation's |a| (GADGET, GADGET2) or potential (EIGEN); zero while |a| is
+// not yet known
+template <int MACT>
+__device__ __forceinline__ double mac_afactor(double agravmag, double gpot) {
+    if (!(agravmag > SMALL_NUMBER)) return 0.0;
+    if (MACT == SERENX_MAC_GADGET) return pow(agravmag, -ONETHIRD);
+    if (MACT == SERENX_MAC_GADGET2) return 1.0 / sqrt(agravmag);
+    return pow(1.0 / gpot, 2.0 * ONETHIRD);
+}
+
 // 1/d for d > 0 to a few ulp: the hardware estimate refined by two Newton
 // steps y <- y + y (1 - d y) (each squares the relative error).
 __device__ __forceinline__ double rcp_fp64(double d) {
@@ -177,11 +188,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
         rpx = a.x; rpy = a.y; rpz = a.z;
         invhp = b.y;
         zo_p = b.z;
-        if (MACT != SERENX_MAC_GEOMETRIC) {   // BHgrav_accel.F90:97-116
-            const double am = agm[p];
-            if (am > SMALL_NUMBER)
-                afactor = (MACT == SERENX_MAC_GADGET) ? pow(am, -ONETHIRD) : 1.0 / sqrt(am);
-        }
+        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
     }
     // one accumulator for cell and pair terms (the reference adds the pairs
     // after the cells, :272-290; the order changes only the rounding)
@@ -314,11 +321,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         rpx = a.x; rpy = a.y; rpz = a.z;
         invhp = (float)b.y;
         zo_p = (float)b.z;
-        if (MACT != SERENX_MAC_GEOMETRIC) {
-            const double am = agm[p];
-            if (am > SMALL_NUMBER)
-                afactor = (MACT == SERENX_MAC_GADGET) ? pow(am, -ONETHIRD) : 1.0 / sqrt(am);
-        }
+        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
     }
     double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
     CSum px, py, pz, pq;                              // pair terms
@@ -449,6 +452,9 @@ int grav_forces(Ctx &c) {
         break;
     case SERENX_MAC_GADGET:
         oct ? launch_grav<1, SERENX_MAC_GADGET>(c) : launch_grav<0, SERENX_MAC_GADGET>(c);
+        break;
+    case SERENX_MAC_EIGEN:
+        oct ? launch_grav<1, SERENX_MAC_EIGEN>(c) : launch_grav<0, SERENX_MAC_EIGEN>(c);
         break;
     default:
         oct ? launch_grav<1, SERENX_MAC_GADGET2>(c) : launch_grav<0, SERENX_MAC_GADGET2>(c);

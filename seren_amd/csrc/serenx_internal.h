@@ -29,6 +29,7 @@ constexpr double H_CON = 0.02;
 constexpr int GRADH_ITERATION_MAX = 20;
 constexpr double ETA_SQD = 0.01;
 constexpr double ONETHIRD = 0.333333333;
+constexpr double TWOTHIRDS_DP = 0.666666666666666666;
 constexpr double PI_REF = 3.1415926536;
 constexpr double INVPI = 0.318309886;
 constexpr int NTAB = 9;            // w0 w1 w2 w3 w4 w5 w6 wh wg

@@ -422,9 +422,20 @@ __global__ void __launch_bounds__(256) k_stock(
     } else if (mac == SERENX_MAC_GADGET) {
         dminsqd = diagonalsqd;
         macv = pow(mc * diagonalsqd * diagonalsqd / abserror, ONETHIRD);
-    } else {
+    } else if (mac == SERENX_MAC_GADGET2) {
         dminsqd = diagonalsqd;
         macv = sqrt(mc * diagonalsqd / abserror);
+    } else {
+        // eigenvalue_mac.F90: largest eigenvalue of the traceless quadrupole
+        dminsqd = diagonalsqd;
+        const double pq = qc[0] * qc[2] - (qc[0] + qc[2]) * (qc[0] + qc[2]) - qc[1] * qc[1] -
+                          qc[3] * qc[3] - qc[4] * qc[4];
+        if (pq >= 0.0) {
+            macv = 0.0;
+        } else {
+            const double lambda = 2.0 * sqrt(-pq / 3.0);
+            macv = pow(0.5 * lambda / abserror, TWOTHIRDS_DP);
+        }
     }
     dminsqd = fmax(dminsqd, rsize * rsize);
     if (dminsqd < daux) dminsqd = daux;

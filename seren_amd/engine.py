@@ -29,12 +29,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libserenx.so")
 
 QUADRUPOLE, OCTUPOLE = 2, 3
-MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2 = 0, 1, 2
+MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2, MAC_EIGEN = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 EVAL_HGUESS, EVAL_NO_GRAV, EVAL_NO_HYDRO = 1, 2, 4
 
 _MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE}
-_MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2}
+_MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2,
+        "eigen": MAC_EIGEN}
 _PREC = {"fp64": FP64, "mixed": MIXED}
 
 EXPORTED = [

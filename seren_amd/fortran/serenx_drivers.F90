@@ -37,6 +37,8 @@ CONTAINS
        opt%mac = SERENX_MAC_GADGET2
 #elif defined(GADGET_MAC)
        opt%mac = SERENX_MAC_GADGET
+#elif defined(EIGEN_MAC)
+       opt%mac = SERENX_MAC_EIGEN
 #else
        opt%mac = SERENX_MAC_GEOMETRIC
 #endif

@@ -9,7 +9,7 @@ MODULE serenx_f
   integer(c_int32_t), parameter :: SERENX_OK = 0
   integer(c_int32_t), parameter :: SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3
   integer(c_int32_t), parameter :: SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, &
-       & SERENX_MAC_GADGET2 = 2
+       & SERENX_MAC_GADGET2 = 2, SERENX_MAC_EIGEN = 3
   integer(c_int32_t), parameter :: SERENX_FP64 = 0, SERENX_MIXED = 1
 
   type, bind(C) :: serenx_options

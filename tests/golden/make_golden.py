@@ -31,6 +31,8 @@ CASES = {
     "uniform1k_quad_geo": (1000, "uniform", 101, 1.0, False, "geometric", 0.25, 1e-4, 1, True),
     "plummer2k_quad_geo": (2048, "plummer", 202, 0.0, False, "geometric", 0.25, 1e-4, 1, True),
     "r2_2k_oct_gadget2_x2": (2048, "r2", 303, 2.0, True, "gadget2", 0.25, 1e-4, 2, True),
+    "r2_2k_oct_gadget_x2": (2048, "r2", 304, 2.0, True, "gadget", 0.25, 1e-4, 2, True),
+    "plummer2k_oct_eigen_x2": (2048, "plummer", 305, 1.0, True, "eigen", 0.25, 1e-4, 2, True),
     # configuration 1 of the benchmark suite: N = 16384 uniform sphere, quadrupole,
     # geometric MAC theta = 0.5, one force evaluation (digests for big arrays)
     "c1_uniform16k_quad_geo": (16384, "uniform", 1, 0.0, False, "geometric", 0.25, 1e-4, 1,

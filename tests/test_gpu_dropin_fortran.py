@@ -25,7 +25,8 @@ GOLD = os.path.join(HERE, "golden")
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF_DIR, "libseren_dropin_quad_geo.so")),
                     reason="drop-in host not built (make -f oracle/Makefile.ref dropin)")
 @pytest.mark.parametrize("case", ["uniform1k_quad_geo", "plummer2k_quad_geo",
-                                  "r2_2k_oct_gadget2_x2"])
+                                  "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
+                                  "plummer2k_oct_eigen_x2"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))

@@ -20,6 +20,8 @@ CASES = [
     (2500, "plummer", 12, False, "geometric", 1),
     (3000, "r2", 13, True, "gadget2", 2),
     (700, "uniform", 14, True, "gadget2", 3),
+    (2000, "r2", 15, True, "gadget", 2),
+    (2000, "plummer", 16, True, "eigen", 2),
 ]
 
 
