@@ -67,11 +67,16 @@ enum {
     SERENX_EVAL_NO_HYDRO = 4  /* skip the hydro pair forces */
 };
 
+/* kernel tables (kernel.F90:14-127): M4 cubic spline, or M4 with the
+ * Thomas-Couchman (1992) gradient in the hydro force (TC_KERNEL: w1 = -1/pi
+ * for s < 2/3) */
+enum { SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1 };
+
 typedef struct serenx_options {
     int32_t multipole;   /* SERENX_QUADRUPOLE or SERENX_OCTUPOLE */
     int32_t mac;         /* SERENX_MAC_* */
     int32_t precision;   /* SERENX_FP64 or SERENX_MIXED */
-    int32_t reserved;
+    int32_t kernel;      /* SERENX_KERNEL_* */
     double h_fac;        /* grad-h h = h_fac (m/rho)^(1/3); params 'h_fac' (1.2) */
     double thetamaxsqd;  /* geometric MAC theta^2; params 'thetamaxsqd' */
     double abserror;     /* multipole MAC tolerance; params 'abserror' */

@@ -36,7 +36,8 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 VARIANTS = {
-    # (octupole, mac) -> reference library variant
+    # (octupole, mac[, kernel]) -> reference library variant
+    (False, "geometric", "m4tc"): "quad_geo_tc",
     (False, "geometric"): "quad_geo",
     (True, "gadget2"): "oct_gadget2",
     (True, "gadget"): "oct_gadget",
@@ -143,11 +144,12 @@ class RefSeren(_Common):
 
     _libs: Dict[str, C.CDLL] = {}
 
-    def __init__(self, octupole: bool, mac: str, dropin: bool = False):
+    def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4"):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
         oracle/Makefile.ref dropin): the reference code as the host."""
-        variant = VARIANTS[(bool(octupole), mac)]
+        key = (bool(octupole), mac) if kernel == "m4" else (bool(octupole), mac, kernel)
+        variant = VARIANTS[key]
         kind = "dropin" if dropin else "ref"
         path = os.path.join(HERE, "_ref", f"libseren_{kind}_{variant}.so")
         if not os.path.exists(path):
@@ -275,7 +277,8 @@ class CSeren(_Common):
             vp = C.c_void_p
             lib.oracle_create.restype = vp
             lib.oracle_create.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
-                                          C.c_double, C.c_double, C.c_double, C.c_double]
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_int]
             lib.oracle_destroy.argtypes = [vp]
             lib.oracle_error.argtypes = [vp]
             for f in ("oracle_hguess", "oracle_sph_update", "oracle_hydro_forces",
@@ -296,10 +299,11 @@ class CSeren(_Common):
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, octupole: bool, mac: str):
+    def __init__(self, octupole: bool, mac: str, kernel: str = "m4"):
         self.lib = self._load()
         self.octupole = bool(octupole)
         self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2, "eigen": 3}[mac]
+        self.tc = {"m4": 0, "m4tc": 1}[kernel]
         self.ctx = None
 
     def __del__(self):
@@ -314,7 +318,8 @@ class CSeren(_Common):
         pos, mass, h, vel = _f64(pos), _f64(mass), _f64(h), _f64(vel)
         self.n = len(mass)
         self.ctx = self.lib.oracle_create(self.n, pos, mass, h, vel, int(self.octupole),
-                                          self.mac, h_fac, thetamaxsqd, abserror, alpha)
+                                          self.mac, h_fac, thetamaxsqd, abserror, alpha,
+                                          self.tc)
         self.pp_limit = self.params()["pp_limit"]
 
     def tree_update(self):

@@ -9,8 +9,8 @@
  * order (compiled with -ffp-contract=off so no FMA contraction happens), the
  * option set GRAD_H_SPH + M4 kernel + AB viscosity + isothermal + KS gravity +
  * BH tree + REORDER_PARTICLES + NEIGHBOUR_LISTS, NDIM=3, DOUBLE precision,
- * with QUADRUPOLE or OCTUPOLE moments and the GEOMETRIC, GADGET, GADGET2 or
- * EIGEN MAC.
+ * with QUADRUPOLE or OCTUPOLE moments, the GEOMETRIC, GADGET, GADGET2 or
+ * EIGEN MAC, and the M4 or M4 Thomas-Couchman (TC_KERNEL) kernel.
  *
  * Arrays are 1-based like the Fortran module arrays they mirror: index 0 of
  * every particle array is unused; tree cell ids start at 0 (root).
@@ -72,7 +72,7 @@ typedef struct {
 } snode;
 
 typedef struct oracle_ctx {
-    int n, pp_gather, pp_limit, octupole, mac;
+    int n, pp_gather, pp_limit, octupole, mac, tc_kernel;
     double h_fac, thetamaxsqd, abserror, alpha, beta, rextent;
     int *porig;
     double (*parray)[6]; /* x y z m h zeta */
@@ -105,12 +105,16 @@ static double ipow(double x, int n) {
 static void oracle_kernel(oracle_ctx *c) {
     const double invkerntot = 1.0 / (double)KERNTOT;
     const int kern1 = (int)HALFKERNTOT;
+    const int kerntc = (int)((double)KERNTOT / 3.0); /* kernel.F90:37 */
     const double ndim = 3.0;
     for (int i = 0; i <= KERNTOT; i++) {
         double s = (KERNRANGE * (double)i + 1.0) * invkerntot;
         double saux = KERNRANGE - s;
         if (i < kern1) {
-            c->w1[i] = -(3.0 * s) + 2.25 * s * s;
+            if (c->tc_kernel && i < kerntc) /* TC_KERNEL, kernel.F90:56-62 */
+                c->w1[i] = -1.0;
+            else
+                c->w1[i] = -(3.0 * s) + 2.25 * s * s;
             c->w0[i] = 1.0 - 1.5 * s * s + 0.75 * ipow(s, 3);
             c->w2[i] = (4.0 / 3.0) * ipow(s, 3) - 1.2 * ipow(s, 5) + 0.5 * ipow(s, 6);
             c->w3[i] = (7.0 / 5.0) * s - (2.0 / 3.0) * ipow(s, 3) + 0.3 * ipow(s, 5) -
@@ -166,9 +170,10 @@ void oracle_destroy(oracle_ctx *c) {
 oracle_ctx *oracle_create(int n, const double *pos, const double *mass,
                           const double *h, const double *vel, int octupole,
                           int mac, double h_fac, double thetamaxsqd,
-                          double abserror, double alpha) {
+                          double abserror, double alpha, int tc_kernel) {
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     c->n = n;
+    c->tc_kernel = tc_kernel;
     c->octupole = octupole;
     c->mac = mac;
     c->h_fac = h_fac;

@@ -32,9 +32,10 @@ static double ipow_seq(double x, int n) {
 }
 
 // kernel.F90:14-127 (M4, GRAD_H_SPH, NDIM = 3); tab[j*KT + i]
-void make_kernel_tables(double *tab) {
+void make_kernel_tables(double *tab, int kernel) {
     const double invkerntot = 1.0 / (double)KERNTOT;
     const int kern1 = (int)HALFKERNTOT;
+    const int kerntc = (int)((double)KERNTOT / 3.0);   // kernel.F90:37
     const double nd = 3.0;
     double *w0 = tab + T_W0 * KT, *w1 = tab + T_W1 * KT, *w2 = tab + T_W2 * KT,
            *w3 = tab + T_W3 * KT, *w4 = tab + T_W4 * KT, *w5 = tab + T_W5 * KT,
@@ -43,7 +44,10 @@ void make_kernel_tables(double *tab) {
         const double s = (KERNRANGE * (double)i + 1.0) * invkerntot;
         const double saux = KERNRANGE - s;
         if (i < kern1) {
-            w1[i] = -(3.0 * s) + 2.25 * s * s;
+            if (kernel == SERENX_KERNEL_M4_TC && i < kerntc)
+                w1[i] = -1.0;   // Thomas-Couchman gradient (kernel.F90:56-62)
+            else
+                w1[i] = -(3.0 * s) + 2.25 * s * s;
             w0[i] = 1.0 - 1.5 * s * s + 0.75 * ipow_seq(s, 3);
             w2[i] = (4.0 / 3.0) * ipow_seq(s, 3) - 1.2 * ipow_seq(s, 5) + 0.5 * ipow_seq(s, 6);
             w3[i] = (7.0 / 5.0) * s - (2.0 / 3.0) * ipow_seq(s, 3) + 0.3 * ipow_seq(s, 5) -
@@ -172,6 +176,8 @@ int check_options(const serenx_options &o) {
         return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET, GADGET2 or EIGEN");
     if (o.precision != SERENX_FP64 && o.precision != SERENX_MIXED)
         return set_error(SERENX_EINVAL, "precision must be SERENX_FP64 or SERENX_MIXED");
+    if (o.kernel != SERENX_KERNEL_M4 && o.kernel != SERENX_KERNEL_M4_TC)
+        return set_error(SERENX_EINVAL, "kernel must be SERENX_KERNEL_M4 or SERENX_KERNEL_M4_TC");
     if (!(o.h_fac > 0.0)) return set_error(SERENX_EINVAL, "h_fac must be positive");
     if (o.mac != SERENX_MAC_GEOMETRIC && !(o.abserror > 0.0))
         return set_error(SERENX_EINVAL, "abserror must be positive for multipole MACs");
@@ -338,7 +344,7 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         delete c;
         return hip_check(e, "serenx_create");
     }
-    make_kernel_tables(c->host_ktab);
+    make_kernel_tables(c->host_ktab, c->opt.kernel);
     {
         std::vector<double> dev(KT_TOTAL, 0.0);
         const double *t = c->host_ktab;

@@ -208,7 +208,7 @@ int hip_check(hipError_t e, const char *what);
     } while (0)
 
 // host-side kernel tables (capi.hip): kernel.F90:14-127, M4, NDIM=3
-void make_kernel_tables(double *tab);
+void make_kernel_tables(double *tab, int kernel);
 
 // tree.hip
 int ensure_tree_buffers(Ctx &c);

@@ -37,6 +37,8 @@ _MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE}
 _MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2,
         "eigen": MAC_EIGEN}
 _PREC = {"fp64": FP64, "mixed": MIXED}
+KERNEL_M4, KERNEL_M4_TC = 0, 1
+_KERNEL = {"m4": KERNEL_M4, "m4tc": KERNEL_M4_TC}
 
 EXPORTED = [
     "serenx_default_options", "serenx_create", "serenx_destroy", "serenx_last_error",
@@ -59,7 +61,7 @@ class SerenxError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("multipole", C.c_int32), ("mac", C.c_int32), ("precision", C.c_int32),
-                ("reserved", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
+                ("kernel", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
                 ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double)]
 
 
@@ -154,13 +156,14 @@ class ForceEngine:
     def __init__(self, multipole: str = "quadrupole", mac: str = "geometric",
                  precision: str = "fp64", h_fac: float = 1.2, thetamaxsqd: float = 0.25,
                  abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
-                 device: int = 0):
+                 device: int = 0, kernel: str = "m4"):
         self.lib = load_library()
         opt = Options()
         _check(self.lib.serenx_default_options(C.byref(opt)))
         opt.multipole = _MULTIPOLE[multipole]
         opt.mac = _MAC[mac]
         opt.precision = _PREC[precision]
+        opt.kernel = _KERNEL[kernel]
         opt.h_fac = h_fac
         opt.thetamaxsqd = thetamaxsqd
         opt.abserror = abserror

@@ -10,10 +10,11 @@ MODULE serenx_f
   integer(c_int32_t), parameter :: SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3
   integer(c_int32_t), parameter :: SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, &
        & SERENX_MAC_GADGET2 = 2, SERENX_MAC_EIGEN = 3
+  integer(c_int32_t), parameter :: SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1
   integer(c_int32_t), parameter :: SERENX_FP64 = 0, SERENX_MIXED = 1
 
   type, bind(C) :: serenx_options
-     integer(c_int32_t) :: multipole, mac, precision, reserved
+     integer(c_int32_t) :: multipole, mac, precision, kernel
      real(c_double) :: h_fac, thetamaxsqd, abserror, alpha, sound_speed
   end type serenx_options
 

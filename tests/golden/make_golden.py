@@ -33,6 +33,9 @@ CASES = {
     "r2_2k_oct_gadget2_x2": (2048, "r2", 303, 2.0, True, "gadget2", 0.25, 1e-4, 2, True),
     "r2_2k_oct_gadget_x2": (2048, "r2", 304, 2.0, True, "gadget", 0.25, 1e-4, 2, True),
     "plummer2k_oct_eigen_x2": (2048, "plummer", 305, 1.0, True, "eigen", 0.25, 1e-4, 2, True),
+    # M4 with the Thomas-Couchman gradient (TC_KERNEL)
+    "uniform1k_quad_geo_tc": (1000, "uniform", 106, 1.0, False, "geometric", 0.25, 1e-4, 1, True,
+                              "m4tc"),
     # configuration 1 of the benchmark suite: N = 16384 uniform sphere, quadrupole,
     # geometric MAC theta = 0.5, one force evaluation (digests for big arrays)
     "c1_uniform16k_quad_geo": (16384, "uniform", 1, 0.0, False, "geometric", 0.25, 1e-4, 1,
@@ -45,9 +48,10 @@ def digest(a: np.ndarray) -> str:
 
 
 def run_case(name, spec):
-    n, prof, seed, mach, octu, mac, th2, ae, nev, full = spec
+    n, prof, seed, mach, octu, mac, th2, ae, nev, full = spec[:10]
+    kernel = spec[10] if len(spec) > 10 else "m4"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac)
+    ref = RefSeren(octu, mac, kernel=kernel)
     ref.setup(pos, mass, h, vel, thetamaxsqd=th2, abserror=ae)
     for e in range(nev):
         ref.tree_update()
@@ -75,6 +79,7 @@ def run_case(name, spec):
     out = dict(
         spec=np.array([n, seed, int(octu), nev], np.int64),
         profile=np.array(prof), mac=np.array(mac), thetamaxsqd=np.array(th2),
+        kernel=np.array(kernel),
         abserror=np.array(ae), mach=np.array(mach),
         pos=pos, vel=vel, mass=mass,
         porig=p["porig"], h=p["h"], rho=p["rho"], omega=p["omega"], zeta=p["zeta"],

@@ -26,12 +26,13 @@ GOLD = os.path.join(HERE, "golden")
                     reason="drop-in host not built (make -f oracle/Makefile.ref dropin)")
 @pytest.mark.parametrize("case", ["uniform1k_quad_geo", "plummer2k_quad_geo",
                                   "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
-                                  "plummer2k_oct_eigen_x2"])
+                                  "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))
     n, seed, octu, nev = (int(v) for v in g["spec"])
-    host = RefSeren(bool(octu), str(g["mac"]), dropin=True)
+    kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
+    host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel)
     host.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], thetamaxsqd=float(g["thetamaxsqd"]),
                abserror=float(g["abserror"]))
     for e in range(nev):

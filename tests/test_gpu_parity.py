@@ -39,10 +39,11 @@ def _ids(c):
     return f"n{c[0]}-{c[1]}-{'oct' if c[3] else 'quad'}-{c[4]}-x{c[7]}"
 
 
-def _engine(octupole, mac, theta2, abserr):
+def _engine(octupole, mac, theta2, abserr, kernel="m4"):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole="octupole" if octupole else "quadrupole", mac=mac,
-                       precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS)
+                       precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
+                       kernel=kernel)
 
 
 def _report(name, a, b):
@@ -175,8 +176,9 @@ def test_gpu_matches_reference_golden_vectors(case):
     g = np.load(os.path.join(GOLD, case + ".npz"))
     n, seed, octu, nev = (int(v) for v in g["spec"])
     mac = str(g["mac"])
+    kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     print(f"golden {case}", flush=True)
-    eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]))
+    eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel)
     eng.load(g["pos"], g["vel"], g["mass"], np.zeros(n))
     for e in range(nev):
         eng.force_eval(hguess=(e == 0))

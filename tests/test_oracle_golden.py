@@ -30,7 +30,8 @@ def _run_oracle(g):
     from oracle_lib import CSeren
     n, seed, octu, nev = (int(v) for v in g["spec"])
     mac = str(g["mac"])
-    o = CSeren(bool(octu), mac)
+    kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
+    o = CSeren(bool(octu), mac, kernel)
     o.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], thetamaxsqd=float(g["thetamaxsqd"]),
             abserror=float(g["abserror"]))
     for e in range(nev):

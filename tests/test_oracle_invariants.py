@@ -214,3 +214,18 @@ def test_hydro_momentum_conservation():
     tot = np.abs((m[:, None] * a_h).sum(0)).max()
     scale = (m[:, None] * np.abs(a_h)).sum(0).max()
     assert tot <= 1e-12 * scale
+
+
+def test_tc_kernel_gradient_table():
+    """TC_KERNEL (kernel.F90:56-62): w1 = -1/pi below s = 2/3 (table index
+    KERNTOT/3), the plain M4 gradient w4 elsewhere; other tables unchanged."""
+    from oracle_lib import CSeren
+    pos, vel, mass, h = ics.make_cloud(64, "uniform", seed=3)
+    plain, tc = CSeren(False, "geometric"), CSeren(False, "geometric", "m4tc")
+    for o in (plain, tc):
+        o.setup(pos, mass, h, vel)
+    tp, tt = plain.kernel(), tc.kernel()
+    k = 10000 // 3
+    assert np.all(tt[1][:k] == -0.318309886)
+    assert np.array_equal(tt[1][k:], tp[4][k:])
+    assert np.array_equal(np.delete(tt, 1, axis=0), np.delete(tp, 1, axis=0))

@@ -22,6 +22,7 @@ CASES = [
     (700, "uniform", 14, True, "gadget2", 3),
     (2000, "r2", 15, True, "gadget", 2),
     (2000, "plummer", 16, True, "eigen", 2),
+    (1500, "uniform", 17, False, "geometric", 1, "m4tc"),
 ]
 
 
@@ -29,11 +30,12 @@ CASES = [
 def test_oracle_bit_exact_with_reference(case):
     from oracle_lib import CSeren, RefSeren
     from seren_amd import ics
-    n, prof, seed, octu, mac, nev = case
+    n, prof, seed, octu, mac, nev = case[:6]
+    kernel = case[6] if len(case) > 6 else "m4"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=1.5, ngrid=16)
     outs = []
     for cls in (RefSeren, CSeren):
-        o = cls(octu, mac)
+        o = cls(octu, mac, kernel=kernel)
         o.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
         for e in range(nev):
             o.tree_update()
