@@ -1,14 +1,18 @@
-"""Benchmark: particles/s of one SEREN force evaluation on one MI355X.
+"""Benchmark: particles/s of SEREN integration steps on one MI355X.
 
-Workload (one "step"): one complete force evaluation -- tree build + stock,
-grad-h h/rho solve with neighbour lists, isothermal thermal update, grad-h
-hydro pair forces, Barnes-Hut gravity -- of a resident cloud of
+Workload (the GPU FP64 configuration of the reference's benchmark suite):
 N = 1,048,576 equal-mass particles, rho ~ r^-2 truncated at r = 1, with a
 divergence-free P(k) ~ k^-4 velocity field on a 128^3 grid at rms Mach 2
 (c_s = 0.2), octupole moments and the GADGET2 multipole MAC (abserror 1e-4),
-FP64 throughout.  Inputs are resident in HBM; the timed region is K force
-evaluations.  Two untimed evaluations come first (the setup evaluation with
-the tree h guess, then one that provides |a| for the multipole MAC).
+FP64 throughout.  One "step" (--mode step, the default) is one kick-drift-
+kick leapfrog step with SEREN's timestep criteria (timestep_size.F90,
+sph_timesteps.F90, one time-step level): two device-resident sph_integrate
+calls, the second holding the force evaluation of every particle -- tree
+build (every 8 steps, as nbuildstep) or restock, grad-h h/rho solve with
+neighbour lists, isothermal thermal update, grad-h hydro pair forces,
+Barnes-Hut gravity -- and the particle advance.  --mode eval times bare force
+evaluations (tree build every time) instead.  Inputs are resident in HBM;
+the timed region is K steps after W untimed ones and the setup evaluation.
 
 Prints one JSON line.  --cpu-baseline times the reference Fortran/OpenMP path
 (the flang build of the reference sources in oracle/_ref) on the same cloud.
@@ -42,8 +46,9 @@ MI355X_FP32_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="step", choices=["step", "eval"])
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--profile", default="r2")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -131,24 +136,50 @@ def main():
     eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=args.precision,
                       thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
     eng.load(pos, vel, mass, h)
-    eng.force_eval(hguess=True)
+    eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
     eng.sync()
-    for _ in range(max(0, args.warmup - 1)):
-        eng.force_eval()
-    eng.sync()
-    print("# warm", file=sys.stderr, flush=True)
+    counts = np.zeros(4)
     grav_ms = []
     phases = np.zeros(4)
+    wstats = None
+
+    def unit():
+        """One timed unit: a leapfrog step (two sph_integrate calls) or a
+        bare force evaluation; returns per-phase ms and interaction counts."""
+        nonlocal wstats
+        ph, cnt, glaunch = np.zeros(4), np.zeros(4), []
+        for _ in range(2 if args.mode == "step" else 1):
+            if args.mode == "step":
+                eng.step(1)
+            else:
+                eng.force_eval()
+            t = eng.timings()
+            ph += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
+            c = np.array(eng.interaction_counts(), np.float64)
+            if c[0] > 0:                          # the call that evaluated forces
+                wstats = eng.walk_stats()
+                glaunch.append(t["grav"])
+            cnt += c
+        return ph, cnt, glaunch
+
+    if args.mode == "step":
+        eng.step_begin("lfkdk", nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
+    else:
+        eng.force_eval()                         # provides |a| for the multipole MAC
+    for _ in range(args.warmup):
+        unit()
+    eng.sync()
+    print("# warm", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.force_eval()
-        t = eng.timings()          # waits for this evaluation (events on its stream)
-        grav_ms.append(t["grav"])
-        phases += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
+        ph, cnt, gl = unit()
+        phases += ph
+        counts += cnt
+        grav_ms.extend(gl)
     eng.sync()
     dt = time.perf_counter() - t0
-    ncell, npp, npp_in_p, npp_in_j = eng.interaction_counts()
-    wstats = eng.walk_stats()
+    step_info = eng.step_info() if args.mode == "step" else None
+    ncell, npp, npp_in_p, npp_in_j = counts
     # accuracy of the tree gravity against the FP64 direct sum on a seeded
     # subsample of targets (one more, gravity-only, evaluation; untimed)
     eng.force_eval(hydro=False)
@@ -160,12 +191,15 @@ def main():
     n = args.n
     value = n * args.steps / dt
     ms_step = 1e3 * dt / args.steps
-    # gravity-walk roofline: algorithmic flops per launch / measured launch time
+    # gravity-walk roofline: algorithmic flops of the force-evaluating
+    # launches / their measured duration (HIP events on the library's stream)
     flops = ncell * F_CELL[args.multipole] + npp * F_PAIR + 8.0 * npp_in_p + 9.0 * npp_in_j
-    gms = float(np.mean(grav_ms))
+    gms = float(np.sum(grav_ms))
     achieved = flops / (gms * 1e-3) / 1e12
     peak = MI355X_FP64_TFLOPS if args.precision == "fp64" else MI355X_FP32_TFLOPS
-    workload = (f"force evaluation, N={n} {args.profile} sphere, Mach {args.mach}, "
+    what = ("KDK leapfrog steps (timestep_size criteria, 1 level, tree build every 8)"
+            if args.mode == "step" else "force evaluation")
+    workload = (f"{what}, N={n} {args.profile} sphere, Mach {args.mach}, "
                 f"{args.multipole} + {args.mac} MAC (abserror {args.abserror}), {args.precision}")
     traffic = measured_traffic(workload)
     out = {
@@ -179,8 +213,10 @@ def main():
                    "mac": args.mac, "abserror": args.abserror, "precision": args.precision},
         "phases_ms": {k: float(v / args.steps) for k, v in
                       zip(("tree", "sph", "hydro", "grav"), phases)},
-        "interactions_per_particle": {"cell": ncell / n, "pair": npp / n},
+        "interactions_per_particle": {"cell": ncell / n / args.steps,
+                                      "pair": npp / n / args.steps},
         "walk_stats": wstats,
+        "step_state": step_info,
         "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
                                          "rms_rel": float(np.sqrt(np.mean(err * err))),
                                          "max_rel": float(err.max())},

@@ -172,6 +172,22 @@ int serenx_grav_forces(serenx_ctx *ctx);
 int serenx_load(serenx_ctx *ctx, int32_t n, const double *pos, const double *vel,
                 const double *mass, const double *h);
 
+/* A forest of ncloud independent clouds in one context (no counterpart in
+ * the reference, which holds one cloud): counts[k] particles of cloud k,
+ * concatenated in the arrays as for serenx_load.  Every cloud gets its own
+ * tree (its own bounding box and root, BHgrav_build.F90:65-67), walks stay
+ * inside a particle's own cloud, and each cloud's results are those of
+ * loading it alone; one launch per phase serves all clouds.  Tree order
+ * permutes particles within their cloud only, so cloud k keeps the
+ * particle range given by serenx_cloud_offsets.  Integration steps
+ * (serenx_step_begin) need a single cloud. */
+int serenx_load_forest(serenx_ctx *ctx, int32_t ncloud, const int32_t *counts,
+                       const double *pos, const double *vel, const double *mass,
+                       const double *h);
+
+/* offsets[0..ncloud]: particle range of every cloud, in input and tree order */
+int serenx_cloud_offsets(const serenx_ctx *ctx, int32_t *offsets);
+
 /* Replaces |a| from a previous evaluation (used by the GADGET/GADGET2 MACs,
  * BHgrav_accel.F90:97-116); values are in the current tree order. */
 int serenx_set_agravmag(serenx_ctx *ctx, const double *agravmag);

@@ -157,7 +157,8 @@ void free_all(Ctx &c) {
                     c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
                     c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
                     c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
-                    c.root, c.bbpart, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
+                    c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,
+                    c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
                     c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount, c.wstat};
     for (void *p : ptrs)
@@ -405,6 +406,10 @@ int serenx_bind_host_state(serenx_ctx *ctx, const serenx_host_state *st) {
     c.hs = *st;
     c.n = st->ptot;
     c.bound = true;
+    if (!same || c.ncloud != 1) {
+        const int32_t one = c.n;
+        SX_TRY(set_clouds(c, 1, &one));
+    }
     if (same) return SERENX_OK;
     c.tree_ok = c.sph_ok = false;
     hipLaunchKernelGGL(k_init_state, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.porig,
@@ -534,12 +539,28 @@ int serenx_grav_forces(serenx_ctx *ctx) {
 // ---- device-resident mode --------------------------------------------------
 int serenx_load(serenx_ctx *ctx, int32_t n, const double *pos, const double *vel,
                 const double *mass, const double *h) {
-    if (!ctx || !pos || !vel || !mass || n <= 0) return set_error(SERENX_EINVAL, "bad load args");
+    return serenx_load_forest(ctx, 1, &n, pos, vel, mass, h);
+}
+
+int serenx_load_forest(serenx_ctx *ctx, int32_t ncloud, const int32_t *counts,
+                       const double *pos, const double *vel, const double *mass,
+                       const double *h) {
+    if (!ctx || !counts || !pos || !vel || !mass || ncloud <= 0)
+        return set_error(SERENX_EINVAL, "bad load args");
+    long long total = 0;
+    for (int k = 0; k < ncloud; k++) {
+        if (counts[k] <= 0) return set_error(SERENX_EINVAL, "empty cloud");
+        total += counts[k];
+    }
+    if (total >= (1LL << 27))
+        return set_error(SERENX_EINVAL, "more than 2^27 - 1 particles on one device");
+    const int n = (int)total;
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
     step_free(c);
     SX_TRY(ensure_capacity(c, n));
     c.n = n;
+    SX_TRY(set_clouds(c, ncloud, counts));
     c.bound = false;
     c.tree_ok = c.sph_ok = false;
     SX_TRY(up_vec3(c, pos, c.x, c.y, c.z));
@@ -774,6 +795,13 @@ int serenx_step(serenx_ctx *ctx, int32_t count) {
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
     return step_run(c, count);
+}
+
+int serenx_cloud_offsets(const serenx_ctx *ctx, int32_t *offsets) {
+    if (!ctx || !offsets) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    for (size_t k = 0; k < c.cloud_off_h.size(); k++) offsets[k] = c.cloud_off_h[k];
+    return SERENX_OK;
 }
 
 int serenx_step_get_info(const serenx_ctx *ctx, serenx_step_info *info) {

@@ -176,6 +176,7 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_fp64(
     int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
@@ -194,8 +195,9 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     // after the cells, :272-290; the order changes only the rounding)
     double gx = 0, gy = 0, gz = 0, gp = 0;
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
+    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
     const unsigned vis = group_walk(
-        gn, nc, valid, lvis,
+        gn, nc, valid, rr.x, rr.y, lvis,
         [&](const GNode &g, bool here) -> int {
             const double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
             const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
@@ -308,6 +310,7 @@ __device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], do
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_mixed(
     int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
@@ -326,8 +329,9 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
     CSum px, py, pz, pq;                              // pair terms
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
+    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
     const unsigned vis = group_walk(
-        gn, nc, valid, lvis,
+        gn, nc, valid, rr.x, rr.y, lvis,
         [&](const GNodeF &g, bool here) -> int {
             const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
             const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
@@ -386,6 +390,8 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
                                                 const double *__restrict__ m,
                                                 const double *__restrict__ h,
                                                 const double *__restrict__ zeta,
+                                                const int *__restrict__ cid,
+                                                const int *__restrict__ cloud_off,
                                                 const double4 *__restrict__ t4,
                                                 double *__restrict__ acc, double *__restrict__ pot) {
     __shared__ double sx[256], sy[256], sz[256], sm[256], sh[256], sih[256], szt[256];
@@ -393,6 +399,8 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
     const bool valid = t < nt;
     const int p = valid ? targets[t] : 0;
     const double rpx = x[p], rpy = y[p], rpz = z[p], invhp = 1.0 / h[p], zo_p = zeta[p];
+    // sources: the target's own cloud
+    const int lo = cloud_off[cid[p]], hi = cloud_off[cid[p] + 1];
     double gx = 0, gy = 0, gz = 0, gp = 0;
     for (int base = 0; base < n; base += 256) {
         const int j = base + threadIdx.x;
@@ -409,7 +417,7 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
         const int lim = min(256, n - base);
         if (valid) {
             for (int k = 0; k < lim; k++) {
-                if (base + k == p) continue;
+                if (base + k == p || base + k < lo || base + k >= hi) continue;
                 (void)pair_fp64(t4, invhp, zo_p, sx[k] - rpx, sy[k] - rpy, sz[k] - rpz, sm[k],
                                 sh[k], sih[k], szt[k], gx, gy, gz, gp);
             }
@@ -431,12 +439,12 @@ void launch_grav(Ctx &c) {
                        c.m, c.h, c.zeta, rec);
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL((k_grav_mixed<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gnf, rec, c.active,
+                           c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.ax, c.ay,
                            c.az, c.gpot, c.agm, c.icount, c.wstat);
     else
         hipLaunchKernelGGL((k_grav_fp64<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gn, rec, c.active,
+                           c.nc, c.gn, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.ax, c.ay,
                            c.az, c.gpot, c.agm, c.icount, c.wstat);
 }
@@ -466,7 +474,7 @@ int grav_forces(Ctx &c) {
 
 int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot) {
     hipLaunchKernelGGL(k_direct, dim3(nblk(nt)), dim3(256), 0, c.stream, c.n, nt, dtargets, c.x,
-                       c.y, c.z, c.m, c.h, c.zeta,
+                       c.y, c.z, c.m, c.h, c.zeta, c.cid, c.cloud_off,
                        reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), dacc, dpot);
     SX_HIP(hipGetLastError());
     return SERENX_OK;

@@ -133,8 +133,18 @@ struct Ctx {
     int *flag = nullptr, *incl = nullptr, *cellof = nullptr;
     int *ncs = nullptr, *off = nullptr;
     int8_t *firstlev = nullptr;
-    double *root = nullptr;      // [rc x y z, cellsize0, rextent, rmin3, rmax3]
-    double *bbpart = nullptr;    // bounding-box block partials
+    // clouds: a forest of independent trees, one root per cloud at level 0;
+    // cloud k owns the particles [cloud_off[k], cloud_off[k+1]) in input and
+    // in tree order (the build permutes within clouds only)
+    int ncloud = 1, max_clouds = 1, max_chunks = 1, nchunk = 0;
+    std::vector<int> cloud_off_h;
+    int *cloud_off = nullptr;    // device, ncloud + 1
+    int *cid = nullptr;          // cloud of each particle
+    int *chunks = nullptr;       // bounding-box chunks [cloud, begin, end]
+    int *chunk_first = nullptr;  // first chunk of each cloud, ncloud + 1
+    int2 *root_range = nullptr;  // DFS range [root, root.next) of each cloud's tree
+    double *root = nullptr;      // per cloud [rc x y z, cellsize0, rextent, rmin3, rmax3]
+    double *bbpart = nullptr;    // bounding-box chunk partials
     void *sorttmp = nullptr;
     size_t sorttmp_bytes = 0;
     int *hostpin = nullptr;      // pinned readback slots
@@ -217,6 +227,8 @@ int stock_tree(Ctx &c);          // BHgrav_stock + BHhydro_stock (fused)
 int hguess(Ctx &c);              // BHhydro_hguess
 int update_hmax(Ctx &c);         // BHhydro_update_hmax
 int permute_particles(Ctx &c, const int *perm);
+// cloud layout of the loaded particles (counts[k] particles in cloud k)
+int set_clouds(Ctx &c, int ncloud, const int *counts);
 
 // sph.hip
 int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles

@@ -81,8 +81,10 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
 // stays in hlist (a lane joins no walk after it has converged).
 __global__ void __launch_bounds__(256) k_hsolve_iter(
     int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
-    int *__restrict__ hlist, const uint8_t *__restrict__ active, const double *__restrict__ ktab,
-    double h_fac, double rextent, int pp_gather, double *__restrict__ h,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
+    const double *__restrict__ roots, int *__restrict__ hlist,
+    const uint8_t *__restrict__ active, const double *__restrict__ ktab,
+    double h_fac, int pp_gather, double *__restrict__ h,
     double *__restrict__ rho, double *__restrict__ hhigh, int *__restrict__ nlist,
     unsigned long long *__restrict__ wstat) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
@@ -90,11 +92,15 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
     int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + (threadIdx.x & (WAVE - 1));
     const double *w0 = ktab + T_W0 * KT;
     const double *wh = ktab + T_WH * KT;
-    double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1;
+    double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1, rextent = 0.0;
+    int2 rr = make_int2(nc, nc);
     if (valid) {
         const double4 a = rec[4 * (size_t)p], b = rec[4 * (size_t)p + 1];
         rpx = a.x; rpy = a.y; rpz = a.z; hp = a.w;
         mp = b.w;
+        const int k = cid[p];
+        rr = root_range[k];
+        rextent = roots[11 * k + 4];   // the particle's own cloud (bounding_box.F90:39)
     }
     double h_high = 0.0, h_old = hp, h_lower = 0.0, h_upper = rextent;
     double rhotemp = 0.0;
@@ -138,7 +144,8 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
                 cnt = 0;
             }
             bool lovf = false;
-            vis += group_walk(hn, nc, iterating, lvis, visit, [&](const HNode &nd, bool mine) {
+            vis += group_walk(hn, nc, iterating, rr.x, rr.y, lvis, visit,
+                              [&](const HNode &nd, bool mine) {
                 if (mine) {
                     cnt += nd.leaf;
                     if (nslot < HL_SLOTS)
@@ -165,7 +172,8 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
                 pair(a.x, a.y, a.z, b.w);
             });
         } else {
-            vis += group_walk(hn, nc, sum, lvis, visit, [&](const HNode &nd, bool mine) {
+            vis += group_walk(hn, nc, sum, rr.x, rr.y, lvis, visit,
+                              [&](const HNode &nd, bool mine) {
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
                     const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
@@ -236,6 +244,7 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
 // sums rho, Omega, div v and zeta over the last potential-neighbour list.
 __global__ void __launch_bounds__(256) k_hsolve_final(
     int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const int *__restrict__ hlist, const uint8_t *__restrict__ active,
     const double *__restrict__ ktab, double h_fac, const double *__restrict__ hhigh,
     const int *__restrict__ nlist, double *__restrict__ h, double *__restrict__ zeta,
@@ -290,8 +299,9 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
     const bool walk = valid && nslot < 0;
     if (__any(walk)) {
         // lists that did not fit: gather over the tree walk at the same radius
+        const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
         vis = group_walk(
-            hn, nc, walk, lvis,
+            hn, nc, walk, rr.x, rr.y, lvis,
             [&](const HNode &nd, bool) { return hgather_visit(nd, rpx, rpy, rpz, R); },
             [&](const HNode &nd, bool mine) {
                 for (int i = 0; i < nd.leaf; i++) {
@@ -326,6 +336,8 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
                                               const double *__restrict__ y,
                                               const double *__restrict__ z,
                                               const double *__restrict__ h,
+                                              const int *__restrict__ cid,
+                                              const int2 *__restrict__ root_range,
                                               const uint8_t *__restrict__ active, int pp_limit,
                                               int *__restrict__ pptot, int *__restrict__ pplist,
                                               const int *__restrict__ ovf_off,
@@ -350,8 +362,9 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
         if (OVERFLOW_PASS) lim = pptot[p];
     }
     unsigned lvis = 0;
+    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
     const unsigned vis = group_walk(
-        hn, nc, valid, lvis,
+        hn, nc, valid, rr.x, rr.y, lvis,
         [&](const HNode &nd, bool) -> int {
             const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
             const double drsqd = dx * dx + dy * dy + dz * dz;
@@ -508,11 +521,11 @@ int sph_hsolve(Ctx &c) {
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
     hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
-                       rec, c.hlist, c.active, c.ktab, c.opt.h_fac, c.rextent, c.pp_gather, c.h,
-                       c.rho, c.tmpd, c.tmpi, c.wstat);
+                       rec, c.cid, c.root_range, c.root, c.hlist, c.active, c.ktab, c.opt.h_fac,
+                       c.pp_gather, c.h, c.rho, c.tmpd, c.tmpi, c.wstat);
     SX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
-                       rec, c.hlist, c.active, c.ktab, c.opt.h_fac, c.tmpd, c.tmpi, c.h, c.zeta,
+                       rec, c.cid, c.root_range, c.hlist, c.active, c.ktab, c.opt.h_fac, c.tmpd, c.tmpi, c.h, c.zeta,
                        c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -523,8 +536,8 @@ int sph_neighbours(Ctx &c) {
     hipStream_t s = c.stream;
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x, c.y,
-                       c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
-                       c.ovf_count, c.wstat);
+                       c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot, c.pplist,
+                       c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
@@ -550,8 +563,8 @@ int sph_neighbours(Ctx &c) {
                 return set_error(SERENX_ENOMEM, "hipMalloc failed for overflow neighbour lists");
         }
         hipLaunchKernelGGL(k_neib<true>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x,
-                           c.y, c.z, c.h, c.active, c.pp_limit, c.pptot, c.pplist, c.ovf_off,
-                           c.ovf_list, c.ovf_count, c.wstat);
+                           c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
+                           c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());
     }
     return SERENX_OK;

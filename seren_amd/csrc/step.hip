@@ -337,6 +337,10 @@ int step_advance(Ctx &c) {
 // sph_integrate.F90:41-113 for one step
 int step_once(Ctx &c) {
     Ctx::Step &s = c.st;
+    // phase markers as serenx_force_eval records them (serenx_last_timings)
+    // and walk statistics of this step's force evaluation
+    SX_HIP(hipMemsetAsync(c.wstat, 0, 16 * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipEventRecord(c.ev[0], c.stream));
     // tree_update.F90:24-88
     const bool build = s.nbuild == s.nsteps;
     if (build) {
@@ -346,11 +350,15 @@ int step_once(Ctx &c) {
     if (s.nstock == s.nsteps) SX_TRY(stock_tree(c));
     if (build) s.nbuild = s.nbuild + s.opt.nbuildstep;
     if (s.nstock == s.nsteps) s.nstock = s.nstock + (s.opt.integrator == SERENX_INT_LFKDK ? 2 : 1);
+    SX_HIP(hipEventRecord(c.ev[1], c.stream));
     // sph_update, thermal, forces for the active particles
     SX_TRY(run_sph_update(c));
     SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    SX_HIP(hipEventRecord(c.ev[2], c.stream));
     SX_TRY(run_hydro(c));
+    SX_HIP(hipEventRecord(c.ev[3], c.stream));
     SX_TRY(run_grav(c));
+    SX_HIP(hipEventRecord(c.ev[4], c.stream));
     SX_TRY(step_timesteps(c));
     SX_TRY(step_advance(c));
     return SERENX_OK;
@@ -405,6 +413,9 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
         return set_error(SERENX_EINVAL, "accel_mult and courant_mult must be positive");
     if (!c.tree_ok || !c.sph_ok)
         return set_error(SERENX_ESTATE, "step_begin needs the setup force evaluation first");
+    if (c.ncloud != 1)
+        return set_error(SERENX_EINVAL, "integration steps run on a single cloud (one context "
+                                        "per cloud)");
     step_free(c);
     Ctx::Step &s = c.st;
     const size_t n = (size_t)c.cap + 1;

@@ -43,21 +43,26 @@ int dalloc(T **p, size_t count) {
     return SERENX_OK;
 }
 
-// ---- bounding box (bounding_box.F90:11-43) ------------------------------
-__global__ void __launch_bounds__(256) k_bbox_partial(int n, const double *__restrict__ x,
-                                                      const double *__restrict__ y,
-                                                      const double *__restrict__ z,
-                                                      double *__restrict__ part) {
+// ---- bounding boxes (bounding_box.F90:11-43), one per cloud -------------
+constexpr int BB_CHUNK = 8192;
+
+// min / max of the positions of one chunk of one cloud (one block per chunk)
+__global__ void __launch_bounds__(256) k_bbox_chunks(const int *__restrict__ chunks,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z,
+                                                     double *__restrict__ part) {
     __shared__ double s[6][256];
+    const int ch = blockIdx.x, b = chunks[3 * ch + 1], e = chunks[3 * ch + 2];
     double mn0 = BIG_NUMBER, mn1 = BIG_NUMBER, mn2 = BIG_NUMBER;
     double mx0 = -BIG_NUMBER, mx1 = -BIG_NUMBER, mx2 = -BIG_NUMBER;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double a = x[i], b = y[i], c = z[i];
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const double a = x[i], bb = y[i], c = z[i];
         mn0 = fmin(mn0, a); mx0 = fmax(mx0, a);
-        mn1 = fmin(mn1, b); mx1 = fmax(mx1, b);
+        mn1 = fmin(mn1, bb); mx1 = fmax(mx1, bb);
         mn2 = fmin(mn2, c); mx2 = fmax(mx2, c);
     }
-    int t = threadIdx.x;
+    const int t = threadIdx.x;
     s[0][t] = mn0; s[1][t] = mn1; s[2][t] = mn2;
     s[3][t] = mx0; s[4][t] = mx1; s[5][t] = mx2;
     __syncthreads();
@@ -68,44 +73,68 @@ __global__ void __launch_bounds__(256) k_bbox_partial(int n, const double *__res
         }
         __syncthreads();
     }
-    if (t < 6) part[blockIdx.x * 6 + t] = s[t][0];
+    if (t < 6) part[6 * ch + t] = s[t][0];
 }
 
-// root cell: centre 0.5 (rmin + rmax), half-size 0.5 max extent
-// (BHgrav_build.F90:65-67); rextent = max extent (bounding_box.F90:39)
-__global__ void k_bbox_final(int nb, const double *__restrict__ part, double *__restrict__ root) {
-    if (threadIdx.x != 0) return;
-    double mn[3] = {BIG_NUMBER, BIG_NUMBER, BIG_NUMBER};
-    double mx[3] = {-BIG_NUMBER, -BIG_NUMBER, -BIG_NUMBER};
-    for (int b = 0; b < nb; b++)
-        for (int k = 0; k < 3; k++) {
-            mn[k] = fmin(mn[k], part[6 * b + k]);
-            mx[k] = fmax(mx[k], part[6 * b + 3 + k]);
+// per cloud (one block): root cell centre 0.5 (rmin + rmax), half-size
+// 0.5 max extent (BHgrav_build.F90:65-67); rextent = max extent
+// (bounding_box.F90:39)
+__global__ void __launch_bounds__(64) k_bbox_clouds(const int *__restrict__ chunk_first,
+                                                    const double *__restrict__ part,
+                                                    double *__restrict__ root) {
+    __shared__ double s[6][64];
+    const int k = blockIdx.x, t = threadIdx.x;
+    double v[6] = {BIG_NUMBER, BIG_NUMBER, BIG_NUMBER, -BIG_NUMBER, -BIG_NUMBER, -BIG_NUMBER};
+    for (int ch = chunk_first[k] + t; ch < chunk_first[k + 1]; ch += 64) {
+        for (int q = 0; q < 3; q++) v[q] = fmin(v[q], part[6 * ch + q]);
+        for (int q = 3; q < 6; q++) v[q] = fmax(v[q], part[6 * ch + q]);
+    }
+    for (int q = 0; q < 6; q++) s[q][t] = v[q];
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int q = 0; q < 3; q++) s[q][t] = fmin(s[q][t], s[q][t + w]);
+            for (int q = 3; q < 6; q++) s[q][t] = fmax(s[q][t], s[q][t + w]);
         }
-    double e0 = mx[0] - mn[0], e1 = mx[1] - mn[1], e2 = mx[2] - mn[2];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double mn[3] = {s[0][0], s[1][0], s[2][0]}, mx[3] = {s[3][0], s[4][0], s[5][0]};
+    const double e0 = mx[0] - mn[0], e1 = mx[1] - mn[1], e2 = mx[2] - mn[2];
     double ext = e0;
     if (e1 > ext) ext = e1;
     if (e2 > ext) ext = e2;
-    for (int k = 0; k < 3; k++) root[k] = 0.5 * (mn[k] + mx[k]);
-    root[3] = 0.5 * ext;
-    root[4] = ext;
-    for (int k = 0; k < 3; k++) {
-        root[5 + k] = mn[k];
-        root[8 + k] = mx[k];
+    double *r = root + 11 * k;
+    for (int q = 0; q < 3; q++) r[q] = 0.5 * (mn[q] + mx[q]);
+    r[3] = 0.5 * ext;
+    r[4] = ext;
+    for (int q = 0; q < 3; q++) {
+        r[5 + q] = mn[q];
+        r[8 + q] = mx[q];
     }
+}
+
+__global__ void k_fill_cid(int ncloud, const int *__restrict__ cloud_off, int *__restrict__ cid) {
+    const int k = blockIdx.y;
+    if (k >= ncloud) return;
+    for (int i = cloud_off[k] + blockIdx.x * blockDim.x + threadIdx.x; i < cloud_off[k + 1];
+         i += gridDim.x * blockDim.x)
+        cid[i] = k;
 }
 
 // ---- octant keys ---------------------------------------------------------
 __global__ void __launch_bounds__(256) k_keys(int n, const double *__restrict__ x,
                                               const double *__restrict__ y,
                                               const double *__restrict__ z,
-                                              const double *__restrict__ root,
+                                              const double *__restrict__ roots,
+                                              const int *__restrict__ cid,
                                               uint64_t *__restrict__ khi,
                                               uint64_t *__restrict__ klo,
                                               int *__restrict__ idx) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = x[i], py = y[i], pz = z[i];
+    const double *root = roots + 11 * cid[i];
     double cx = root[0], cy = root[1], cz = root[2], cs = root[3];
     uint64_t hi = 0, lo = 0;
 #pragma unroll
@@ -132,19 +161,22 @@ __global__ void k_gather_u64(int n, const int *__restrict__ idx, const uint64_t 
     if (i < n) dst[i] = src[idx[i]];
 }
 
-// any run of > LEAFMAX equal 21-level keys needs the deeper 9 levels sorted
-__global__ void k_deep_check(int n, const uint64_t *__restrict__ khi, int *__restrict__ flag) {
+// any run of > LEAFMAX equal 21-level keys in one cloud needs the deeper 9
+// levels sorted
+__global__ void k_deep_check(int n, const uint64_t *__restrict__ khi, const int *__restrict__ cid,
+                             int *__restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i + LEAFMAX < n && khi[i] == khi[i + LEAFMAX]) *flag = 1;
+    if (i + LEAFMAX < n && khi[i] == khi[i + LEAFMAX] && cid[i] == cid[i + LEAFMAX]) *flag = 1;
 }
 
-// common key prefix of sorted neighbours i-1, i in levels (0..30)
+// common key prefix of sorted neighbours i-1, i in levels (0..30); zero at
+// the first particle of every cloud (its root starts there)
 __global__ void k_cpl(int n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
-                      uint8_t *__restrict__ cpl) {
+                      const int *__restrict__ cid, uint8_t *__restrict__ cpl) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (i == 0) {
-        cpl[0] = 0;
+    if (i == 0 || cid[i] != cid[i - 1]) {
+        cpl[i] = 0;
         return;
     }
     uint64_t a = khi[i - 1] ^ khi[i];
@@ -158,24 +190,37 @@ __global__ void k_cpl(int n, const uint64_t *__restrict__ khi, const uint64_t *_
     cpl[i] = (uint8_t)lv;
 }
 
-__global__ void k_init_levels(int n, uint8_t *pact, int *cellof, int *ncs, int8_t *firstlev,
-                              int *c_start, int *c_end, int *c_parent, int *c_leaf, int *c_level) {
+// level 0: one root cell per cloud, cell id = cloud id
+__global__ void k_init_levels(int n, int ncloud, const int *__restrict__ cloud_off,
+                              const int *__restrict__ cid, uint8_t *pact, int *cellof, int *ncs,
+                              int8_t *firstlev, int *c_start, int *c_end, int *c_parent,
+                              int *c_leaf, int *c_level) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= n) {
-        ncs[i] = (i == 0) ? 1 : 0;
-        firstlev[i] = (i == 0) ? 0 : -1;
+        const bool first = (i < n) && (i == 0 || cid[i] != cid[i - 1]);
+        ncs[i] = first ? 1 : 0;
+        firstlev[i] = first ? 0 : -1;
     }
     if (i < n) {
         pact[i] = 1;
-        cellof[i] = 0;
+        cellof[i] = cid[i];
     }
-    if (i == 0) {
-        c_start[0] = 0;
-        c_end[0] = n;
-        c_parent[0] = -1;
-        c_leaf[0] = 0;  // the root is always divided (BHgrav_build.F90:74)
-        c_level[0] = 0;
+    if (i < ncloud) {
+        c_start[i] = cloud_off[i];
+        c_end[i] = cloud_off[i + 1];
+        c_parent[i] = -1;
+        c_leaf[i] = 0;  // a root is always divided (BHgrav_build.F90:74)
+        c_level[i] = 0;
     }
+}
+
+// DFS range of each cloud's tree, [root, root.next)
+__global__ void k_root_range(int ncloud, const int *__restrict__ bfs2dfs,
+                             const GNode *__restrict__ gn, int2 *__restrict__ rr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncloud) return;
+    const int d = bfs2dfs[k];
+    rr[k] = make_int2(d, gn[d].next);
 }
 
 // child-start flags at level L: an active particle starts a cell when its
@@ -184,7 +229,7 @@ __global__ void k_mark(int n, int L, const uint8_t *__restrict__ pact,
                        const uint8_t *__restrict__ cpl, int *__restrict__ f) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    f[i] = (pact[i] && (i == 0 || cpl[i] < L)) ? 1 : 0;
+    f[i] = (pact[i] && cpl[i] < L) ? 1 : 0;
 }
 
 __global__ void k_emit(int n, int L, int base, const uint8_t *__restrict__ pact,
@@ -545,9 +590,17 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.ncs, nn));
     SX_TRY(dalloc(&c.off, nn));
     SX_TRY(dalloc(&c.firstlev, nn));
-    SX_TRY(dalloc(&c.root, 16));
-    SX_TRY(dalloc(&c.bbpart, 6 * 1024));
-    c.ccap = 2 * n + 2;
+    // a forest holds at most one cloud per pp_gather particles (set_clouds)
+    c.max_clouds = n / std::max(c.pp_gather, 1) + 1;
+    c.max_chunks = n / BB_CHUNK + c.max_clouds + 1;
+    SX_TRY(dalloc(&c.root, 11 * (size_t)c.max_clouds + 16));
+    SX_TRY(dalloc(&c.root_range, (size_t)c.max_clouds));
+    SX_TRY(dalloc(&c.cloud_off, (size_t)c.max_clouds + 1));
+    SX_TRY(dalloc(&c.chunk_first, (size_t)c.max_clouds + 1));
+    SX_TRY(dalloc(&c.chunks, 3 * (size_t)c.max_chunks));
+    SX_TRY(dalloc(&c.bbpart, 6 * (size_t)c.max_chunks));
+    SX_TRY(dalloc(&c.cid, nn));
+    c.ccap = 2 * n + 2 + c.max_clouds;
     const size_t cc = (size_t)c.ccap;
     SX_TRY(dalloc(&c.c_start, cc));
     SX_TRY(dalloc(&c.c_end, cc));
@@ -602,24 +655,85 @@ int permute_particles(Ctx &c, const int *perm) {
     return permute_step(c, perm);
 }
 
+int set_clouds(Ctx &c, int ncloud, const int *counts) {
+    if (ncloud < 1 || ncloud > c.max_clouds || ncloud > 65535)
+        return set_error(SERENX_EINVAL, "cloud count out of range for the particle capacity");
+    c.cloud_off_h.assign(ncloud + 1, 0);
+    std::vector<int> chunks, first(ncloud + 1, 0);
+    for (int k = 0; k < ncloud; k++) {
+        // (a single cloud is checked when its h solve runs, as before)
+        if (ncloud > 1 && counts[k] < c.pp_gather)
+            return set_error(SERENX_EFEW, "a cloud has fewer particles than pp_gather: the "
+                                          "grad-h solve cannot reach its neighbour number");
+        c.cloud_off_h[k + 1] = c.cloud_off_h[k] + counts[k];
+        first[k] = (int)chunks.size() / 3;
+        for (int b = c.cloud_off_h[k]; b < c.cloud_off_h[k + 1]; b += BB_CHUNK) {
+            chunks.push_back(k);
+            chunks.push_back(b);
+            chunks.push_back(std::min(b + BB_CHUNK, c.cloud_off_h[k + 1]));
+        }
+    }
+    first[ncloud] = (int)chunks.size() / 3;
+    if (c.cloud_off_h[ncloud] != c.n)
+        return set_error(SERENX_EINVAL, "cloud counts do not add up to the particle count");
+    if (first[ncloud] > c.max_chunks) return set_error(SERENX_EINVAL, "too many clouds");
+    c.ncloud = ncloud;
+    c.nchunk = first[ncloud];
+    SX_HIP(hipMemcpy(c.cloud_off, c.cloud_off_h.data(), sizeof(int) * (ncloud + 1),
+                     hipMemcpyHostToDevice));
+    SX_HIP(hipMemcpy(c.chunk_first, first.data(), sizeof(int) * (ncloud + 1),
+                     hipMemcpyHostToDevice));
+    SX_HIP(hipMemcpy(c.chunks, chunks.data(), sizeof(int) * chunks.size(),
+                     hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_cid, dim3(64, ncloud), dim3(TB), 0, c.stream, ncloud, c.cloud_off,
+                       c.cid);
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return SERENX_OK;
+}
+
 int build_tree(Ctx &c) {
     const int n = c.n;
     hipStream_t s = c.stream;
     c.tree_ok = false;
     c.sph_ok = false;
-    // bounding box and root cell
-    int nb = std::min(1024, nblk(n));
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, n, c.x, c.y, c.z, c.bbpart);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, nb, c.bbpart, c.root);
-    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(TB), 0, s, n, c.x, c.y, c.z, c.root, c.khi,
-                       c.klo, c.idx);
+    // bounding box and root cell of every cloud
+    hipLaunchKernelGGL(k_bbox_chunks, dim3(c.nchunk), dim3(256), 0, s, c.chunks, c.x, c.y, c.z,
+                       c.bbpart);
+    hipLaunchKernelGGL(k_bbox_clouds, dim3(c.ncloud), dim3(64), 0, s, c.chunk_first, c.bbpart,
+                       c.root);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n)), dim3(TB), 0, s, n, c.x, c.y, c.z, c.root, c.cid,
+                       c.khi, c.klo, c.idx);
     SX_HIP(hipGetLastError());
-    // sort by the 21-level key; redo with all 30 levels only when needed
-    size_t tb = c.sorttmp_bytes;
-    SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0,
-                                     63, s));
+    // sort by the 21-level key within each cloud; redo with all 30 levels
+    // only when needed
+    auto sort_keys = [&](uint64_t *kin, uint64_t *kout, int *vin, int *vout,
+                         int bits) -> hipError_t {
+        size_t bytes = c.sorttmp_bytes;
+        if (c.ncloud == 1)
+            return rocprim::radix_sort_pairs(c.sorttmp, bytes, kin, kout, vin, vout, (unsigned)n,
+                                              0, bits, s);
+        return rocprim::segmented_radix_sort_pairs(c.sorttmp, bytes, kin, kout, vin, vout,
+                                                   (unsigned)n, (unsigned)c.ncloud, c.cloud_off,
+                                                   c.cloud_off + 1, 0, bits, s);
+    };
+    if (c.ncloud > 1) {
+        size_t need = 0;
+        SX_HIP(rocprim::segmented_radix_sort_pairs(nullptr, need, c.khi, c.khi2, c.idx, c.idx2,
+                                                   (unsigned)n, (unsigned)c.ncloud, c.cloud_off,
+                                                   c.cloud_off + 1, 0, 63, s));
+        if (need > c.sorttmp_bytes) {
+            SX_HIP(hipStreamSynchronize(s));
+            if (c.sorttmp) (void)hipFree(c.sorttmp);
+            c.sorttmp = nullptr;
+            if (hipMalloc(&c.sorttmp, need + 256) != hipSuccess)
+                return set_error(SERENX_ENOMEM, "hipMalloc failed for sort scratch");
+            c.sorttmp_bytes = need + 256;
+        }
+    }
+    SX_HIP(sort_keys(c.khi, c.khi2, c.idx, c.idx2, 63));
     SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_deep_check, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.flag);
+    hipLaunchKernelGGL(k_deep_check, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.cid, c.flag);
     SX_HIP(hipMemcpyAsync(c.hostpin, c.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     SX_HIP(hipMemcpyAsync(c.hostpin_d, c.root, 11 * sizeof(double), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
@@ -627,13 +741,9 @@ int build_tree(Ctx &c) {
     const bool deep = c.hostpin[0] != 0;
     if (deep) {
         // stable LSD order: deep 9 levels first, then the top 21 levels
-        tb = c.sorttmp_bytes;
-        SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.klo, c.klo2, c.idx, c.idx2,
-                                         (unsigned)n, 0, 27, s));
+        SX_HIP(sort_keys(c.klo, c.klo2, c.idx, c.idx2, 27));
         hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx2, c.khi, c.khi2);
-        tb = c.sorttmp_bytes;
-        SX_HIP(rocprim::radix_sort_pairs(c.sorttmp, tb, c.khi2, c.khi, c.idx2, c.idx,
-                                         (unsigned)n, 0, 63, s));
+        SX_HIP(sort_keys(c.khi2, c.khi, c.idx2, c.idx, 63));
         // khi = sorted keys, idx = order; gather the deep keys in that order
         hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx, c.klo, c.klo2);
         std::swap(c.khi, c.khi2);  // khi2 <- sorted hi keys
@@ -641,13 +751,15 @@ int build_tree(Ctx &c) {
         std::swap(c.idx, c.idx2);  // idx <- sorted order
         hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n)), dim3(TB), 0, s, n, c.idx, c.klo, c.klo2);
     }
-    hipLaunchKernelGGL(k_cpl, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.klo2, c.cpl);
-    hipLaunchKernelGGL(k_init_levels, dim3(nblk(n + 1)), dim3(TB), 0, s, n, c.pact, c.cellof,
-                       c.ncs, c.firstlev, c.c_start, c.c_end, c.c_parent, c.c_leaf, c.c_level);
+    hipLaunchKernelGGL(k_cpl, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.klo2, c.cid, c.cpl);
+    hipLaunchKernelGGL(k_init_levels, dim3(nblk(std::max(n + 1, c.ncloud))), dim3(TB), 0, s, n,
+                       c.ncloud, c.cloud_off, c.cid, c.pact, c.cellof, c.ncs, c.firstlev,
+                       c.c_start, c.c_end, c.c_parent, c.c_leaf, c.c_level);
     SX_HIP(hipGetLastError());
     c.level_off[0] = 0;
-    c.level_cnt[0] = 1;
-    int base = 1, ltot = 0;
+    c.level_cnt[0] = c.ncloud;
+    int base = c.ncloud, ltot = 0;
+    size_t tb = 0;
     for (int L = 1; L <= LMAX + 1; L++) {
         hipLaunchKernelGGL(k_mark, dim3(nblk(n)), dim3(TB), 0, s, n, L, c.pact, c.cpl, c.flag);
         tb = c.sorttmp_bytes;
@@ -688,6 +800,8 @@ int build_tree(Ctx &c) {
                        c.bfs2dfs, c.dfs_parent);
     hipLaunchKernelGGL(k_leafsort, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.c_start, c.c_leaf,
                        c.idx);
+    hipLaunchKernelGGL(k_root_range, dim3(nblk(c.ncloud)), dim3(TB), 0, s, c.ncloud, c.bfs2dfs,
+                       c.gn, c.root_range);
     SX_HIP(hipGetLastError());
     SX_TRY(permute_particles(c, c.idx));
     c.tree_ok = true;

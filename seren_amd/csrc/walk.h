@@ -75,9 +75,10 @@ __device__ __forceinline__ double ref_pow_third(double x) {
     return c0 + __fma_rn(c0, g, clo);
 }
 
-// Leaf-group walk over a DFS-ordered tree.  Every lane of the wave owns one
-// target; the wave visits, in DFS order, the union of the cells its lanes
-// visit on their own stackless walks (nextcell / ifopen = c + 1).  A lane is
+// Leaf-group walk over a DFS-ordered forest.  Every lane of the wave owns one
+// target and walks the tree [start, end) of its own cloud; the wave visits,
+// in DFS order, the union of the cells its lanes visit on their own
+// stackless walks (nextcell / ifopen = c + 1).  A lane is
 // active at cell c when its own walk would be at c (resume == c); it then
 // tests c and returns WALK_DESCEND (open a branch: resume = c + 1),
 // WALK_LEAF (open a leaf) or WALK_SKIP (cell used or rejected), the latter two
@@ -101,9 +102,10 @@ enum { WALK_SKIP = 0, WALK_DESCEND = 1, WALK_LEAF = 2 };
 
 template <class Node, class Visit, class Leaf>
 __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, int nc,
-                                               bool walking, unsigned &lane_visits,
-                                               Visit &&visit, Leaf &&leaf) {
-    int resume = walking ? 0 : nc;
+                                               bool walking, int start, int end,
+                                               unsigned &lane_visits, Visit &&visit,
+                                               Leaf &&leaf) {
+    int resume = walking ? start : nc;
     int c = uniform(wave_min_int(resume));
     unsigned visits = 0;
     while (c < nc) {
@@ -116,6 +118,7 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
         lane_visits += (unsigned)__popcll(__ballot(here));
         if (here) {
             resume = (act == WALK_DESCEND) ? c + 1 : nd.next;
+            if (resume >= end) resume = nc;   // past the end of the lane's own tree
         } else {
             act = WALK_SKIP;
         }

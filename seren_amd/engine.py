@@ -49,7 +49,7 @@ EXPORTED = [
     "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
     "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
     "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
-    "serenx_fetch_step",
+    "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
               "serenx_grav_forces", "serenx_sync"):
         getattr(lib, f).argtypes = [vp]
     lib.serenx_load.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp]
+    lib.serenx_load_forest.argtypes = [vp, C.c_int32, _ip, _dp, _dp, _dp, _dp]
+    lib.serenx_cloud_offsets.argtypes = [vp, _ip]
     lib.serenx_set_agravmag.argtypes = [vp, _dp]
     lib.serenx_force_eval.argtypes = [vp, C.c_int32]
     lib.serenx_fetch.argtypes = [vp, _ip] + [_dp] * 14 + [_ip, _ip]
@@ -197,6 +199,25 @@ class ForceEngine:
         self.n = len(mass)
         _check(self.lib.serenx_load(self.ctx, self.n, _dptr(pos), _dptr(vel), _dptr(mass),
                                     _dptr(h)))
+
+    def load_forest(self, clouds) -> np.ndarray:
+        """Loads independent clouds [(pos, vel, mass, h or None), ...] as one
+        forest; returns the particle offsets of the clouds (ncloud + 1)."""
+        counts = np.array([len(c[2]) for c in clouds], np.int32)
+        pos = np.ascontiguousarray(np.concatenate([c[0] for c in clouds]), np.float64)
+        vel = np.ascontiguousarray(np.concatenate([c[1] for c in clouds]), np.float64)
+        mass = np.ascontiguousarray(np.concatenate([c[2] for c in clouds]), np.float64)
+        hs = [c[3] if len(c) > 3 and c[3] is not None else np.zeros(len(c[2])) for c in clouds]
+        h = np.ascontiguousarray(np.concatenate(hs), np.float64)
+        _check(self.lib.serenx_load_forest(self.ctx, len(counts), _iptr(counts), _dptr(pos),
+                                           _dptr(vel), _dptr(mass), _dptr(h)))
+        self.n = int(counts.sum())
+        return self.cloud_offsets(len(counts))
+
+    def cloud_offsets(self, ncloud: int) -> np.ndarray:
+        off = np.zeros(ncloud + 1, np.int32)
+        _check(self.lib.serenx_cloud_offsets(self.ctx, _iptr(off)))
+        return off
 
     def set_agravmag(self, am) -> None:
         am = np.ascontiguousarray(am, np.float64)
