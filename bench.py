@@ -48,7 +48,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="step", choices=["step", "eval"])
+    ap.add_argument("--mode", default="step", choices=["step", "eval", "forest", "serve"])
+    ap.add_argument("--clouds", type=int, default=256, help="forest mode: number of clouds")
+    ap.add_argument("--requests", type=int, default=2000, help="serve mode: requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve mode: requests per second")
+    ap.add_argument("--slots", type=int, default=8, help="serve mode: concurrent requests")
+    ap.add_argument("--pool", type=int, default=4, help="serve mode: resident clouds per size")
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--profile", default="r2")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -118,6 +123,109 @@ def cpu_baseline(pos, vel, mass, h, args):
                       f"untimed setup evaluation), {dt:.2f} s wall"}
 
 
+def bench_forest(args):
+    """Batched ensemble: independent uniform clouds, N log-uniform in
+    [4096, 262144], each with its own P(k) ~ k^-4 turbulent seed at Mach
+    U(1, 5), loaded as one forest; a step is one force evaluation of the
+    whole forest (every phase one launch over all clouds)."""
+    from seren_amd import ics
+    from seren_amd.engine import ForceEngine
+    rng = np.random.default_rng(args.seed)
+    ns = np.round(np.exp(rng.uniform(np.log(4096), np.log(262144), args.clouds))).astype(int)
+    machs = rng.uniform(1.0, 5.0, args.clouds)
+    print(f"# generating {args.clouds} clouds, {ns.sum()} particles", file=sys.stderr, flush=True)
+    clouds = [ics.make_cloud(int(n), "uniform", seed=args.seed + 1 + k, mach=float(m), cs=0.2,
+                             ngrid=32) for k, (n, m) in enumerate(zip(ns, machs))]
+    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=args.precision,
+                      thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
+    off = eng.load_forest(clouds)
+    eng.force_eval(hguess=True)
+    for _ in range(max(1, args.warmup)):
+        eng.force_eval()
+    eng.sync()
+    phases = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.force_eval()
+        t = eng.timings()
+        phases += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
+    eng.sync()
+    dt = time.perf_counter() - t0
+    ntot = int(off[-1])
+    tg = np.sort(rng.choice(ntot, min(ntot, args.direct_targets), replace=False))
+    eng.force_eval(hydro=False)
+    a_tree = eng.fetch(neighbours=False)["a"]
+    a_dir, _ = eng.direct_gravity(tg.astype(np.int32))
+    err = np.linalg.norm(a_tree[tg] - a_dir, axis=1) / np.linalg.norm(a_dir, axis=1)
+    eng.close()
+    out = {"metric": METRIC, "value": ntot * args.steps / dt, "unit": "particles/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None,
+           "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
+           "data": "synthetic (seeded uniform spheres, P(k)~k^-4 turbulent velocities)",
+           "config": {"workload": f"forest of {args.clouds} clouds, N log-uniform "
+                                  f"[4096, 262144] (total {ntot}), Mach U(1,5), "
+                                  f"{args.multipole} + {args.mac} MAC, one force evaluation "
+                                  f"of the forest per step, {args.precision}",
+                      "clouds": args.clouds, "n_total": ntot, "precision": args.precision},
+           "phases_ms": {k: float(v / args.steps) for k, v in
+                         zip(("tree", "sph", "hydro", "grav"), phases)},
+           "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
+                                            "rms_rel": float(np.sqrt(np.mean(err * err))),
+                                            "max_rel": float(err.max())}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Serving: Poisson requests (one KDK leapfrog step each) on resident
+    clouds of N in {2k, 16k, 128k, 1M} drawn with p = {0.4, 0.3, 0.2, 0.1},
+    profiles uniform / r^-2 / Plummer, mixed precision with an FP64 guard
+    step every 4, admission by HBM budget; p50 / p99 latency and throughput."""
+    from seren_amd import ics
+    from seren_amd.serve import CloudServer, poisson_run
+    sizes, probs = [2048, 16384, 131072, 1048576], [0.4, 0.3, 0.2, 0.1]
+    profiles = ["uniform", "r2", "plummer"]
+    srv = CloudServer(hbm_budget=200e9, slots=args.slots, precision=args.precision,
+                      guard_every=4, engine_kw=dict(multipole=args.multipole, mac=args.mac,
+                                                    abserror=args.abserror, sound_speed=0.2))
+    rng = np.random.default_rng(args.seed)
+    ids, p_cloud = [], []
+    print(f"# admitting {len(sizes) * args.pool} clouds", file=sys.stderr, flush=True)
+    for si, (n, pr) in enumerate(zip(sizes, probs)):
+        for j in range(args.pool):
+            cid = len(ids)
+            prof = profiles[int(rng.integers(3))]
+            pos, vel, mass, h = ics.make_cloud(n, prof, seed=args.seed + 100 + cid, mach=2.0,
+                                               cs=0.2, ngrid=32 if n < 200000 else 64)
+            srv.admit(cid, pos, vel, mass, h)
+            ids.append(cid)
+            p_cloud.append(pr / args.pool)
+    srv.start()
+    # warm every cloud once
+    for f in [srv.submit(c) for c in ids]:
+        f.result()
+    print("# warm", file=sys.stderr, flush=True)
+    res = poisson_run(srv, args.rate, args.requests, ids, p_cloud, seed=args.seed)
+    used = srv.used
+    srv.close()
+    out = {"metric": METRIC, "value": res["particles_per_s"], "unit": "particles/s",
+           "n_gpus": 1, "steps": args.requests, "warmup": len(ids),
+           "ms_per_step": res["mean_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None,
+           "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
+           "data": "synthetic (seeded uniform / r^-2 / Plummer clouds, Mach 2)",
+           "config": {"workload": f"serving: Poisson {args.rate:g} req/s, {args.requests} "
+                                  f"requests (one KDK step each) on {len(ids)} resident "
+                                  f"clouds, N in {sizes} with p {probs}, {args.slots} slots, "
+                                  f"{args.precision} with FP64 guard step every 4",
+                      "precision": args.precision, "slots": args.slots},
+           "latency_ms": {"p50": res["p50_ms"], "p99": res["p99_ms"], "mean": res["mean_ms"]},
+           "offered_particles_per_s": res["offered_particles_per_s"],
+           "resident_bytes": used, "wall_s": res["wall_s"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -127,6 +235,10 @@ def main():
         print(json.dumps({"error": "this force path runs on one GPU; sharding over "
                                    "several GPUs is not built, use --gpus 1"}))
         sys.exit(2)
+    if args.mode == "forest":
+        return bench_forest(args)
+    if args.mode == "serve":
+        return bench_serve(args)
     from seren_amd import ics
     from seren_amd.engine import ForceEngine
     print(f"# generating {args.n} particles ({args.profile}, Mach {args.mach})", file=sys.stderr,

@@ -188,6 +188,17 @@ int serenx_load_forest(serenx_ctx *ctx, int32_t ncloud, const int32_t *counts,
 /* offsets[0..ncloud]: particle range of every cloud, in input and tree order */
 int serenx_cloud_offsets(const serenx_ctx *ctx, int32_t *offsets);
 
+/* Switches the gravity precision of later evaluations (SERENX_FP64 or
+ * SERENX_MIXED); takes effect at the next tree stock (every force
+ * evaluation, and every force-evaluating integration step).  Lets a serving
+ * loop interleave FP64 guard steps with mixed-precision steps. */
+int serenx_set_precision(serenx_ctx *ctx, int32_t precision);
+
+/* Free and total device memory (hipMemGetInfo) of the context's device, and
+ * the bytes this context holds. */
+int serenx_memory(const serenx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes,
+                  int64_t *ctx_bytes);
+
 /* Replaces |a| from a previous evaluation (used by the GADGET/GADGET2 MACs,
  * BHgrav_accel.F90:97-116); values are in the current tree order. */
 int serenx_set_agravmag(serenx_ctx *ctx, const double *agravmag);

@@ -185,6 +185,20 @@ int check_options(const serenx_options &o) {
     return SERENX_OK;
 }
 
+// device bytes held by a context, from the allocation sizes
+size_t ctx_device_bytes(const Ctx &c) {
+    const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
+    size_t b = 0;
+    b += nn * (22 * 8 + 16 * 8);                            // particle SoA, tmpd, records
+    b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
+    b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
+    b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
+    b += cc * (sizeof(GNode) + sizeof(GNodeF) + sizeof(HNode) + 8 * 4 + 7 * 8);
+    b += c.sorttmp_bytes + c.ovf_cap * 4 + sizeof(double) * KT_TOTAL;
+    if (c.st.on) b += nn * (11 * 8 + 8 + 4);                 // step state
+    return b;
+}
+
 Ctx *as_ctx(serenx_ctx *p) { return reinterpret_cast<Ctx *>(p); }
 const Ctx *as_ctx(const serenx_ctx *p) { return reinterpret_cast<const Ctx *>(p); }
 
@@ -830,6 +844,31 @@ int serenx_fetch_step(serenx_ctx *ctx, int32_t *nlevel, int64_t *nlast, double *
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
     return step_fetch(c, nlevel, nlast, laststep, active, r_old, v_old, v_half);
+}
+
+int serenx_set_precision(serenx_ctx *ctx, int32_t precision) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    if (precision != SERENX_FP64 && precision != SERENX_MIXED)
+        return set_error(SERENX_EINVAL, "precision must be SERENX_FP64 or SERENX_MIXED");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    const bool to_mixed = precision == SERENX_MIXED && c.opt.precision != SERENX_MIXED;
+    c.opt.precision = precision;
+    if (to_mixed) SX_TRY(refresh_f32_nodes(c));
+    return SERENX_OK;
+}
+
+int serenx_memory(const serenx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes,
+                  int64_t *ctx_bytes) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    const Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    size_t fr = 0, tot = 0;
+    SX_HIP(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    if (ctx_bytes) *ctx_bytes = (int64_t)ctx_device_bytes(c);
+    return SERENX_OK;
 }
 
 }  // extern "C"

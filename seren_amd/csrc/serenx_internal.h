@@ -224,6 +224,7 @@ void make_kernel_tables(double *tab, int kernel);
 int ensure_tree_buffers(Ctx &c);
 int build_tree(Ctx &c);          // bbox, keys, sort, cells, DFS, permutation
 int stock_tree(Ctx &c);          // BHgrav_stock + BHhydro_stock (fused)
+int refresh_f32_nodes(Ctx &c);   // mixed-precision node copy of the stocked tree
 int hguess(Ctx &c);              // BHhydro_hguess
 int update_hmax(Ctx &c);         // BHhydro_update_hmax
 int permute_particles(Ctx &c, const int *perm);

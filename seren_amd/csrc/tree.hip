@@ -824,6 +824,14 @@ int stock_tree(Ctx &c) {
     return SERENX_OK;
 }
 
+int refresh_f32_nodes(Ctx &c) {
+    if (!c.tree_ok || c.nc <= 0) return SERENX_OK;
+    hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, c.gn,
+                       c.gnf);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
 int hguess(Ctx &c) {
     const int thresh = (int)(0.5 * (double)c.pp_gather);
     hipLaunchKernelGGL(k_hguess, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, thresh, c.c_leaf,

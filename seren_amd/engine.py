@@ -49,7 +49,8 @@ EXPORTED = [
     "serenx_fetch", "serenx_direct_gravity", "serenx_tree_sizes", "serenx_export_tree",
     "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
     "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
-    "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets",
+    "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets", "serenx_set_precision",
+    "serenx_memory",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -113,6 +114,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_load.argtypes = [vp, C.c_int32, _dp, _dp, _dp, _dp]
     lib.serenx_load_forest.argtypes = [vp, C.c_int32, _ip, _dp, _dp, _dp, _dp]
     lib.serenx_cloud_offsets.argtypes = [vp, _ip]
+    lib.serenx_set_precision.argtypes = [vp, C.c_int32]
+    lib.serenx_memory.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
     lib.serenx_set_agravmag.argtypes = [vp, _dp]
     lib.serenx_force_eval.argtypes = [vp, C.c_int32]
     lib.serenx_fetch.argtypes = [vp, _ip] + [_dp] * 14 + [_ip, _ip]
@@ -218,6 +221,16 @@ class ForceEngine:
         off = np.zeros(ncloud + 1, np.int32)
         _check(self.lib.serenx_cloud_offsets(self.ctx, _iptr(off)))
         return off
+
+    def set_precision(self, precision: str) -> None:
+        """'fp64' or 'mixed' for later evaluations (guard steps)."""
+        _check(self.lib.serenx_set_precision(self.ctx, _PREC[precision]))
+
+    def memory(self) -> Dict[str, int]:
+        """Free / total device bytes and the bytes this context holds."""
+        fr, tot, mine = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(self.lib.serenx_memory(self.ctx, C.byref(fr), C.byref(tot), C.byref(mine)))
+        return {"free": fr.value, "total": tot.value, "context": mine.value}
 
     def set_agravmag(self, am) -> None:
         am = np.ascontiguousarray(am, np.float64)
