@@ -72,7 +72,8 @@ class CloudServer:
     def estimate_bytes(n: int) -> int:
         """Device bytes of a resident cloud of n particles (particle and step
         state, records, neighbour rows, h-solve lists, tree of <= 2n cells),
-        rounded up; the exact figure is charged once the cloud is loaded."""
+        rounded up; a loaded cloud is charged the larger of this and the
+        bytes its context holds."""
         return int(n * 2800 + (96 << 20))
 
     def admit(self, cloud: int, pos, vel, mass, h=None, timeout: Optional[float] = None) -> None:
@@ -103,12 +104,15 @@ class CloudServer:
                 self.used -= need
                 self.mem.notify_all()
             raise
+        # charge the larger of the estimate and the bytes held now: the
+        # estimate covers buffers that grow later (overflow lists, sort space)
+        charged = max(need, actual)
         with self.mem:
-            self.used += actual - need
+            self.used += charged - need
         self.engines[cloud] = eng
         self.nparticles[cloud] = len(mass)
         self.nsteps[cloud] = 0
-        self.bytes[cloud] = actual
+        self.bytes[cloud] = charged
         self.locks[cloud] = threading.Lock()
 
     def release(self, cloud: int) -> None:
