@@ -47,8 +47,13 @@ enum {
     SERENX_EFEW = 7      /* fewer particles than pp_gather: h solve cannot close */
 };
 
-/* MULTIPOLE option (Makefile: MULTIPOLE = QUADRUPOLE | OCTUPOLE) */
-enum { SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3 };
+/* MULTIPOLE option (Makefile: MULTIPOLE = QUADRUPOLE | OCTUPOLE).
+ * SERENX_OCTUPOLE reproduces BHgrav_stock.F90 exactly, including its branch
+ * parallel-axis shift (:263-297), whose quadrupole terms are half the exact
+ * ones and whose xyz term takes 25 for 5.  SERENX_OCTUPOLE_EXACT (no
+ * counterpart in the reference) stocks exact octupole moments at every level;
+ * serenx_export_tree reports them in the same units. */
+enum { SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3, SERENX_OCTUPOLE_EXACT = 4 };
 /* MAC option (Makefile: MAC = GEOMETRIC | GADGET | GADGET2) */
 enum {
     SERENX_MAC_GEOMETRIC = 0, /* GEOMETRIC_MAC: d^2 > diag^2 / theta^2 */

@@ -299,9 +299,11 @@ class CSeren(_Common):
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, octupole: bool, mac: str, kernel: str = "m4"):
+    def __init__(self, octupole, mac: str, kernel: str = "m4"):
+        """octupole: False (quadrupole), True (the reference's octupole
+        stocking) or "exact" (exact branch parallel-axis shift)."""
         self.lib = self._load()
-        self.octupole = bool(octupole)
+        self.octupole = 2 if octupole == "exact" else int(bool(octupole))
         self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2, "eigen": 3}[mac]
         self.tc = {"m4": 0, "m4tc": 1}[kernel]
         self.ctx = None

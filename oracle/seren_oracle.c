@@ -92,6 +92,48 @@ typedef struct oracle_ctx {
     int err; /* 0 ok, 1 LMAX reached, 2 too few particles */
 } oracle_ctx;
 
+/* Exact parallel-axis shift of a child's traceless octupole (the library's
+ * SERENX_OCTUPOLE_EXACT mode; no counterpart in the reference, whose shift
+ * BHgrav_stock.F90:263-297 halves the quadrupole terms and takes 25 for 5 on
+ * the xyz term).  From the moments' definitions,
+ *   O_ijk = O'_ijk + m [15 d_i d_j d_k - 3 d^2 (d_i delta_jk + 2 perms)]
+ *         + 5 (d_i Q'_jk + 2 perms) - 2 ((Q'd)_i delta_jk + 2 perms),
+ * in the reference's s-units (s0 = O_xxx/3, s1 = O_xxy, s2 = O_xyy,
+ * s3 = O_yyy/3, s4 = O_xzz, s5 = O_yzz, s6 = O_zzz/3, s7 = O_xxz, s8 = O_yyz)
+ * with a stored cell holding s9 = 2 O_xyz, which makes the walk's expansion
+ * (coefficient 1 on s9 xyz) the exact series.  Parity unpinned against the
+ * reference (it has no such mode); pinned against moments summed directly
+ * over each cell's particles and the far-field error order. */
+static void oct_shift_exact(double *sc, const double *cs, const double *q, double mcc,
+                            const double *dr, double drsqd) {
+    const double qzz = -(q[0] + q[2]);
+    const double qd0 = q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2];
+    const double qd1 = q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2];
+    const double qd2 = q[3] * dr[0] + q[4] * dr[1] + qzz * dr[2];
+    for (int d = 0; d < 9; d++) sc[d] = sc[d] + cs[d];
+    sc[9] = sc[9] + 0.5 * cs[9];
+    sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * dr[0] * q[0] - 2.0 * qd0);
+    sc[1] = sc[1] + mcc * dr[1] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[0] * q[1] + dr[1] * q[0]) - 2.0 * qd1);
+    sc[2] = sc[2] + mcc * dr[0] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[1] * q[1] + dr[0] * q[2]) - 2.0 * qd0);
+    sc[3] = sc[3] + mcc * dr[1] * (5.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * dr[1] * q[2] - 2.0 * qd1);
+    sc[4] = sc[4] + mcc * dr[0] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[2] * q[3] + dr[0] * qzz) - 2.0 * qd0);
+    sc[5] = sc[5] + mcc * dr[1] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[2] * q[4] + dr[1] * qzz) - 2.0 * qd1);
+    sc[6] = sc[6] + mcc * dr[2] * (5.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * dr[2] * qzz - 2.0 * qd2);
+    sc[7] = sc[7] + mcc * dr[2] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[0] * q[3] + dr[2] * q[0]) - 2.0 * qd2);
+    sc[8] = sc[8] + mcc * dr[2] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[1] * q[4] + dr[2] * q[2]) - 2.0 * qd2);
+    sc[9] = sc[9] + 15.0 * mcc * dr[0] * dr[1] * dr[2] +
+            5.0 * (dr[0] * q[4] + dr[1] * q[3] + dr[2] * q[1]);
+}
+
 /* x**n for a small integer constant n.  The reference compiler expands
  * integer powers as a left-to-right product ((x*x)*x)*...; the kernel tables
  * only reproduce bit-for-bit with that association. */
@@ -491,6 +533,7 @@ static void BHgrav_stock(oracle_ctx *c) {
                 for (int k = 0; k < 3; k++) g->r[k] = rc[k];
                 for (int k = 0; k < 5; k++) g->q[k] = qc[k];
                 for (int k = 0; k < 10; k++) g->s[k] = c->octupole ? sc[k] : 0.0;
+                if (c->octupole == 2) g->s[9] = 2.0 * sc[9];
             } else if (g->leaf == 0) {
                 int cfirst = g->ifopen, cend = g->nextcell, cc = cfirst;
                 for (;;) {
@@ -524,7 +567,9 @@ static void BHgrav_stock(oracle_ctx *c) {
                     qc[2] = qc[2] + mcc * (3.0 * dr[1] * dr[1] - drsqd);
                     qc[3] = qc[3] + mcc * (3.0 * dr[2] * dr[0]);
                     qc[4] = qc[4] + mcc * (3.0 * dr[2] * dr[1]);
-                    if (c->octupole) { /* :263-297 (shift kept as written) */
+                    if (c->octupole == 2) {
+                        oct_shift_exact(sc, h->s, q, mcc, dr, drsqd);
+                    } else if (c->octupole) { /* :263-297 (shift kept as written) */
                         for (int k = 0; k < 10; k++) sc[k] = sc[k] + h->s[k];
                         sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
                                 1.5 * dr[0] * q[0] - dr[1] * q[1] - dr[2] * q[3];
@@ -560,6 +605,7 @@ static void BHgrav_stock(oracle_ctx *c) {
                 for (int k = 0; k < 3; k++) g->r[k] = rc[k];
                 for (int k = 0; k < 5; k++) g->q[k] = qc[k];
                 for (int k = 0; k < 10; k++) g->s[k] = c->octupole ? sc[k] : 0.0;
+                if (c->octupole == 2) g->s[9] = 2.0 * sc[9];
             }
             /* MAC quantities :317-348 */
             for (int k = 0; k < 3; k++)
@@ -1304,7 +1350,8 @@ void oracle_get_grav_tree(const oracle_ctx *c, int nc, int *leaf, int *plist, in
         dminsqd[ci] = g->dminsqd;
         mac[ci] = g->mac;
         for (int k = 0; k < 5; k++) q[5 * ci + k] = g->q[k];
-        for (int k = 0; k < 10; k++) s[10 * ci + k] = g->s[k];
+        for (int k = 0; k < 10; k++)
+            s[10 * ci + k] = (k == 9 && c->octupole == 2) ? 0.5 * g->s[k] : g->s[k];
         hmax[ci] = c->BHstock[ci].hmax;
     }
 }

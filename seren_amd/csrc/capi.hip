@@ -171,8 +171,11 @@ void free_all(Ctx &c) {
 }
 
 int check_options(const serenx_options &o) {
-    if (o.multipole != SERENX_QUADRUPOLE && o.multipole != SERENX_OCTUPOLE)
-        return set_error(SERENX_EINVAL, "multipole must be SERENX_QUADRUPOLE or SERENX_OCTUPOLE");
+    if (o.multipole != SERENX_QUADRUPOLE && o.multipole != SERENX_OCTUPOLE &&
+        o.multipole != SERENX_OCTUPOLE_EXACT)
+        return set_error(SERENX_EINVAL,
+                         "multipole must be SERENX_QUADRUPOLE, SERENX_OCTUPOLE or "
+                         "SERENX_OCTUPOLE_EXACT");
     if (o.mac < SERENX_MAC_GEOMETRIC || o.mac > SERENX_MAC_EIGEN)
         return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET, GADGET2 or EIGEN");
     if (o.precision != SERENX_FP64 && o.precision != SERENX_MIXED)
@@ -775,7 +778,9 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int
         if (q)
             for (int k = 0; k < 5; k++) q[5 * b + k] = gg.q[k];
         if (s)
-            for (int k = 0; k < 10; k++) s[10 * b + k] = gg.s[k];
+            for (int k = 0; k < 10; k++)
+                s[10 * b + k] = (k == 9 && c.opt.multipole == SERENX_OCTUPOLE_EXACT)
+                                    ? 0.5 * gg.s[k] : gg.s[k];
         if (rmax) rmax[b] = hh[d].rmax;
         if (hrangemax) hrangemax[b] = hh[d].hrangemax;
     }

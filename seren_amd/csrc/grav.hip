@@ -453,7 +453,7 @@ void launch_grav(Ctx &c) {
 
 int grav_forces(Ctx &c) {
     SX_HIP(hipMemsetAsync(c.icount, 0, 4 * sizeof(unsigned long long), c.stream));
-    const bool oct = c.opt.multipole == SERENX_OCTUPOLE;
+    const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
     switch (c.opt.mac) {
     case SERENX_MAC_GEOMETRIC:
         oct ? launch_grav<1, SERENX_MAC_GEOMETRIC>(c) : launch_grav<0, SERENX_MAC_GEOMETRIC>(c);

@@ -330,6 +330,48 @@ __global__ void k_gather(int n, const int *__restrict__ perm, const T *__restric
     if (i < n) dst[i] = src[perm[i]];
 }
 
+// Exact parallel-axis shift of a child's traceless octupole to the parent's
+// COM (multipole SERENX_OCTUPOLE_EXACT).  With d the child's COM offset, Q'
+// its traceless quadrupole and m its mass,
+//   O_ijk = O'_ijk + m [15 d_i d_j d_k - 3 d^2 (d_i delta_jk + 2 perms)]
+//         + 5 (d_i Q'_jk + 2 perms) - 2 ((Q'd)_i delta_jk + 2 perms).
+// s[] holds the reference's units (BHgrav_stock.F90:194-206): s0 = O_xxx/3,
+// s1 = O_xxy, s2 = O_xyy, s3 = O_yyy/3, s4 = O_xzz, s5 = O_yzz, s6 = O_zzz/3,
+// s7 = O_xxz, s8 = O_yyz, and a stored cell carries s9 = 2 O_xyz (the walk's
+// expansion has coefficient 1 on s9 xyz where the series has 2 O_xyz/3).
+// The reference's branch shift (:263-297) halves the quadrupole terms and
+// takes 25 for 5 on the xyz term; SERENX_OCTUPOLE keeps that form.
+__device__ __forceinline__ void octupole_shift_exact(double *sc, const double *cs,
+                                                     const double *q, double mcc,
+                                                     const double *dr, double drsqd) {
+    const double qzz = -(q[0] + q[2]);
+    const double qd0 = q[0] * dr[0] + q[1] * dr[1] + q[3] * dr[2];
+    const double qd1 = q[1] * dr[0] + q[2] * dr[1] + q[4] * dr[2];
+    const double qd2 = q[3] * dr[0] + q[4] * dr[1] + qzz * dr[2];
+    for (int d = 0; d < 9; d++) sc[d] = sc[d] + cs[d];
+    sc[9] = sc[9] + 0.5 * cs[9];
+    sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * dr[0] * q[0] - 2.0 * qd0);
+    sc[1] = sc[1] + mcc * dr[1] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[0] * q[1] + dr[1] * q[0]) - 2.0 * qd1);
+    sc[2] = sc[2] + mcc * dr[0] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[1] * q[1] + dr[0] * q[2]) - 2.0 * qd0);
+    sc[3] = sc[3] + mcc * dr[1] * (5.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * dr[1] * q[2] - 2.0 * qd1);
+    sc[4] = sc[4] + mcc * dr[0] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[2] * q[3] + dr[0] * qzz) - 2.0 * qd0);
+    sc[5] = sc[5] + mcc * dr[1] * (15.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[2] * q[4] + dr[1] * qzz) - 2.0 * qd1);
+    sc[6] = sc[6] + mcc * dr[2] * (5.0 * dr[2] * dr[2] - 3.0 * drsqd) +
+            (5.0 * dr[2] * qzz - 2.0 * qd2);
+    sc[7] = sc[7] + mcc * dr[2] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[0] * q[3] + dr[2] * q[0]) - 2.0 * qd2);
+    sc[8] = sc[8] + mcc * dr[2] * (15.0 * dr[1] * dr[1] - 3.0 * drsqd) +
+            (5.0 * (2.0 * dr[1] * q[4] + dr[2] * q[2]) - 2.0 * qd2);
+    sc[9] = sc[9] + 15.0 * mcc * dr[0] * dr[1] * dr[2] +
+            5.0 * (dr[0] * q[4] + dr[1] * q[3] + dr[2] * q[1]);
+}
+
 // ---- stocking: BHgrav_stock.F90:53-354 and BHhydro_stock.F90:40-159 -------
 // One thread per cell of one level, deepest level first.  Arithmetic follows
 // the reference's operation order (built without FMA contraction) so that
@@ -424,7 +466,9 @@ __global__ void __launch_bounds__(256) k_stock(
             qc[2] = qc[2] + mcc * (3.0 * dr[1] * dr[1] - drsqd);
             qc[3] = qc[3] + mcc * (3.0 * dr[2] * dr[0]);
             qc[4] = qc[4] + mcc * (3.0 * dr[2] * dr[1]);
-            if (octupole) {
+            if (octupole == 2) {
+                octupole_shift_exact(sc, k.s, q, mcc, dr, drsqd);
+            } else if (octupole) {
                 // parallel-axis shift exactly as BHgrav_stock.F90:263-297
                 for (int d = 0; d < 10; d++) sc[d] = sc[d] + k.s[d];
                 sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
@@ -491,6 +535,7 @@ __global__ void __launch_bounds__(256) k_stock(
     g.mac = macv;
     for (int k = 0; k < 5; k++) g.q[k] = qc[k];
     for (int k = 0; k < 10; k++) g.s[k] = octupole ? sc[k] : 0.0;
+    if (octupole == 2) g.s[9] = 2.0 * sc[9];   // the walk's xyz coefficient is 1
     for (int k = 0; k < 3; k++) {
         bbmin[3 * c + k] = bmn[k];
         bbmax[3 * c + k] = bmx[k];
@@ -810,7 +855,8 @@ int build_tree(Ctx &c) {
 
 int stock_tree(Ctx &c) {
     hipStream_t s = c.stream;
-    const int oct = c.opt.multipole == SERENX_OCTUPOLE;
+    const int oct = c.opt.multipole == SERENX_OCTUPOLE_EXACT ? 2
+                    : c.opt.multipole == SERENX_OCTUPOLE ? 1 : 0;
     for (int L = c.ltot; L >= 0; L--) {
         const int cnt = c.level_cnt[L];
         if (cnt == 0) continue;

@@ -28,12 +28,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libserenx.so")
 
-QUADRUPOLE, OCTUPOLE = 2, 3
+QUADRUPOLE, OCTUPOLE, OCTUPOLE_EXACT = 2, 3, 4
 MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2, MAC_EIGEN = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 EVAL_HGUESS, EVAL_NO_GRAV, EVAL_NO_HYDRO = 1, 2, 4
 
-_MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE}
+_MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE, "octupole_exact": OCTUPOLE_EXACT}
 _MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2,
         "eigen": MAC_EIGEN}
 _PREC = {"fp64": FP64, "mixed": MIXED}

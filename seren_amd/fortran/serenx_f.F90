@@ -7,7 +7,7 @@ MODULE serenx_f
   implicit none
 
   integer(c_int32_t), parameter :: SERENX_OK = 0
-  integer(c_int32_t), parameter :: SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3
+  integer(c_int32_t), parameter :: SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3, SERENX_OCTUPOLE_EXACT = 4
   integer(c_int32_t), parameter :: SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, &
        & SERENX_MAC_GADGET2 = 2, SERENX_MAC_EIGEN = 3
   integer(c_int32_t), parameter :: SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1

@@ -32,16 +32,21 @@ CASES = [
     (2000, "uniform", 3, False, "geometric", 0.25, 1e-4, 1),
     (6000, "plummer", 5, False, "geometric", 0.25, 1e-4, 1),
     (5000, "r2", 7, True, "gadget2", 0.25, 1e-4, 2),
+    # exact branch octupole shift (no reference counterpart; the oracle's
+    # moments are pinned against direct per-cell sums)
+    (4000, "r2", 8, "exact", "gadget2", 0.25, 1e-4, 2),
 ]
+
+_MULTI = {False: "quadrupole", True: "octupole", "exact": "octupole_exact"}
 
 
 def _ids(c):
-    return f"n{c[0]}-{c[1]}-{'oct' if c[3] else 'quad'}-{c[4]}-x{c[7]}"
+    return f"n{c[0]}-{c[1]}-{_MULTI[c[3]]}-{c[4]}-x{c[7]}"
 
 
 def _engine(octupole, mac, theta2, abserr, kernel="m4"):
     from seren_amd.engine import ForceEngine
-    return ForceEngine(multipole="octupole" if octupole else "quadrupole", mac=mac,
+    return ForceEngine(multipole=_MULTI[octupole], mac=mac,
                        precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
                        kernel=kernel)
 

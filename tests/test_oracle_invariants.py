@@ -229,3 +229,118 @@ def test_tc_kernel_gradient_table():
     assert np.all(tt[1][:k] == -0.318309886)
     assert np.array_equal(tt[1][k:], tp[4][k:])
     assert np.array_equal(np.delete(tt, 1, axis=0), np.delete(tp, 1, axis=0))
+
+
+# ---- octupole moments ------------------------------------------------------
+# s-units of the stocked octupole (BHgrav_stock.F90:194-206) as entries of the
+# traceless tensor O_ijk = sum m (15 x_i x_j x_k - 3 r^2 (x_i d_jk + x_j d_ik
+# + x_k d_ij)): (index triple, divisor)
+_S_UNITS = [((0, 0, 0), 3.0), ((0, 0, 1), 1.0), ((0, 1, 1), 1.0), ((1, 1, 1), 3.0),
+            ((0, 2, 2), 1.0), ((1, 2, 2), 1.0), ((2, 2, 2), 3.0), ((0, 0, 2), 1.0),
+            ((1, 1, 2), 1.0), ((0, 1, 2), 1.0)]
+
+
+def _octupole_tensor(d, ms):
+    eye = np.eye(3)
+    r2 = (d * d).sum(1)
+    t = 15.0 * np.einsum("p,pi,pj,pk->ijk", ms, d, d, d)
+    v = np.einsum("p,pi->i", ms * r2, d)
+    t -= 3.0 * (np.einsum("i,jk->ijk", v, eye) + np.einsum("j,ik->ijk", v, eye) +
+                np.einsum("k,ij->ijk", v, eye))
+    return t
+
+
+def _s_from_tensor(t):
+    return np.array([t[ijk] / div for ijk, div in _S_UNITS])
+
+
+def _tensor_from_s(s):
+    t = np.zeros((3, 3, 3))
+    for v, (ijk, div) in zip(s, _S_UNITS):
+        i, j, k = ijk
+        for p in {(i, j, k), (i, k, j), (j, i, k), (j, k, i), (k, i, j), (k, j, i)}:
+            t[p] = v * div
+    return t
+
+
+def _lopsided_cloud(n=3000, seed=23):
+    """Two unequal Plummer-like clumps plus a sheet: strong octupoles at
+    every level."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_t(3, size=(n // 2, 3)) * 0.05 + [0.3, 0.1, -0.2]
+    b = rng.standard_t(3, size=(n // 3, 3)) * 0.12 - [0.2, 0.3, 0.0]
+    c = np.column_stack([rng.uniform(-0.5, 0.5, n - len(a) - len(b)),
+                         rng.uniform(-0.5, 0.5, n - len(a) - len(b)) * 0.1 + 0.4,
+                         rng.uniform(-0.5, 0.5, n - len(a) - len(b))])
+    pos = np.clip(np.vstack([a, b, c]), -2.0, 2.0)
+    mass = rng.uniform(0.5, 1.5, n) / n
+    h = np.full(n, 0.05)
+    return pos, np.zeros_like(pos), mass, h
+
+
+def _stocked(octupole):
+    from oracle_lib import CSeren
+    pos, vel, mass, h = _lopsided_cloud()
+    o = CSeren(octupole, "gadget2")
+    o.setup(pos, mass, h, vel, abserror=1e-4)
+    o.tree_update()
+    return o.particles(), o.grav_tree()
+
+
+def test_exact_octupole_moments_equal_direct_sums():
+    """Exact-shift octupole (multipole 'octupole_exact'): every cell's
+    stocked octupole equals the traceless octupole summed directly over the
+    cell's particles about its COM.  The reference's shift
+    (BHgrav_stock.F90:263-297, kept as written in 'octupole' mode) matches
+    the direct sums on leaves but not on branches with structured children."""
+    p, t = _stocked("exact")
+    pr, tr = _stocked(True)
+    order, rng = _cell_ranges(t)
+    x, m, leaf = p["pos"], p["m"], t["leaf"]
+    worst_exact, worst_ref_branch = 0.0, 0.0
+    for c in order:
+        if leaf[c] == 1:
+            assert np.all(t["s"][c] == 0.0)
+            continue
+        a, b = rng[c]
+        ms, d = m[a:b + 1], x[a:b + 1] - t["r"][c]
+        scale = (ms * ((d * d).sum(1)) ** 1.5).sum() + 1e-300
+        want = _s_from_tensor(_octupole_tensor(d, ms))
+        err = np.max(np.abs(t["s"][c] - want)) / scale
+        worst_exact = max(worst_exact, err)
+        eref = np.max(np.abs(tr["s"][c] - want)) / scale
+        if leaf[c] > 1:
+            assert eref <= 1e-11                    # particle sums are exact
+        else:
+            worst_ref_branch = max(worst_ref_branch, eref)
+    assert worst_exact <= 1e-11
+    assert worst_ref_branch > 1e-3                  # the reference's shift defect
+    # everything but the octupole is the same in both modes
+    for k in ("leaf", "nextcell", "r", "m", "q", "dminsqd", "mac"):
+        assert np.array_equal(t[k], tr[k]), k
+
+
+def test_octupole_expansion_error_order():
+    """Root-cell potential at distance R with monopole + quadrupole +
+    octupole: the series error relative to the direct sum falls as R^-4
+    (x1/16 per doubling) with exact moments; the reference's shifted
+    octupole leaves an R^-3 error (x1/8)."""
+    ratios = {}
+    for mode in ("exact", True):
+        p, t = _stocked(mode)
+        x, m = p["pos"], p["m"]
+        q = t["q"][0]
+        Q = np.array([[q[0], q[1], q[3]], [q[1], q[2], q[4]], [q[3], q[4], -(q[0] + q[2])]])
+        O = _tensor_from_s(t["s"][0])
+        errs = []
+        for R in (16.0, 32.0, 64.0):
+            pt = t["r"][0] + R * np.array([0.48, -0.6, 0.64])
+            direct = np.sum(m / np.linalg.norm(x - pt, axis=1))
+            dr = pt - t["r"][0]
+            r = np.linalg.norm(dr)
+            approx = (t["m"][0] / r + 0.5 * dr @ Q @ dr / r ** 5 +
+                      np.einsum("ijk,i,j,k->", O, dr, dr, dr) / (6.0 * r ** 7))
+            errs.append(abs(approx - direct) / direct)
+        ratios[mode] = (errs[1] / errs[0], errs[2] / errs[1])
+    assert max(ratios["exact"]) < 1.0 / 12.0, ratios
+    assert min(ratios[True]) > 1.0 / 11.0, ratios
