@@ -272,13 +272,33 @@ enum {
                              (the reference leaves dt_half unset there) */
 };
 
+/* Makefile TIMESTEP: how the largest step is chosen at a resynchronisation
+ * (sph_timesteps.F90:84-91) */
+enum {
+    SERENX_LEVELS_ADAPTIVE = 0,    /* dt_max = dt_min 2^level_max */
+    SERENX_LEVELS_RESTRICTED = 1,  /* dt_min rounded down to dt_fixed / 2^k */
+    SERENX_LEVELS_FIXED = 2        /* dt_min = dt_fixed / 2^level_max */
+};
+/* Makefile CHECK_NEIB_TIMESTEP: neighbour time-step limiter
+ * (check_neighbour_timesteps.F90, reduce_timesteps.F90) */
+enum {
+    SERENX_NEIB_CHECK_OFF = 0,
+    SERENX_NEIB_CHECK_END = 1,       /* levels lowered at the end of a step */
+    SERENX_NEIB_CHECK_IMMEDIATE = 2  /* ... or at once when synchronised
+                                        (IMMEDIATE_TIMESTEP_REDUCTION) */
+};
+
 typedef struct serenx_step_options {
     int32_t integrator;   /* SERENX_INT_RK or SERENX_INT_LFKDK */
     int32_t nlevels;      /* block time-step levels (parameter nlevels, default 1) */
     int32_t nbuildstep;   /* integer steps between tree builds (default 8) */
-    int32_t reserved;
+    int32_t levels;       /* SERENX_LEVELS_* (Makefile default RESTRICTED) */
     double accel_mult;    /* timestep_size acceleration factor (default 0.5) */
     double courant_mult;  /* timestep_size Courant factor (default 0.2) */
+    double dt_fixed;      /* reference step for RESTRICTED / FIXED levels
+                             (parameter dt_fixed, default 1.0) */
+    int32_t neib_check;   /* SERENX_NEIB_CHECK_* (Makefile default END) */
+    int32_t reserved;
 } serenx_step_options;
 
 typedef struct serenx_step_info {
@@ -314,9 +334,12 @@ int serenx_step(serenx_ctx *ctx, int32_t count);
 int serenx_step_get_info(const serenx_ctx *ctx, serenx_step_info *info);
 
 /* Per-particle step state in tree order; any pointer may be NULL.  Vectors
- * row-major [n][3]; active = accdo (1/0). */
+ * row-major [n][3]; active = accdo (1/0); nminneib = the deepest level among
+ * the particles that hold this one as a neighbour (zero without the
+ * neighbour check). */
 int serenx_fetch_step(serenx_ctx *ctx, int32_t *nlevel, int64_t *nlast, double *laststep,
-                      int32_t *active, double *r_old, double *v_old, double *v_half);
+                      int32_t *active, double *r_old, double *v_old, double *v_half,
+                      int32_t *nminneib);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,9 @@ VARIANTS = {
     (True, "gadget2"): "oct_gadget2",
     (True, "gadget"): "oct_gadget",
     (True, "eigen"): "oct_eigen",
+    # time-step options (Makefile TIMESTEP, CHECK_NEIB_TIMESTEP)
+    (False, "geometric", "m4", "restricted_1"): "quad_geo_tsr1",
+    (False, "geometric", "m4", "fixed_2"): "quad_geo_tsf2",
 }
 
 
@@ -144,11 +147,18 @@ class RefSeren(_Common):
 
     _libs: Dict[str, C.CDLL] = {}
 
-    def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4"):
+    def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4",
+                 timestep: str = ""):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
-        oracle/Makefile.ref dropin): the reference code as the host."""
-        key = (bool(octupole), mac) if kernel == "m4" else (bool(octupole), mac, kernel)
+        oracle/Makefile.ref dropin): the reference code as the host.
+        timestep selects a build with the Makefile's TIMESTEP and
+        CHECK_NEIB_TIMESTEP options ("restricted_1", "fixed_2"); the default
+        builds have neither."""
+        if timestep:
+            key = (bool(octupole), mac, kernel, timestep)
+        else:
+            key = (bool(octupole), mac) if kernel == "m4" else (bool(octupole), mac, kernel)
         variant = VARIANTS[key]
         kind = "dropin" if dropin else "ref"
         path = os.path.join(HERE, "_ref", f"libseren_{kind}_{variant}.so")
@@ -170,10 +180,12 @@ class RefSeren(_Common):
             lib.ref_get_grav_tree.argtypes = [C.c_int, _ip, _ip, _ip, _ip] + [_dp] * 7
             lib.ref_get_hydro_tree.argtypes = [C.c_int, _dp, _dp, _dp]
             lib.ref_get_kernel.argtypes = [_dp]
-            lib.ref_step_setup.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+            lib.ref_step_setup.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                           C.c_double]
             lib.ref_integrate.argtypes = [C.c_double]
             lib.ref_get_step.argtypes = [_lp, _dp]
-            lib.ref_get_step_particles.argtypes = [C.c_int, _lp, _lp, _dp, _ip, _dp, _dp, _dp]
+            lib.ref_get_step_particles.argtypes = [C.c_int, _lp, _lp, _dp, _ip, _dp, _dp, _dp,
+                                                   _lp]
             self._libs[variant] = lib
         self.lib = self._libs[variant]
         self.variant = variant
@@ -204,9 +216,10 @@ class RefSeren(_Common):
         self.lib.ref_grav_forces()
 
     # ---- integration steps (sph_integrate.F90 with RUNGE_KUTTA) ---------
-    def step_setup(self, nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2):
+    def step_setup(self, nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
+                   dt_fixed=1.0):
         """Step state after the setup evaluation (initialize_sph_variables_2)."""
-        self.lib.ref_step_setup(nlevels, nbuildstep, accel_mult, courant_mult)
+        self.lib.ref_step_setup(nlevels, nbuildstep, accel_mult, courant_mult, dt_fixed)
 
     def integrate(self, cs):
         """One sph_integrate: tree_update, sph_update, thermal, forces,
@@ -227,9 +240,10 @@ class RefSeren(_Common):
         nlevel, nlast = np.zeros(n, np.int64), np.zeros(n, np.int64)
         laststep, accdo = np.zeros(n), np.zeros(n, np.int32)
         vh, ro, vo = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
-        self.lib.ref_get_step_particles(n, nlevel, nlast, laststep, accdo, vh, ro, vo)
+        nmn = np.zeros(n, np.int64)
+        self.lib.ref_get_step_particles(n, nlevel, nlast, laststep, accdo, vh, ro, vo, nmn)
         return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=accdo,
-                    v_half=vh, r_old=ro, v_old=vo)
+                    v_half=vh, r_old=ro, v_old=vo, nminneib=nmn)
 
     def set_agravmag(self, am):
         self.lib.ref_set_agravmag(self.n, _f64(am))

@@ -56,6 +56,9 @@ CONTAINS
     deallocate(rho, temp, press, sound, div_v, rho_old, drhodt, omega)
     deallocate(accdo, nlast, nlevel, laststep)
     deallocate(v_half)
+#if defined(CHECK_NEIGHBOUR_TIMESTEPS)
+    deallocate(nminneib)
+#endif
     deallocate(first_cell_grav, last_cell_grav, BHgrav)
     deallocate(first_cell_hydro, last_cell_hydro, BHhydro)
     deallocate(cellof, BHnextptcl, whichchild, BHtemp, BHstock)
@@ -98,6 +101,10 @@ CONTAINS
     allocate(div_v(1:pmax), rho_old(1:pmax), drhodt(1:pmax), omega(1:pmax))
     allocate(accdo(1:pmax), nlast(1:pmax), nlevel(1:pmax), laststep(1:pmax))
     allocate(v_half(1:VDIM,1:pmax))
+#if defined(CHECK_NEIGHBOUR_TIMESTEPS)
+    allocate(nminneib(1:pmax))
+    nminneib(1:pmax) = 0
+#endif
 
     cmax_grav  = 2*(pgas + pcdm)
     cmax_hydro = 2*pmax
@@ -352,10 +359,10 @@ CONTAINS
   end subroutine ref_get_kernel
 
 ! ----------------------------------------------------------------------------
-  subroutine ref_step_setup(nlevels_in, nbuildstep_in, accel_in, courant_in) &
+  subroutine ref_step_setup(nlevels_in, nbuildstep_in, accel_in, courant_in, dtfixed_in) &
        & bind(C, name="ref_step_setup")
     integer(c_int), value :: nlevels_in, nbuildstep_in
-    real(c_double), value :: accel_in, courant_in
+    real(c_double), value :: accel_in, courant_in, dtfixed_in
     r_old(1:NDIM,1:ptot) = parray(1:NDIM,1:ptot)
     v_old(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
     v_half(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
@@ -365,7 +372,7 @@ CONTAINS
     nbuildstep = nbuildstep_in
     accel_mult = accel_in
     courant_mult = courant_in
-    dt_fixed = 1.0_DP
+    dt_fixed = dtfixed_in
     n = 0
     nresync = n
     time = 0.0_DP
@@ -385,7 +392,13 @@ CONTAINS
     call sph_hydro_forces
     call sph_grav_forces
     call sph_timesteps
+#if defined(CHECK_NEIGHBOUR_TIMESTEPS)
+    call check_neighbour_timesteps
+#endif
     call sph_advance
+#if defined(CHECK_NEIGHBOUR_TIMESTEPS) && defined(IMMEDIATE_TIMESTEP_REDUCTION)
+    call reduce_timesteps
+#endif
   end subroutine ref_integrate
 
   subroutine ref_get_step(istate, rstate) bind(C, name="ref_get_step")
@@ -397,14 +410,19 @@ CONTAINS
     rstate(1) = time ; rstate(2) = timestep ; rstate(3) = dt_max ; rstate(4) = 0.0_DP
   end subroutine ref_get_step
 
-  subroutine ref_get_step_particles(np, nlev, nl, lstep, acc, vh, ro, vo) &
+  subroutine ref_get_step_particles(np, nlev, nl, lstep, acc, vh, ro, vo, nmn) &
        & bind(C, name="ref_get_step_particles")
     integer(c_int), value :: np
-    integer(c_long_long), intent(out) :: nlev(np), nl(np)
+    integer(c_long_long), intent(out) :: nlev(np), nl(np), nmn(np)
     real(c_double), intent(out) :: lstep(np), vh(3,np), ro(3,np), vo(3,np)
     integer(c_int), intent(out) :: acc(np)
     integer :: p
     do p=1,np
+#if defined(CHECK_NEIGHBOUR_TIMESTEPS)
+       nmn(p) = nminneib(p)
+#else
+       nmn(p) = 0
+#endif
        nlev(p) = nlevel(p)
        nl(p) = nlast(p)
        lstep(p) = laststep(p)

@@ -800,6 +800,9 @@ void serenx_default_step_options(serenx_step_options *opt) {
     opt->nbuildstep = 8;                 // default_parameters.F90:117
     opt->accel_mult = 0.5;               // default_parameters.F90:63
     opt->courant_mult = 0.2;             // default_parameters.F90:62
+    opt->levels = SERENX_LEVELS_RESTRICTED;   // Makefile:100 TIMESTEP = RESTRICTED
+    opt->dt_fixed = 1.0;                      // default_parameters.F90:67
+    opt->neib_check = SERENX_NEIB_CHECK_END;  // Makefile:101 CHECK_NEIB_TIMESTEP = 1
 }
 
 int serenx_step_begin(serenx_ctx *ctx, const serenx_step_options *opt) {
@@ -844,11 +847,12 @@ int serenx_step_get_info(const serenx_ctx *ctx, serenx_step_info *info) {
 }
 
 int serenx_fetch_step(serenx_ctx *ctx, int32_t *nlevel, int64_t *nlast, double *laststep,
-                      int32_t *active, double *r_old, double *v_old, double *v_half) {
+                      int32_t *active, double *r_old, double *v_old, double *v_half,
+                      int32_t *nminneib) {
     if (!ctx) return set_error(SERENX_EINVAL, "null context");
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
-    return step_fetch(c, nlevel, nlast, laststep, active, r_old, v_old, v_half);
+    return step_fetch(c, nlevel, nlast, laststep, active, r_old, v_old, v_half, nminneib);
 }
 
 int serenx_set_precision(serenx_ctx *ctx, int32_t precision) {

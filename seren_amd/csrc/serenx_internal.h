@@ -190,7 +190,9 @@ struct Ctx {
         double *laststep = nullptr, *dt = nullptr;
         long long *nlast = nullptr;
         int *nlevel = nullptr;
-        unsigned long long *red = nullptr;   // device reductions [dt_min bits, p_update, level max, nactive]
+        int *nminneib = nullptr;             // neighbour check: deepest level among neighbours
+        unsigned long long *red = nullptr;   // device reductions [dt_min bits, p_update, level
+                                             // max, nactive, nminneib max]
         long long n = 0, nsteps = 0, nresync = 0, level_max = 0, level_step = 0;
         long long nbuild = 0, nstock = 0, nactive = 0;
         double time = 0.0, timestep = 0.0, dt_max = 0.0;
@@ -241,7 +243,7 @@ int hydro_forces(Ctx &c);        // hydro_gradh over active particles
 int step_begin(Ctx &c, const serenx_step_options &opt);
 int step_run(Ctx &c, int count);
 int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_t *active,
-               double *r_old, double *v_old, double *v_half);
+               double *r_old, double *v_old, double *v_half, int32_t *nminneib);
 void step_free(Ctx &c);
 int permute_step(Ctx &c, const int *perm);
 

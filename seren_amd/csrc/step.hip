@@ -27,7 +27,9 @@ constexpr double INVLOGETWO = 1.442695041;   // macros.h:15 (1/ln 2 to 10 digits
 constexpr double TVISC_FAC = 0.6;            // macros.h:133
 constexpr double SMALL_NUMBER_DP = 1.0e-20;  // macros.h:82
 
-enum { R_DTMIN = 0, R_PUPDATE = 1, R_LEVMAX = 2, R_NACTIVE = 3 };
+constexpr int TIMESTEP_LEVEL_DIFF_MAX = 2;   // macros.h:185
+
+enum { R_DTMIN = 0, R_PUPDATE = 1, R_LEVMAX = 2, R_NACTIVE = 3, R_NMAX = 4, R_COUNT = 5 };
 
 // timestep_size.F90:92-114: acceleration and Courant criteria with the AB
 // viscosity terms (beta = 2 alpha, hydro_gradh.F90:227)
@@ -80,13 +82,124 @@ __global__ void k_ts_all(int n, const double *__restrict__ ax, const double *__r
 }
 
 __global__ void k_ts_levels(int n, double dt_max, long long level_max,
-                            const double *__restrict__ dt, int *__restrict__ nlevel) {
+                            const double *__restrict__ dt, int *__restrict__ nlevel,
+                            int *__restrict__ nminneib) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     long long level = level_of(dt_max, dt[p]);
     level = level < level_max ? level : level_max;
     level = level > 0 ? level : 0;
     nlevel[p] = (int)level;
+    if (nminneib) nminneib[p] = (int)level;   // :104-106
+}
+
+// 2^e divides nint, as mod(n, 2**e) == 0 tests it; a negative e makes the
+// reference's 2**e zero and its mod divide by zero, and counts as false here
+__device__ __forceinline__ bool synced(long long nint, long long e) {
+    return e >= 0 && e < 63 && nint % (1LL << e) == 0;
+}
+
+// sph_timesteps.F90:226-240: the neighbour check after a level update.
+// level_step is the value before this update, level_max the new one.
+__global__ void k_ts_neib(int n, long long nint, long long level_step, long long level_max,
+                          const long long *__restrict__ nlast, int *__restrict__ nlevel,
+                          int *__restrict__ nminneib) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    long long nm = nminneib[p];
+    nm = nm < level_max ? nm : level_max;
+    const long long lev = nlevel[p];
+    if (nm - lev > TIMESTEP_LEVEL_DIFF_MAX && nlast[p] == nint &&
+        synced(nint, level_step - nm + TIMESTEP_LEVEL_DIFF_MAX)) {
+        long long nl = nm - TIMESTEP_LEVEL_DIFF_MAX;
+        nl = nl < level_max ? nl : level_max;
+        nlevel[p] = (int)nl;
+        nm = nl;
+    } else if (nlast[p] == nint) {
+        nm = lev;
+    }
+    nminneib[p] = (int)nm;
+}
+
+// check_neighbour_timesteps.F90:43-66: every active particle raises the
+// nminneib of each of its neighbours to its own level (a maximum, so the
+// order of the updates does not matter; without sinks nminneib is never -1)
+__global__ void __launch_bounds__(256) k_neib_levels(
+    int n, const uint8_t *__restrict__ active, const int *__restrict__ nlevel,
+    const int *__restrict__ pptot, const int *__restrict__ pplist,
+    const int *__restrict__ ovf_off, const int *__restrict__ ovf_list, int pp_limit,
+    int *__restrict__ nminneib) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const int levelp = nlevel[p];
+    const int nn = pptot[p];
+    const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
+    for (int i = 0; i < nn; i++) {
+        const int j = lst[i];
+        if (levelp > nminneib[j]) atomicMax(nminneib + j, levelp);
+    }
+}
+
+// reduce_timesteps.F90:28-40 with reduce_particle_timestep.F90:31-66 (after
+// the advance): a particle whose neighbours step more than
+// TIMESTEP_LEVEL_DIFF_MAX levels deeper restarts its step at a deeper level
+// when the integer time allows.  Also the largest nminneib, which must not
+// exceed level_max (:30-33).
+template <int KDK>
+__global__ void k_reduce_steps(int n, long long nint, long long level_step, double timestep,
+                               double *__restrict__ x, double *__restrict__ y,
+                               double *__restrict__ z, const double *__restrict__ vx,
+                               const double *__restrict__ vy, const double *__restrict__ vz,
+                               const double *__restrict__ ax, const double *__restrict__ ay,
+                               const double *__restrict__ az, double *__restrict__ rox,
+                               double *__restrict__ roy, double *__restrict__ roz,
+                               double *__restrict__ vox, double *__restrict__ voy,
+                               double *__restrict__ voz, double *__restrict__ vhx,
+                               double *__restrict__ vhy, double *__restrict__ vhz,
+                               long long *__restrict__ nlast, int *__restrict__ nlevel,
+                               int *__restrict__ nminneib, double *__restrict__ laststep,
+                               uint8_t *__restrict__ active,
+                               unsigned long long *__restrict__ red) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned added = 0;
+    int nmax = 0;
+    if (p < n) {
+        const long long nm = nminneib[p];
+        nmax = (int)nm;
+        if (nm - nlevel[p] > TIMESTEP_LEVEL_DIFF_MAX && nm != -1 && nlast[p] != nint &&
+            synced(nint, level_step - nm + TIMESTEP_LEVEL_DIFF_MAX)) {
+            const long long lnew = nm - TIMESTEP_LEVEL_DIFF_MAX;
+            const double r[3] = {x[p], y[p], z[p]};
+            const double v[3] = {vx[p], vy[p], vz[p]};
+            rox[p] = r[0]; roy[p] = r[1]; roz[p] = r[2];
+            vox[p] = v[0]; voy[p] = v[1]; voz[p] = v[2];
+            const double ls = timestep * (double)(nint - nlast[p]);
+            laststep[p] = ls;
+            if (KDK) {
+                const double a[3] = {ax[p], ay[p], az[p]};
+                double vh[3];
+                for (int k = 0; k < 3; k++) vh[k] = v[k] + 0.5 * a[k] * ls;
+                vhx[p] = vh[0]; vhy[p] = vh[1]; vhz[p] = vh[2];
+                x[p] = r[0] + vh[0] * ls;
+                y[p] = r[1] + vh[1] * ls;
+                z[p] = r[2] + vh[2] * ls;
+            }
+            nlevel[p] = (int)lnew;
+            nlast[p] = nint;
+            nminneib[p] = (int)lnew;
+            added = active[p] ? 0u : 1u;
+            active[p] = 1;   // RUNGE_KUTTA and LEAPFROG_KDK
+        }
+    }
+    unsigned long long u = added;
+    for (int o = 32; o > 0; o >>= 1) {
+        u += __shfl_xor(u, o, 64);
+        nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(red + R_NACTIVE, u);
+        atomicMax(red + R_NMAX, (unsigned long long)(nmax > 0 ? nmax : 0));
+    }
 }
 
 // sph_timesteps.F90:151-170: particles at the end of their step take a new
@@ -244,13 +357,14 @@ int dmalloc(T **p, size_t count) {
     return SERENX_OK;
 }
 
-// sph_timesteps.F90:30-337 (no sinks, no neighbour-step checks)
+// sph_timesteps.F90:30-337 (no sinks)
 int step_timesteps(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
     hipStream_t st = c.stream;
-    unsigned long long init[4] = {(unsigned long long)__builtin_bit_cast(long long, 9.9e20), 0,
-                                  0, 0};
+    const bool neib = s.opt.neib_check != SERENX_NEIB_CHECK_OFF;
+    unsigned long long init[R_COUNT] = {
+        (unsigned long long)__builtin_bit_cast(long long, 9.9e20), 0, 0, 0, 0};
     SX_HIP(hipMemcpyAsync(s.red, init, sizeof(init), hipMemcpyHostToDevice, st));
     const double alpha = c.opt.alpha;
     if (s.n == s.nresync) {
@@ -259,14 +373,21 @@ int step_timesteps(Ctx &c) {
                            c.divv, c.sound, alpha, s.opt.accel_mult, s.opt.courant_mult, s.dt,
                            s.nlast, s.red);
         SX_HIP(hipGetLastError());
-        unsigned long long r[4];
+        unsigned long long r[R_COUNT];
         SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
         SX_HIP(hipStreamSynchronize(st));
-        const double dt_min = __builtin_bit_cast(double, (long long)r[R_DTMIN]);
+        double dt_min = __builtin_bit_cast(double, (long long)r[R_DTMIN]);
         s.level_max = s.opt.nlevels - 1;
+        // :84-91; 2.0**level is a power of two, so ldexp gives the same quotient
+        if (s.opt.levels == SERENX_LEVELS_RESTRICTED) {
+            const long long level = (long long)(INVLOGETWO * std::log(s.opt.dt_fixed / dt_min)) + 1;
+            dt_min = s.opt.dt_fixed / std::ldexp(1.0, (int)level);
+        } else if (s.opt.levels == SERENX_LEVELS_FIXED) {
+            dt_min = s.opt.dt_fixed / std::ldexp(1.0, (int)s.level_max);
+        }
         s.dt_max = dt_min * std::ldexp(1.0, (int)s.level_max);
         hipLaunchKernelGGL(k_ts_levels, dim3(nblk(n)), dim3(TB), 0, st, n, s.dt_max,
-                           s.level_max, s.dt, s.nlevel);
+                           s.level_max, s.dt, s.nlevel, neib ? s.nminneib : nullptr);
         SX_HIP(hipGetLastError());
         s.level_step = s.level_max + 1;   // :131-135, RK and KDK take half steps
         s.nresync = 1LL << s.level_step;
@@ -277,7 +398,7 @@ int step_timesteps(Ctx &c) {
                            s.dt_max, c.ax, c.ay, c.az, c.h, c.divv, c.sound, alpha,
                            s.opt.accel_mult, s.opt.courant_mult, s.nlast, s.nlevel, s.red);
         SX_HIP(hipGetLastError());
-        unsigned long long r[4];
+        unsigned long long r[R_COUNT];
         SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
         SX_HIP(hipStreamSynchronize(st));
         if (r[R_PUPDATE] > 0) {
@@ -287,6 +408,11 @@ int step_timesteps(Ctx &c) {
                 level_max = level_old - 1;
             else if (level_max < level_old)
                 level_max = level_old;
+            if (neib) {
+                hipLaunchKernelGGL(k_ts_neib, dim3(nblk(n)), dim3(TB), 0, st, n, s.n,
+                                   s.level_step, level_max, s.nlast, s.nlevel, s.nminneib);
+                SX_HIP(hipGetLastError());
+            }
             const int shift = (int)(level_max - level_old);
             if (shift != 0) {
                 hipLaunchKernelGGL(k_rescale_nlast, dim3(nblk(n)), dim3(TB), 0, st, n, shift,
@@ -311,11 +437,21 @@ int step_timesteps(Ctx &c) {
     return SERENX_OK;
 }
 
+// check_neighbour_timesteps.F90 (sph_integrate.F90:90-92)
+int step_neib_levels(Ctx &c) {
+    Ctx::Step &s = c.st;
+    hipLaunchKernelGGL(k_neib_levels, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active,
+                       s.nlevel, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit,
+                       s.nminneib);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
 int step_advance(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
     hipStream_t st = c.stream;
-    SX_HIP(hipMemsetAsync(s.red + R_NACTIVE, 0, sizeof(unsigned long long), st));
+    SX_HIP(hipMemsetAsync(s.red + R_NACTIVE, 0, 2 * sizeof(unsigned long long), st));
     if (s.opt.integrator == SERENX_INT_LFKDK)
         hipLaunchKernelGGL(k_advance<1>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
                            s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
@@ -327,10 +463,28 @@ int step_advance(Ctx &c) {
                            s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz, s.nlast,
                            s.nlevel, s.laststep, c.active, s.red);
     SX_HIP(hipGetLastError());
-    unsigned long long na = 0;
-    SX_HIP(hipMemcpyAsync(&na, s.red + R_NACTIVE, sizeof(na), hipMemcpyDeviceToHost, st));
+    const bool reduce = s.opt.neib_check == SERENX_NEIB_CHECK_IMMEDIATE;
+    if (reduce) {   // sph_integrate.F90:103-105
+        if (s.opt.integrator == SERENX_INT_LFKDK)
+            hipLaunchKernelGGL(k_reduce_steps<1>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n,
+                               s.level_step, s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz,
+                               c.ax, c.ay, c.az, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz,
+                               s.vhx, s.vhy, s.vhz, s.nlast, s.nlevel, s.nminneib, s.laststep,
+                               c.active, s.red);
+        else
+            hipLaunchKernelGGL(k_reduce_steps<0>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n,
+                               s.level_step, s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz,
+                               c.ax, c.ay, c.az, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz,
+                               s.vhx, s.vhy, s.vhz, s.nlast, s.nlevel, s.nminneib, s.laststep,
+                               c.active, s.red);
+        SX_HIP(hipGetLastError());
+    }
+    unsigned long long r2[2] = {0, 0};
+    SX_HIP(hipMemcpyAsync(r2, s.red + R_NACTIVE, sizeof(r2), hipMemcpyDeviceToHost, st));
     SX_HIP(hipStreamSynchronize(st));
-    s.nactive = (long long)na;
+    s.nactive = (long long)r2[0];
+    if (reduce && (long long)r2[1] > s.level_max)
+        return set_error(SERENX_ESTATE, "Problem with nminneib > level_max (reduce_timesteps)");
     return SERENX_OK;
 }
 
@@ -360,6 +514,7 @@ int step_once(Ctx &c) {
     SX_TRY(run_grav(c));
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     SX_TRY(step_timesteps(c));
+    if (s.opt.neib_check != SERENX_NEIB_CHECK_OFF) SX_TRY(step_neib_levels(c));
     SX_TRY(step_advance(c));
     return SERENX_OK;
 }
@@ -369,7 +524,7 @@ int step_once(Ctx &c) {
 void step_free(Ctx &c) {
     Ctx::Step &s = c.st;
     void *ptrs[] = {s.rox, s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz,
-                    s.laststep, s.dt, s.nlast, s.nlevel, s.red};
+                    s.laststep, s.dt, s.nlast, s.nlevel, s.nminneib, s.red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     s = Ctx::Step{};
@@ -399,6 +554,10 @@ int permute_step(Ctx &c, const int *perm) {
     // nlevel's old buffer (n ints) is too small to serve as the double
     // scratch, so copy back rather than swap
     SX_HIP(hipMemcpyAsync(s.nlevel, ti, sizeof(int) * n, hipMemcpyDeviceToDevice, c.stream));
+    // nminneib (reorder_array.F90:234-237)
+    hipLaunchKernelGGL(k_gather_step<int>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
+                       (const int *)s.nminneib, ti);
+    SX_HIP(hipMemcpyAsync(s.nminneib, ti, sizeof(int) * n, hipMemcpyDeviceToDevice, c.stream));
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
@@ -411,6 +570,14 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
     if (opt.nbuildstep < 1) return set_error(SERENX_EINVAL, "nbuildstep must be positive");
     if (!(opt.accel_mult > 0.0) || !(opt.courant_mult > 0.0))
         return set_error(SERENX_EINVAL, "accel_mult and courant_mult must be positive");
+    if (opt.levels < SERENX_LEVELS_ADAPTIVE || opt.levels > SERENX_LEVELS_FIXED)
+        return set_error(SERENX_EINVAL, "levels must be SERENX_LEVELS_ADAPTIVE, _RESTRICTED or "
+                                        "_FIXED");
+    if (opt.levels != SERENX_LEVELS_ADAPTIVE && !(opt.dt_fixed > 0.0))
+        return set_error(SERENX_EINVAL, "dt_fixed must be positive");
+    if (opt.neib_check < SERENX_NEIB_CHECK_OFF || opt.neib_check > SERENX_NEIB_CHECK_IMMEDIATE)
+        return set_error(SERENX_EINVAL, "neib_check must be SERENX_NEIB_CHECK_OFF, _END or "
+                                        "_IMMEDIATE");
     if (!c.tree_ok || !c.sph_ok)
         return set_error(SERENX_ESTATE, "step_begin needs the setup force evaluation first");
     if (c.ncloud != 1)
@@ -424,7 +591,9 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
     for (double **a : arrs) SX_TRY(dmalloc(a, n));
     SX_TRY(dmalloc(&s.nlast, n));
     SX_TRY(dmalloc(&s.nlevel, n));
-    SX_TRY(dmalloc(&s.red, 4));
+    SX_TRY(dmalloc(&s.nminneib, n));
+    SX_TRY(dmalloc(&s.red, R_COUNT));
+    SX_HIP(hipMemsetAsync(s.nminneib, 0, sizeof(int) * n, c.stream));
     s.opt = opt;
     s.on = true;
     hipLaunchKernelGGL(k_step_init, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
@@ -455,12 +624,14 @@ int step_run(Ctx &c, int count) {
 }
 
 int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_t *active,
-               double *r_old, double *v_old, double *v_half) {
+               double *r_old, double *v_old, double *v_half, int32_t *nminneib) {
     Ctx::Step &s = c.st;
     if (!s.on) return set_error(SERENX_ESTATE, "no step state (serenx_step_begin)");
     const int n = c.n;
     SX_HIP(hipStreamSynchronize(c.stream));
     if (nlevel) SX_HIP(hipMemcpy(nlevel, s.nlevel, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (nminneib)
+        SX_HIP(hipMemcpy(nminneib, s.nminneib, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (nlast) SX_HIP(hipMemcpy(nlast, s.nlast, sizeof(long long) * n, hipMemcpyDeviceToHost));
     if (laststep)
         SX_HIP(hipMemcpy(laststep, s.laststep, sizeof(double) * n, hipMemcpyDeviceToHost));

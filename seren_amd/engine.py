@@ -54,6 +54,9 @@ EXPORTED = [
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
+# Makefile TIMESTEP and CHECK_NEIB_TIMESTEP options
+_LEVELS = {"adaptive": 0, "restricted": 1, "fixed": 2}
+_NEIB_CHECK = {0: 0, 1: 1, 2: 2, "off": 0, "end": 1, "immediate": 2}
 
 
 class SerenxError(RuntimeError):
@@ -79,8 +82,9 @@ class HostState(C.Structure):
 
 class StepOptions(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("nlevels", C.c_int32), ("nbuildstep", C.c_int32),
-                ("reserved", C.c_int32), ("accel_mult", C.c_double),
-                ("courant_mult", C.c_double)]
+                ("levels", C.c_int32), ("accel_mult", C.c_double),
+                ("courant_mult", C.c_double), ("dt_fixed", C.c_double),
+                ("neib_check", C.c_int32), ("reserved", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -130,7 +134,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_step_begin.argtypes = [vp, C.POINTER(StepOptions)]
     lib.serenx_step.argtypes = [vp, C.c_int32]
     lib.serenx_step_get_info.argtypes = [vp, C.POINTER(StepInfo)]
-    lib.serenx_fetch_step.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _ip, _dp, _dp, _dp]
+    lib.serenx_fetch_step.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _ip, _dp, _dp, _dp,
+                                      _ip]
     _lib = lib
     return lib
 
@@ -311,13 +316,20 @@ class ForceEngine:
 
     # ---- device-resident integration (sph_integrate.F90) ----------------
     def step_begin(self, integrator: str = "rk", nlevels: int = 1, nbuildstep: int = 8,
-                   accel_mult: float = 0.5, courant_mult: float = 0.2) -> None:
-        """Start stepping after the setup evaluation (force_eval(hguess=True))."""
+                   accel_mult: float = 0.5, courant_mult: float = 0.2,
+                   levels: str = "restricted", dt_fixed: float = 1.0,
+                   neib_check=1) -> None:
+        """Start stepping after the setup evaluation (force_eval(hguess=True)).
+        levels and neib_check are the Makefile's TIMESTEP (adaptive,
+        restricted, fixed) and CHECK_NEIB_TIMESTEP (0, 1, 2) options, with
+        the reference Makefile's defaults."""
         o = StepOptions()
         self.lib.serenx_default_step_options(C.byref(o))
         o.integrator = _INTEGRATOR[integrator]
         o.nlevels, o.nbuildstep = nlevels, nbuildstep
         o.accel_mult, o.courant_mult = accel_mult, courant_mult
+        o.levels, o.dt_fixed = _LEVELS[levels], dt_fixed
+        o.neib_check = _NEIB_CHECK[neib_check]
         _check(self.lib.serenx_step_begin(self.ctx, C.byref(o)))
 
     def step(self, count: int = 1) -> None:
@@ -337,11 +349,13 @@ class ForceEngine:
         laststep = np.zeros(n)
         active = np.zeros(n, np.int32)
         r_old, v_old, v_half = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        nminneib = np.zeros(n, np.int32)
         _check(self.lib.serenx_fetch_step(
             self.ctx, _iptr(nlevel), nlast.ctypes.data_as(C.POINTER(C.c_int64)),
-            _dptr(laststep), _iptr(active), _dptr(r_old), _dptr(v_old), _dptr(v_half)))
+            _dptr(laststep), _iptr(active), _dptr(r_old), _dptr(v_old), _dptr(v_half),
+            _iptr(nminneib)))
         return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=active,
-                    r_old=r_old, v_old=v_old, v_half=v_half)
+                    r_old=r_old, v_old=v_old, v_half=v_half, nminneib=nminneib)
 
     def interaction_counts(self):
         """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""

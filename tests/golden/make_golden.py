@@ -107,20 +107,29 @@ def run_case(name, spec):
 # Integration steps (sph_integrate.F90 with RUNGE_KUTTA) after the setup
 # evaluation: name: (n, profile, seed, mach, octupole, mac, nlevels, steps)
 STEP_CASES = {
+    # n, profile, seed, Mach, octupole, MAC, nlevels, steps[, time-step build, dt_fixed]
     "step_rk_uniform2k_quad_geo_nlev3": (2000, "uniform", 11, 1.0, False, "geometric", 3, 6),
     "step_rk_r2_2k_oct_gadget2_nlev1": (2048, "r2", 12, 2.0, True, "gadget2", 1, 4),
+    # Makefile TIMESTEP=RESTRICTED CHECK_NEIB_TIMESTEP=1 (the Makefile's defaults)
+    "step_rk_r2_2k_quad_geo_nlev6_restricted_1": (2048, "r2", 13, 3.0, False, "geometric", 6,
+                                                  24, "restricted_1", 1.0),
+    # TIMESTEP=FIXED CHECK_NEIB_TIMESTEP=2 (IMMEDIATE_TIMESTEP_REDUCTION)
+    "step_rk_r2_2k_quad_geo_nlev6_fixed_2": (2048, "r2", 13, 3.0, False, "geometric", 6, 24,
+                                             "fixed_2", 0.02),
 }
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
 
 def step_case_data(spec):
     """Runs one step case on the reference and returns the fixture arrays."""
-    n, prof, seed, mach, octu, mac, nlevels, nsteps = spec
+    n, prof, seed, mach, octu, mac, nlevels, nsteps = spec[:8]
+    timestep, dt_fixed = spec[8:] if len(spec) > 8 else ("", 1.0)
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac)
+    ref = RefSeren(octu, mac, timestep=timestep)
     ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
     ref.force_eval(CS, hguess=True)
-    ref.step_setup(nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
+    ref.step_setup(nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
+                   dt_fixed=dt_fixed)
     ints, reals, nact = [], [], []
     for _ in range(nsteps):
         ref.integrate(CS)
@@ -133,6 +142,7 @@ def step_case_data(spec):
     return dict(
         spec=np.array([n, seed, int(octu), nlevels, nsteps], np.int64), profile=np.array(prof),
         mac=np.array(mac), mach=np.array(mach), pos0=pos, vel0=vel, mass=mass,
+        timestep=np.array(timestep), dt_fixed=np.array(dt_fixed), nminneib=sp["nminneib"],
         step_ints=np.array(ints, np.int64), step_reals=np.array(reals),
         step_nactive=np.array(nact, np.int64),
         porig=p["porig"], pos=p["pos"], v=p["v"], a=p["a"], h=p["h"], rho=p["rho"],

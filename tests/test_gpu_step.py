@@ -14,6 +14,13 @@ Bar:
     |da| <= 1e-10 |a|.  Forces agree with the reference to ~1e-15 relative
     (see test_gpu_parity), and a few steps carry that into the state.
 
+The time-step options of the reference Makefile are covered by fixtures from
+builds with them: TIMESTEP=RESTRICTED with CHECK_NEIB_TIMESTEP=1 (the
+Makefile's defaults) and TIMESTEP=FIXED with CHECK_NEIB_TIMESTEP=2
+(check_neighbour_timesteps.F90, reduce_timesteps.F90); nminneib is compared
+exactly as well.  The other fixtures come from a build with neither option
+(adaptive levels, no neighbour check).
+
 The leapfrog KDK integrator has no reference counterpart (the reference's
 advance_leapfrog_kdk.F90 uses dt_half before setting it); it is checked
 against a numpy restatement of its update with the fix, driven by the
@@ -31,7 +38,11 @@ from helpers import CS, rel_err  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STEP_CASES = ["step_rk_uniform2k_quad_geo_nlev3", "step_rk_r2_2k_oct_gadget2_nlev1"]
+STEP_CASES = ["step_rk_uniform2k_quad_geo_nlev3", "step_rk_r2_2k_oct_gadget2_nlev1",
+              "step_rk_r2_2k_quad_geo_nlev6_restricted_1", "step_rk_r2_2k_quad_geo_nlev6_fixed_2"]
+# reference build -> (levels, neib_check)
+TIMESTEP_BUILDS = {"": ("adaptive", 0), "restricted_1": ("restricted", 1),
+                   "fixed_2": ("fixed", 2)}
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
 
@@ -46,12 +57,14 @@ def test_rk_steps_match_reference_golden(name):
     print(f"case {name}", flush=True)
     g = np.load(os.path.join(GOLD, name + ".npz"))
     n, seed, octu, nlevels, nsteps = (int(v) for v in g["spec"])
+    levels, check = TIMESTEP_BUILDS[str(g["timestep"])]
     eng = _engine(bool(octu), str(g["mac"]))
     try:
         h0 = np.zeros(n)
         eng.load(g["pos0"], g["vel0"], g["mass"], h0)
         eng.force_eval(hguess=True)
-        eng.step_begin("rk", nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
+        eng.step_begin("rk", nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
+                       levels=levels, dt_fixed=float(g["dt_fixed"]), neib_check=check)
         for k in range(nsteps):
             eng.step(1)
             info = eng.step_info()
@@ -77,6 +90,8 @@ def test_rk_steps_match_reference_golden(name):
         assert np.abs(st[key] - g[key]).max() <= 1e-12 * vscale
     assert np.array_equal(st["nlevel"], g["nlevel"])
     assert np.array_equal(st["nlast"], g["nlast"])
+    if check:
+        assert np.array_equal(st["nminneib"], g["nminneib"])
     assert np.array_equal(st["active"], g["active"])
     assert rel_err(st["laststep"], g["laststep"]).max() <= 1e-12
     assert rel_err(d["h"], g["h"]).max() <= 1e-11
@@ -126,7 +141,7 @@ def test_kdk_advance_matches_restatement():
     try:
         eng.load(pos, vel, mass, h)
         eng.force_eval(hguess=True)
-        eng.step_begin("lfkdk", nlevels=2, nbuildstep=8)
+        eng.step_begin("lfkdk", nlevels=2, nbuildstep=8, levels="adaptive", neib_check=0)
         prev = eng.fetch(neighbours=False)
         sprev = eng.fetch_step()
         iprev = eng.step_info()
