@@ -353,9 +353,10 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     for (auto &ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * KT_TOTAL);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->icount, 4 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc((void **)&c->wstat, 16 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->wstat, 0, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->icount, CTR_WORDS * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->icount, 0, CTR_WORDS * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->wstat, CTR_WORDS * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->wstat, 0, CTR_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&c->ovf_count, sizeof(int));
     if (e != hipSuccess) {
         free_all(*c);
@@ -607,7 +608,7 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     Ctx &c = *as_ctx(ctx);
     if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
     SX_HIP(hipSetDevice(c.device));
-    SX_HIP(hipMemsetAsync(c.wstat, 0, 16 * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.wstat, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     SX_TRY(run_tree_update(c));
     if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
@@ -649,9 +650,12 @@ int serenx_last_timings(const serenx_ctx *ctx, double *ms4) {
 int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4) {
     if (!ctx || !counts4) return set_error(SERENX_EINVAL, "null argument");
     Ctx &c = *as_ctx(ctx);
-    unsigned long long v[4] = {0, 0, 0, 0};
+    std::vector<unsigned long long> rows(CTR_WORDS);
+    unsigned long long v[4];
     SX_HIP(hipStreamSynchronize(c.stream));
-    SX_HIP(hipMemcpy(v, c.icount, sizeof(v), hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(rows.data(), c.icount, CTR_WORDS * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    ctr_fold(rows.data(), v, 4);
     for (int k = 0; k < 4; k++) counts4[k] = (int64_t)v[k];
     return SERENX_OK;
 }
@@ -659,9 +663,12 @@ int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4) {
 int serenx_walk_stats(serenx_ctx *ctx, int64_t *stats16) {
     if (!ctx || !stats16) return set_error(SERENX_EINVAL, "null argument");
     Ctx &c = *as_ctx(ctx);
+    std::vector<unsigned long long> rows(CTR_WORDS);
     unsigned long long v[16];
     SX_HIP(hipStreamSynchronize(c.stream));
-    SX_HIP(hipMemcpy(v, c.wstat, sizeof(v), hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(rows.data(), c.wstat, CTR_WORDS * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    ctr_fold(rows.data(), v, 16, 0, WSTAT_MAX_MASK);
     for (int k = 0; k < 16; k++) stats16[k] = (int64_t)v[k];
     return SERENX_OK;
 }

@@ -234,10 +234,11 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(icount + 0, (unsigned long long)ncell);
-        atomicAdd(icount + 1, (unsigned long long)npp);
-        atomicAdd(icount + 2, (unsigned long long)nin_p);
-        atomicAdd(icount + 3, (unsigned long long)nin_j);
+        unsigned long long *ic = ctr_row(icount);
+        atomicAdd(ic + 0, (unsigned long long)ncell);
+        atomicAdd(ic + 1, (unsigned long long)npp);
+        atomicAdd(ic + 2, (unsigned long long)nin_p);
+        atomicAdd(ic + 3, (unsigned long long)nin_j);
     }
     wave_stats(wstat, 0, vis, lvis);
 }
@@ -373,10 +374,11 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(icount + 0, (unsigned long long)ncell);
-        atomicAdd(icount + 1, (unsigned long long)npp);
-        atomicAdd(icount + 2, (unsigned long long)nin_p);
-        atomicAdd(icount + 3, (unsigned long long)nin_j);
+        unsigned long long *ic = ctr_row(icount);
+        atomicAdd(ic + 0, (unsigned long long)ncell);
+        atomicAdd(ic + 1, (unsigned long long)npp);
+        atomicAdd(ic + 2, (unsigned long long)nin_p);
+        atomicAdd(ic + 3, (unsigned long long)nin_j);
     }
     wave_stats(wstat, 0, vis, lvis);
 }
@@ -452,7 +454,7 @@ void launch_grav(Ctx &c) {
 }  // namespace
 
 int grav_forces(Ctx &c) {
-    SX_HIP(hipMemsetAsync(c.icount, 0, 4 * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
     const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
     switch (c.opt.mac) {
     case SERENX_MAC_GEOMETRIC:

@@ -47,6 +47,42 @@ constexpr int WAVE = 64;
 // potential-neighbour list slots (opened leaves) per particle in the h solve
 constexpr int HL_SLOTS = 96;
 
+// Counters that every wave of a launch updates (walk statistics, interaction
+// counts, the step's reductions) are spread over CTR_ROWS rows of CTR_SLOTS
+// 64-bit slots, one 128-byte line per row.  A wave updates the row of its own
+// index: device atomics on one address serialise in L2 (about 12 ns each, so
+// a single counter updated by the 16k waves of a 1M-particle launch costs
+// about 0.2 ms).  The host folds the rows (ctr_fold).
+constexpr int CTR_ROWS = 128;
+constexpr int CTR_SLOTS = 16;
+constexpr size_t CTR_WORDS = (size_t)CTR_ROWS * CTR_SLOTS;
+
+__device__ __forceinline__ unsigned long long *ctr_row(unsigned long long *base) {
+    const unsigned w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    return base + (size_t)(w % CTR_ROWS) * CTR_SLOTS;
+}
+
+// Folds counter rows into nslots values: the minimum for the slots in
+// min_mask, the maximum for those in max_mask, the sum for the others.
+inline void ctr_fold(const unsigned long long *rows, unsigned long long *out, int nslots,
+                     unsigned min_mask = 0, unsigned max_mask = 0) {
+    for (int k = 0; k < nslots; k++) {
+        unsigned long long v = rows[k];
+        for (int r = 1; r < CTR_ROWS; r++) {
+            const unsigned long long x = rows[(size_t)r * CTR_SLOTS + k];
+            if (min_mask >> k & 1u)
+                v = x < v ? x : v;
+            else if (max_mask >> k & 1u)
+                v = x > v ? x : v;
+            else
+                v += x;
+        }
+        out[k] = v;
+    }
+}
+// walk-statistics slots that hold maxima (serenx_walk_stats)
+constexpr unsigned WSTAT_MAX_MASK = (1u << 5) | (1u << 12);
+
 // Gravity node, DFS pre-order.  Read with wave-uniform indices in the walk, so
 // a node is a run of scalar loads; everything a visit needs (MAC data and
 // links) sits in the first 64 bytes, one s_load_dwordx16, and the moments
@@ -191,8 +227,9 @@ struct Ctx {
         long long *nlast = nullptr;
         int *nlevel = nullptr;
         int *nminneib = nullptr;             // neighbour check: deepest level among neighbours
-        unsigned long long *red = nullptr;   // device reductions [dt_min bits, p_update, level
-                                             // max, nactive, nminneib max]
+        unsigned long long *red = nullptr;   // device reductions in counter rows [dt_min bits,
+                                             // p_update, level max, nactive, nminneib max]
+        std::vector<unsigned long long> red_host;
         long long n = 0, nsteps = 0, nresync = 0, level_max = 0, level_step = 0;
         long long nbuild = 0, nstock = 0, nactive = 0;
         double time = 0.0, timestep = 0.0, dt_max = 0.0;

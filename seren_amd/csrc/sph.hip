@@ -230,13 +230,14 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
     for (int o = 32; o > 0; o >>= 1) it += __shfl_xor(it, o, 64);
     const int smx = wave_max_int(slot_max);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(wstat + 4, (unsigned long long)passes);
-        atomicMax(wstat + 5, (unsigned long long)passes);
-        atomicAdd(wstat + 6, it);
-        atomicAdd(wstat + 7, 1ull);
-        atomicAdd(wstat + 10, (unsigned long long)walks);
-        atomicAdd(wstat + 11, (unsigned long long)fallback);
-        atomicMax(wstat + 12, (unsigned long long)smx);
+        unsigned long long *ws = ctr_row(wstat);
+        atomicAdd(ws + 4, (unsigned long long)passes);
+        atomicMax(ws + 5, (unsigned long long)passes);
+        atomicAdd(ws + 6, it);
+        atomicAdd(ws + 7, 1ull);
+        atomicAdd(ws + 10, (unsigned long long)walks);
+        atomicAdd(ws + 11, (unsigned long long)fallback);
+        atomicMax(ws + 12, (unsigned long long)smx);
     }
 }
 
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
                 }
             });
         wave_stats(wstat, 2, vis, lvis);
-        if ((threadIdx.x & 63) == 0) atomicAdd(wstat + 11, 1ull);
+        if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(wstat) + 11, 1ull);
     }
     if (valid) {
         const double om = 1.0 + ((hp * som) / (3.0 * srho));

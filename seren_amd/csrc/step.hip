@@ -29,7 +29,18 @@ constexpr double SMALL_NUMBER_DP = 1.0e-20;  // macros.h:82
 
 constexpr int TIMESTEP_LEVEL_DIFF_MAX = 2;   // macros.h:185
 
+// reductions, in counter rows (ctr_row): folded by minimum, maximum or sum
 enum { R_DTMIN = 0, R_PUPDATE = 1, R_LEVMAX = 2, R_NACTIVE = 3, R_NMAX = 4, R_COUNT = 5 };
+constexpr unsigned R_MIN_MASK = 1u << R_DTMIN;
+constexpr unsigned R_MAX_MASK = (1u << R_LEVMAX) | (1u << R_NMAX);
+constexpr double DT_INIT = 9.9e20;
+
+__global__ void k_red_init(unsigned long long *__restrict__ red) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int)CTR_WORDS)
+        red[i] = (i % CTR_SLOTS == R_DTMIN)
+                     ? (unsigned long long)__double_as_longlong(DT_INIT) : 0ull;
+}
 
 // timestep_size.F90:92-114: acceleration and Courant criteria with the AB
 // viscosity terms (beta = 2 alpha, hydro_gradh.F90:227)
@@ -59,7 +70,7 @@ __device__ __forceinline__ void wave_min_bits(unsigned long long *dst, double v)
         const unsigned long long t = __shfl_xor(b, o, 64);
         b = t < b ? t : b;
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(dst, b);
+    if ((threadIdx.x & 63) == 0) atomicMin(ctr_row(dst), b);
 }
 
 // sph_timesteps.F90:59-66 and :94-102: every particle's step size, the
@@ -71,7 +82,7 @@ __global__ void k_ts_all(int n, const double *__restrict__ ax, const double *__r
                          double *__restrict__ dt, long long *__restrict__ nlast,
                          unsigned long long *__restrict__ red) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 9.9e20;
+    double d = DT_INIT;
     if (p < n) {
         nlast[p] = 0;
         d = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha, 2.0 * alpha,
@@ -132,6 +143,7 @@ __global__ void __launch_bounds__(256) k_neib_levels(
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || !active[p]) return;
     const int levelp = nlevel[p];
+    if (levelp == 0) return;   // nminneib >= 0 already
     const int nn = pptot[p];
     const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
     for (int i = 0; i < nn; i++) {
@@ -197,8 +209,9 @@ __global__ void k_reduce_steps(int n, long long nint, long long level_step, doub
         nmax = max(nmax, __shfl_xor(nmax, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(red + R_NACTIVE, u);
-        atomicMax(red + R_NMAX, (unsigned long long)(nmax > 0 ? nmax : 0));
+        unsigned long long *row = ctr_row(red);
+        atomicAdd(row + R_NACTIVE, u);
+        atomicMax(row + R_NMAX, (unsigned long long)(nmax > 0 ? nmax : 0));
     }
 }
 
@@ -239,8 +252,9 @@ __global__ void k_ts_update(int n, long long nint, long long level_step, double 
         lm = max(lm, __shfl_xor(lm, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(red + R_PUPDATE, u);
-        atomicMax(red + R_LEVMAX, (unsigned long long)lm);
+        unsigned long long *row = ctr_row(red);
+        atomicAdd(row + R_PUPDATE, u);
+        atomicMax(row + R_LEVMAX, (unsigned long long)lm);
     }
 }
 
@@ -319,7 +333,7 @@ __global__ void k_advance(int n, long long nint, long long level_step, double ti
     }
     unsigned long long u = act;
     for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(red + R_NACTIVE, u);
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(red) + R_NACTIVE, u);
 }
 
 __global__ void k_step_init(int n, const double *__restrict__ x, const double *__restrict__ y,
@@ -357,15 +371,29 @@ int dmalloc(T **p, size_t count) {
     return SERENX_OK;
 }
 
+int red_init(Ctx &c) {
+    hipLaunchKernelGGL(k_red_init, dim3(nblk((long)CTR_WORDS)), dim3(TB), 0, c.stream, c.st.red);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+// waits for the stream and folds the reduction rows
+int red_read(Ctx &c, unsigned long long *r) {
+    Ctx::Step &s = c.st;
+    SX_HIP(hipMemcpyAsync(s.red_host.data(), s.red, CTR_WORDS * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c.stream));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    ctr_fold(s.red_host.data(), r, R_COUNT, R_MIN_MASK, R_MAX_MASK);
+    return SERENX_OK;
+}
+
 // sph_timesteps.F90:30-337 (no sinks)
 int step_timesteps(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
     hipStream_t st = c.stream;
     const bool neib = s.opt.neib_check != SERENX_NEIB_CHECK_OFF;
-    unsigned long long init[R_COUNT] = {
-        (unsigned long long)__builtin_bit_cast(long long, 9.9e20), 0, 0, 0, 0};
-    SX_HIP(hipMemcpyAsync(s.red, init, sizeof(init), hipMemcpyHostToDevice, st));
+    SX_TRY(red_init(c));
     const double alpha = c.opt.alpha;
     if (s.n == s.nresync) {
         s.n = 0;
@@ -374,8 +402,7 @@ int step_timesteps(Ctx &c) {
                            s.nlast, s.red);
         SX_HIP(hipGetLastError());
         unsigned long long r[R_COUNT];
-        SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
-        SX_HIP(hipStreamSynchronize(st));
+        SX_TRY(red_read(c, r));
         double dt_min = __builtin_bit_cast(double, (long long)r[R_DTMIN]);
         s.level_max = s.opt.nlevels - 1;
         // :84-91; 2.0**level is a power of two, so ldexp gives the same quotient
@@ -399,8 +426,7 @@ int step_timesteps(Ctx &c) {
                            s.opt.accel_mult, s.opt.courant_mult, s.nlast, s.nlevel, s.red);
         SX_HIP(hipGetLastError());
         unsigned long long r[R_COUNT];
-        SX_HIP(hipMemcpyAsync(r, s.red, sizeof(r), hipMemcpyDeviceToHost, st));
-        SX_HIP(hipStreamSynchronize(st));
+        SX_TRY(red_read(c, r));
         if (r[R_PUPDATE] > 0) {
             long long level_max = (long long)r[R_LEVMAX];
             if (level_max <= level_old - 1 && level_old > 1 &&
@@ -451,7 +477,7 @@ int step_advance(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
     hipStream_t st = c.stream;
-    SX_HIP(hipMemsetAsync(s.red + R_NACTIVE, 0, 2 * sizeof(unsigned long long), st));
+    SX_TRY(red_init(c));
     if (s.opt.integrator == SERENX_INT_LFKDK)
         hipLaunchKernelGGL(k_advance<1>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
                            s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
@@ -479,11 +505,10 @@ int step_advance(Ctx &c) {
                                c.active, s.red);
         SX_HIP(hipGetLastError());
     }
-    unsigned long long r2[2] = {0, 0};
-    SX_HIP(hipMemcpyAsync(r2, s.red + R_NACTIVE, sizeof(r2), hipMemcpyDeviceToHost, st));
-    SX_HIP(hipStreamSynchronize(st));
-    s.nactive = (long long)r2[0];
-    if (reduce && (long long)r2[1] > s.level_max)
+    unsigned long long r[R_COUNT];
+    SX_TRY(red_read(c, r));
+    s.nactive = (long long)r[R_NACTIVE];
+    if (reduce && (long long)r[R_NMAX] > s.level_max)
         return set_error(SERENX_ESTATE, "Problem with nminneib > level_max (reduce_timesteps)");
     return SERENX_OK;
 }
@@ -493,7 +518,8 @@ int step_once(Ctx &c) {
     Ctx::Step &s = c.st;
     // phase markers as serenx_force_eval records them (serenx_last_timings)
     // and walk statistics of this step's force evaluation
-    SX_HIP(hipMemsetAsync(c.wstat, 0, 16 * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.wstat, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     // tree_update.F90:24-88
     const bool build = s.nbuild == s.nsteps;
@@ -505,16 +531,25 @@ int step_once(Ctx &c) {
     if (build) s.nbuild = s.nbuild + s.opt.nbuildstep;
     if (s.nstock == s.nsteps) s.nstock = s.nstock + (s.opt.integrator == SERENX_INT_LFKDK ? 2 : 1);
     SX_HIP(hipEventRecord(c.ev[1], c.stream));
-    // sph_update, thermal, forces for the active particles
-    SX_TRY(run_sph_update(c));
-    SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    // sph_update, thermal, forces for the active particles; with none active
+    // (the first step, the half steps of a one-level leapfrog) they change
+    // nothing but the tree's hrangemax, which the next sph_update recomputes
+    // from the same h, and it is kept current here all the same
+    if (s.nactive > 0) {
+        SX_TRY(run_sph_update(c));
+        SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    } else {
+        SX_TRY(update_hmax(c));
+    }
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
-    SX_TRY(run_hydro(c));
+    if (s.nactive > 0) SX_TRY(run_hydro(c));
     SX_HIP(hipEventRecord(c.ev[3], c.stream));
-    SX_TRY(run_grav(c));
+    if (s.nactive > 0) SX_TRY(run_grav(c));
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     SX_TRY(step_timesteps(c));
-    if (s.opt.neib_check != SERENX_NEIB_CHECK_OFF) SX_TRY(step_neib_levels(c));
+    // every level 0 (and nminneib >= 0 without sinks): the check changes nothing
+    if (s.opt.neib_check != SERENX_NEIB_CHECK_OFF && s.nactive > 0 && s.level_max > 0)
+        SX_TRY(step_neib_levels(c));
     SX_TRY(step_advance(c));
     return SERENX_OK;
 }
@@ -592,7 +627,8 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
     SX_TRY(dmalloc(&s.nlast, n));
     SX_TRY(dmalloc(&s.nlevel, n));
     SX_TRY(dmalloc(&s.nminneib, n));
-    SX_TRY(dmalloc(&s.red, R_COUNT));
+    SX_TRY(dmalloc(&s.red, CTR_WORDS));
+    s.red_host.assign(CTR_WORDS, 0);
     SX_HIP(hipMemsetAsync(s.nminneib, 0, sizeof(int) * n, c.stream));
     s.opt = opt;
     s.on = true;

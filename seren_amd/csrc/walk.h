@@ -136,8 +136,9 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
 __device__ __forceinline__ void wave_stats(unsigned long long *stats, int k0, unsigned visits,
                                            unsigned lane_visits) {
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(stats + k0, (unsigned long long)visits);
-        atomicAdd(stats + k0 + 1, (unsigned long long)lane_visits);
+        unsigned long long *row = ctr_row(stats);
+        atomicAdd(row + k0, (unsigned long long)visits);
+        atomicAdd(row + k0 + 1, (unsigned long long)lane_visits);
     }
 }
 
