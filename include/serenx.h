@@ -229,6 +229,11 @@ int serenx_fetch(serenx_ctx *ctx, int32_t *porig, double *pos, double *vel,
 int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets,
                           double *acc, double *pot);
 
+/* y[i] = x[i]**0.333333333 as the h solve evaluates it on the device (the
+ * reference's (m/rho)**ONETHIRD, all_sph_gradh.F90:253,317, correctly
+ * rounded); n values, host arrays.  For checking the device arithmetic. */
+int serenx_pow_third(serenx_ctx *ctx, int32_t n, const double *x, double *y);
+
 /* Tree sizes: number of cells and the deepest level (ctot_grav + 1, ltot_grav). */
 int serenx_tree_sizes(const serenx_ctx *ctx, int32_t *ncells, int32_t *ltot);
 

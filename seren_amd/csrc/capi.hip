@@ -739,6 +739,26 @@ int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets, d
     return rc;
 }
 
+int serenx_pow_third(serenx_ctx *ctx, int32_t n, const double *x, double *y) {
+    if (!ctx || !x || !y || n < 0) return set_error(SERENX_EINVAL, "bad args");
+    if (n == 0) return SERENX_OK;
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    double *d = nullptr;
+    SX_HIP(hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+    int rc = SERENX_OK;
+    if (hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "copy of x failed");
+    if (rc == SERENX_OK) rc = eval_pow_third(c, n, d, d + n);
+    if (rc == SERENX_OK && hipStreamSynchronize(c.stream) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "pow_third failed");
+    if (rc == SERENX_OK &&
+        hipMemcpy(y, d + n, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "copy of y failed");
+    (void)hipFree(d);
+    return rc;
+}
+
 int serenx_tree_sizes(const serenx_ctx *ctx, int32_t *ncells, int32_t *ltot) {
     if (!ctx) return set_error(SERENX_EINVAL, "null context");
     const Ctx &c = *as_ctx(ctx);

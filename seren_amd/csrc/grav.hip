@@ -293,20 +293,21 @@ __device__ __forceinline__ unsigned pair_fp32(const float4 *__restrict__ t4, flo
     return inside;
 }
 
-// FP32 multipole terms on the FP32 cell offset, accumulated in FP64
+// FP32 multipole terms on the FP32 cell offset, added to an FP32 batch sum
 template <int OCT>
-__device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], double &ax,
-                                          double &ay, double &az, double &pot) {
+__device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], float &ax,
+                                          float &ay, float &az, float &pot) {
 #pragma clang fp contract(fast)
     const float drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     const float ir = __builtin_amdgcn_rsqf(drsqd + 1.0e-30f);
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f, fp = 0.0f;
-    cell_terms<OCT, float>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, fx, fy, fz, fp);
-    ax += (double)fx;
-    ay += (double)fy;
-    az += (double)fz;
-    pot += (double)fp;
+    cell_terms<OCT, float>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
 }
+
+// Cell terms are summed in FP32 over batches of CELL_BATCH cell visits of the
+// wave and each batch is added to FP64 sums: a batch sum carries at most a
+// few FP32 roundings per term, and the conversions and FP64 additions are
+// paid once per batch instead of once per cell.
+constexpr int CELL_BATCH = 8;
 
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_mixed(
@@ -328,6 +329,8 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
     }
     double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
+    float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
+    int nbatch = 0;                                   // wave-uniform
     CSum px, py, pz, pq;                              // pair terms
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
     const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
@@ -342,10 +345,16 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
             else
                 accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
             accept = accept && g.leaf != 1;
-            ncell += lanes(here && accept);
+            const unsigned nacc = lanes(here && accept);
+            ncell += nacc;
             if (here && accept) {
                 const float dr[3] = {(float)d0, (float)d1, (float)d2};
-                cell_fp32<OCT>(g, dr, cx, cy, cz, cp);
+                cell_fp32<OCT>(g, dr, bx, by, bz, bp);
+            }
+            if (nacc && ++nbatch == CELL_BATCH) {
+                cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
+                bx = by = bz = bp = 0.0f;
+                nbatch = 0;
             }
             if (accept) return WALK_SKIP;
             return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
@@ -365,6 +374,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 nin_j += lanes(in_ & 2);
             }
         });
+    cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
     if (valid) {
         const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
         ax[p] = ax[p] + gx;

@@ -266,6 +266,7 @@ int stock_tree(Ctx &c);          // BHgrav_stock + BHhydro_stock (fused)
 int refresh_f32_nodes(Ctx &c);   // mixed-precision node copy of the stocked tree
 int hguess(Ctx &c);              // BHhydro_hguess
 int update_hmax(Ctx &c);         // BHhydro_update_hmax
+int eval_pow_third(Ctx &c, int n, const double *x, double *y);   // device x**ONETHIRD
 int permute_particles(Ctx &c, const int *perm);
 // cloud layout of the loaded particles (counts[k] particles in cloud k)
 int set_clouds(Ctx &c, int ncloud, const int *counts);

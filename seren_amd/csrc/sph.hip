@@ -509,7 +509,18 @@ __global__ void __launch_bounds__(256) k_hydro(
     az[p] = ahz + adz;
 }
 
+__global__ void k_pow_third(int n, const double *__restrict__ x, double *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = ref_pow_third(x[i]);
+}
+
 }  // namespace
+
+int eval_pow_third(Ctx &c, int n, const double *x, double *y) {
+    hipLaunchKernelGGL(k_pow_third, dim3(nblk(n)), dim3(TB), 0, c.stream, n, x, y);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
 
 int sph_hsolve(Ctx &c) {
     if (c.n < c.pp_gather)

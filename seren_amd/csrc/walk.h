@@ -52,25 +52,63 @@ __device__ __forceinline__ double cr_pow4(double x) {
 }
 
 // x**0.333333333 (the reference's ONETHIRD, macros.h:18) correctly rounded,
-// as libm pow returns it on the host.  With y = 0.333333333 = 1/3 - delta,
+// as libm pow returns it on the host, for 2^-1000 <= x <= 2^1000 (other x go
+// to the device pow).  With y = 0.333333333 = 1/3 - delta,
 //   x^y = cbrt(x) * exp(-delta ln x).
-// cbrt(x) is taken to ~2^-100 relative by one Newton step whose residual
-// c^3 - x is formed exactly with FMA products; |delta ln x| < 2e-8, so its
-// exponential needs only three Taylor terms, and a double-precision ln x
-// leaves ~2^-79 relative error in the sum.  One final rounding follows.
-__device__ __forceinline__ double ref_pow_third(double x) {
-    if (!(x > 0.0) || __builtin_isinf(x)) return pow(x, ONETHIRD);
+// * cube root: x = m 2^(3k) with m in [1, 8).  The hardware FP32 log2 and
+//   exp2 give 2^(log2(m)/3) to ~2^-22; one Newton step c <- c - (c^3 - m) /
+//   (3 c^2) in FP64 takes it to ~2^-44, and a second one on x, whose residual
+//   c^3 - x is formed exactly with FMA products, to ~2^-88.
+// * ln x = e ln 2 + 2 atanh(s), s = (f - 1)/(f + 1) with f in [sqrt(1/2),
+//   sqrt(2)), from the odd series of atanh to s^17 (|s| < 0.172: truncation
+//   below 1e-15).  |delta ln x| < 3e-7, so exp(-delta ln x) - 1 needs three
+//   Taylor terms, and the error of ln x enters the result below 2^-70
+//   relative.
+// One final rounding follows.  It stays a call: inlined into the h-solve
+// iteration kernel it raised that kernel from 84 to 123 VGPRs (3 waves per
+// SIMD instead of 6 on a latency-bound walk).
+static __device__ __noinline__ double ref_pow_third(double x) {
+    if (!(x >= 0x1p-1000 && x <= 0x1p1000)) return pow(x, ONETHIRD);
     constexpr double THIRD_HI = 0.3333333333333333;       // nearest double to 1/3
     constexpr double THIRD_LO = 1.850371707708594e-17;    // 1/3 - THIRD_HI
     constexpr double DELTA = (THIRD_HI - ONETHIRD) + THIRD_LO;
-    const double c0 = cbrt(x);
+    constexpr double LN2 = 0.6931471805599453;            // nearest double to ln 2
+    int ex;
+    const double fm = frexp(x, &ex);                      // x = fm 2^ex, fm in [1/2, 1)
+    // cube root
+    const int t = ex - 1;
+    const int r = ((t % 3) + 3) % 3;
+    const int k = (t - r) / 3;                            // x = (2 fm 2^r) 2^(3k)
+    const double m = ldexp(2.0 * fm, r);
+    double c = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)m) * (1.0f / 3.0f));
+    c = c - (c * c * c - m) / (3.0 * (c * c));
+    const double c0 = ldexp(c, k);
     const double p = c0 * c0;
     const double e1 = __fma_rn(c0, c0, -p);
     const double q = p * c0;
     const double e2 = __fma_rn(p, c0, -q);
     const double res = (q - x) + __fma_rn(e1, c0, e2);   // c0^3 - x
     const double clo = -res / (3.0 * p);
-    const double d = -DELTA * log(x);
+    // ln x
+    double f = fm;
+    int e = ex;
+    if (f < 0.7071067811865476) {
+        f = 2.0 * f;
+        e = e - 1;
+    }
+    const double s = (f - 1.0) / (f + 1.0);
+    const double s2 = s * s;
+    double poly = 1.0 / 17.0;
+    poly = poly * s2 + 1.0 / 15.0;
+    poly = poly * s2 + 1.0 / 13.0;
+    poly = poly * s2 + 1.0 / 11.0;
+    poly = poly * s2 + 1.0 / 9.0;
+    poly = poly * s2 + 1.0 / 7.0;
+    poly = poly * s2 + 1.0 / 5.0;
+    poly = poly * s2 + 1.0 / 3.0;
+    poly = poly * s2 + 1.0;
+    const double lnx = (double)e * LN2 + 2.0 * s * poly;
+    const double d = -DELTA * lnx;
     const double g = d + d * d * (0.5 + d * (1.0 / 6.0));
     return c0 + __fma_rn(c0, g, clo);
 }

@@ -50,7 +50,7 @@ EXPORTED = [
     "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
     "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
     "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets", "serenx_set_precision",
-    "serenx_memory",
+    "serenx_memory", "serenx_pow_third",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -134,6 +134,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_step_begin.argtypes = [vp, C.POINTER(StepOptions)]
     lib.serenx_step.argtypes = [vp, C.c_int32]
     lib.serenx_step_get_info.argtypes = [vp, C.POINTER(StepInfo)]
+    lib.serenx_pow_third.argtypes = [vp, C.c_int32, _dp, _dp]
     lib.serenx_fetch_step.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _ip, _dp, _dp, _dp,
                                       _ip]
     _lib = lib
@@ -356,6 +357,13 @@ class ForceEngine:
             _iptr(nminneib)))
         return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=active,
                     r_old=r_old, v_old=v_old, v_half=v_half, nminneib=nminneib)
+
+    def pow_third(self, x: np.ndarray) -> np.ndarray:
+        """x**0.333333333 as the device h solve evaluates it."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        _check(self.lib.serenx_pow_third(self.ctx, len(x), _dptr(x), _dptr(y)))
+        return y
 
     def interaction_counts(self):
         """(cell terms, pair terms, pairs inside 2h_p, pairs inside 2h_j)."""
