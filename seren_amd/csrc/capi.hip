@@ -144,6 +144,7 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.pptot, nn));
     SX_TRY(dalloc(&c.pplist, nn * (size_t)c.pp_limit));
     SX_TRY(dalloc(&c.ovf_off, nn));
+    SX_TRY(dalloc(&c.ovf_idx, nn));
     SX_TRY(dalloc(&c.hrec, nn * 16));
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     SX_TRY(ensure_tree_buffers(c));
@@ -155,6 +156,7 @@ void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
                     c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
+                    c.ovf_idx,
                     c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
                     c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
                     c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,

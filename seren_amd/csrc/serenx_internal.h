@@ -205,6 +205,7 @@ struct Ctx {
     int *ovf_off = nullptr, *ovf_list = nullptr;  // rows with pptot > pp_limit
     size_t ovf_cap = 0;
     int *ovf_count = nullptr;     // device scalar: number of overflow particles
+    int *ovf_idx = nullptr;       // the overflow particles, in tree order
 
     // kernel tables [NTAB][KT] (kernel.F90)
     double *ktab = nullptr;

@@ -327,12 +327,15 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
     }
 }
 
+constexpr int OVF_PER_WAVE = 1;
+
 // get_neib.F90:46-118: gather-or-scatter neighbours in walk (DFS) order.
 // Rows of pp_limit ids; particles with more neighbours only get their count
 // here (the reference does not store their list either, :115-118) and are
 // listed completely by the OVERFLOW_PASS instance.
 template <bool OVERFLOW_PASS>
-__global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__restrict__ hn,
+__global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf_idx, int nc,
+                                              const HNode *__restrict__ hn,
                                               const double *__restrict__ x,
                                               const double *__restrict__ y,
                                               const double *__restrict__ z,
@@ -345,9 +348,14 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
                                               int *__restrict__ ovf_list,
                                               int *__restrict__ ovf_count,
                                               unsigned long long *__restrict__ wstat) {
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
-    bool valid = p < n && active[p];
-    if (OVERFLOW_PASS && valid) valid = pptot[p] > pp_limit;
+    // the overflow pass runs over the list of overflow particles (n of them),
+    // OVF_PER_WAVE to a wave: their walks are long, and a wave walks the union
+    // of its lanes' walks, so a few lanes per wave keep many waves in flight
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int ot = (t / WAVE) * OVF_PER_WAVE + (t % WAVE);
+    const bool ovf_lane = (t % WAVE) < OVF_PER_WAVE && ot < n;
+    const int p = OVERFLOW_PASS ? (ovf_lane ? ovf_idx[ot] : 0) : t;
+    const bool valid = OVERFLOW_PASS ? ovf_lane : (p < n && active[p]);
     double rpx = 0, rpy = 0, rpz = 0, hp = 0;
     if (valid) {
         rpx = x[p]; rpy = y[p]; rpz = z[p];
@@ -397,9 +405,20 @@ __global__ void __launch_bounds__(256) k_neib(int n, int nc, const HNode *__rest
 }
 
 __global__ void k_ovf_sizes(int n, int pp_limit, const int *__restrict__ pptot,
-                            const uint8_t *__restrict__ active, int *__restrict__ sz) {
+                            const uint8_t *__restrict__ active, int *__restrict__ sz,
+                            int *__restrict__ is_ovf) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sz[i] = (active[i] && pptot[i] > pp_limit) ? pptot[i] : 0;
+    if (i >= n) return;
+    const bool o = active[i] && pptot[i] > pp_limit;
+    sz[i] = o ? pptot[i] : 0;
+    is_ovf[i] = o ? 1 : 0;
+}
+
+// overflow particles in tree order: ovf_idx[pos[i]] = i
+__global__ void k_ovf_compact(int n, const int *__restrict__ is_ovf, const int *__restrict__ pos,
+                              int *__restrict__ ovf_idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && is_ovf[i]) ovf_idx[pos[i]] = i;
 }
 
 // thermal.F90:52-60 with the sound speed given: P = cs^2 rho, c = cs
@@ -547,9 +566,9 @@ int sph_hsolve(Ctx &c) {
 int sph_neighbours(Ctx &c) {
     hipStream_t s = c.stream;
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x, c.y,
-                       c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot, c.pplist,
-                       c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
+    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.ovf_idx, c.nc, c.hn,
+                       c.x, c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
+                       c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
@@ -557,12 +576,18 @@ int sph_neighbours(Ctx &c) {
         return set_error(SERENX_EHNEG, "Negative smoothing lengths (all_sph_gradh)");
     if (c.hostpin[9] > 0) {
         // complete lists for particles above pp_limit, packed by offset
+        // sizes (flag) and overflow marks (incl; both tree-build scratch)
         hipLaunchKernelGGL(k_ovf_sizes, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.pp_limit, c.pptot,
-                           c.active, c.flag);
+                           c.active, c.flag, c.incl);
         SX_HIP(hipMemsetAsync(c.flag + c.n, 0, sizeof(int), s));
         size_t tb = c.sorttmp_bytes;
         SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.flag, c.ovf_off, 0, (size_t)c.n + 1,
                                        rocprim::plus<int>(), s));
+        tb = c.sorttmp_bytes;
+        SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.incl, c.cellof, 0, (size_t)c.n,
+                                       rocprim::plus<int>(), s));
+        hipLaunchKernelGGL(k_ovf_compact, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.incl, c.cellof,
+                           c.ovf_idx);
         SX_HIP(hipMemcpyAsync(c.hostpin + 10, c.ovf_off + c.n, sizeof(int),
                               hipMemcpyDeviceToHost, s));
         SX_HIP(hipStreamSynchronize(s));
@@ -574,9 +599,11 @@ int sph_neighbours(Ctx &c) {
             if (hipMalloc(&c.ovf_list, sizeof(int) * c.ovf_cap) != hipSuccess)
                 return set_error(SERENX_ENOMEM, "hipMalloc failed for overflow neighbour lists");
         }
-        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.nc, c.hn, c.x,
-                           c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
-                           c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
+        const int nov = c.hostpin[9];   // overflow particles, counted by k_neib<false>
+        const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
+        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, nov, c.ovf_idx, c.nc,
+                           c.hn, c.x, c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit,
+                           c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());
     }
     return SERENX_OK;
