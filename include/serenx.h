@@ -77,6 +77,14 @@ enum {
  * for s < 2/3) */
 enum { SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1 };
 
+/* THERMAL option (Makefile THERMAL): ISOTHERMAL (P = cs^2 rho, c = cs with
+ * the given sound speed), or ENERGY_EQN (INTERNAL_ENERGY): the ideal gas
+ * P = (gamma - 1) rho u, c = sqrt(P / rho) (thermal.F90:108-113), du/dt from
+ * the pressure and viscous-heating terms (hydro_gradh.F90:258-259,326-332),
+ * u advanced with the particles and the energy time-step criterion
+ * (timestep_size.F90:117-122). */
+enum { SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1 };
+
 typedef struct serenx_options {
     int32_t multipole;   /* SERENX_QUADRUPOLE or SERENX_OCTUPOLE */
     int32_t mac;         /* SERENX_MAC_* */
@@ -87,11 +95,15 @@ typedef struct serenx_options {
     double abserror;     /* multipole MAC tolerance; params 'abserror' */
     double alpha;        /* AB viscosity alpha (beta := 2 alpha); params 'alpha' */
     double sound_speed;  /* isothermal c_s used by serenx_force_eval */
+    int32_t thermal;     /* SERENX_THERMAL_* */
+    int32_t reserved;
+    double gamma;        /* ratio of specific heats; params 'gamma' (1.4) */
 } serenx_options;
 
-/* Fills SEREN's parameter defaults (default_parameters.F90:62-117):
+/* Fills SEREN's parameter defaults (default_parameters.F90:62-117,173):
  * quadrupole, geometric MAC, h_fac 1.2, thetamaxsqd 0.5, abserror 0.1,
- * alpha 1, FP64, sound speed 1 (DIMENSIONLESS isothermal, thermal.F90:52-60). */
+ * alpha 1, FP64, isothermal with sound speed 1 (DIMENSIONLESS isothermal,
+ * thermal.F90:52-60), gamma 1.4. */
 int serenx_default_options(serenx_options *opt);
 
 int serenx_create(const serenx_options *opt, int device, serenx_ctx **out);
@@ -130,6 +142,8 @@ typedef struct serenx_host_state {
     int32_t *pplist;
     int32_t *accdo;
     double *rextent;     /* scalar out: max extent (bounding_box.F90:39) */
+    double *du_dt;       /* SERENX_THERMAL_ENERGY_EQN: du_dt(ptot), in and out of
+                            serenx_hydro_forces; may be NULL otherwise */
 } serenx_host_state;
 
 /* Binding a state with a different ptot resets the device state; re-binding
@@ -228,6 +242,13 @@ int serenx_fetch(serenx_ctx *ctx, int32_t *porig, double *pos, double *vel,
  * indices in tree order): the semantics of direct_sph_gravity.F90:10-107. */
 int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets,
                           double *acc, double *pot);
+
+/* SERENX_THERMAL_ENERGY_EQN, device-resident mode: the specific internal
+ * energy u of every particle, in the current particle order (the load order
+ * until the first tree build, tree order after it; serenx_fetch's porig
+ * gives it), and the fetch of u and du/dt in the same order. */
+int serenx_set_energy(serenx_ctx *ctx, const double *u);
+int serenx_fetch_energy(serenx_ctx *ctx, double *u, double *du_dt);
 
 /* y[i] = x[i]**0.333333333 as the h solve evaluates it on the device (the
  * reference's (m/rho)**ONETHIRD, all_sph_gradh.F90:253,317, correctly

@@ -45,6 +45,8 @@ VARIANTS = {
     # time-step options (Makefile TIMESTEP, CHECK_NEIB_TIMESTEP)
     (False, "geometric", "m4", "restricted_1"): "quad_geo_tsr1",
     (False, "geometric", "m4", "fixed_2"): "quad_geo_tsf2",
+    # THERMAL=ENERGY_EQN
+    (False, "geometric", "m4", "energy"): "quad_geo_ee",
 }
 
 
@@ -148,14 +150,18 @@ class RefSeren(_Common):
     _libs: Dict[str, C.CDLL] = {}
 
     def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4",
-                 timestep: str = ""):
+                 timestep: str = "", thermal: str = "isothermal"):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
         oracle/Makefile.ref dropin): the reference code as the host.
         timestep selects a build with the Makefile's TIMESTEP and
         CHECK_NEIB_TIMESTEP options ("restricted_1", "fixed_2"); the default
-        builds have neither."""
-        if timestep:
+        builds have neither.  thermal="energy" selects the THERMAL=ENERGY_EQN
+        build (set u with set_energy; thermal_iso then runs the reference's
+        update_thermal_properties)."""
+        if thermal == "energy":
+            key = (bool(octupole), mac, kernel, "energy")
+        elif timestep:
             key = (bool(octupole), mac, kernel, timestep)
         else:
             key = (bool(octupole), mac) if kernel == "m4" else (bool(octupole), mac, kernel)
@@ -182,6 +188,8 @@ class RefSeren(_Common):
             lib.ref_get_kernel.argtypes = [_dp]
             lib.ref_step_setup.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.c_double]
+            lib.ref_set_energy.argtypes = [C.c_int, _dp, C.c_double]
+            lib.ref_get_energy.argtypes = [C.c_int, _dp, _dp]
             lib.ref_integrate.argtypes = [C.c_double]
             lib.ref_get_step.argtypes = [_lp, _dp]
             lib.ref_get_step_particles.argtypes = [C.c_int, _lp, _lp, _dp, _ip, _dp, _dp, _dp,
@@ -248,6 +256,16 @@ class RefSeren(_Common):
     def set_agravmag(self, am):
         self.lib.ref_set_agravmag(self.n, _f64(am))
 
+    def set_energy(self, u, gamma=1.4):
+        """Specific internal energy per particle (current order) and gamma."""
+        self.lib.ref_set_energy(self.n, _f64(u), gamma)
+
+    def energy(self):
+        """(u, du_dt) in the current (tree) order."""
+        u, du = np.zeros(self.n), np.zeros(self.n)
+        self.lib.ref_get_energy(self.n, u, du)
+        return u, du
+
     def set_h(self, h):
         self.lib.ref_set_h(self.n, _f64(h))
 
@@ -310,6 +328,8 @@ class CSeren(_Common):
             lib.oracle_get_grav_tree.argtypes = [vp, C.c_int, _ip, _ip, _ip, _ip] + [_dp] * 7
             lib.oracle_get_hydro_tree.argtypes = [vp, C.c_int, _dp, _dp, _dp]
             lib.oracle_get_kernel.argtypes = [vp, _dp]
+            lib.oracle_set_energy.argtypes = [vp, _dp, C.c_double]
+            lib.oracle_get_energy.argtypes = [vp, _dp, _dp]
             cls._lib = lib
         return cls._lib
 
@@ -360,6 +380,15 @@ class CSeren(_Common):
 
     def set_agravmag(self, am):
         self.lib.oracle_set_agravmag(self.ctx, _f64(am))
+
+    def set_energy(self, u, gamma=1.4):
+        """THERMAL=ENERGY_EQN: u per particle (current order) and gamma."""
+        self.lib.oracle_set_energy(self.ctx, _f64(u), gamma)
+
+    def energy(self):
+        u, du = np.zeros(self.n), np.zeros(self.n)
+        self.lib.oracle_get_energy(self.ctx, u, du)
+        return u, du
 
     def set_h(self, h):
         self.lib.oracle_set_h(self.ctx, _f64(h))

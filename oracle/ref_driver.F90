@@ -59,6 +59,9 @@ CONTAINS
 #if defined(CHECK_NEIGHBOUR_TIMESTEPS)
     deallocate(nminneib)
 #endif
+#if defined(INTERNAL_ENERGY)
+    deallocate(u, u_old, du_dt)
+#endif
     deallocate(first_cell_grav, last_cell_grav, BHgrav)
     deallocate(first_cell_hydro, last_cell_hydro, BHhydro)
     deallocate(cellof, BHnextptcl, whichchild, BHtemp, BHstock)
@@ -104,6 +107,16 @@ CONTAINS
 #if defined(CHECK_NEIGHBOUR_TIMESTEPS)
     allocate(nminneib(1:pmax))
     nminneib(1:pmax) = 0
+#endif
+#if defined(INTERNAL_ENERGY)
+    ! energy equation: ideal gas with gamma (params default 1.4), DIMENSIONLESS
+    ! units (units.F90:58 Pconst = 1); u is set by ref_set_energy
+    allocate(u(1:pmax), u_old(1:pmax), du_dt(1:pmax))
+    u(1:pmax) = 0.0_PR
+    u_old(1:pmax) = 0.0_PR
+    du_dt(1:pmax) = 0.0_PR
+    gamma = 1.4_PR
+    Pconst = 1.0_PR
 #endif
 
     cmax_grav  = 2*(pgas + pcdm)
@@ -194,6 +207,12 @@ CONTAINS
   subroutine ref_thermal_iso(cs) bind(C, name="ref_thermal_iso")
     real(c_double), value :: cs
     integer :: p
+#if defined(INTERNAL_ENERGY)
+    ! the energy equation's ideal-gas branch (thermal.F90:108-113) through
+    ! the reference's own driver; cs is not used
+    call update_thermal_properties
+    return
+#endif
     do p=1,ptot
        press(p) = cs*cs*rho(p)
        sound(p) = cs
@@ -237,6 +256,28 @@ CONTAINS
     agravmag(1:n) = am(1:n)
 #endif
   end subroutine ref_set_agravmag
+
+  subroutine ref_set_energy(n, uin, gam) bind(C, name="ref_set_energy")
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: uin(n)
+    real(c_double), value :: gam
+#if defined(INTERNAL_ENERGY)
+    u(1:n) = uin(1:n)
+    gamma = gam
+#endif
+  end subroutine ref_set_energy
+
+  subroutine ref_get_energy(n, uout, dudt) bind(C, name="ref_get_energy")
+    integer(c_int), value :: n
+    real(c_double), intent(out) :: uout(n), dudt(n)
+#if defined(INTERNAL_ENERGY)
+    uout(1:n) = u(1:n)
+    dudt(1:n) = du_dt(1:n)
+#else
+    uout(1:n) = 0.0_DP
+    dudt(1:n) = 0.0_DP
+#endif
+  end subroutine ref_get_energy
 
   subroutine ref_set_h(n, h) bind(C, name="ref_set_h")
     integer(c_int), value :: n
@@ -367,6 +408,9 @@ CONTAINS
     v_old(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
     v_half(1:VDIM,1:ptot) = v(1:VDIM,1:ptot)
     rho_old(1:ptot) = rho(1:ptot)
+#if defined(INTERNAL_ENERGY)
+    u_old(1:ptot) = u(1:ptot)            ! initialize_sph_variables_2.F90:75-77
+#endif
     accdo(1:ptot) = .false.
     nlevels = nlevels_in
     nbuildstep = nbuildstep_in

@@ -73,6 +73,9 @@ typedef struct {
 
 typedef struct oracle_ctx {
     int n, pp_gather, pp_limit, octupole, mac, tc_kernel;
+    int energy;          /* THERMAL=ENERGY_EQN (INTERNAL_ENERGY) */
+    double gamma;
+    double *u, *du_dt;   /* 1-based, allocated by oracle_set_energy */
     double h_fac, thetamaxsqd, abserror, alpha, beta, rextent;
     int *porig;
     double (*parray)[6]; /* x y z m h zeta */
@@ -199,6 +202,7 @@ static void oracle_kernel(oracle_ctx *c) {
 /* ------------------------------------------------------------------------ */
 void oracle_destroy(oracle_ctx *c) {
     if (!c) return;
+    free(c->u); free(c->du_dt);
     free(c->porig); free(c->parray); free(c->v); free(c->a); free(c->gpot);
     free(c->agravmag); free(c->rho); free(c->omega); free(c->press);
     free(c->sound); free(c->div_v); free(c->drhodt); free(c->pptot);
@@ -460,6 +464,10 @@ static void BH_reorder_particles(oracle_ctx *c) {
     PERMUTE(c->sound, double, 1);
     PERMUTE(c->div_v, double, 1);
     PERMUTE(c->omega, double, 1);
+    if (c->energy) {   /* reorder_array.F90:98-104 */
+        PERMUTE(c->u, double, 1);
+        PERMUTE(c->du_dt, double, 1);
+    }
 #undef PERMUTE
     free(dummy);
 }
@@ -1006,13 +1014,43 @@ void oracle_sph_update(oracle_ctx *c) {
 
 /* thermal.F90:52-60 with injected sound speed (P = cs^2 rho, c = cs) */
 void oracle_thermal_iso(oracle_ctx *c, double cs) {
+    if (c->energy) {   /* thermal.F90:108-113, ideal gas from u (DIMENSIONLESS) */
+        for (int p = 1; p <= c->n; p++) {
+            double rho_p = c->rho[p];
+            double press_p = (c->gamma - 1.0) * rho_p * c->u[p];
+            c->press[p] = press_p;
+            c->sound[p] = sqrt(press_p / rho_p);
+        }
+        return;
+    }
     for (int p = 1; p <= c->n; p++) {
         c->press[p] = cs * cs * c->rho[p];
         c->sound[p] = cs;
     }
 }
 
-/* hydro_gradh.F90:9-339 (AB viscosity, no VISC_TD/BALSARA, isothermal) */
+/* THERMAL=ENERGY_EQN: u per particle (current order) and gamma; from then on
+ * oracle_thermal_iso applies the ideal-gas branch and hydro forces form
+ * du/dt */
+void oracle_set_energy(oracle_ctx *c, const double *u, double gamma) {
+    if (!c->u) {
+        c->u = (double *)calloc(c->n + 1, sizeof(double));
+        c->du_dt = (double *)calloc(c->n + 1, sizeof(double));
+    }
+    c->energy = 1;
+    c->gamma = gamma;
+    for (int p = 1; p <= c->n; p++) c->u[p] = u[p - 1];
+}
+
+void oracle_get_energy(const oracle_ctx *c, double *u, double *du_dt) {
+    for (int p = 1; p <= c->n; p++) {
+        u[p - 1] = c->energy ? c->u[p] : 0.0;
+        du_dt[p - 1] = c->energy ? c->du_dt[p] : 0.0;
+    }
+}
+
+/* hydro_gradh.F90:9-339 (AB viscosity, no VISC_TD/BALSARA); with the energy
+ * equation du/dt (:119-124, :258-259, :326-332) */
 static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **big) {
     double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
     double hp = PX(p, SMOO), invhp = 1.0 / hp, hfactor_p = pow(invhp, 4.0);
@@ -1020,6 +1058,7 @@ static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **bi
     double sound_p = c->sound[p], rho_p = c->rho[p];
     double pfactor_p = c->press[p] / rho_p / rho_p / c->omega[p];
     double ah[3] = {0, 0, 0}, ad[3] = {0, 0, 0};
+    double dudt_diss = 0.0;
     int pp_numb = c->pptot[p];
     const int *lst;
     if (pp_numb <= c->pp_limit) {
@@ -1066,9 +1105,17 @@ static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **bi
                         (drmag * drmag + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
             double visc_factor = (-alpha_mean * vsignal * mu + beta * mu * mu) / rhomean;
             for (int k = 0; k < 3; k++) ad[k] = ad[k] + mpp * visc_factor * wmean * du[k];
+            if (c->energy)
+                dudt_diss = dudt_diss + 0.5 * mpp * visc_factor * wmean * dvdr * invdrmag;
         }
     }
     for (int k = 0; k < 3; k++) c->a[p][k] = ah[k] + ad[k];
+    if (c->energy) {
+        double dudt = 0.0;
+        dudt = dudt + dudt_diss;
+        dudt = dudt - c->press[p] * c->div_v[p] / c->omega[p] / rho_p;
+        c->du_dt[p] = dudt;
+    }
 }
 
 /* sph_hydro_forces.F90:9-120 */

@@ -136,8 +136,10 @@ int ensure_capacity(Ctx &c, int n) {
     const size_t nn = (size_t)n + 1;
     double **arrs[] = {&c.x, &c.y, &c.z, &c.m, &c.h, &c.zeta, &c.vx, &c.vy, &c.vz,
                        &c.rho, &c.omega, &c.press, &c.sound, &c.divv, &c.drhodt,
-                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm, &c.tmpd};
+                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm, &c.tmpd, &c.u, &c.dudt};
     for (double **a : arrs) SX_TRY(dalloc(a, nn));
+    SX_HIP(hipMemset(c.u, 0, sizeof(double) * nn));
+    SX_HIP(hipMemset(c.dudt, 0, sizeof(double) * nn));
     SX_TRY(dalloc(&c.porig, nn));
     SX_TRY(dalloc(&c.tmpi, nn));
     SX_TRY(dalloc(&c.active, nn));
@@ -155,6 +157,7 @@ void free_all(Ctx &c) {
     step_free(c);
     void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
+                    c.u, c.dudt,
                     c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
                     c.ovf_idx,
                     c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
@@ -187,6 +190,11 @@ int check_options(const serenx_options &o) {
     if (!(o.h_fac > 0.0)) return set_error(SERENX_EINVAL, "h_fac must be positive");
     if (o.mac != SERENX_MAC_GEOMETRIC && !(o.abserror > 0.0))
         return set_error(SERENX_EINVAL, "abserror must be positive for multipole MACs");
+    if (o.thermal != SERENX_THERMAL_ISOTHERMAL && o.thermal != SERENX_THERMAL_ENERGY_EQN)
+        return set_error(SERENX_EINVAL,
+                         "thermal must be SERENX_THERMAL_ISOTHERMAL or SERENX_THERMAL_ENERGY_EQN");
+    if (o.thermal == SERENX_THERMAL_ENERGY_EQN && !(o.gamma > 1.0))
+        return set_error(SERENX_EINVAL, "gamma must exceed 1 for the energy equation");
     return SERENX_OK;
 }
 
@@ -194,7 +202,7 @@ int check_options(const serenx_options &o) {
 size_t ctx_device_bytes(const Ctx &c) {
     const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
     size_t b = 0;
-    b += nn * (22 * 8 + 16 * 8);                            // particle SoA, tmpd, records
+    b += nn * (24 * 8 + 16 * 8);                            // particle SoA, tmpd, records
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
@@ -333,6 +341,8 @@ int serenx_default_options(serenx_options *opt) {
     opt->abserror = 0.1;
     opt->alpha = 1.0;
     opt->sound_speed = 1.0;
+    opt->thermal = SERENX_THERMAL_ISOTHERMAL;
+    opt->gamma = 1.4;                    // default_parameters.F90:173
     return SERENX_OK;
 }
 
@@ -529,10 +539,18 @@ int serenx_hydro_forces(serenx_ctx *ctx) {
     SX_TRY(up_scalar(c, c.hs.omega, c.omega));
     SX_TRY(up_scalar(c, c.hs.press, c.press));
     SX_TRY(up_scalar(c, c.hs.sound, c.sound));
+    const bool energy = c.opt.thermal == SERENX_THERMAL_ENERGY_EQN;
+    if (energy) {   // du/dt reads div v of p; inactive particles keep their du/dt
+        if (!c.hs.du_dt)
+            return set_error(SERENX_EINVAL, "the energy equation needs du_dt in the host state");
+        SX_TRY(up_scalar(c, c.hs.div_v, c.divv));
+        SX_TRY(up_scalar(c, c.hs.du_dt, c.dudt));
+    }
     SX_TRY(up_accdo(c));
     SX_TRY(run_hydro(c));
     SX_HIP(hipStreamSynchronize(c.stream));
     SX_TRY(down_vec3(c, c.hs.a, c.ax, c.ay, c.az));
+    if (energy) SX_TRY(down_scalar(c, c.hs.du_dt, c.dudt));
     return SERENX_OK;
 }
 
@@ -616,7 +634,7 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
     SX_HIP(hipEventRecord(c.ev[1], c.stream));
     SX_TRY(run_sph_update(c));
-    SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+    SX_TRY(sph_thermal(c));
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
     if (flags & SERENX_EVAL_NO_HYDRO) {
         hipLaunchKernelGGL(k_zero_active, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active,
@@ -739,6 +757,26 @@ int serenx_direct_gravity(serenx_ctx *ctx, int32_t nt, const int32_t *targets, d
     (void)hipFree(da);
     (void)hipFree(dp);
     return rc;
+}
+
+int serenx_set_energy(serenx_ctx *ctx, const double *u) {
+    if (!ctx || !u) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    SX_HIP(hipMemcpy(c.u, u, sizeof(double) * c.n, hipMemcpyHostToDevice));
+    return SERENX_OK;
+}
+
+int serenx_fetch_energy(serenx_ctx *ctx, double *u, double *du_dt) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    if (u) SX_HIP(hipMemcpy(u, c.u, sizeof(double) * c.n, hipMemcpyDeviceToHost));
+    if (du_dt) SX_HIP(hipMemcpy(du_dt, c.dudt, sizeof(double) * c.n, hipMemcpyDeviceToHost));
+    return SERENX_OK;
 }
 
 int serenx_pow_third(serenx_ctx *ctx, int32_t n, const double *x, double *y) {

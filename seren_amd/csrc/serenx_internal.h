@@ -155,6 +155,7 @@ struct Ctx {
            *rho = nullptr, *omega = nullptr, *press = nullptr, *sound = nullptr,
            *divv = nullptr, *drhodt = nullptr, *ax = nullptr, *ay = nullptr,
            *az = nullptr, *gpot = nullptr, *agm = nullptr;
+    double *u = nullptr, *dudt = nullptr;   // energy equation: u, du/dt
     int *porig = nullptr;
     uint8_t *active = nullptr;   // accdo
     double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
@@ -225,6 +226,7 @@ struct Ctx {
         double *vox = nullptr, *voy = nullptr, *voz = nullptr;   // v_old
         double *vhx = nullptr, *vhy = nullptr, *vhz = nullptr;   // v_half
         double *laststep = nullptr, *dt = nullptr;
+        double *u_old = nullptr;             // energy equation
         long long *nlast = nullptr;
         int *nlevel = nullptr;
         int *nminneib = nullptr;             // neighbour check: deepest level among neighbours
@@ -276,6 +278,7 @@ int set_clouds(Ctx &c, int ncloud, const int *counts);
 int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles
 int sph_neighbours(Ctx &c);      // get_neib over active particles
 int sph_thermal_iso(Ctx &c, double cs);
+int sph_thermal(Ctx &c);          // thermal for the context's THERMAL option
 int hydro_forces(Ctx &c);        // hydro_gradh over active particles
 
 // step.hip

@@ -456,12 +456,17 @@ __global__ void k_pack_hydro(int n, const double *__restrict__ x, const double *
     r[3] = make_double4(cr_pow4(invh), 0.0, 0.0, 0.0);
 }
 
-// hydro_gradh.F90:87-297 (AB viscosity, beta := 2 alpha, isothermal)
+// hydro_gradh.F90:87-297 (AB viscosity, beta := 2 alpha).  ENERGY: the
+// INTERNAL_ENERGY terms, du/dt = sum 0.5 m_j visc W_mean (dv.dr) / |dr|
+// (:258-259) - P div v / (Omega rho) (:326-332)
+template <bool ENERGY>
 __global__ void __launch_bounds__(256) k_hydro(
     int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
     const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
     const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab, double alpha,
-    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ press, const double *__restrict__ divv,
+    const double *__restrict__ omega, double *__restrict__ dudt) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n || !active[p]) return;
     const double *w1 = ktab + T_W1 * KT;
@@ -473,6 +478,7 @@ __global__ void __launch_bounds__(256) k_hydro(
     const double hfactor_p = r3.x;
     const double beta = 2.0 * alpha;
     double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0;
+    double dudt_diss = 0.0;
     const int nn = pptot[p];
     const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
     for (int i = 0; i < nn; i++) {
@@ -521,11 +527,29 @@ __global__ void __launch_bounds__(256) k_hydro(
             adx = adx + f * ux;
             ady = ady + f * uy;
             adz = adz + f * uz;
+            if (ENERGY) dudt_diss = dudt_diss + 0.5 * mpp * visc * wmean * dvdr * invdrmag;
         }
     }
     ax[p] = ahx + adx;
     ay[p] = ahy + ady;
     az[p] = ahz + adz;
+    if (ENERGY) dudt[p] = (0.0 + dudt_diss) - press[p] * divv[p] / omega[p] / rho_p;
+}
+
+// thermal.F90:108-113 (INTERNAL_ENERGY, ideal gas): P = (gamma - 1) rho u,
+// c = sqrt(P / rho), for every particle: update_thermal_properties.F90:76-81
+// loops over all particles once any is active (its active-list loop is
+// commented out), so inactive particles take P and c from their advanced u.
+// The temperature is not used on the force path.
+__global__ void k_thermal_energy(int n, double gamma, const double *__restrict__ rho,
+                                 const double *__restrict__ u, double *__restrict__ press,
+                                 double *__restrict__ sound) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rho_p = rho[i];
+    const double press_p = (gamma - 1.0) * rho_p * u[i];
+    press[i] = press_p;
+    sound[i] = sqrt(press_p / rho_p);
 }
 
 __global__ void k_pow_third(int n, const double *__restrict__ x, double *__restrict__ y) {
@@ -609,6 +633,16 @@ int sph_neighbours(Ctx &c) {
     return SERENX_OK;
 }
 
+int sph_thermal(Ctx &c) {
+    if (c.opt.thermal == SERENX_THERMAL_ENERGY_EQN) {
+        hipLaunchKernelGGL(k_thermal_energy, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
+                           c.opt.gamma, c.rho, c.u, c.press, c.sound);
+        SX_HIP(hipGetLastError());
+        return SERENX_OK;
+    }
+    return sph_thermal_iso(c, c.opt.sound_speed);
+}
+
 int sph_thermal_iso(Ctx &c, double cs) {
     hipLaunchKernelGGL(k_thermal_iso, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, cs, c.active,
                        c.rho, c.press, c.sound);
@@ -621,9 +655,14 @@ int hydro_forces(Ctx &c) {
     hipLaunchKernelGGL(k_pack_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.omega, c.press, c.sound, rec);
     SX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec, c.active,
-                       c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab, c.opt.alpha,
-                       c.ax, c.ay, c.az);
+    if (c.opt.thermal == SERENX_THERMAL_ENERGY_EQN)
+        hipLaunchKernelGGL(k_hydro<true>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec,
+                           c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
+                           c.opt.alpha, c.ax, c.ay, c.az, c.press, c.divv, c.omega, c.dudt);
+    else
+        hipLaunchKernelGGL(k_hydro<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec,
+                           c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
+                           c.opt.alpha, c.ax, c.ay, c.az, c.press, c.divv, c.omega, c.dudt);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }

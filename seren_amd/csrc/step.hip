@@ -42,12 +42,15 @@ __global__ void k_red_init(unsigned long long *__restrict__ red) {
                      ? (unsigned long long)__double_as_longlong(DT_INIT) : 0ull;
 }
 
-// timestep_size.F90:92-114: acceleration and Courant criteria with the AB
-// viscosity terms (beta = 2 alpha, hydro_gradh.F90:227)
+// timestep_size.F90:92-122: acceleration and Courant criteria with the AB
+// viscosity terms (beta = 2 alpha, hydro_gradh.F90:227), and with the energy
+// equation the energy criterion, which the reference applies to
+// p < phydroend only (:118), i.e. not to the last particle
 __device__ __forceinline__ double timestep_size(double ax, double ay, double az, double hp,
                                                 double divv, double vsignal, double alpha_p,
                                                 double beta_p, double accel_mult,
-                                                double courant_mult) {
+                                                double courant_mult, bool energy_p, double up,
+                                                double dudt_p) {
     const double div_v_p = fabs(divv);
     const double amag = sqrt(ax * ax + ay * ay + az * az);
     const double tacc = accel_mult * sqrt(hp / (amag + SMALL_NUMBER_DP));
@@ -56,6 +59,11 @@ __device__ __forceinline__ double timestep_size(double ax, double ay, double az,
                          (vsignal + hp * div_v_p +
                           TVISC_FAC * (alpha_p * vsignal + beta_p * hp * div_v_p));
     dt = (tcour < dt) ? tcour : dt;
+    if (energy_p) {
+        const double m = (accel_mult < courant_mult) ? accel_mult : courant_mult;
+        const double tenergy = m * up / (fabs(dudt_p) + SMALL_NUMBER_DP);
+        dt = (tenergy < dt) ? tenergy : dt;
+    }
     return dt;
 }
 
@@ -78,6 +86,7 @@ __device__ __forceinline__ void wave_min_bits(unsigned long long *dst, double v)
 __global__ void k_ts_all(int n, const double *__restrict__ ax, const double *__restrict__ ay,
                          const double *__restrict__ az, const double *__restrict__ h,
                          const double *__restrict__ divv, const double *__restrict__ sound,
+                         const double *__restrict__ u, const double *__restrict__ dudt,
                          double alpha, double accel_mult, double courant_mult,
                          double *__restrict__ dt, long long *__restrict__ nlast,
                          unsigned long long *__restrict__ red) {
@@ -85,8 +94,9 @@ __global__ void k_ts_all(int n, const double *__restrict__ ax, const double *__r
     double d = DT_INIT;
     if (p < n) {
         nlast[p] = 0;
+        const bool en = u && p < n - 1;
         d = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha, 2.0 * alpha,
-                          accel_mult, courant_mult);
+                          accel_mult, courant_mult, en, en ? u[p] : 0.0, en ? dudt[p] : 0.0);
         dt[p] = d;
     }
     wave_min_bits(red + R_DTMIN, d);
@@ -170,7 +180,8 @@ __global__ void k_reduce_steps(int n, long long nint, long long level_step, doub
                                double *__restrict__ vhy, double *__restrict__ vhz,
                                long long *__restrict__ nlast, int *__restrict__ nlevel,
                                int *__restrict__ nminneib, double *__restrict__ laststep,
-                               uint8_t *__restrict__ active,
+                               uint8_t *__restrict__ active, const double *__restrict__ u,
+                               double *__restrict__ u_old,
                                unsigned long long *__restrict__ red) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned added = 0;
@@ -196,6 +207,7 @@ __global__ void k_reduce_steps(int n, long long nint, long long level_step, doub
                 y[p] = r[1] + vh[1] * ls;
                 z[p] = r[2] + vh[2] * ls;
             }
+            if (u) u_old[p] = u[p];   // INTERNAL_ENERGY (:40-42)
             nlevel[p] = (int)lnew;
             nlast[p] = nint;
             nminneib[p] = (int)lnew;
@@ -203,14 +215,14 @@ __global__ void k_reduce_steps(int n, long long nint, long long level_step, doub
             active[p] = 1;   // RUNGE_KUTTA and LEAPFROG_KDK
         }
     }
-    unsigned long long u = added;
+    unsigned long long cnt = added;
     for (int o = 32; o > 0; o >>= 1) {
-        u += __shfl_xor(u, o, 64);
+        cnt += __shfl_xor(cnt, o, 64);
         nmax = max(nmax, __shfl_xor(nmax, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *row = ctr_row(red);
-        atomicAdd(row + R_NACTIVE, u);
+        atomicAdd(row + R_NACTIVE, cnt);
         atomicMax(row + R_NMAX, (unsigned long long)(nmax > 0 ? nmax : 0));
     }
 }
@@ -222,6 +234,7 @@ __global__ void k_ts_update(int n, long long nint, long long level_step, double 
                             const double *__restrict__ ax, const double *__restrict__ ay,
                             const double *__restrict__ az, const double *__restrict__ h,
                             const double *__restrict__ divv, const double *__restrict__ sound,
+                            const double *__restrict__ u, const double *__restrict__ dudt,
                             double alpha, double accel_mult, double courant_mult,
                             const long long *__restrict__ nlast, int *__restrict__ nlevel,
                             unsigned long long *__restrict__ red) {
@@ -233,8 +246,10 @@ __global__ void k_ts_update(int n, long long nint, long long level_step, double 
         if (nint == nlast[p]) {
             upd = 1;
             const long long nlastlevel = lev;
+            const bool en = u && p < n - 1;
             const double dt = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha,
-                                            2.0 * alpha, accel_mult, courant_mult);
+                                            2.0 * alpha, accel_mult, courant_mult, en,
+                                            en ? u[p] : 0.0, en ? dudt[p] : 0.0);
             long long level = level_of(dt_max, dt);
             level = level > 0 ? level : 0;
             if (level < nlastlevel && nlastlevel > 1 &&
@@ -245,15 +260,15 @@ __global__ void k_ts_update(int n, long long nint, long long level_step, double 
             nlevel[p] = lev;
         }
     }
-    unsigned long long u = upd;
+    unsigned long long cnt = upd;
     int lm = lev;
     for (int o = 32; o > 0; o >>= 1) {
-        u += __shfl_xor(u, o, 64);
+        cnt += __shfl_xor(cnt, o, 64);
         lm = max(lm, __shfl_xor(lm, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *row = ctr_row(red);
-        atomicAdd(row + R_PUPDATE, u);
+        atomicAdd(row + R_PUPDATE, cnt);
         atomicMax(row + R_LEVMAX, (unsigned long long)lm);
     }
 }
@@ -282,7 +297,9 @@ __global__ void k_advance(int n, long long nint, long long level_step, double ti
                           double *__restrict__ vhx, double *__restrict__ vhy,
                           double *__restrict__ vhz, long long *__restrict__ nlast,
                           const int *__restrict__ nlevel, double *__restrict__ laststep,
-                          uint8_t *__restrict__ active, unsigned long long *__restrict__ red) {
+                          uint8_t *__restrict__ active, double *__restrict__ u,
+                          double *__restrict__ u_old, const double *__restrict__ dudt,
+                          unsigned long long *__restrict__ red) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned act = 0;
     if (p < n) {
@@ -322,18 +339,26 @@ __global__ void k_advance(int n, long long nint, long long level_step, double ti
         }
         x[p] = rp[0]; y[p] = rp[1]; z[p] = rp[2];
         vx[p] = vp[0]; vy[p] = vp[1]; vz[p] = vp[2];
+        double up = 0.0;
+        if (u) {   // INTERNAL_ENERGY (RK :79-81, KDK :86-89)
+            const double uo = u_old[p], du = dudt[p];
+            up = uo + du * dt;
+            if (KDK && up < SMALL_NUMBER) up = uo * exp(-uo / du);
+            u[p] = up;
+        }
         if (dn == nfull) {
             act = 1;
             nlast[p] = nint;
             laststep[p] = timestep * (double)nfull;
             rox[p] = rp[0]; roy[p] = rp[1]; roz[p] = rp[2];
             vox[p] = vp[0]; voy[p] = vp[1]; voz[p] = vp[2];
+            if (u) u_old[p] = up;
         }
         active[p] = (uint8_t)act;
     }
-    unsigned long long u = act;
-    for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(red) + R_NACTIVE, u);
+    unsigned long long cnt = act;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(red) + R_NACTIVE, cnt);
 }
 
 __global__ void k_step_init(int n, const double *__restrict__ x, const double *__restrict__ y,
@@ -393,13 +418,15 @@ int step_timesteps(Ctx &c) {
     const int n = c.n;
     hipStream_t st = c.stream;
     const bool neib = s.opt.neib_check != SERENX_NEIB_CHECK_OFF;
+    const bool energy = c.opt.thermal == SERENX_THERMAL_ENERGY_EQN;
+    const double *u = energy ? c.u : nullptr, *dudt = energy ? c.dudt : nullptr;
     SX_TRY(red_init(c));
     const double alpha = c.opt.alpha;
     if (s.n == s.nresync) {
         s.n = 0;
         hipLaunchKernelGGL(k_ts_all, dim3(nblk(n)), dim3(TB), 0, st, n, c.ax, c.ay, c.az, c.h,
-                           c.divv, c.sound, alpha, s.opt.accel_mult, s.opt.courant_mult, s.dt,
-                           s.nlast, s.red);
+                           c.divv, c.sound, u, dudt, alpha, s.opt.accel_mult,
+                           s.opt.courant_mult, s.dt, s.nlast, s.red);
         SX_HIP(hipGetLastError());
         unsigned long long r[R_COUNT];
         SX_TRY(red_read(c, r));
@@ -422,7 +449,7 @@ int step_timesteps(Ctx &c) {
     } else {
         const long long level_old = s.level_max;
         hipLaunchKernelGGL(k_ts_update, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
-                           s.dt_max, c.ax, c.ay, c.az, c.h, c.divv, c.sound, alpha,
+                           s.dt_max, c.ax, c.ay, c.az, c.h, c.divv, c.sound, u, dudt, alpha,
                            s.opt.accel_mult, s.opt.courant_mult, s.nlast, s.nlevel, s.red);
         SX_HIP(hipGetLastError());
         unsigned long long r[R_COUNT];
@@ -477,17 +504,19 @@ int step_advance(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
     hipStream_t st = c.stream;
+    const bool energy = c.opt.thermal == SERENX_THERMAL_ENERGY_EQN;
+    double *u = energy ? c.u : nullptr, *u_old = energy ? s.u_old : nullptr;
     SX_TRY(red_init(c));
     if (s.opt.integrator == SERENX_INT_LFKDK)
         hipLaunchKernelGGL(k_advance<1>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
                            s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
                            s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz, s.nlast,
-                           s.nlevel, s.laststep, c.active, s.red);
+                           s.nlevel, s.laststep, c.active, u, u_old, c.dudt, s.red);
     else
         hipLaunchKernelGGL(k_advance<0>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n, s.level_step,
                            s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, s.rox,
                            s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz, s.nlast,
-                           s.nlevel, s.laststep, c.active, s.red);
+                           s.nlevel, s.laststep, c.active, u, u_old, c.dudt, s.red);
     SX_HIP(hipGetLastError());
     const bool reduce = s.opt.neib_check == SERENX_NEIB_CHECK_IMMEDIATE;
     if (reduce) {   // sph_integrate.F90:103-105
@@ -496,13 +525,13 @@ int step_advance(Ctx &c) {
                                s.level_step, s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz,
                                c.ax, c.ay, c.az, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz,
                                s.vhx, s.vhy, s.vhz, s.nlast, s.nlevel, s.nminneib, s.laststep,
-                               c.active, s.red);
+                               c.active, u, u_old, s.red);
         else
             hipLaunchKernelGGL(k_reduce_steps<0>, dim3(nblk(n)), dim3(TB), 0, st, n, s.n,
                                s.level_step, s.timestep, c.x, c.y, c.z, c.vx, c.vy, c.vz,
                                c.ax, c.ay, c.az, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz,
                                s.vhx, s.vhy, s.vhz, s.nlast, s.nlevel, s.nminneib, s.laststep,
-                               c.active, s.red);
+                               c.active, u, u_old, s.red);
         SX_HIP(hipGetLastError());
     }
     unsigned long long r[R_COUNT];
@@ -537,7 +566,7 @@ int step_once(Ctx &c) {
     // from the same h, and it is kept current here all the same
     if (s.nactive > 0) {
         SX_TRY(run_sph_update(c));
-        SX_TRY(sph_thermal_iso(c, c.opt.sound_speed));
+        SX_TRY(sph_thermal(c));
     } else {
         SX_TRY(update_hmax(c));
     }
@@ -559,7 +588,7 @@ int step_once(Ctx &c) {
 void step_free(Ctx &c) {
     Ctx::Step &s = c.st;
     void *ptrs[] = {s.rox, s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy, s.vhz,
-                    s.laststep, s.dt, s.nlast, s.nlevel, s.nminneib, s.red};
+                    s.laststep, s.dt, s.u_old, s.nlast, s.nlevel, s.nminneib, s.red};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     s = Ctx::Step{};
@@ -570,7 +599,7 @@ int permute_step(Ctx &c, const int *perm) {
     if (!s.on) return SERENX_OK;
     const int n = c.n;
     double **arrs[] = {&s.rox, &s.roy, &s.roz, &s.vox, &s.voy, &s.voz, &s.vhx, &s.vhy, &s.vhz,
-                       &s.laststep};
+                       &s.laststep, &s.u_old};
     for (double **a : arrs) {
         hipLaunchKernelGGL(k_gather_step<double>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
                            (const double *)*a, s.dt);
@@ -622,7 +651,7 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
     Ctx::Step &s = c.st;
     const size_t n = (size_t)c.cap + 1;
     double **arrs[] = {&s.rox, &s.roy, &s.roz, &s.vox, &s.voy, &s.voz, &s.vhx, &s.vhy, &s.vhz,
-                       &s.laststep, &s.dt};
+                       &s.laststep, &s.dt, &s.u_old};
     for (double **a : arrs) SX_TRY(dmalloc(a, n));
     SX_TRY(dmalloc(&s.nlast, n));
     SX_TRY(dmalloc(&s.nlevel, n));
@@ -636,6 +665,9 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
                        c.vx, c.vy, c.vz, s.rox, s.roy, s.roz, s.vox, s.voy, s.voz, s.vhx, s.vhy,
                        s.vhz, s.nlast, s.nlevel, s.laststep, c.active);
     SX_HIP(hipGetLastError());
+    // u_old = u (initialize_sph_variables_2.F90:75-77)
+    SX_HIP(hipMemcpyAsync(s.u_old, c.u, sizeof(double) * c.n, hipMemcpyDeviceToDevice,
+                          c.stream));
     SX_HIP(hipStreamSynchronize(c.stream));
     // initialize_seren_variables_2.F90:60-61, initialize_sph_variables_2.F90:165-175
     s.n = 0;

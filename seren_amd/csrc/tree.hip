@@ -682,9 +682,10 @@ int ensure_tree_buffers(Ctx &c) {
 
 int permute_particles(Ctx &c, const int *perm) {
     const int n = c.n;
+    // (u and du_dt: reorder_array.F90:98-104)
     double **arrs[] = {&c.x, &c.y, &c.z, &c.m, &c.h, &c.zeta, &c.vx, &c.vy, &c.vz,
                        &c.rho, &c.omega, &c.press, &c.sound, &c.divv, &c.drhodt,
-                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm};
+                       &c.ax, &c.ay, &c.az, &c.gpot, &c.agm, &c.u, &c.dudt};
     for (double **a : arrs) {
         hipLaunchKernelGGL(k_gather<double>, dim3(nblk(n)), dim3(TB), 0, c.stream, n, perm,
                            (const double *)*a, c.tmpd);

@@ -50,13 +50,14 @@ EXPORTED = [
     "serenx_last_timings", "serenx_interaction_counts", "serenx_walk_stats",
     "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
     "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets", "serenx_set_precision",
-    "serenx_memory", "serenx_pow_third",
+    "serenx_memory", "serenx_pow_third", "serenx_set_energy", "serenx_fetch_energy",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
 # Makefile TIMESTEP and CHECK_NEIB_TIMESTEP options
 _LEVELS = {"adaptive": 0, "restricted": 1, "fixed": 2}
 _NEIB_CHECK = {0: 0, 1: 1, 2: 2, "off": 0, "end": 1, "immediate": 2}
+_THERMAL = {"isothermal": 0, "energy": 1}
 
 
 class SerenxError(RuntimeError):
@@ -66,7 +67,8 @@ class SerenxError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("multipole", C.c_int32), ("mac", C.c_int32), ("precision", C.c_int32),
                 ("kernel", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
-                ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double)]
+                ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double),
+                ("thermal", C.c_int32), ("reserved", C.c_int32), ("gamma", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -77,7 +79,8 @@ class HostState(C.Structure):
     _fields_ = [("ptot", C.c_int32), ("pp_limit", C.c_int32), ("parray", _dp), ("v", _dp),
                 ("a", _dp), ("gpot", _dp), ("agravmag", _dp), ("rho", _dp), ("omega", _dp),
                 ("press", _dp), ("sound", _dp), ("div_v", _dp), ("drhodt", _dp),
-                ("pptot", _ip), ("pplist", _ip), ("accdo", _ip), ("rextent", _dp)]
+                ("pptot", _ip), ("pplist", _ip), ("accdo", _ip), ("rextent", _dp),
+                ("du_dt", _dp)]
 
 
 class StepOptions(C.Structure):
@@ -135,6 +138,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_step.argtypes = [vp, C.c_int32]
     lib.serenx_step_get_info.argtypes = [vp, C.POINTER(StepInfo)]
     lib.serenx_pow_third.argtypes = [vp, C.c_int32, _dp, _dp]
+    lib.serenx_set_energy.argtypes = [vp, _dp]
+    lib.serenx_fetch_energy.argtypes = [vp, _dp, _dp]
     lib.serenx_fetch_step.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _ip, _dp, _dp, _dp,
                                       _ip]
     _lib = lib
@@ -167,7 +172,10 @@ class ForceEngine:
     def __init__(self, multipole: str = "quadrupole", mac: str = "geometric",
                  precision: str = "fp64", h_fac: float = 1.2, thetamaxsqd: float = 0.25,
                  abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
-                 device: int = 0, kernel: str = "m4"):
+                 device: int = 0, kernel: str = "m4", thermal: str = "isothermal",
+                 gamma: float = 1.4):
+        """thermal: "isothermal" (P = c_s^2 rho with sound_speed) or "energy"
+        (the energy equation: set u with set_energy)."""
         self.lib = load_library()
         opt = Options()
         _check(self.lib.serenx_default_options(C.byref(opt)))
@@ -180,6 +188,8 @@ class ForceEngine:
         opt.abserror = abserror
         opt.alpha = alpha
         opt.sound_speed = sound_speed
+        opt.thermal = _THERMAL[thermal]
+        opt.gamma = gamma
         self.opt = opt
         self.ctx = C.c_void_p()
         _check(self.lib.serenx_create(C.byref(opt), device, C.byref(self.ctx)))
@@ -357,6 +367,20 @@ class ForceEngine:
             _iptr(nminneib)))
         return dict(nlevel=nlevel, nlast=nlast, laststep=laststep, active=active,
                     r_old=r_old, v_old=v_old, v_half=v_half, nminneib=nminneib)
+
+    def set_energy(self, u) -> None:
+        """Specific internal energy per particle in the current particle order
+        (load order before the first evaluation, tree order after)."""
+        u = np.ascontiguousarray(u, np.float64)
+        if len(u) != self.n:
+            raise ValueError("u must have one value per particle")
+        _check(self.lib.serenx_set_energy(self.ctx, _dptr(u)))
+
+    def fetch_energy(self):
+        """(u, du_dt) in tree order."""
+        u, du = np.empty(self.n), np.empty(self.n)
+        _check(self.lib.serenx_fetch_energy(self.ctx, _dptr(u), _dptr(du)))
+        return u, du
 
     def pow_third(self, x: np.ndarray) -> np.ndarray:
         """x**0.333333333 as the device h solve evaluates it."""

@@ -6,8 +6,10 @@
 ! src/BHtree/BHhydro_hguess.F90) runs its force evaluation on the GPU through
 ! libserenx.so.  The module arrays are bound by address before every call,
 ! so reallocation between steps is harmless.  Option set: GRAD_H_SPH, M4,
-! ISOTHERMAL, AB viscosity, KS gravity, BH tree, REORDER_PARTICLES,
-! NEIGHBOUR_LISTS, DOUBLE_PRECISION, NDIM=3.
+! ISOTHERMAL or ENERGY_EQN (INTERNAL_ENERGY: du_dt comes back from
+! sph_hydro_forces; thermal and the advance stay on the host), AB viscosity,
+! KS gravity, BH tree, REORDER_PARTICLES, NEIGHBOUR_LISTS, DOUBLE_PRECISION,
+! NDIM=3.
 ! ============================================================================
 
 #include "macros.h"
@@ -49,6 +51,10 @@ CONTAINS
        opt%thetamaxsqd = thetamaxsqd
        opt%abserror = abserror
        opt%alpha = alpha
+#if defined(INTERNAL_ENERGY)
+       opt%thermal = SERENX_THERMAL_ENERGY_EQN
+       opt%gamma = gamma
+#endif
        call serenx_f_check(serenx_create(opt, 0, serenx_ctx), "create")
     end if
     st%ptot = ptot
@@ -63,6 +69,11 @@ CONTAINS
     st%agravmag = c_null_ptr
 #endif
     call address_of_scalar(rextent, st%rextent)
+#if defined(INTERNAL_ENERGY)
+    call address_of(du_dt, st%du_dt)
+#else
+    st%du_dt = c_null_ptr
+#endif
     call serenx_f_check(serenx_bind_host_state(serenx_ctx, st), "bind_host_state")
   contains
     subroutine addresses(x1, x2, x3, x4, x5, x6, x7, x8, x9, x10, i1, i2, l1)

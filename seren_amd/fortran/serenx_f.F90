@@ -12,16 +12,19 @@ MODULE serenx_f
        & SERENX_MAC_GADGET2 = 2, SERENX_MAC_EIGEN = 3
   integer(c_int32_t), parameter :: SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1
   integer(c_int32_t), parameter :: SERENX_FP64 = 0, SERENX_MIXED = 1
+  integer(c_int32_t), parameter :: SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1
 
   type, bind(C) :: serenx_options
      integer(c_int32_t) :: multipole, mac, precision, kernel
      real(c_double) :: h_fac, thetamaxsqd, abserror, alpha, sound_speed
+     integer(c_int32_t) :: thermal, reserved
+     real(c_double) :: gamma
   end type serenx_options
 
   type, bind(C) :: serenx_host_state
      integer(c_int32_t) :: ptot, pp_limit
      type(c_ptr) :: parray, v, a, gpot, agravmag, rho, omega, press, sound, &
-          & div_v, drhodt, pptot, pplist, accdo, rextent
+          & div_v, drhodt, pptot, pplist, accdo, rextent, du_dt
   end type serenx_host_state
 
   interface

@@ -25,8 +25,18 @@ from oracle_lib import RefSeren  # noqa: E402
 from seren_amd import ics  # noqa: E402
 
 CS = 0.2
+GAMMA = 1.4
 
-# name: (n, profile, seed, mach, octupole, mac, thetamaxsqd, abserror, evaluations, full)
+
+def energy0(n, seed):
+    """Seeded specific internal energies around c_s^2 / (gamma - 1) for the
+    energy-equation cases (THERMAL=ENERGY_EQN)."""
+    rng = np.random.default_rng(seed + 1000)
+    return CS * CS / (GAMMA - 1.0) * (1.0 + 0.5 * rng.random(n))
+
+
+# name: (n, profile, seed, mach, octupole, mac, thetamaxsqd, abserror, evaluations, full
+#        [, kernel [, thermal]])
 CASES = {
     "uniform1k_quad_geo": (1000, "uniform", 101, 1.0, False, "geometric", 0.25, 1e-4, 1, True),
     "plummer2k_quad_geo": (2048, "plummer", 202, 0.0, False, "geometric", 0.25, 1e-4, 1, True),
@@ -40,6 +50,9 @@ CASES = {
     # geometric MAC theta = 0.5, one force evaluation (digests for big arrays)
     "c1_uniform16k_quad_geo": (16384, "uniform", 1, 0.0, False, "geometric", 0.25, 1e-4, 1,
                                False),
+    # THERMAL=ENERGY_EQN: ideal gas from u, du/dt
+    "r2_2k_quad_geo_energy_x2": (2048, "r2", 107, 2.0, False, "geometric", 0.25, 1e-4, 2, True,
+                                 "m4", "energy"),
 }
 
 
@@ -50,9 +63,12 @@ def digest(a: np.ndarray) -> str:
 def run_case(name, spec):
     n, prof, seed, mach, octu, mac, th2, ae, nev, full = spec[:10]
     kernel = spec[10] if len(spec) > 10 else "m4"
+    thermal = spec[11] if len(spec) > 11 else "isothermal"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, kernel=kernel)
+    ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal)
     ref.setup(pos, mass, h, vel, thetamaxsqd=th2, abserror=ae)
+    if thermal == "energy":
+        ref.set_energy(energy0(n, seed), GAMMA)
     for e in range(nev):
         ref.tree_update()
         if e == 0:
@@ -88,6 +104,12 @@ def run_case(name, spec):
         ctot=np.array(par["ctot_grav"]), ltot=np.array(par["ltot_grav"]),
         rextent=np.array(par["rextent"]),
     )
+    if thermal == "energy":
+        out["thermal"] = np.array(thermal)
+        out["gamma"] = np.array(GAMMA)
+        out["u0"] = energy0(n, seed)
+        out["u"], out["du_dt"] = ref.energy()
+        out["press"], out["sound"] = p["press"], p["sound"]
     tree = dict(leaf=t["leaf"], nextcell=t["nextcell"], ifopen=ifopen, plist=plist,
                 r=t["r"], m=t["m"], dminsqd=t["dminsqd"], mac_=t["mac"], q=t["q"], s=t["s"],
                 hmax=t["hmax"], hr=ht["r"], rmax=ht["rmax"], hrangemax=ht["hrangemax"])
@@ -116,6 +138,9 @@ STEP_CASES = {
     # TIMESTEP=FIXED CHECK_NEIB_TIMESTEP=2 (IMMEDIATE_TIMESTEP_REDUCTION)
     "step_rk_r2_2k_quad_geo_nlev6_fixed_2": (2048, "r2", 13, 3.0, False, "geometric", 6, 24,
                                              "fixed_2", 0.02),
+    # THERMAL=ENERGY_EQN: u advanced, the energy time-step criterion
+    "step_rk_uniform2k_quad_geo_energy_nlev3": (2000, "uniform", 15, 2.0, False, "geometric", 3,
+                                                8, "", 1.0, "energy"),
 }
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
@@ -123,10 +148,13 @@ STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nst
 def step_case_data(spec):
     """Runs one step case on the reference and returns the fixture arrays."""
     n, prof, seed, mach, octu, mac, nlevels, nsteps = spec[:8]
-    timestep, dt_fixed = spec[8:] if len(spec) > 8 else ("", 1.0)
+    timestep, dt_fixed = spec[8:10] if len(spec) > 8 else ("", 1.0)
+    thermal = spec[10] if len(spec) > 10 else "isothermal"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, timestep=timestep)
+    ref = RefSeren(octu, mac, timestep=timestep, thermal=thermal)
     ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+    if thermal == "energy":
+        ref.set_energy(energy0(n, seed), GAMMA)
     ref.force_eval(CS, hguess=True)
     ref.step_setup(nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
                    dt_fixed=dt_fixed)
@@ -143,11 +171,19 @@ def step_case_data(spec):
         spec=np.array([n, seed, int(octu), nlevels, nsteps], np.int64), profile=np.array(prof),
         mac=np.array(mac), mach=np.array(mach), pos0=pos, vel0=vel, mass=mass,
         timestep=np.array(timestep), dt_fixed=np.array(dt_fixed), nminneib=sp["nminneib"],
+        thermal=np.array(thermal), **energy_out(ref, n, seed, thermal),
         step_ints=np.array(ints, np.int64), step_reals=np.array(reals),
         step_nactive=np.array(nact, np.int64),
         porig=p["porig"], pos=p["pos"], v=p["v"], a=p["a"], h=p["h"], rho=p["rho"],
         nlevel=sp["nlevel"], nlast=sp["nlast"], laststep=sp["laststep"], active=sp["active"],
         r_old=sp["r_old"], v_old=sp["v_old"], v_half=sp["v_half"])
+
+
+def energy_out(ref, n, seed, thermal):
+    if thermal != "energy":
+        return {}
+    u, du = ref.energy()
+    return dict(gamma=np.array(GAMMA), u0=energy0(n, seed), u=u, du_dt=du)
 
 
 def run_step_case(name, spec):

@@ -26,15 +26,19 @@ GOLD = os.path.join(HERE, "golden")
                     reason="drop-in host not built (make -f oracle/Makefile.ref dropin)")
 @pytest.mark.parametrize("case", ["uniform1k_quad_geo", "plummer2k_quad_geo",
                                   "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
-                                  "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc"])
+                                  "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc",
+                                  "r2_2k_quad_geo_energy_x2"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))
     n, seed, octu, nev = (int(v) for v in g["spec"])
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
-    host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel)
+    thermal = str(g["thermal"]) if "thermal" in g.files else "isothermal"
+    host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel, thermal=thermal)
     host.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], thetamaxsqd=float(g["thetamaxsqd"]),
                abserror=float(g["abserror"]))
+    if thermal == "energy":
+        host.set_energy(g["u0"], float(g["gamma"]))
     for e in range(nev):
         host.tree_update()
         if e == 0:
@@ -57,3 +61,7 @@ def test_fortran_dropin_matches_reference(case):
     print(f"{case}: max |da|/|a_grav| {ea.max():.3e}")
     assert ea.max() <= 1e-12
     assert rel_err(p["gpot"], g["gpot"]).max() <= 1e-12
+    if thermal == "energy":   # du_dt comes back from the library's sph_hydro_forces
+        u, du = host.energy()
+        assert np.array_equal(u, g["u"])
+        assert np.abs(du - g["du_dt"]).max() <= 1e-11 * np.abs(g["du_dt"]).max()

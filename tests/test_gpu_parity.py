@@ -1,7 +1,8 @@
 """GPU parity: the gfx950 force path against the CPU restatement oracle on
 identical seeded inputs.
 
-Bar (per quantity):
+Bar (per quantity; the reference-vector test adds u, du/dt, P and c for
+the energy equation, bar in that test):
   * tree topology (BFS ids, leaf counts, plist, nextcell, ifopen, levels),
     tree order (porig), neighbour counts and lists: bit-exact;
   * stocked moments, COM, opening distances, hydro-cell centres, radii,
@@ -44,11 +45,11 @@ def _ids(c):
     return f"n{c[0]}-{c[1]}-{_MULTI[c[3]]}-{c[4]}-x{c[7]}"
 
 
-def _engine(octupole, mac, theta2, abserr, kernel="m4"):
+def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal"):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole=_MULTI[octupole], mac=mac,
                        precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
-                       kernel=kernel)
+                       kernel=kernel, thermal=thermal)
 
 
 def _report(name, a, b):
@@ -182,9 +183,14 @@ def test_gpu_matches_reference_golden_vectors(case):
     n, seed, octu, nev = (int(v) for v in g["spec"])
     mac = str(g["mac"])
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
+    energy = "thermal" in g.files and str(g["thermal"]) == "energy"
     print(f"golden {case}", flush=True)
-    eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel)
+    eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel,
+                  "energy" if energy else "isothermal")
     eng.load(g["pos"], g["vel"], g["mass"], np.zeros(n))
+    if energy:
+        assert float(g["gamma"]) == eng.opt.gamma
+        eng.set_energy(g["u0"])
     for e in range(nev):
         eng.force_eval(hguess=(e == 0))
     d = eng.fetch()
@@ -219,4 +225,15 @@ def test_gpu_matches_reference_golden_vectors(case):
     print(f"  a: max |da|/|a_grav| {ea.max():.3e}", flush=True)
     assert ea.max() <= 1e-12
     assert rel_err(d["gpot"], g["gpot"]).max() <= 1e-12
+    if energy:
+        # u moves with the tree order only; du/dt = viscous heating - P div v /
+        # (Omega rho) carries the ~1e-12 relative h-solve differences of div v
+        u, du = eng.fetch_energy()
+        assert np.array_equal(u, g["u"])
+        edu = np.abs(du - g["du_dt"]).max() / np.abs(g["du_dt"]).max()
+        print(f"  du/dt: max |d(du/dt)| / max |du/dt| {edu:.3e}, "
+              f"{int(np.sum(du != g['du_dt']))} not bit-exact", flush=True)
+        assert edu <= 1e-11
+        for k in ("press", "sound"):
+            assert rel_err(d[k], g[k]).max() <= 1e-12, k
     eng.close()

@@ -19,7 +19,9 @@ builds with them: TIMESTEP=RESTRICTED with CHECK_NEIB_TIMESTEP=1 (the
 Makefile's defaults) and TIMESTEP=FIXED with CHECK_NEIB_TIMESTEP=2
 (check_neighbour_timesteps.F90, reduce_timesteps.F90); nminneib is compared
 exactly as well.  The other fixtures come from a build with neither option
-(adaptive levels, no neighbour check).
+(adaptive levels, no neighbour check); one of them with THERMAL=ENERGY_EQN
+(u advanced with du/dt, the energy time-step criterion), where u is bound
+to 1e-11 relative and du/dt to 1e-10 of its largest value.
 
 The leapfrog KDK integrator has no reference counterpart (the reference's
 advance_leapfrog_kdk.F90 uses dt_half before setting it); it is checked
@@ -39,17 +41,19 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STEP_CASES = ["step_rk_uniform2k_quad_geo_nlev3", "step_rk_r2_2k_oct_gadget2_nlev1",
-              "step_rk_r2_2k_quad_geo_nlev6_restricted_1", "step_rk_r2_2k_quad_geo_nlev6_fixed_2"]
+              "step_rk_r2_2k_quad_geo_nlev6_restricted_1", "step_rk_r2_2k_quad_geo_nlev6_fixed_2",
+              "step_rk_uniform2k_quad_geo_energy_nlev3"]
 # reference build -> (levels, neib_check)
 TIMESTEP_BUILDS = {"": ("adaptive", 0), "restricted_1": ("restricted", 1),
                    "fixed_2": ("fixed", 2)}
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
 
-def _engine(octupole, mac):
+def _engine(octupole, mac, thermal="isothermal"):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole="octupole" if octupole else "quadrupole", mac=mac,
-                       precision="fp64", thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS)
+                       precision="fp64", thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS,
+                       thermal=thermal)
 
 
 @pytest.mark.parametrize("name", STEP_CASES)
@@ -58,10 +62,13 @@ def test_rk_steps_match_reference_golden(name):
     g = np.load(os.path.join(GOLD, name + ".npz"))
     n, seed, octu, nlevels, nsteps = (int(v) for v in g["spec"])
     levels, check = TIMESTEP_BUILDS[str(g["timestep"])]
-    eng = _engine(bool(octu), str(g["mac"]))
+    energy = str(g["thermal"]) == "energy"
+    eng = _engine(bool(octu), str(g["mac"]), "energy" if energy else "isothermal")
     try:
         h0 = np.zeros(n)
         eng.load(g["pos0"], g["vel0"], g["mass"], h0)
+        if energy:
+            eng.set_energy(g["u0"])
         eng.force_eval(hguess=True)
         eng.step_begin("rk", nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
                        levels=levels, dt_fixed=float(g["dt_fixed"]), neib_check=check)
@@ -76,8 +83,12 @@ def test_rk_steps_match_reference_golden(name):
                 assert rel_err(info[key], gr[j]) <= 1e-12, (k, key, info[key], gr[j])
         d = eng.fetch(neighbours=False)
         st = eng.fetch_step()
+        en = eng.fetch_energy() if energy else None
     finally:
         eng.close()
+    if energy:   # u advanced with du/dt (advance_runge_kutta.F90:79-81)
+        assert rel_err(en[0], g["u"]).max() <= 1e-11
+        assert np.abs(en[1] - g["du_dt"]).max() <= 1e-10 * np.abs(g["du_dt"]).max()
     # tree order after the last build equals the reference's
     assert np.array_equal(d["porig"], g["porig"])
     scale = np.abs(g["pos"]).max()

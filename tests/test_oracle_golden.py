@@ -5,7 +5,7 @@ themselves (the flang build of the reference sources, see
 tests/golden/make_golden.py).  The oracle must reproduce every stored
 quantity bit for bit: tree topology and moments, tree order, h, rho, Omega,
 zeta, div v, neighbour lists, hydro and total accelerations, potentials and
-the kernel tables.
+the kernel tables; with the energy equation also u, du/dt, P and c.
 """
 import glob
 import hashlib
@@ -34,6 +34,8 @@ def _run_oracle(g):
     o = CSeren(bool(octu), mac, kernel)
     o.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], thetamaxsqd=float(g["thetamaxsqd"]),
             abserror=float(g["abserror"]))
+    if "thermal" in g.files and str(g["thermal"]) == "energy":
+        o.set_energy(g["u0"], float(g["gamma"]))
     for e in range(nev):
         o.tree_update()
         if e == 0:
@@ -62,6 +64,10 @@ def test_oracle_matches_reference_fixture(case):
     for k in ("porig", "h", "rho", "omega", "zeta", "div_v", "drhodt", "a", "gpot", "agravmag"):
         assert np.array_equal(p[k], g[k]), k
     assert np.array_equal(a_hydro, g["a_hydro"])
+    if "du_dt" in g.files:
+        u, du = o.energy()
+        assert np.array_equal(u, g["u"]) and np.array_equal(du, g["du_dt"])
+        assert np.array_equal(p["press"], g["press"]) and np.array_equal(p["sound"], g["sound"])
     assert np.array_equal(pptot, g["pptot"])
     valid = pptot <= pplist.shape[1]
     cols = np.arange(pplist.shape[1])[None, :] < pptot[:, None]
