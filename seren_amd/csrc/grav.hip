@@ -15,7 +15,9 @@
 //
 // Mixed precision (SERENX_MIXED): the cell offset rp - rc is formed in FP64
 // and the multipole and pair terms are evaluated in FP32 on that
-// cell-relative offset, accumulated with Neumaier-compensated FP32 sums.
+// cell-relative offset; cell terms are summed in FP32 batches added to FP64
+// sums, pair terms in Neumaier-compensated FP32 sums.  Its walk uses a group
+// MAC (see k_grav_mixed).
 #include "walk.h"
 
 namespace serenx {
@@ -309,6 +311,14 @@ __device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], fl
 // paid once per batch instead of once per cell.
 constexpr int CELL_BATCH = 8;
 
+// The mixed walk uses a group MAC: a wave of 64 tree-ordered particles
+// accepts a cell only when the MAC of BHgrav_accel.F90:154-158 accepts it for
+// every one of its particles, and otherwise opens it for all of them (leaves
+// give pair terms to all).  Every accepted cell therefore satisfies each
+// particle's own MAC, so the truncation error is at most the per-particle
+// walk's; the walk is wave-uniform (no per-lane resume state), and no cell
+// term is issued for a cell that only some lanes use.  The FP64 walk keeps
+// the reference's per-particle interaction sets.
 template <int OCT, int MACT>
 __global__ void __launch_bounds__(256) k_grav_mixed(
     int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
@@ -332,34 +342,38 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
     int nbatch = 0;                                   // wave-uniform
     CSum px, py, pz, pq;                              // pair terms
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, vis = 0, lvis = 0;   // wave totals
+    // the walk covers the lanes' trees (one DFS range, or a few adjacent ones
+    // when the wave straddles clouds); a lane takes part only inside its own
     const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
-    const unsigned vis = group_walk(
-        gn, nc, valid, rr.x, rr.y, lvis,
-        [&](const GNodeF &g, bool here) -> int {
-            const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
-            const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
-            bool accept;
-            if (MACT == SERENX_MAC_GEOMETRIC)
-                accept = drsqd > g.dminsqd;
-            else
-                accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-            accept = accept && g.leaf != 1;
-            const unsigned nacc = lanes(here && accept);
-            ncell += nacc;
-            if (here && accept) {
+    int c = uniform(wave_min_int(valid ? rr.x : nc));
+    const int cend = uniform(wave_max_int(valid ? rr.y : 0));
+    while (c < cend) {
+        const GNodeF g = gn[c];
+        vis++;
+        const bool mine = valid && c >= rr.x && c < rr.y;
+        lvis += lanes(mine);
+        const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
+        const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
+        bool accept;
+        if (MACT == SERENX_MAC_GEOMETRIC)
+            accept = drsqd > g.dminsqd;
+        else
+            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
+        accept = accept && g.leaf != 1;                  // :128-136
+        if (__ballot(mine && !accept) == 0) {            // every lane accepts
+            ncell += lanes(mine);
+            if (mine) {
                 const float dr[3] = {(float)d0, (float)d1, (float)d2};
                 cell_fp32<OCT>(g, dr, bx, by, bz, bp);
             }
-            if (nacc && ++nbatch == CELL_BATCH) {
+            if (++nbatch == CELL_BATCH) {
                 cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
                 bx = by = bz = bp = 0.0f;
                 nbatch = 0;
             }
-            if (accept) return WALK_SKIP;
-            return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
-        },
-        [&](const GNodeF &g, bool mine) {
+            c = g.next;
+        } else if (g.leaf > 0) {
             for (int i = 0; i < g.leaf; i++) {
                 const int j = g.first + i;
                 const double4 a = rec[4 * (size_t)j + 2], b = rec[4 * (size_t)j + 3];
@@ -373,7 +387,11 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 nin_p += lanes(in_ & 1);
                 nin_j += lanes(in_ & 2);
             }
-        });
+            c = g.next;
+        } else {
+            c = c + 1;
+        }
+    }
     cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
     if (valid) {
         const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();

@@ -8,8 +8,10 @@ Tolerances, stated here and asserted below:
   * FP64 tree vs FP64 direct sum, octupole + GADGET2 MAC (abserror 1e-4)
     after one evaluation that supplies |a|: rms |da|/|a| <= 2e-3 (the MAC
     bounds the error per cell relative to |a|);
-  * mixed vs FP64 tree (same tree, FP64 MAC tests): rms |da|/|a| <= 1e-5 and
-    the mixed tree vs direct sum within 1.05x of the FP64 tree's rms error.
+  * mixed walk (same tree; group MAC, so every cell it accepts satisfies
+    each particle's own MAC): mixed tree vs direct sum within 1.05x of the
+    FP64 tree's rms error, and the mixed tree within 2x that error of the
+    FP64 tree (the two differ by the cells the group opens).
 """
 import os
 import sys
@@ -88,5 +90,5 @@ def test_tree_and_mixed_vs_direct_sum(profile):
           f"mixed vs fp64 rms {rms(emx):.3e} max {emx.max():.3e}; "
           f"counts fp64 {res['fp64_counts']} mixed {res['mixed_counts']}")
     assert rms(e64) <= 2e-3
-    assert rms(emx) <= 1e-5
     assert rms(e32) <= 1.05 * rms(e64) + 1e-6
+    assert rms(emx) <= 2.0 * rms(e64)
