@@ -38,6 +38,13 @@ __global__ void k_pack_hsolve(int n, const double *__restrict__ x, const double 
     r[1] = make_double4(vx[i], vy[i], vz[i], m[i]);
 }
 
+__global__ void k_pack_xyzh(int n, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const double *__restrict__ h,
+                            double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rec[4 * (size_t)i] = make_double4(x[i], y[i], z[i], h[i]);
+}
+
 // HNode visit of BHhydrowalk_hgather.F90:40 for a search radius R
 __device__ __forceinline__ int hgather_visit(const HNode &nd, double rpx, double rpy, double rpz,
                                              double R) {
@@ -336,10 +343,7 @@ constexpr int OVF_PER_WAVE = 1;
 template <bool OVERFLOW_PASS>
 __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf_idx, int nc,
                                               const HNode *__restrict__ hn,
-                                              const double *__restrict__ x,
-                                              const double *__restrict__ y,
-                                              const double *__restrict__ z,
-                                              const double *__restrict__ h,
+                                              const double4 *__restrict__ rec,
                                               const int *__restrict__ cid,
                                               const int2 *__restrict__ root_range,
                                               const uint8_t *__restrict__ active, int pp_limit,
@@ -358,8 +362,9 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
     const bool valid = OVERFLOW_PASS ? ovf_lane : (p < n && active[p]);
     double rpx = 0, rpy = 0, rpz = 0, hp = 0;
     if (valid) {
-        rpx = x[p]; rpy = y[p]; rpz = z[p];
-        hp = h[p];
+        const double4 a = rec[4 * (size_t)p];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        hp = a.w;
     }
     const double hrangesqd_p = KERNRANGESQD * hp * hp;
     const double R = KERNRANGE * hp;
@@ -386,7 +391,8 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
             const int k = nd.leaf, f0 = nd.first;
             for (int i = 0; i < k; i++) {
                 const int j = f0 + i;
-                const double xj = x[j], yj = y[j], zj = z[j], hj = h[j];
+                const double4 aj = rec[4 * (size_t)j];   // one scalar load per candidate
+                const double xj = aj.x, yj = aj.y, zj = aj.z, hj = aj.w;
                 if (!mine || j == p) continue;
                 const double hrangesqd_pp = KERNRANGESQD * hj * hj;
                 const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
@@ -589,9 +595,12 @@ int sph_hsolve(Ctx &c) {
 
 int sph_neighbours(Ctx &c) {
     hipStream_t s = c.stream;
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    // [0] x y z h with the solved h
+    hipLaunchKernelGGL(k_pack_xyzh, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.x, c.y, c.z, c.h, rec);
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.ovf_idx, c.nc, c.hn,
-                       c.x, c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
+                       rec, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
                        c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -626,7 +635,7 @@ int sph_neighbours(Ctx &c) {
         const int nov = c.hostpin[9];   // overflow particles, counted by k_neib<false>
         const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
         hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, nov, c.ovf_idx, c.nc,
-                           c.hn, c.x, c.y, c.z, c.h, c.cid, c.root_range, c.active, c.pp_limit,
+                           c.hn, rec, c.cid, c.root_range, c.active, c.pp_limit,
                            c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());
     }
