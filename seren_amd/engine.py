@@ -26,7 +26,8 @@ from typing import Dict, Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libserenx.so")
+# SERENX_LIB names another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("SERENX_LIB") or os.path.join(HERE, "libserenx.so")
 
 QUADRUPOLE, OCTUPOLE, OCTUPOLE_EXACT = 2, 3, 4
 MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2, MAC_EIGEN = 0, 1, 2, 3
