@@ -384,12 +384,17 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         for (int i = 0; i < NTAB * KT; i++) tf[i] = (float)t[i];
         double *g4 = dev.data() + KT_G4_OFF;
         float *g4f = reinterpret_cast<float *>(dev.data() + KT_G4F_OFF);
+        double *h4 = dev.data() + KT_H4_OFF;
         for (int k = 0; k < KT; k++) {
             const double w[3] = {t[T_W4 * KT + k], t[T_W5 * KT + k], t[T_W6 * KT + k]};
             for (int q = 0; q < 3; q++) {
                 g4[4 * k + q] = w[q];
                 g4f[4 * k + q] = (float)w[q];
             }
+            h4[4 * k + 0] = t[T_W0 * KT + k];
+            h4[4 * k + 1] = t[T_WH * KT + k];
+            h4[4 * k + 2] = t[T_W4 * KT + k];
+            h4[4 * k + 3] = t[T_WG * KT + k];
         }
         e = hipMemcpy(c->ktab, dev.data(), sizeof(double) * KT_TOTAL, hipMemcpyHostToDevice);
     }

@@ -41,7 +41,10 @@ enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
 constexpr size_t KT_F32_OFF = (size_t)NTAB * KT;
 constexpr size_t KT_G4_OFF = ((KT_F32_OFF + ((size_t)NTAB * KT + 1) / 2) + 3) / 4 * 4;
 constexpr size_t KT_G4F_OFF = KT_G4_OFF + 4 * (size_t)KT;
-constexpr size_t KT_TOTAL = KT_G4F_OFF + 2 * (size_t)KT;
+// (w0, wh, w4, wg) per index for the h solve: one 16- or 32-byte load per
+// candidate
+constexpr size_t KT_H4_OFF = (KT_G4F_OFF + 2 * (size_t)KT + 3) / 4 * 4;
+constexpr size_t KT_TOTAL = KT_H4_OFF + 4 * (size_t)KT;
 
 constexpr int WAVE = 64;
 // potential-neighbour list slots (opened leaves) per particle in the h solve

@@ -23,26 +23,26 @@ inline int nblk(long n, int tb = TB) { return (int)((n + tb - 1) / tb); }
 // lane modes of the h solve
 enum { M_DONE = 0, M_ITER = 1, M_FINAL = 2 };
 
-// Positions, velocities and masses as 64-byte records for the per-lane
-// candidate loops of the h solve: [0] x y z h  [1] vx vy vz m (the first half
-// of the hydro pair record, which the hydro pass rewrites in full).
+// Compact 32-byte records for the per-lane candidate loops of the h solve,
+// in the record buffer (which the hydro pass later rewrites): rec[i] =
+// (x, y, z, m) and rec[n + i] = (vx, vy, vz, 0).  Four particles share a
+// cache line, so a leaf's candidates are two lines, and the loops' working
+// set stays in L2.
 __global__ void k_pack_hsolve(int n, const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ z, const double *__restrict__ h,
-                              const double *__restrict__ vx, const double *__restrict__ vy,
-                              const double *__restrict__ vz, const double *__restrict__ m,
-                              double4 *__restrict__ rec) {
+                              const double *__restrict__ z, const double *__restrict__ vx,
+                              const double *__restrict__ vy, const double *__restrict__ vz,
+                              const double *__restrict__ m, double4 *__restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double4 *r = rec + 4 * (size_t)i;
-    r[0] = make_double4(x[i], y[i], z[i], h[i]);
-    r[1] = make_double4(vx[i], vy[i], vz[i], m[i]);
+    rec[i] = make_double4(x[i], y[i], z[i], m[i]);
+    rec[(size_t)n + i] = make_double4(vx[i], vy[i], vz[i], 0.0);
 }
 
 __global__ void k_pack_xyzh(int n, const double *__restrict__ x, const double *__restrict__ y,
                             const double *__restrict__ z, const double *__restrict__ h,
                             double4 *__restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) rec[4 * (size_t)i] = make_double4(x[i], y[i], z[i], h[i]);
+    if (i < n) rec[i] = make_double4(x[i], y[i], z[i], h[i]);
 }
 
 // HNode visit of BHhydrowalk_hgather.F90:40 for a search radius R
@@ -99,12 +99,13 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
     int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + (threadIdx.x & (WAVE - 1));
     const double *w0 = ktab + T_W0 * KT;
     const double *wh = ktab + T_WH * KT;
+    const double *h4 = ktab + KT_H4_OFF;   // (w0, wh, w4, wg) per index
     double rpx = 0, rpy = 0, rpz = 0, mp = 0, hp = 1, rextent = 0.0;
     int2 rr = make_int2(nc, nc);
     if (valid) {
-        const double4 a = rec[4 * (size_t)p], b = rec[4 * (size_t)p + 1];
-        rpx = a.x; rpy = a.y; rpz = a.z; hp = a.w;
-        mp = b.w;
+        const double4 a = rec[p];
+        rpx = a.x; rpy = a.y; rpz = a.z; mp = a.w;
+        hp = h[p];
         const int k = cid[p];
         rr = root_range[k];
         rextent = roots[11 * k + 4];   // the particle's own cloud (bounding_box.F90:39)
@@ -139,8 +140,9 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
             const double drmag = sqrt(dsq) + SMALL_NUMBER;
             const int kern = (int)(HALFKERNTOT * drmag * invhp);
             if (kern >= KERNTOT) return;                        // :227
-            srho = srho + mj * hfactor_p * w0[kern];
-            som = som + mj * hfactor_p * invhp * wh[kern];
+            const double2 w = *reinterpret_cast<const double2 *>(h4 + 4 * (size_t)kern);
+            srho = srho + mj * hfactor_p * w.x;
+            som = som + mj * hfactor_p * invhp * w.y;
         };
 
         if (!have_list || __any(newlist)) {
@@ -175,16 +177,16 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
         const bool sum = iterating && !grow;
         if (!wave_ovf) {
             list_loop(wl, nslot, sum, p, [&](int j) {
-                const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
-                pair(a.x, a.y, a.z, b.w);
+                const double4 a = rec[j];
+                pair(a.x, a.y, a.z, a.w);
             });
         } else {
             vis += group_walk(hn, nc, sum, rr.x, rr.y, lvis, visit,
                               [&](const HNode &nd, bool mine) {
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
-                    const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
-                    if (mine && j != p) pair(a.x, a.y, a.z, b.w);
+                    const double4 a = rec[j];
+                    if (mine && j != p) pair(a.x, a.y, a.z, a.w);
                 }
             });
             fallback++;
@@ -262,15 +264,16 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
     const bool valid = p < n && active[p];
     const int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + (threadIdx.x & (WAVE - 1));
     const double *w0 = ktab + T_W0 * KT;
-    const double *w4 = ktab + T_W4 * KT;
     const double *wh = ktab + T_WH * KT;
     const double *wg = ktab + T_WG * KT;
+    const double *h4 = ktab + KT_H4_OFF;   // (w0, wh, w4, wg) per index
+    const double4 *vrec = rec + n;
     double rpx = 0, rpy = 0, rpz = 0, mp = 0, vpx = 0, vpy = 0, vpz = 0, R = 0, hp = 1;
     int nslot = 0;
     if (valid) {
-        const double4 a = rec[4 * (size_t)p], b = rec[4 * (size_t)p + 1];
-        rpx = a.x; rpy = a.y; rpz = a.z;
-        vpx = b.x; vpy = b.y; vpz = b.z; mp = b.w;
+        const double4 a = rec[p], b = vrec[p];
+        rpx = a.x; rpy = a.y; rpz = a.z; mp = a.w;
+        vpx = b.x; vpy = b.y; vpz = b.z;
         hp = h_fac * ref_pow_third(mp / rho[p]);                 // :317
         R = KERNRANGE * hhigh[p];
         nslot = nlist[p];
@@ -294,14 +297,15 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
         if (kern > KERNTOT) return;                             // :368
         const double dvx = vxj - vpx, dvy = vyj - vpy, dvz = vzj - vpz;
         const double dvdr = dvx * ex + dvy * ey + dvz * ez;
-        srho = srho + mj * hfactor_p * w0[kern];
-        som = som + mj * hfactor_p * invhp * wh[kern];
-        sdiv = sdiv - mj * dvdr * w4[kern] * hfactor_p * invhp / drmag;
-        szeta = szeta + mj * invhp * invhp * wg[kern];
+        const double4 w = *reinterpret_cast<const double4 *>(h4 + 4 * (size_t)kern);
+        srho = srho + mj * hfactor_p * w.x;
+        som = som + mj * hfactor_p * invhp * w.y;
+        sdiv = sdiv - mj * dvdr * w.z * hfactor_p * invhp / drmag;
+        szeta = szeta + mj * invhp * invhp * w.w;
     };
     list_loop(wl, nslot, valid && nslot >= 0, p, [&](int j) {
-        const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
-        pair(a.x, a.y, a.z, b.w, b.x, b.y, b.z);
+        const double4 a = rec[j], b = vrec[j];
+        pair(a.x, a.y, a.z, a.w, b.x, b.y, b.z);
     });
     unsigned vis = 0, lvis = 0;
     const bool walk = valid && nslot < 0;
@@ -314,8 +318,8 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
             [&](const HNode &nd, bool mine) {
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
-                    const double4 a = rec[4 * (size_t)j], b = rec[4 * (size_t)j + 1];
-                    if (mine && j != p) pair(a.x, a.y, a.z, b.w, b.x, b.y, b.z);
+                    const double4 a = rec[j], b = vrec[j];
+                    if (mine && j != p) pair(a.x, a.y, a.z, a.w, b.x, b.y, b.z);
                 }
             });
         wave_stats(wstat, 2, vis, lvis);
@@ -362,7 +366,7 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
     const bool valid = OVERFLOW_PASS ? ovf_lane : (p < n && active[p]);
     double rpx = 0, rpy = 0, rpz = 0, hp = 0;
     if (valid) {
-        const double4 a = rec[4 * (size_t)p];
+        const double4 a = rec[p];
         rpx = a.x; rpy = a.y; rpz = a.z;
         hp = a.w;
     }
@@ -391,7 +395,7 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
             const int k = nd.leaf, f0 = nd.first;
             for (int i = 0; i < k; i++) {
                 const int j = f0 + i;
-                const double4 aj = rec[4 * (size_t)j];   // one scalar load per candidate
+                const double4 aj = rec[j];   // one scalar load per candidate
                 const double xj = aj.x, yj = aj.y, zj = aj.z, hj = aj.w;
                 if (!mine || j == p) continue;
                 const double hrangesqd_pp = KERNRANGESQD * hj * hj;
@@ -578,7 +582,7 @@ int sph_hsolve(Ctx &c) {
     SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
-                       c.h, c.vx, c.vy, c.vz, c.m, rec);
+                       c.vx, c.vy, c.vz, c.m, rec);
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
     hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
