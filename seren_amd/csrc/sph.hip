@@ -56,20 +56,39 @@ __device__ __forceinline__ int hgather_visit(const HNode &nd, double rpx, double
 }
 
 // Loops every summing lane over its own potential-neighbour list, leaf by
-// leaf in DFS order, calling pair(j) for each listed particle j != p.
-template <class Pair>
+// leaf in DFS order, calling use(j, load(j)) for each listed particle j != p.
+// Each lane walks its own stream of candidates and issues the load of the
+// next candidate (and the next list slot) before using the current one, so
+// the record's memory latency overlaps the current candidate's table lookup
+// and arithmetic.  The order of the candidates is the list order.
+template <class Load, class Use>
 __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot, bool sum, int p,
-                                          Pair &&pair) {
-    const int smax = uniform(wave_max_int(sum ? nslot : 0));
-    for (int sl = 0; sl < smax; sl++) {
-        if (sum && sl < nslot) {
+                                          Load &&load, Use &&use) {
+    int sl = 0, jcur = 0, jend = 0;
+    // moves to the next listed particle; false when the list is done
+    auto advance = [&]() -> bool {
+        while (jcur >= jend) {
+            if (sl >= nslot) return false;
             const int d = wl[sl * WAVE];
-            const int f0 = d >> 4, k = d & 15;
-            for (int i = 0; i < k; i++) {
-                const int j = f0 + i;
-                if (j != p) pair(j);
-            }
+            sl++;
+            jcur = d >> 4;
+            jend = jcur + (d & 15);
         }
+        return true;
+    };
+    bool have = sum && advance();
+    decltype(load(0)) next{};
+    if (have) next = load(jcur);
+    while (__any(have)) {
+        const auto cur = next;
+        const int jc = jcur;
+        const bool hc = have;
+        if (have) {
+            jcur++;
+            have = advance();
+            if (have) next = load(jcur);
+        }
+        if (hc && jc != p) use(jc, cur);
     }
 }
 
@@ -176,10 +195,8 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
         const bool grow = iterating && newlist && cnt < pp_gather;
         const bool sum = iterating && !grow;
         if (!wave_ovf) {
-            list_loop(wl, nslot, sum, p, [&](int j) {
-                const double4 a = rec[j];
-                pair(a.x, a.y, a.z, a.w);
-            });
+            list_loop(wl, nslot, sum, p, [&](int j) { return rec[j]; },
+                      [&](int, const double4 &a) { pair(a.x, a.y, a.z, a.w); });
         } else {
             vis += group_walk(hn, nc, sum, rr.x, rr.y, lvis, visit,
                               [&](const HNode &nd, bool mine) {
@@ -287,15 +304,15 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
     double sdiv = 0.0;
     double szeta = mp * invhp * invhp * wg[0];
     // :356-381 for one potential neighbour
-    auto pair = [&](double xj, double yj, double zj, double mj, double vxj, double vyj,
-                    double vzj) {
+    auto pair = [&](double xj, double yj, double zj, double mj, int j) {
         const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
         const double dsq = ex * ex + ey * ey + ez * ez;
         if (dsq > hrangesqd) return;                            // :364
         const double drmag = sqrt(dsq) + SMALL_NUMBER;
         const int kern = (int)(HALFKERNTOT * drmag * invhp);
         if (kern > KERNTOT) return;                             // :368
-        const double dvx = vxj - vpx, dvy = vyj - vpy, dvz = vzj - vpz;
+        const double4 vj = vrec[j];
+        const double dvx = vj.x - vpx, dvy = vj.y - vpy, dvz = vj.z - vpz;
         const double dvdr = dvx * ex + dvy * ey + dvz * ez;
         const double4 w = *reinterpret_cast<const double4 *>(h4 + 4 * (size_t)kern);
         srho = srho + mj * hfactor_p * w.x;
@@ -303,10 +320,8 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
         sdiv = sdiv - mj * dvdr * w.z * hfactor_p * invhp / drmag;
         szeta = szeta + mj * invhp * invhp * w.w;
     };
-    list_loop(wl, nslot, valid && nslot >= 0, p, [&](int j) {
-        const double4 a = rec[j], b = vrec[j];
-        pair(a.x, a.y, a.z, a.w, b.x, b.y, b.z);
-    });
+    list_loop(wl, nslot, valid && nslot >= 0, p, [&](int j) { return rec[j]; },
+              [&](int j, const double4 &a) { pair(a.x, a.y, a.z, a.w, j); });
     unsigned vis = 0, lvis = 0;
     const bool walk = valid && nslot < 0;
     if (__any(walk)) {
@@ -318,8 +333,8 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
             [&](const HNode &nd, bool mine) {
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
-                    const double4 a = rec[j], b = vrec[j];
-                    if (mine && j != p) pair(a.x, a.y, a.z, a.w, b.x, b.y, b.z);
+                    const double4 a = rec[j];
+                    if (mine && j != p) pair(a.x, a.y, a.z, a.w, j);
                 }
             });
         wave_stats(wstat, 2, vis, lvis);
