@@ -72,10 +72,18 @@ enum {
     SERENX_EVAL_NO_HYDRO = 4  /* skip the hydro pair forces */
 };
 
-/* kernel tables (kernel.F90:14-127): M4 cubic spline, or M4 with the
- * Thomas-Couchman (1992) gradient in the hydro force (TC_KERNEL: w1 = -1/pi
- * for s < 2/3) */
-enum { SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1 };
+/* kernel (Makefile KERNEL, kernel.F90:14-236, macros.h:101-118): M4 cubic
+ * spline (range 2h), M4 with the Thomas-Couchman (1992) gradient in the
+ * hydro force (M4TC: w1 = -1/pi for s < 2/3), the quintic spline of Morris
+ * (1996) (range 3h, table index 3333 r/h), or the quintic with its
+ * Thomas-Couchman gradient (QUINTICTC).  The kernel range sets the
+ * neighbour radius, so pp_gather = 4 pi/3 (range h_fac)^3. */
+enum {
+    SERENX_KERNEL_M4 = 0,
+    SERENX_KERNEL_M4_TC = 1,
+    SERENX_KERNEL_QUINTIC = 2,
+    SERENX_KERNEL_QUINTIC_TC = 3
+};
 
 /* THERMAL option (Makefile THERMAL): ISOTHERMAL (P = cs^2 rho, c = cs with
  * the given sound speed), or ENERGY_EQN (INTERNAL_ENERGY): the ideal gas

@@ -35,9 +35,15 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
+# kernel name -> the kernel option of the restatement and of the GPU library
+KERNELS = {"m4": 0, "m4tc": 1, "quintic": 2, "quintictc": 3}
+
 VARIANTS = {
     # (octupole, mac[, kernel]) -> reference library variant
     (False, "geometric", "m4tc"): "quad_geo_tc",
+    # Makefile KERNEL=QUINTIC and QUINTICTC
+    (False, "geometric", "quintic"): "quad_geo_quintic",
+    (True, "gadget2", "quintictc"): "oct_gadget2_quintictc",
     (False, "geometric"): "quad_geo",
     (True, "gadget2"): "oct_gadget2",
     (True, "gadget"): "oct_gadget",
@@ -339,7 +345,7 @@ class CSeren(_Common):
         self.lib = self._load()
         self.octupole = 2 if octupole == "exact" else int(bool(octupole))
         self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2, "eigen": 3}[mac]
-        self.tc = {"m4": 0, "m4tc": 1}[kernel]
+        self.kernel_id = KERNELS[kernel]
         self.ctx = None
 
     def __del__(self):
@@ -355,7 +361,7 @@ class CSeren(_Common):
         self.n = len(mass)
         self.ctx = self.lib.oracle_create(self.n, pos, mass, h, vel, int(self.octupole),
                                           self.mac, h_fac, thetamaxsqd, abserror, alpha,
-                                          self.tc)
+                                          self.kernel_id)
         self.pp_limit = self.params()["pp_limit"]
 
     def tree_update(self):

@@ -7,10 +7,11 @@
  *
  * It restates, in plain C99 with FP64 arithmetic in the reference's operation
  * order (compiled with -ffp-contract=off so no FMA contraction happens), the
- * option set GRAD_H_SPH + M4 kernel + AB viscosity + isothermal + KS gravity +
+ * option set GRAD_H_SPH + AB viscosity + isothermal + KS gravity +
  * BH tree + REORDER_PARTICLES + NEIGHBOUR_LISTS, NDIM=3, DOUBLE precision,
  * with QUADRUPOLE or OCTUPOLE moments, the GEOMETRIC, GADGET, GADGET2 or
- * EIGEN MAC, and the M4 or M4 Thomas-Couchman (TC_KERNEL) kernel.
+ * EIGEN MAC, and the M4 or quintic kernel, each with or without the
+ * Thomas-Couchman gradient (TC_KERNEL).
  *
  * Arrays are 1-based like the Fortran module arrays they mirror: index 0 of
  * every particle array is unused; tree cell ids start at 0 (root).
@@ -36,10 +37,6 @@
 #define LMAX 30
 #define NCHILD 8
 #define KERNTOT 10000
-#define HALFKERNTOT 5000.0
-#define KERNRANGE 2.0
-#define KERNRANGESQD 4.0
-#define INVKERNRANGE 0.5
 #define SMALL_NUMBER 1.0e-20
 #define BIG_NUMBER 9.9e20
 #define HMULT 1.1
@@ -72,7 +69,9 @@ typedef struct {
 } snode;
 
 typedef struct oracle_ctx {
-    int n, pp_gather, pp_limit, octupole, mac, tc_kernel;
+    int n, pp_gather, pp_limit, octupole, mac, kernel;
+    /* kernel range constants, macros.h:101-118 */
+    double halfkerntot, kernrange, kernrangesqd, invkernrange;
     int energy;          /* THERMAL=ENERGY_EQN (INTERNAL_ENERGY) */
     double gamma;
     double *u, *du_dt;   /* 1-based, allocated by oracle_set_energy */
@@ -147,16 +146,16 @@ static double ipow(double x, int n) {
 }
 
 /* kernel.F90:14-127 (M4, GRAD_H_SPH, NDIM=3) */
-static void oracle_kernel(oracle_ctx *c) {
+static void kernel_m4(oracle_ctx *c) {
     const double invkerntot = 1.0 / (double)KERNTOT;
-    const int kern1 = (int)HALFKERNTOT;
+    const int kern1 = 5000; /* int(HALFKERNTOT) */
     const int kerntc = (int)((double)KERNTOT / 3.0); /* kernel.F90:37 */
     const double ndim = 3.0;
     for (int i = 0; i <= KERNTOT; i++) {
-        double s = (KERNRANGE * (double)i + 1.0) * invkerntot;
-        double saux = KERNRANGE - s;
+        double s = (c->kernrange * (double)i + 1.0) * invkerntot;
+        double saux = c->kernrange - s;
         if (i < kern1) {
-            if (c->tc_kernel && i < kerntc) /* TC_KERNEL, kernel.F90:56-62 */
+            if (c->kernel == 1 && i < kerntc) /* TC_KERNEL, kernel.F90:56-62 */
                 c->w1[i] = -1.0;
             else
                 c->w1[i] = -(3.0 * s) + 2.25 * s * s;
@@ -194,6 +193,76 @@ static void oracle_kernel(oracle_ctx *c) {
     }
 }
 
+/* kernel.F90:130-236 (QUINTIC, GRAD_H_SPH, NDIM=3): three polynomial
+ * pieces split at entries int(ONETHIRD KERNTOT) and int(2 ONETHIRD KERNTOT);
+ * w2 = w3 = 0; the TC gradient is the constant -55.2040 below
+ * int(0.75929848074 KERNTOT / 3). */
+static void kernel_quintic(oracle_ctx *c) {
+    const double invkerntot = 1.0 / (double)KERNTOT;
+    const int kern1 = (int)(ONETHIRD * (double)KERNTOT);
+    const int kern2 = (int)(2.0 * ONETHIRD * (double)KERNTOT);
+    const int kerntc = (int)(0.75929848074 * (double)KERNTOT / 3.0);
+    const double d = 3.0; /* NDIMPR */
+    for (int i = 0; i <= KERNTOT; i++) {
+        const double s = (3.0 * (double)i + 1.0) * invkerntot;
+        const double s2 = ipow(s, 2), s3 = ipow(s, 3), s4 = ipow(s, 4), s5 = ipow(s, 5),
+                     s6 = ipow(s, 6), s7 = ipow(s, 7);
+        double w0, w1, w4, w5, w6, wh, wg;
+        if (i < kern1) {
+            w4 = -(120.0 * s) + 120.0 * s3 - 50.0 * s4;
+            w1 = (c->kernel == 3 && i < kerntc) ? -55.2040 : w4;
+            w0 = 66.0 - 60.0 * s * s + 30.0 * s4 - 10.0 * s5;
+            w5 = 22.0 * s - 12.0 * s3 + (30.0 / 7.0) * s5 - (5.0 / 4.0) * s6;
+            w6 = -(11.0 * s * s) + 3.0 * s4 - (5.0 / 7.0) * s6 + (5.0 / 28.0) * s7 +
+                 (478.0 / 14.0);
+            wh = -(66.0 * d) + 60.0 * (d + 2.0) * s * s - 30.0 * (d + 4.0) * s4 +
+                 10.0 * (d + 5.0) * s5;
+            wg = 33.0 * s * s - 15.0 * s4 + 5.0 * s6 - (10.0 / 7.0) * s7 - (478.0 / 14.0);
+        } else if (i < kern2) {
+            w0 = 51.0 + 75.0 * s - 210.0 * s * s + 150.0 * s3 - 45.0 * s4 + 5.0 * s5;
+            w4 = 75.0 - 420.0 * s + 450.0 * s * s - 180.0 * s3 + 25.0 * s4;
+            w1 = w4;
+            w5 = 17.0 * s + (75.0 / 4.0) * s2 - 42.0 * s3 + 25.0 * s4 - (45.0 / 7.0) * s5 +
+                 (5.0 / 8.0) * s6 + (5.0 / 56.0) / s2;
+            w6 = -((17.0 / 2.0) * s * s) - (25.0 / 4.0) * s3 + (21.0 / 2.0) * s4 - 5.0 * s5 +
+                 (15.0 / 14.0) * s6 - (5.0 / 56.0) * s7 + (473.0 / 14.0) + (5.0 / 56.0) / s;
+            wh = -(51.0 * d) - 75.0 * (d + 1.0) * s + 210.0 * (d + 2.0) * s2 -
+                 150.0 * (d + 3.0) * s3 + 45.0 * (d + 4.0) * s4 - 5.0 * (d + 5.0) * s5;
+            wg = (51.0 / 2.0) * s * s + 25.0 * s3 - (105.0 / 2.0) * s4 + 30.0 * s5 -
+                 (15.0 / 2.0) * s6 + (5.0 / 7.0) * s7 - (473.0 / 14.0);
+        } else {
+            w0 = 243.0 - 405.0 * s + 270.0 * s * s - 90.0 * s3 + 15.0 * s4 - s5;
+            w4 = -405.0 + 540.0 * s - 270.0 * s * s + 60.0 * s3 - 5.0 * s4;
+            w1 = w4;
+            w5 = 81.0 * s - (405.0 / 4.0) * s2 + 54.0 * s3 - 15.0 * s4 + (15.0 / 7.0) * s5 -
+                 (1.0 / 8.0) * s6 - (507.0 / 56.0) / s2;
+            w6 = -((81.0 / 2.0) * s * s) + (135.0 / 4.0) * s3 - (27.0 / 2.0) * s4 + 3.0 * s5 -
+                 (5.0 / 14.0) * s6 + (1.0 / 56.0) * s7 + (729.0 / 14.0) - (507.0 / 56.0) / s;
+            wh = -(243.0 * d) + 405.0 * (d + 1.0) * s - 270.0 * (d + 2.0) * s2 +
+                 90.0 * (d + 3.0) * s3 - 15.0 * (d + 4.0) * s4 + (d + 5.0) * s5;
+            wg = (243.0 / 2.0) * s * s - 135.0 * s3 + (135.0 / 2.0) * s4 - 18.0 * s5 +
+                 (5.0 / 2.0) * s6 - (1.0 / 7.0) * s7 - (729.0 / 14.0);
+        }
+        /* normalisation for NDIM = 3, evaluated left to right */
+        c->w0[i] = w0 * INVPI * 3.0 / 359.0;
+        c->w1[i] = w1 * INVPI * 3.0 / 359.0;
+        c->w2[i] = 0.0;
+        c->w3[i] = 0.0;
+        c->w4[i] = w4 * INVPI * 3.0 / 359.0;
+        c->w5[i] = w5 * 12.0 / 359.0;
+        c->w6[i] = w6 * 12.0 / 359.0;
+        c->wh[i] = wh * INVPI * 3.0 / 359.0;
+        c->wg[i] = -wg * 12.0 / 359.0;
+    }
+}
+
+static void oracle_kernel(oracle_ctx *c) {
+    if (c->kernel >= 2)
+        kernel_quintic(c);
+    else
+        kernel_m4(c);
+}
+
 #define PX(p, k) (c->parray[(p)][(k)])
 #define MASS 3
 #define SMOO 4
@@ -216,10 +285,21 @@ void oracle_destroy(oracle_ctx *c) {
 oracle_ctx *oracle_create(int n, const double *pos, const double *mass,
                           const double *h, const double *vel, int octupole,
                           int mac, double h_fac, double thetamaxsqd,
-                          double abserror, double alpha, int tc_kernel) {
+                          double abserror, double alpha, int kernel) {
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     c->n = n;
-    c->tc_kernel = tc_kernel;
+    c->kernel = kernel; /* 0 M4, 1 M4 TC, 2 QUINTIC, 3 QUINTIC TC */
+    if (kernel >= 2) {
+        c->halfkerntot = 3333.0;
+        c->kernrange = 3.0;
+        c->kernrangesqd = 9.0;
+        c->invkernrange = 0.3333333333333;
+    } else {
+        c->halfkerntot = 5000.0;
+        c->kernrange = 2.0;
+        c->kernrangesqd = 4.0;
+        c->invkernrange = 0.5;
+    }
     c->octupole = octupole;
     c->mac = mac;
     c->h_fac = h_fac;
@@ -227,7 +307,7 @@ oracle_ctx *oracle_create(int n, const double *pos, const double *mass,
     c->abserror = abserror;
     c->alpha = alpha;
     c->beta = 2.0 * alpha;
-    c->pp_gather = (int)((4.0 * PI_ / 3.0) * ipow(KERNRANGE * h_fac, 3));
+    c->pp_gather = (int)((4.0 * PI_ / 3.0) * ipow(c->kernrange * h_fac, 3));
     c->pp_limit = 2 * c->pp_gather;
     c->porig = (int *)calloc(n + 1, sizeof(int));
     c->parray = calloc(n + 1, sizeof(*c->parray));
@@ -619,7 +699,7 @@ static void BHgrav_stock(oracle_ctx *c) {
             for (int k = 0; k < 3; k++)
                 dr[k] = fmax(st->bbmax[k] - rc[k], rc[k] - st->bbmin[k]);
             double diagonalsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-            double rsize = sqrt(diagonalsqd) + KERNRANGE * st->hmax;
+            double rsize = sqrt(diagonalsqd) + c->kernrange * st->hmax;
             if (c->mac == MAC_GEOMETRIC) {
                 g->dminsqd = (c->thetamaxsqd > 0.0) ? diagonalsqd / c->thetamaxsqd : BIG_NUMBER;
                 g->mac = 0.0;
@@ -765,7 +845,7 @@ void oracle_hguess(oracle_ctx *c) {
                 if (pstack[laux] > thresh || laux == 0) break;
                 laux--;
             }
-            double hguess = INVKERNRANGE * rstack[laux];
+            double hguess = c->invkernrange * rstack[laux];
             for (int i = 0; i < hn->leaf; i++) PX(hn->plist[i], SMOO) = hguess;
             ci = hn->nextcell;
         } else if (hn->leaf == 0) {
@@ -827,7 +907,7 @@ static void all_sph_gradh(oracle_ctx *c, int p, int *pp_potlist, int **bigbuf) {
             if (hp > h_high) {
                 pp_pot = 0;
                 h_high = hp * HMULT;
-                pp_pot = hydro_walk(c, rp, KERNRANGE * h_high, pp_max, list, 0);
+                pp_pot = hydro_walk(c, rp, c->kernrange * h_high, pp_max, list, 0);
             }
             if (pp_pot < 0) {
                 h_high = 0.0;
@@ -845,7 +925,7 @@ static void all_sph_gradh(oracle_ctx *c, int p, int *pp_potlist, int **bigbuf) {
         h_new = hp;
         invhp = 1.0 / hp;
         hfactor_p = pow(invhp, 3.0);
-        hrangesqd = KERNRANGESQD * hp * hp;
+        hrangesqd = c->kernrangesqd * hp * hp;
         rhotemp = mp * c->w0[0] * hfactor_p;
         omega_p = mp * hfactor_p * invhp * c->wh[0];
         for (int i = 0; i < pp_pot; i++) {
@@ -855,7 +935,7 @@ static void all_sph_gradh(oracle_ctx *c, int p, int *pp_potlist, int **bigbuf) {
             double drsqd = dr0 * dr0 + dr1 * dr1 + dr2 * dr2;
             if (drsqd >= hrangesqd) continue;
             double drmag = sqrt(drsqd) + SMALL_NUMBER;
-            double skern = HALFKERNTOT * drmag * invhp;
+            double skern = c->halfkerntot * drmag * invhp;
             int kern = (int)skern;
             if (kern >= KERNTOT) continue;
             double mpp = PX(pp, MASS);
@@ -894,7 +974,7 @@ static void all_sph_gradh(oracle_ctx *c, int p, int *pp_potlist, int **bigbuf) {
     hp = c->h_fac * pow(mp / rhotemp, ONETHIRD);
     invhp = 1.0 / hp;
     hfactor_p = pow(invhp, 3.0);
-    hrangesqd = KERNRANGESQD * hp * hp;
+    hrangesqd = c->kernrangesqd * hp * hp;
     rhotemp = mp * c->w0[0] * hfactor_p;
     omega_p = mp * hfactor_p * invhp * c->wh[0];
     div_v_p = 0.0;
@@ -906,7 +986,7 @@ static void all_sph_gradh(oracle_ctx *c, int p, int *pp_potlist, int **bigbuf) {
         double drmag = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
         if (drmag > hrangesqd) continue;
         drmag = sqrt(drmag) + SMALL_NUMBER;
-        double skern = HALFKERNTOT * drmag * invhp;
+        double skern = c->halfkerntot * drmag * invhp;
         int kern = (int)skern;
         if (kern > KERNTOT) continue;
         double mpp = PX(pp, MASS);
@@ -935,9 +1015,9 @@ static void BHhydro_update_hmax(oracle_ctx *c) {
             hnode *hn = &c->BHhydro[ci];
             double hr = 0.0;
             if (hn->leaf == 1) {
-                hn->hrangemax = KERNRANGE * PX(hn->plist[0], SMOO);
+                hn->hrangemax = c->kernrange * PX(hn->plist[0], SMOO);
             } else if (hn->leaf > 0) {
-                for (int i = 0; i < hn->leaf; i++) hr = fmax(hr, KERNRANGE * PX(hn->plist[i], SMOO));
+                for (int i = 0; i < hn->leaf; i++) hr = fmax(hr, c->kernrange * PX(hn->plist[i], SMOO));
                 hn->hrangemax = hr;
             } else if (hn->leaf == 0) {
                 int cc = hn->ifopen, cend = hn->nextcell;
@@ -957,20 +1037,20 @@ static void BHhydro_update_hmax(oracle_ctx *c) {
 static int get_neib(oracle_ctx *c, int p, double hp, int *potlist, int **bigbuf,
                     int *out, int outmax) {
     double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
-    double hrangesqd_p = KERNRANGESQD * hp * hp;
+    double hrangesqd_p = c->kernrangesqd * hp * hp;
     int pp_max = (LISTSIZE < c->n) ? LISTSIZE : c->n;
     int *list = potlist;
-    int pp_pot = hydro_walk(c, rp, KERNRANGE * hp, pp_max, list, 1);
+    int pp_pot = hydro_walk(c, rp, c->kernrange * hp, pp_max, list, 1);
     if (pp_pot < 0) {
         if (!*bigbuf) *bigbuf = (int *)malloc(sizeof(int) * (c->n + 1));
         list = *bigbuf;
-        pp_pot = hydro_walk(c, rp, KERNRANGE * hp, c->n, list, 1);
+        pp_pot = hydro_walk(c, rp, c->kernrange * hp, c->n, list, 1);
     }
     int pp_tot = 0;
     for (int i = 0; i < pp_pot; i++) {
         int pp = list[i];
         if (pp == p) continue;
-        double hrangesqd_pp = KERNRANGESQD * PX(pp, SMOO) * PX(pp, SMOO);
+        double hrangesqd_pp = c->kernrangesqd * PX(pp, SMOO) * PX(pp, SMOO);
         double dr0 = PX(pp, 0) - rp[0], dr1 = PX(pp, 1) - rp[1], dr2 = PX(pp, 2) - rp[2];
         double drsqd = dr0 * dr0 + dr1 * dr1 + dr2 * dr2;
         if (drsqd <= hrangesqd_p || drsqd <= hrangesqd_pp) {
@@ -1073,24 +1153,24 @@ static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **bi
         double wmean = 0.0, rho_pp = c->rho[pp], mpp = PX(pp, MASS), hpp = PX(pp, SMOO);
         double du[3] = {PX(pp, 0) - rp[0], PX(pp, 1) - rp[1], PX(pp, 2) - rp[2]};
         double drmag = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
-        if (drmag >= KERNRANGESQD * hp * hp && drmag >= KERNRANGESQD * hpp * hpp) continue;
+        if (drmag >= c->kernrangesqd * hp * hp && drmag >= c->kernrangesqd * hpp * hpp) continue;
         drmag = sqrt(drmag) + SMALL_NUMBER;
         double dv[3] = {c->v[pp][0] - vp[0], c->v[pp][1] - vp[1], c->v[pp][2] - vp[2]};
         double dvdr = dv[0] * du[0] + dv[1] * du[1] + dv[2] * du[2];
         double invdrmag = 1.0 / drmag;
         for (int k = 0; k < 3; k++) du[k] = du[k] * invdrmag;
         double pfactor_pp = c->press[pp] / rho_pp / rho_pp / c->omega[pp];
-        if (drmag < KERNRANGE * hp) {
-            double skern = HALFKERNTOT * drmag * invhp;
+        if (drmag < c->kernrange * hp) {
+            double skern = c->halfkerntot * drmag * invhp;
             int kern = (int)skern;
             if (kern > KERNTOT) kern = KERNTOT;
             wmean = 0.5 * hfactor_p * c->w1[kern];
             for (int k = 0; k < 3; k++)
                 ah[k] = ah[k] + mpp * pfactor_p * hfactor_p * c->w1[kern] * du[k];
         }
-        if (drmag < KERNRANGE * hpp) {
+        if (drmag < c->kernrange * hpp) {
             double invhpp = 1.0 / hpp, hfactor_pp = pow(invhpp, 4.0);
-            double skern = HALFKERNTOT * drmag * invhpp;
+            double skern = c->halfkerntot * drmag * invhpp;
             int kern = (int)skern;
             if (kern > KERNTOT) kern = KERNTOT;
             wmean = wmean + 0.5 * hfactor_pp * c->w1[kern];
@@ -1144,8 +1224,8 @@ static inline void gravity_gradh(const oracle_ctx *c, double invhp, double hpp,
     double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     double drmag = sqrt(drsqd) + SMALL_NUMBER;
     double invdrmag = 1.0 / drmag, grav, dpot;
-    if (invdrmag > INVKERNRANGE * invhp) {
-        int kern = (int)(HALFKERNTOT * drmag * invhp);
+    if (invdrmag > c->invkernrange * invhp) {
+        int kern = (int)(c->halfkerntot * drmag * invhp);
         if (kern > KERNTOT) kern = KERNTOT;
         grav = invhp * invhp * (c->w5[kern] + zo_p * c->w4[kern] * invhp * invhp);
         dpot = invhp * c->w6[kern];
@@ -1153,9 +1233,9 @@ static inline void gravity_gradh(const oracle_ctx *c, double invhp, double hpp,
         grav = invdrmag * invdrmag;
         dpot = invdrmag;
     }
-    if (drmag < KERNRANGE * hpp) {
+    if (drmag < c->kernrange * hpp) {
         double invhpp = 1.0 / hpp;
-        int kern = (int)(HALFKERNTOT * drmag * invhpp);
+        int kern = (int)(c->halfkerntot * drmag * invhpp);
         if (kern > KERNTOT) kern = KERNTOT;
         grav = grav + invhpp * invhpp * (c->w5[kern] + zo_pp * c->w4[kern] * invhpp * invhpp);
         dpot = dpot + invhpp * c->w6[kern];

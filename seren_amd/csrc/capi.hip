@@ -31,18 +31,95 @@ static double ipow_seq(double x, int n) {
     return y;
 }
 
-// kernel.F90:14-127 (M4, GRAD_H_SPH, NDIM = 3); tab[j*KT + i]
-void make_kernel_tables(double *tab, int kernel) {
+// kernel.F90:130-236, the quintic spline (GRAD_H_SPH, NDIM = 3), entry i at
+// s = (3 i + 1) / KERNTOT; w2 and w3 are zero for this kernel
+static void quintic_tables(double *tab, bool tc) {
     const double invkerntot = 1.0 / (double)KERNTOT;
-    const int kern1 = (int)HALFKERNTOT;
+    const int kern1 = (int)(ONETHIRD * (double)KERNTOT);          // kernel.F90:40-41
+    const int kern2 = (int)(2.0 * ONETHIRD * (double)KERNTOT);
+    const int kerntc = (int)(0.75929848074 * (double)KERNTOT / 3.0);   // :43
+    const double nd = 3.0;
+    double *w0 = tab + T_W0 * KT, *w1 = tab + T_W1 * KT, *w2 = tab + T_W2 * KT,
+           *w3 = tab + T_W3 * KT, *w4 = tab + T_W4 * KT, *w5 = tab + T_W5 * KT,
+           *w6 = tab + T_W6 * KT, *wh = tab + T_WH * KT, *wg = tab + T_WG * KT;
+    auto P = ipow_seq;
+    for (int i = 0; i <= KERNTOT; i++) {
+        const double s = (3.0 * (double)i + 1.0) * invkerntot;
+        w2[i] = 0.0;
+        w3[i] = 0.0;
+        if (i < kern1) {
+            const double g = -(120.0 * s) + 120.0 * P(s, 3) - 50.0 * P(s, 4);
+            w1[i] = (tc && i < kerntc) ? -55.2040 : g;
+            w0[i] = 66.0 - 60.0 * s * s + 30.0 * P(s, 4) - 10.0 * P(s, 5);
+            w4[i] = g;
+            w5[i] = 22.0 * s - 12.0 * P(s, 3) + (30.0 / 7.0) * P(s, 5) - (5.0 / 4.0) * P(s, 6);
+            w6[i] = -(11.0 * s * s) + 3.0 * P(s, 4) - (5.0 / 7.0) * P(s, 6) +
+                    (5.0 / 28.0) * P(s, 7) + (478.0 / 14.0);
+            wh[i] = -(66.0 * nd) + 60.0 * (nd + 2.0) * s * s - 30.0 * (nd + 4.0) * P(s, 4) +
+                    10.0 * (nd + 5.0) * P(s, 5);
+            wg[i] = 33.0 * s * s - 15.0 * P(s, 4) + 5.0 * P(s, 6) - (10.0 / 7.0) * P(s, 7) -
+                    (478.0 / 14.0);
+        } else if (i < kern2) {
+            w0[i] = 51.0 + 75.0 * s - 210.0 * s * s + 150.0 * P(s, 3) - 45.0 * P(s, 4) +
+                    5.0 * P(s, 5);
+            const double g = 75.0 - 420.0 * s + 450.0 * s * s - 180.0 * P(s, 3) + 25.0 * P(s, 4);
+            w1[i] = g;
+            w4[i] = g;
+            w5[i] = 17.0 * s + (75.0 / 4.0) * P(s, 2) - 42.0 * P(s, 3) + 25.0 * P(s, 4) -
+                    (45.0 / 7.0) * P(s, 5) + (5.0 / 8.0) * P(s, 6) + (5.0 / 56.0) / P(s, 2);
+            w6[i] = -((17.0 / 2.0) * s * s) - (25.0 / 4.0) * P(s, 3) + (21.0 / 2.0) * P(s, 4) -
+                    5.0 * P(s, 5) + (15.0 / 14.0) * P(s, 6) - (5.0 / 56.0) * P(s, 7) +
+                    (473.0 / 14.0) + (5.0 / 56.0) / s;
+            wh[i] = -(51.0 * nd) - 75.0 * (nd + 1.0) * s + 210.0 * (nd + 2.0) * P(s, 2) -
+                    150.0 * (nd + 3.0) * P(s, 3) + 45.0 * (nd + 4.0) * P(s, 4) -
+                    5.0 * (nd + 5.0) * P(s, 5);
+            wg[i] = (51.0 / 2.0) * s * s + 25.0 * P(s, 3) - (105.0 / 2.0) * P(s, 4) +
+                    30.0 * P(s, 5) - (15.0 / 2.0) * P(s, 6) + (5.0 / 7.0) * P(s, 7) -
+                    (473.0 / 14.0);
+        } else {
+            w0[i] = 243.0 - 405.0 * s + 270.0 * s * s - 90.0 * P(s, 3) + 15.0 * P(s, 4) - P(s, 5);
+            const double g = -405.0 + 540.0 * s - 270.0 * s * s + 60.0 * P(s, 3) - 5.0 * P(s, 4);
+            w1[i] = g;
+            w4[i] = g;
+            w5[i] = 81.0 * s - (405.0 / 4.0) * P(s, 2) + 54.0 * P(s, 3) - 15.0 * P(s, 4) +
+                    (15.0 / 7.0) * P(s, 5) - (1.0 / 8.0) * P(s, 6) - (507.0 / 56.0) / P(s, 2);
+            w6[i] = -((81.0 / 2.0) * s * s) + (135.0 / 4.0) * P(s, 3) - (27.0 / 2.0) * P(s, 4) +
+                    3.0 * P(s, 5) - (5.0 / 14.0) * P(s, 6) + (1.0 / 56.0) * P(s, 7) +
+                    (729.0 / 14.0) - (507.0 / 56.0) / s;
+            wh[i] = -(243.0 * nd) + 405.0 * (nd + 1.0) * s - 270.0 * (nd + 2.0) * P(s, 2) +
+                    90.0 * (nd + 3.0) * P(s, 3) - 15.0 * (nd + 4.0) * P(s, 4) +
+                    (nd + 5.0) * P(s, 5);
+            wg[i] = (243.0 / 2.0) * s * s - 135.0 * P(s, 3) + (135.0 / 2.0) * P(s, 4) -
+                    18.0 * P(s, 5) + (5.0 / 2.0) * P(s, 6) - (1.0 / 7.0) * P(s, 7) -
+                    (729.0 / 14.0);
+        }
+        // normalisation, NDIM = 3 (kernel.F90:220-236): left to right
+        w0[i] = w0[i] * INVPI * 3.0 / 359.0;
+        w1[i] = w1[i] * INVPI * 3.0 / 359.0;
+        w4[i] = w4[i] * INVPI * 3.0 / 359.0;
+        w5[i] = w5[i] * 12.0 / 359.0;
+        w6[i] = w6[i] * 12.0 / 359.0;
+        wh[i] = wh[i] * INVPI * 3.0 / 359.0;
+        wg[i] = -wg[i] * 12.0 / 359.0;
+    }
+}
+
+// kernel.F90:14-127 (M4, GRAD_H_SPH, NDIM = 3), or the quintic; tab[j*KT + i]
+void make_kernel_tables(double *tab, int kernel) {
+    if (kernel == SERENX_KERNEL_QUINTIC || kernel == SERENX_KERNEL_QUINTIC_TC) {
+        quintic_tables(tab, kernel == SERENX_KERNEL_QUINTIC_TC);
+        return;
+    }
+    const double invkerntot = 1.0 / (double)KERNTOT;
+    const int kern1 = 5000;                              // int(HALFKERNTOT)
     const int kerntc = (int)((double)KERNTOT / 3.0);   // kernel.F90:37
     const double nd = 3.0;
     double *w0 = tab + T_W0 * KT, *w1 = tab + T_W1 * KT, *w2 = tab + T_W2 * KT,
            *w3 = tab + T_W3 * KT, *w4 = tab + T_W4 * KT, *w5 = tab + T_W5 * KT,
            *w6 = tab + T_W6 * KT, *wh = tab + T_WH * KT, *wg = tab + T_WG * KT;
     for (int i = 0; i <= KERNTOT; i++) {
-        const double s = (KERNRANGE * (double)i + 1.0) * invkerntot;
-        const double saux = KERNRANGE - s;
+        const double s = (2.0 * (double)i + 1.0) * invkerntot;
+        const double saux = 2.0 - s;
         if (i < kern1) {
             if (kernel == SERENX_KERNEL_M4_TC && i < kerntc)
                 w1[i] = -1.0;   // Thomas-Couchman gradient (kernel.F90:56-62)
@@ -185,8 +262,8 @@ int check_options(const serenx_options &o) {
         return set_error(SERENX_EINVAL, "mac must be GEOMETRIC, GADGET, GADGET2 or EIGEN");
     if (o.precision != SERENX_FP64 && o.precision != SERENX_MIXED)
         return set_error(SERENX_EINVAL, "precision must be SERENX_FP64 or SERENX_MIXED");
-    if (o.kernel != SERENX_KERNEL_M4 && o.kernel != SERENX_KERNEL_M4_TC)
-        return set_error(SERENX_EINVAL, "kernel must be SERENX_KERNEL_M4 or SERENX_KERNEL_M4_TC");
+    if (o.kernel < SERENX_KERNEL_M4 || o.kernel > SERENX_KERNEL_QUINTIC_TC)
+        return set_error(SERENX_EINVAL, "kernel must be one of SERENX_KERNEL_*");
     if (!(o.h_fac > 0.0)) return set_error(SERENX_EINVAL, "h_fac must be positive");
     if (o.mac != SERENX_MAC_GEOMETRIC && !(o.abserror > 0.0))
         return set_error(SERENX_EINVAL, "abserror must be positive for multipole MACs");
@@ -358,7 +435,8 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     Ctx *c = new Ctx();
     c->opt = *opt;
     c->device = device;
-    c->pp_gather = (int)((4.0 * PI_REF / 3.0) * ipow_seq(KERNRANGE * opt->h_fac, 3));
+    c->kg = kernel_geom(opt->kernel);
+    c->pp_gather = (int)((4.0 * PI_REF / 3.0) * ipow_seq(c->kg.range * opt->h_fac, 3));
     c->pp_limit = 2 * c->pp_gather;
     c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

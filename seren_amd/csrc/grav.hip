@@ -38,6 +38,19 @@ __device__ __forceinline__ double mac_afactor(double agravmag, double gpot) {
     return pow(1.0 / gpot, 2.0 * ONETHIRD);
 }
 
+// the kernel-range constants in FP32
+struct KGeomF {
+    float half, range, invrange;
+};
+
+// The gravity walks take the kernel range as a template argument (QUINTIC:
+// the quintic's range 3, else M4's 2), so that the constants are immediates:
+// as kernel arguments they cost the walks 2-6 VGPRs and a wave of occupancy.
+template <bool QUINTIC>
+__host__ __device__ constexpr KGeom walk_geom() {
+    return kernel_geom(QUINTIC ? SERENX_KERNEL_QUINTIC : SERENX_KERNEL_M4);
+}
+
 // 1/d for d > 0 to a few ulp: the hardware estimate refined by two Newton
 // steps y <- y + y (1 - d y) (each squares the relative error).
 __device__ __forceinline__ double rcp_fp64(double d) {
@@ -49,12 +62,15 @@ __device__ __forceinline__ double rcp_fp64(double d) {
 }
 
 // gravity_gradh.F90:42-86 for one pair, (dx, dy, dz) = r_j - r_p; t4 holds
-// (w4, w5, w6, 0) per table index; invhj = 1/h_j.  The kernel index uses the
-// reference's drmag = sqrt(r^2) + 1e-20; 1/drmag comes from rcp_fp64.
-// Returns bit 0: r inside 2 h_p, bit 1: r inside 2 h_j.
-__device__ __forceinline__ unsigned pair_fp64(const double4 *__restrict__ t4, double invhp,
+// (w4, w5, w6, 0) per table index; rangehj = KERNRANGE h_j (the reference's
+// product), invhj = 1/h_j.  The kernel index uses the reference's drmag =
+// sqrt(r^2) + 1e-20; 1/drmag comes from rcp_fp64.
+// Returns bit 0: r inside the kernel range of h_p, bit 1: of h_j.
+__device__ __forceinline__ unsigned pair_fp64(const KGeom &kg, const double4 *__restrict__ t4,
+                                               double invhp,
                                                double zo_p, double dx, double dy, double dz,
-                                               double mj, double hj, double invhj, double zetaj,
+                                               double mj, double rangehj, double invhj,
+                                               double zetaj,
                                                double &ax, double &ay, double &az,
                                                double &pot) {
 #pragma clang fp contract(fast)
@@ -64,9 +80,9 @@ __device__ __forceinline__ unsigned pair_fp64(const double4 *__restrict__ t4, do
     const double id2 = invdrmag * invdrmag;
     double grav, dpot;
     unsigned inside = 0;
-    if (invdrmag > INVKERNRANGE * invhp) {
+    if (invdrmag > kg.invrange * invhp) {
         inside = 1;
-        const int kern = min((int)(HALFKERNTOT * drmag * invhp), KERNTOT);
+        const int kern = min((int)(kg.half * drmag * invhp), KERNTOT);
         const double4 w = t4[kern];
         grav = invhp * invhp * (w.y + zo_p * w.x * invhp * invhp);
         dpot = invhp * w.z;
@@ -74,9 +90,9 @@ __device__ __forceinline__ unsigned pair_fp64(const double4 *__restrict__ t4, do
         grav = id2;
         dpot = invdrmag;
     }
-    if (drmag < KERNRANGE * hj) {
+    if (drmag < rangehj) {
         inside |= 2;
-        const int kern = min((int)(HALFKERNTOT * drmag * invhj), KERNTOT);
+        const int kern = min((int)(kg.half * drmag * invhj), KERNTOT);
         const double4 w = t4[kern];
         grav = grav + invhj * invhj * (w.y + zetaj * w.x * invhj * invhj);
         dpot = dpot + invhj * w.z;
@@ -93,8 +109,9 @@ __device__ __forceinline__ unsigned pair_fp64(const double4 *__restrict__ t4, do
 }
 
 // Gravity source record of particle j, 64 bytes in the second half of the
-// 128-byte particle record: [2] x y z m  [3] h 1/h zeta 0
-__global__ void k_pack_grav(int n, const double *__restrict__ x, const double *__restrict__ y,
+// 128-byte particle record: [2] x y z m  [3] h 1/h zeta KERNRANGE*h
+__global__ void k_pack_grav(int n, double kernrange, const double *__restrict__ x,
+                            const double *__restrict__ y,
                             const double *__restrict__ z, const double *__restrict__ m,
                             const double *__restrict__ h, const double *__restrict__ zeta,
                             double4 *__restrict__ rec) {
@@ -102,7 +119,7 @@ __global__ void k_pack_grav(int n, const double *__restrict__ x, const double *_
     if (i >= n) return;
     double4 *r = rec + 4 * (size_t)i;
     r[2] = make_double4(x[i], y[i], z[i], m[i]);
-    r[3] = make_double4(h[i], 1.0 / h[i], zeta[i], 0.0);
+    r[3] = make_double4(h[i], 1.0 / h[i], zeta[i], kernrange * h[i]);
 }
 
 // 1/sqrt(a) for a > 0 to a few ulp: the hardware estimate refined by two
@@ -175,7 +192,7 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
     cell_terms<OCT, double>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
 }
 
-template <int OCT, int MACT>
+template <int OCT, int MACT, bool QUINTIC>
 __global__ void __launch_bounds__(256) k_grav_fp64(
     int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
@@ -183,6 +200,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
+    constexpr KGeom kg = walk_geom<QUINTIC>();
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
@@ -221,7 +239,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
                 const bool act = mine && j != p;
                 unsigned in_ = 0;
                 if (act)
-                    in_ = pair_fp64(t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w, b.x,
+                    in_ = pair_fp64(kg, t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w, b.w,
                                     b.y, b.z, gx, gy, gz, gp);
                 npp += lanes(act);
                 nin_p += lanes(in_ & 1);
@@ -257,9 +275,10 @@ struct CSum {
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
 };
 
-__device__ __forceinline__ unsigned pair_fp32(const float4 *__restrict__ t4, float invhp,
+__device__ __forceinline__ unsigned pair_fp32(const KGeomF &kg, const float4 *__restrict__ t4,
+                                               float invhp,
                                                float zo_p, float dx, float dy, float dz, float mj,
-                                               float hj, float invhj, float zj, CSum &ax,
+                                               float rangehj, float invhj, float zj, CSum &ax,
                                                CSum &ay, CSum &az, CSum &pot) {
 #pragma clang fp contract(fast)
     const float drsqd = dx * dx + dy * dy + dz * dz;
@@ -267,9 +286,9 @@ __device__ __forceinline__ unsigned pair_fp32(const float4 *__restrict__ t4, flo
     const float invdrmag = __builtin_amdgcn_rcpf(drmag);
     float grav, dpot;
     unsigned inside = 0;
-    if (invdrmag > 0.5f * invhp) {
+    if (invdrmag > kg.invrange * invhp) {
         inside = 1;
-        const int kern = min((int)(5000.0f * drmag * invhp), KERNTOT);
+        const int kern = min((int)(kg.half * drmag * invhp), KERNTOT);
         const float4 w = t4[kern];
         grav = invhp * invhp * (w.y + zo_p * w.x * invhp * invhp);
         dpot = invhp * w.z;
@@ -277,9 +296,9 @@ __device__ __forceinline__ unsigned pair_fp32(const float4 *__restrict__ t4, flo
         grav = invdrmag * invdrmag;
         dpot = invdrmag;
     }
-    if (drmag < 2.0f * hj) {
+    if (drmag < rangehj) {
         inside |= 2;
-        const int kern = min((int)(5000.0f * drmag * invhj), KERNTOT);
+        const int kern = min((int)(kg.half * drmag * invhj), KERNTOT);
         const float4 w = t4[kern];
         grav = grav + invhj * invhj * (w.y + zj * w.x * invhj * invhj);
         dpot = dpot + invhj * w.z;
@@ -319,7 +338,7 @@ constexpr int CELL_BATCH = 8;
 // walk's; the walk is wave-uniform (no per-lane resume state), and no cell
 // term is issued for a cell that only some lanes use.  The FP64 walk keeps
 // the reference's per-particle interaction sets.
-template <int OCT, int MACT>
+template <int OCT, int MACT, bool QUINTIC>
 __global__ void __launch_bounds__(256) k_grav_mixed(
     int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
@@ -327,6 +346,8 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
+    constexpr KGeom kg64 = walk_geom<QUINTIC>();
+    constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
@@ -380,8 +401,8 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 const bool act = mine && j != p;
                 unsigned in_ = 0;
                 if (act)
-                    in_ = pair_fp32(t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
-                                    (float)(a.z - rpz), (float)a.w, (float)b.x, (float)b.y,
+                    in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
+                                    (float)(a.z - rpz), (float)a.w, (float)b.w, (float)b.y,
                                     (float)b.z, px, py, pz, pq);
                 npp += lanes(act);
                 nin_p += lanes(in_ & 1);
@@ -413,7 +434,8 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
 
 // direct_sph_gravity.F90:63-79: sum of gravity_gradh over all other
 // particles, FP64, sources staged through LDS
-__global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__restrict__ targets,
+__global__ void __launch_bounds__(256) k_direct(KGeom kg, int n, int nt,
+                                                const int *__restrict__ targets,
                                                 const double *__restrict__ x,
                                                 const double *__restrict__ y,
                                                 const double *__restrict__ z,
@@ -439,7 +461,7 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
             sy[threadIdx.x] = y[j];
             sz[threadIdx.x] = z[j];
             sm[threadIdx.x] = m[j];
-            sh[threadIdx.x] = h[j];
+            sh[threadIdx.x] = kg.range * h[j];
             sih[threadIdx.x] = 1.0 / h[j];
             szt[threadIdx.x] = zeta[j];
         }
@@ -448,7 +470,7 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
         if (valid) {
             for (int k = 0; k < lim; k++) {
                 if (base + k == p || base + k < lo || base + k >= hi) continue;
-                (void)pair_fp64(t4, invhp, zo_p, sx[k] - rpx, sy[k] - rpy, sz[k] - rpz, sm[k],
+                (void)pair_fp64(kg, t4, invhp, zo_p, sx[k] - rpx, sy[k] - rpy, sz[k] - rpz, sm[k],
                                 sh[k], sih[k], szt[k], gx, gy, gz, gp);
             }
         }
@@ -462,21 +484,29 @@ __global__ void __launch_bounds__(256) k_direct(int n, int nt, const int *__rest
     }
 }
 
-template <int OCT, int MACT>
-void launch_grav(Ctx &c) {
-    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
-    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
-                       c.m, c.h, c.zeta, rec);
+template <int OCT, int MACT, bool QUINTIC>
+void launch_walk(Ctx &c, const double4 *rec) {
     if (c.opt.precision == SERENX_MIXED)
-        hipLaunchKernelGGL((k_grav_mixed<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
+        hipLaunchKernelGGL((k_grav_mixed<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
+                           c.stream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.ax, c.ay,
                            c.az, c.gpot, c.agm, c.icount, c.wstat);
     else
-        hipLaunchKernelGGL((k_grav_fp64<OCT, MACT>), dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
-                           c.nc, c.gn, rec, c.cid, c.root_range, c.active,
+        hipLaunchKernelGGL((k_grav_fp64<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
+                           c.stream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.ax, c.ay,
                            c.az, c.gpot, c.agm, c.icount, c.wstat);
+}
+
+template <int OCT, int MACT>
+void launch_grav(Ctx &c) {
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.kg.range, c.x,
+                       c.y, c.z, c.m, c.h, c.zeta, rec);
+    if (c.kg.range == walk_geom<true>().range)
+        launch_walk<OCT, MACT, true>(c, rec);
+    else
+        launch_walk<OCT, MACT, false>(c, rec);
 }
 
 }  // namespace
@@ -503,7 +533,7 @@ int grav_forces(Ctx &c) {
 }
 
 int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot) {
-    hipLaunchKernelGGL(k_direct, dim3(nblk(nt)), dim3(256), 0, c.stream, c.n, nt, dtargets, c.x,
+    hipLaunchKernelGGL(k_direct, dim3(nblk(nt)), dim3(256), 0, c.stream, c.kg, c.n, nt, dtargets, c.x,
                        c.y, c.z, c.m, c.h, c.zeta, c.cid, c.cloud_off,
                        reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), dacc, dpot);
     SX_HIP(hipGetLastError());

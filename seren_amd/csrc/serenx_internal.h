@@ -18,10 +18,6 @@ namespace serenx {
 constexpr int LEAFMAX = 8;
 constexpr int LMAX = 30;
 constexpr int KERNTOT = 10000;
-constexpr double HALFKERNTOT = 5000.0;
-constexpr double KERNRANGE = 2.0;
-constexpr double KERNRANGESQD = 4.0;
-constexpr double INVKERNRANGE = 0.5;
 constexpr double SMALL_NUMBER = 1.0e-20;
 constexpr double BIG_NUMBER = 9.9e20;
 constexpr double HMULT = 1.1;
@@ -32,6 +28,20 @@ constexpr double ONETHIRD = 0.333333333;
 constexpr double TWOTHIRDS_DP = 0.666666666666666666;
 constexpr double PI_REF = 3.1415926536;
 constexpr double INVPI = 0.318309886;
+
+// Range constants of the kernel (macros.h:101-118): the table index per unit
+// r/h (HALFKERNTOT), the range in units of h (KERNRANGE), its square and the
+// reference's rounded inverse (INVKERNRANGE).  Passed to every kernel that
+// tests a pair against the kernel range.
+struct KGeom {
+    double half, range, rangesqd, invrange;
+};
+__host__ __device__ constexpr KGeom kernel_geom(int kernel) {
+    if (kernel == SERENX_KERNEL_QUINTIC || kernel == SERENX_KERNEL_QUINTIC_TC)
+        return KGeom{3333.0, 3.0, 9.0, 0.3333333333333};
+    return KGeom{5000.0, 2.0, 4.0, 0.5};
+}
+
 constexpr int NTAB = 9;            // w0 w1 w2 w3 w4 w5 w6 wh wg
 constexpr int KT = KERNTOT + 1;    // entries per table
 enum { T_W0 = 0, T_W1, T_W2, T_W3, T_W4, T_W5, T_W6, T_WH, T_WG };
@@ -150,6 +160,7 @@ struct Ctx {
     int n = 0;          // particles loaded
     int cap = 0;        // particle capacity of the device arrays
     int pp_gather = 0, pp_limit = 0;
+    KGeom kg{};         // range constants of opt.kernel
     double daux = 0.0;  // 10 huge^-0.2 (BHgrav_stock.F90:347)
 
     // particle SoA, tree order

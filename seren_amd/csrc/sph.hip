@@ -106,7 +106,7 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
 // list length in nlist (-1: list incomplete) for k_hsolve_final; its list
 // stays in hlist (a lane joins no walk after it has converged).
 __global__ void __launch_bounds__(256) k_hsolve_iter(
-    int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const double *__restrict__ roots, int *__restrict__ hlist,
     const uint8_t *__restrict__ active, const double *__restrict__ ktab,
@@ -144,10 +144,10 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
             h_high = hp * HMULT;
             newlist = true;
         }
-        const double R = KERNRANGE * h_high;
+        const double R = kg.range * h_high;
         const double invhp = 1.0 / hp;
         const double hfactor_p = cr_pow3(invhp);
-        const double hrangesqd = KERNRANGESQD * hp * hp;
+        const double hrangesqd = kg.rangesqd * hp * hp;
         double srho = mp * w0[0] * hfactor_p;                  // :209-210
         double som = mp * hfactor_p * invhp * wh[0];
         auto visit = [&](const HNode &nd, bool) { return hgather_visit(nd, rpx, rpy, rpz, R); };
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
             const double dsq = ex * ex + ey * ey + ez * ez;
             if (dsq >= hrangesqd) return;                       // :223
             const double drmag = sqrt(dsq) + SMALL_NUMBER;
-            const int kern = (int)(HALFKERNTOT * drmag * invhp);
+            const int kern = (int)(kg.half * drmag * invhp);
             if (kern >= KERNTOT) return;                        // :227
             const double2 w = *reinterpret_cast<const double2 *>(h4 + 4 * (size_t)kern);
             srho = srho + mj * hfactor_p * w.x;
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
 // all_sph_gradh.F90:310-489: with h from the converged density, the gather
 // sums rho, Omega, div v and zeta over the last potential-neighbour list.
 __global__ void __launch_bounds__(256) k_hsolve_final(
-    int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const int *__restrict__ hlist, const uint8_t *__restrict__ active,
     const double *__restrict__ ktab, double h_fac, const double *__restrict__ hhigh,
@@ -292,12 +292,12 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
         rpx = a.x; rpy = a.y; rpz = a.z; mp = a.w;
         vpx = b.x; vpy = b.y; vpz = b.z;
         hp = h_fac * ref_pow_third(mp / rho[p]);                 // :317
-        R = KERNRANGE * hhigh[p];
+        R = kg.range * hhigh[p];
         nslot = nlist[p];
     }
     const double invhp = 1.0 / hp;
     const double hfactor_p = cr_pow3(invhp);
-    const double hrangesqd = KERNRANGESQD * hp * hp;
+    const double hrangesqd = kg.rangesqd * hp * hp;
     // self contributions (:323-346)
     double srho = mp * w0[0] * hfactor_p;
     double som = mp * hfactor_p * invhp * wh[0];
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
         const double dsq = ex * ex + ey * ey + ez * ez;
         if (dsq > hrangesqd) return;                            // :364
         const double drmag = sqrt(dsq) + SMALL_NUMBER;
-        const int kern = (int)(HALFKERNTOT * drmag * invhp);
+        const int kern = (int)(kg.half * drmag * invhp);
         if (kern > KERNTOT) return;                             // :368
         const double4 vj = vrec[j];
         const double dvx = vj.x - vpx, dvy = vj.y - vpy, dvz = vj.z - vpz;
@@ -360,7 +360,8 @@ constexpr int OVF_PER_WAVE = 1;
 // here (the reference does not store their list either, :115-118) and are
 // listed completely by the OVERFLOW_PASS instance.
 template <bool OVERFLOW_PASS>
-__global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf_idx, int nc,
+__global__ void __launch_bounds__(256) k_neib(KGeom kg, int n, const int *__restrict__ ovf_idx,
+                                              int nc,
                                               const HNode *__restrict__ hn,
                                               const double4 *__restrict__ rec,
                                               const int *__restrict__ cid,
@@ -385,8 +386,8 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
         rpx = a.x; rpy = a.y; rpz = a.z;
         hp = a.w;
     }
-    const double hrangesqd_p = KERNRANGESQD * hp * hp;
-    const double R = KERNRANGE * hp;
+    const double hrangesqd_p = kg.rangesqd * hp * hp;
+    const double R = kg.range * hp;
     int tot = 0;
     int *row = nullptr;
     int lim = pp_limit;   // entries this lane may store
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(256) k_neib(int n, const int *__restrict__ ovf
                 const double4 aj = rec[j];   // one scalar load per candidate
                 const double xj = aj.x, yj = aj.y, zj = aj.z, hj = aj.w;
                 if (!mine || j == p) continue;
-                const double hrangesqd_pp = KERNRANGESQD * hj * hj;
+                const double hrangesqd_pp = kg.rangesqd * hj * hj;
                 const double ex = xj - rpx, ey = yj - rpy, ez = zj - rpz;
                 const double dsq = ex * ex + ey * ey + ez * ez;
                 if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
@@ -486,7 +487,7 @@ __global__ void k_pack_hydro(int n, const double *__restrict__ x, const double *
 // (:258-259) - P div v / (Omega rho) (:326-332)
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) k_hydro(
-    int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
+    KGeom kg, int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
     const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
     const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab, double alpha,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
@@ -513,7 +514,7 @@ __global__ void __launch_bounds__(256) k_hydro(
         const double hpp = q0.w, mpp = q1.w, rho_pp = q2.x;
         double ux = q0.x - rpx, uy = q0.y - rpy, uz = q0.z - rpz;
         double drmag = ux * ux + uy * uy + uz * uz;
-        if (drmag >= KERNRANGESQD * hp * hp && drmag >= KERNRANGESQD * hpp * hpp) continue;
+        if (drmag >= kg.rangesqd * hp * hp && drmag >= kg.rangesqd * hpp * hpp) continue;
         drmag = sqrt(drmag) + SMALL_NUMBER;
         const double dvx = q1.x - vpx, dvy = q1.y - vpy, dvz = q1.z - vpz;
         const double dvdr = dvx * ux + dvy * uy + dvz * uz;
@@ -521,8 +522,8 @@ __global__ void __launch_bounds__(256) k_hydro(
         ux = ux * invdrmag;
         uy = uy * invdrmag;
         uz = uz * invdrmag;
-        if (drmag < KERNRANGE * hp) {
-            int kern = (int)(HALFKERNTOT * drmag * invhp);
+        if (drmag < kg.range * hp) {
+            int kern = (int)(kg.half * drmag * invhp);
             kern = min(kern, KERNTOT);
             const double wk = w1[kern];
             wmean = 0.5 * hfactor_p * wk;
@@ -531,9 +532,9 @@ __global__ void __launch_bounds__(256) k_hydro(
             ahy = ahy + f * uy;
             ahz = ahz + f * uz;
         }
-        if (drmag < KERNRANGE * hpp) {
+        if (drmag < kg.range * hpp) {
             const double invhpp = q2.w, hfactor_pp = rq[3].x, pfactor_pp = q2.y;
-            int kern = (int)(HALFKERNTOT * drmag * invhpp);
+            int kern = (int)(kg.half * drmag * invhpp);
             kern = min(kern, KERNTOT);
             const double wk = w1[kern];
             wmean = wmean + 0.5 * hfactor_pp * wk;
@@ -600,11 +601,11 @@ int sph_hsolve(Ctx &c) {
                        c.vx, c.vy, c.vz, c.m, rec);
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
-    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
+    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.root, c.hlist, c.active, c.ktab, c.opt.h_fac,
                        c.pp_gather, c.h, c.rho, c.tmpd, c.tmpi, c.wstat);
     SX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.nc, c.hn,
+    hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.hlist, c.active, c.ktab, c.opt.h_fac, c.tmpd, c.tmpi, c.h, c.zeta,
                        c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
@@ -618,7 +619,7 @@ int sph_neighbours(Ctx &c) {
     // [0] x y z h with the solved h
     hipLaunchKernelGGL(k_pack_xyzh, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.x, c.y, c.z, c.h, rec);
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.ovf_idx, c.nc, c.hn,
+    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.kg, c.n, c.ovf_idx, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
                        c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
@@ -653,7 +654,7 @@ int sph_neighbours(Ctx &c) {
         }
         const int nov = c.hostpin[9];   // overflow particles, counted by k_neib<false>
         const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
-        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, nov, c.ovf_idx, c.nc,
+        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, nov, c.ovf_idx, c.nc,
                            c.hn, rec, c.cid, c.root_range, c.active, c.pp_limit,
                            c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());
@@ -684,11 +685,11 @@ int hydro_forces(Ctx &c) {
                        c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.omega, c.press, c.sound, rec);
     SX_HIP(hipGetLastError());
     if (c.opt.thermal == SERENX_THERMAL_ENERGY_EQN)
-        hipLaunchKernelGGL(k_hydro<true>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec,
+        hipLaunchKernelGGL(k_hydro<true>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, rec,
                            c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
                            c.opt.alpha, c.ax, c.ay, c.az, c.press, c.divv, c.omega, c.dudt);
     else
-        hipLaunchKernelGGL(k_hydro<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, rec,
+        hipLaunchKernelGGL(k_hydro<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, rec,
                            c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
                            c.opt.alpha, c.ax, c.ay, c.az, c.press, c.divv, c.omega, c.dudt);
     SX_HIP(hipGetLastError());

@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_stock(
     HNode *__restrict__ hn, double *__restrict__ bbmin, double *__restrict__ bbmax,
     double *__restrict__ hmaxs, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ h,
-    int octupole, int mac, double thetamaxsqd, double abserror, double daux) {
+    int octupole, int mac, double thetamaxsqd, double abserror, double daux, double kernrange) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt) return;
     const int c = bfs2dfs[base + j];
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(256) k_stock(
     // particle position for a one-particle leaf
     for (int k = 0; k < 3; k++) dr[k] = fmax(bmx[k] - rc[k], rc[k] - bmn[k]);
     const double diagonalsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-    const double rsize = sqrt(diagonalsqd) + KERNRANGE * hmax;
+    const double rsize = sqrt(diagonalsqd) + kernrange * hmax;
     double dminsqd, macv = 0.0;
     if (mac == SERENX_MAC_GEOMETRIC) {
         dminsqd = (thetamaxsqd > 0.0) ? diagonalsqd / thetamaxsqd : BIG_NUMBER;
@@ -577,7 +577,7 @@ __global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn, GNodeF *_
 }
 
 // ---- BHhydro_hguess.F90:108-155 -----------------------------------------
-__global__ void k_hguess(int nc, int thresh, const int *__restrict__ c_leaf,
+__global__ void k_hguess(int nc, int thresh, double invkernrange, const int *__restrict__ c_leaf,
                          const int *__restrict__ c_start, const int *__restrict__ c_end,
                          const int *__restrict__ c_parent, const int *__restrict__ c_level,
                          const int *__restrict__ bfs2dfs, const HNode *__restrict__ hn,
@@ -592,23 +592,23 @@ __global__ void k_hguess(int nc, int thresh, const int *__restrict__ c_leaf,
         a = c_parent[a];
         l--;
     }
-    const double hg = INVKERNRANGE * hn[bfs2dfs[a]].rmax;
+    const double hg = invkernrange * hn[bfs2dfs[a]].rmax;
     const int s = c_start[c];
     for (int i = 0; i < k; i++) h[s + i] = hg;
 }
 
 // ---- BHhydro_update_hmax.F90:29-88 --------------------------------------
 __global__ void k_hmax(int nc, int base, int cnt, const int *__restrict__ bfs2dfs, HNode *__restrict__ hn,
-                       const double *__restrict__ h) {
+                       const double *__restrict__ h, double kernrange) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt) return;
     const int c = bfs2dfs[base + j];
     HNode &hh = hn[c];
     double hr = 0.0;
     if (hh.leaf == 1) {
-        hr = KERNRANGE * h[hh.first];
+        hr = kernrange * h[hh.first];
     } else if (hh.leaf > 0) {
-        for (int i = 0; i < hh.leaf; i++) hr = fmax(hr, KERNRANGE * h[hh.first + i]);
+        for (int i = 0; i < hh.leaf; i++) hr = fmax(hr, kernrange * h[hh.first + i]);
     } else {
         for (int cc = c + 1; cc != hh.next && cc < nc; cc = hn[cc].next) hr = fmax(hr, hn[cc].hrangemax);
     }
@@ -863,7 +863,8 @@ int stock_tree(Ctx &c) {
         if (cnt == 0) continue;
         hipLaunchKernelGGL(k_stock, dim3(nblk(cnt)), dim3(TB), 0, s, c.nc, c.level_off[L], cnt,
                            c.bfs2dfs, c.gn, c.hn, c.bbmin, c.bbmax, c.hmaxs, c.x, c.y, c.z, c.m,
-                           c.h, oct, c.opt.mac, c.opt.thetamaxsqd, c.opt.abserror, c.daux);
+                           c.h, oct, c.opt.mac, c.opt.thetamaxsqd, c.opt.abserror, c.daux,
+                           c.kg.range);
     }
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gnf);
@@ -881,7 +882,7 @@ int refresh_f32_nodes(Ctx &c) {
 
 int hguess(Ctx &c) {
     const int thresh = (int)(0.5 * (double)c.pp_gather);
-    hipLaunchKernelGGL(k_hguess, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, thresh, c.c_leaf,
+    hipLaunchKernelGGL(k_hguess, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, thresh, c.kg.invrange, c.c_leaf,
                        c.c_start, c.c_end, c.c_parent, c.c_level, c.bfs2dfs, c.hn, c.h);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
@@ -892,7 +893,7 @@ int update_hmax(Ctx &c) {
         const int cnt = c.level_cnt[L];
         if (cnt == 0) continue;
         hipLaunchKernelGGL(k_hmax, dim3(nblk(cnt)), dim3(TB), 0, c.stream, c.nc, c.level_off[L], cnt,
-                           c.bfs2dfs, c.hn, c.h);
+                           c.bfs2dfs, c.hn, c.h, c.kg.range);
     }
     SX_HIP(hipGetLastError());
     return SERENX_OK;

@@ -38,8 +38,9 @@ _MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE, "octupole_exact": 
 _MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2,
         "eigen": MAC_EIGEN}
 _PREC = {"fp64": FP64, "mixed": MIXED}
-KERNEL_M4, KERNEL_M4_TC = 0, 1
-_KERNEL = {"m4": KERNEL_M4, "m4tc": KERNEL_M4_TC}
+KERNEL_M4, KERNEL_M4_TC, KERNEL_QUINTIC, KERNEL_QUINTIC_TC = 0, 1, 2, 3
+_KERNEL = {"m4": KERNEL_M4, "m4tc": KERNEL_M4_TC, "quintic": KERNEL_QUINTIC,
+           "quintictc": KERNEL_QUINTIC_TC}
 
 EXPORTED = [
     "serenx_default_options", "serenx_create", "serenx_destroy", "serenx_last_error",

@@ -30,7 +30,11 @@ CONTAINS
     type(serenx_host_state) :: st
     if (.not. c_associated(serenx_ctx)) then
        call serenx_f_check(serenx_default_options(opt), "default_options")
-#if defined(TC_KERNEL)
+#if defined(QUINTIC_KERNEL) && defined(TC_KERNEL)
+       opt%kernel = SERENX_KERNEL_QUINTIC_TC
+#elif defined(QUINTIC_KERNEL)
+       opt%kernel = SERENX_KERNEL_QUINTIC
+#elif defined(TC_KERNEL)
        opt%kernel = SERENX_KERNEL_M4_TC
 #endif
 #if defined(OCTUPOLE)

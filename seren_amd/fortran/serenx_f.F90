@@ -10,7 +10,8 @@ MODULE serenx_f
   integer(c_int32_t), parameter :: SERENX_QUADRUPOLE = 2, SERENX_OCTUPOLE = 3, SERENX_OCTUPOLE_EXACT = 4
   integer(c_int32_t), parameter :: SERENX_MAC_GEOMETRIC = 0, SERENX_MAC_GADGET = 1, &
        & SERENX_MAC_GADGET2 = 2, SERENX_MAC_EIGEN = 3
-  integer(c_int32_t), parameter :: SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1
+  integer(c_int32_t), parameter :: SERENX_KERNEL_M4 = 0, SERENX_KERNEL_M4_TC = 1, &
+       & SERENX_KERNEL_QUINTIC = 2, SERENX_KERNEL_QUINTIC_TC = 3
   integer(c_int32_t), parameter :: SERENX_FP64 = 0, SERENX_MIXED = 1
   integer(c_int32_t), parameter :: SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1
 

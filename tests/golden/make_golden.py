@@ -36,7 +36,7 @@ def energy0(n, seed):
 
 
 # name: (n, profile, seed, mach, octupole, mac, thetamaxsqd, abserror, evaluations, full
-#        [, kernel [, thermal]])
+#        [, kernel [, thermal [, h_fac]]])
 CASES = {
     "uniform1k_quad_geo": (1000, "uniform", 101, 1.0, False, "geometric", 0.25, 1e-4, 1, True),
     "plummer2k_quad_geo": (2048, "plummer", 202, 0.0, False, "geometric", 0.25, 1e-4, 1, True),
@@ -53,6 +53,12 @@ CASES = {
     # THERMAL=ENERGY_EQN: ideal gas from u, du/dt
     "r2_2k_quad_geo_energy_x2": (2048, "r2", 107, 2.0, False, "geometric", 0.25, 1e-4, 2, True,
                                  "m4", "energy"),
+    # KERNEL=QUINTIC (range 3h): h_fac 1.0, pp_gather 113
+    "r2_2k_quad_geo_quintic_x2": (2048, "r2", 109, 2.0, False, "geometric", 0.25, 1e-4, 2, True,
+                                  "quintic", "isothermal", 1.0),
+    # KERNEL=QUINTICTC, octupole + GADGET2: h_fac 0.8, pp_gather 57
+    "plummer2k_oct_gadget2_quintictc_x2": (2048, "plummer", 110, 1.0, True, "gadget2", 0.25,
+                                           1e-4, 2, True, "quintictc", "isothermal", 0.8),
 }
 
 
@@ -64,9 +70,10 @@ def run_case(name, spec):
     n, prof, seed, mach, octu, mac, th2, ae, nev, full = spec[:10]
     kernel = spec[10] if len(spec) > 10 else "m4"
     thermal = spec[11] if len(spec) > 11 else "isothermal"
+    h_fac = spec[12] if len(spec) > 12 else 1.2
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
     ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal)
-    ref.setup(pos, mass, h, vel, thetamaxsqd=th2, abserror=ae)
+    ref.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=th2, abserror=ae)
     if thermal == "energy":
         ref.set_energy(energy0(n, seed), GAMMA)
     for e in range(nev):
@@ -104,6 +111,8 @@ def run_case(name, spec):
         ctot=np.array(par["ctot_grav"]), ltot=np.array(par["ltot_grav"]),
         rextent=np.array(par["rextent"]),
     )
+    if h_fac != 1.2:
+        out["h_fac"] = np.array(h_fac)
     if thermal == "energy":
         out["thermal"] = np.array(thermal)
         out["gamma"] = np.array(GAMMA)
@@ -129,7 +138,8 @@ def run_case(name, spec):
 # Integration steps (sph_integrate.F90 with RUNGE_KUTTA) after the setup
 # evaluation: name: (n, profile, seed, mach, octupole, mac, nlevels, steps)
 STEP_CASES = {
-    # n, profile, seed, Mach, octupole, MAC, nlevels, steps[, time-step build, dt_fixed]
+    # n, profile, seed, Mach, octupole, MAC, nlevels, steps[, time-step build, dt_fixed
+    # [, thermal [, kernel, h_fac]]]
     "step_rk_uniform2k_quad_geo_nlev3": (2000, "uniform", 11, 1.0, False, "geometric", 3, 6),
     "step_rk_r2_2k_oct_gadget2_nlev1": (2048, "r2", 12, 2.0, True, "gadget2", 1, 4),
     # Makefile TIMESTEP=RESTRICTED CHECK_NEIB_TIMESTEP=1 (the Makefile's defaults)
@@ -141,6 +151,9 @@ STEP_CASES = {
     # THERMAL=ENERGY_EQN: u advanced, the energy time-step criterion
     "step_rk_uniform2k_quad_geo_energy_nlev3": (2000, "uniform", 15, 2.0, False, "geometric", 3,
                                                 8, "", 1.0, "energy"),
+    # KERNEL=QUINTIC, h_fac 1.0
+    "step_rk_r2_2k_quad_geo_quintic_nlev3": (2048, "r2", 16, 2.0, False, "geometric", 3, 8, "",
+                                             1.0, "isothermal", "quintic", 1.0),
 }
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
@@ -150,9 +163,10 @@ def step_case_data(spec):
     n, prof, seed, mach, octu, mac, nlevels, nsteps = spec[:8]
     timestep, dt_fixed = spec[8:10] if len(spec) > 8 else ("", 1.0)
     thermal = spec[10] if len(spec) > 10 else "isothermal"
+    kernel, h_fac = spec[11:13] if len(spec) > 11 else ("m4", 1.2)
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, timestep=timestep, thermal=thermal)
-    ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+    ref = RefSeren(octu, mac, timestep=timestep, thermal=thermal, kernel=kernel)
+    ref.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=0.25, abserror=1e-4)
     if thermal == "energy":
         ref.set_energy(energy0(n, seed), GAMMA)
     ref.force_eval(CS, hguess=True)
@@ -172,6 +186,7 @@ def step_case_data(spec):
         mac=np.array(mac), mach=np.array(mach), pos0=pos, vel0=vel, mass=mass,
         timestep=np.array(timestep), dt_fixed=np.array(dt_fixed), nminneib=sp["nminneib"],
         thermal=np.array(thermal), **energy_out(ref, n, seed, thermal),
+        **({} if kernel == "m4" else dict(kernel=np.array(kernel), h_fac=np.array(h_fac))),
         step_ints=np.array(ints, np.int64), step_reals=np.array(reals),
         step_nactive=np.array(nact, np.int64),
         porig=p["porig"], pos=p["pos"], v=p["v"], a=p["a"], h=p["h"], rho=p["rho"],
@@ -192,18 +207,32 @@ def run_step_case(name, spec):
     print(name, "levels", np.bincount(d["nlevel"]), "time", d["step_reals"][-1][0])
 
 
-def main():
-    for name, spec in CASES.items():
-        run_case(name, spec)
-    for name, spec in STEP_CASES.items():
-        run_step_case(name, spec)
-    # kernel tables (kernel.F90): full, they pin the tabulated M4 functions
-    ref = RefSeren(False, "geometric")
+def kernel_tables(kernel):
+    """The reference's kernel tables (kernel.F90) for one KERNEL option."""
+    octu, mac = (True, "gadget2") if kernel == "quintictc" else (False, "geometric")
+    ref = RefSeren(octu, mac, kernel=kernel)
     pos, vel, mass, h = ics.make_cloud(100, "uniform", seed=9, mach=0.0)
     ref.setup(pos, mass, h, vel)
-    np.savez_compressed(os.path.join(HERE, "kernel_tables.npz"), tables=ref.kernel())
-    print("kernel tables written")
+    return ref.kernel()
+
+
+# kernel-table fixtures: full, they pin the tabulated functions
+TABLES = {"kernel_tables": "m4", "kernel_tables_quintic": "quintic"}
+
+
+def main(names=None):
+    """Writes every fixture, or only the named ones."""
+    for name, spec in CASES.items():
+        if not names or name in names:
+            run_case(name, spec)
+    for name, spec in STEP_CASES.items():
+        if not names or name in names:
+            run_step_case(name, spec)
+    for name, kernel in TABLES.items():
+        if not names or name in names:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), tables=kernel_tables(kernel))
+            print(name, "written")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -45,11 +45,11 @@ def _ids(c):
     return f"n{c[0]}-{c[1]}-{_MULTI[c[3]]}-{c[4]}-x{c[7]}"
 
 
-def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal"):
+def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal", h_fac=1.2):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole=_MULTI[octupole], mac=mac,
                        precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
-                       kernel=kernel, thermal=thermal)
+                       kernel=kernel, thermal=thermal, h_fac=h_fac)
 
 
 def _report(name, a, b):
@@ -171,7 +171,7 @@ def test_host_state_dropin_matches_oracle():
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 GOLDEN_CASES = sorted(f[:-4] for f in os.listdir(GOLD)
-                      if f.endswith(".npz") and f != "kernel_tables.npz"
+                      if f.endswith(".npz") and not f.startswith("kernel_tables")
                       and not f.startswith("step_"))
 
 
@@ -185,8 +185,9 @@ def test_gpu_matches_reference_golden_vectors(case):
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     energy = "thermal" in g.files and str(g["thermal"]) == "energy"
     print(f"golden {case}", flush=True)
+    h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
     eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel,
-                  "energy" if energy else "isothermal")
+                  "energy" if energy else "isothermal", h_fac)
     eng.load(g["pos"], g["vel"], g["mass"], np.zeros(n))
     if energy:
         assert float(g["gamma"]) == eng.opt.gamma

@@ -42,18 +42,18 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STEP_CASES = ["step_rk_uniform2k_quad_geo_nlev3", "step_rk_r2_2k_oct_gadget2_nlev1",
               "step_rk_r2_2k_quad_geo_nlev6_restricted_1", "step_rk_r2_2k_quad_geo_nlev6_fixed_2",
-              "step_rk_uniform2k_quad_geo_energy_nlev3"]
+              "step_rk_uniform2k_quad_geo_energy_nlev3", "step_rk_r2_2k_quad_geo_quintic_nlev3"]
 # reference build -> (levels, neib_check)
 TIMESTEP_BUILDS = {"": ("adaptive", 0), "restricted_1": ("restricted", 1),
                    "fixed_2": ("fixed", 2)}
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
 
-def _engine(octupole, mac, thermal="isothermal"):
+def _engine(octupole, mac, thermal="isothermal", kernel="m4", h_fac=1.2):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole="octupole" if octupole else "quadrupole", mac=mac,
                        precision="fp64", thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS,
-                       thermal=thermal)
+                       thermal=thermal, kernel=kernel, h_fac=h_fac)
 
 
 @pytest.mark.parametrize("name", STEP_CASES)
@@ -63,7 +63,9 @@ def test_rk_steps_match_reference_golden(name):
     n, seed, octu, nlevels, nsteps = (int(v) for v in g["spec"])
     levels, check = TIMESTEP_BUILDS[str(g["timestep"])]
     energy = str(g["thermal"]) == "energy"
-    eng = _engine(bool(octu), str(g["mac"]), "energy" if energy else "isothermal")
+    kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
+    h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
+    eng = _engine(bool(octu), str(g["mac"]), "energy" if energy else "isothermal", kernel, h_fac)
     try:
         h0 = np.zeros(n)
         eng.load(g["pos0"], g["vel0"], g["mass"], h0)

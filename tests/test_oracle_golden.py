@@ -17,7 +17,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz"))
-               if not p.endswith("kernel_tables.npz")
+               if not os.path.basename(p).startswith("kernel_tables")
                and not os.path.basename(p).startswith("step_"))
 CS = 0.2
 
@@ -32,7 +32,9 @@ def _run_oracle(g):
     mac = str(g["mac"])
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     o = CSeren(bool(octu), mac, kernel)
-    o.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], thetamaxsqd=float(g["thetamaxsqd"]),
+    h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
+    o.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], h_fac=h_fac,
+            thetamaxsqd=float(g["thetamaxsqd"]),
             abserror=float(g["abserror"]))
     if "thermal" in g.files and str(g["thermal"]) == "energy":
         o.set_energy(g["u0"], float(g["gamma"]))
@@ -88,11 +90,13 @@ def test_oracle_matches_reference_fixture(case):
             assert _digest(v) == str(g["tree_" + k + "_sha256"]), k
 
 
-def test_kernel_tables_match_reference():
+@pytest.mark.parametrize("kernel,fixture", [("m4", "kernel_tables"),
+                                            ("quintic", "kernel_tables_quintic")])
+def test_kernel_tables_match_reference(kernel, fixture):
     from oracle_lib import CSeren
     from seren_amd import ics
     pos, vel, mass, h = ics.make_cloud(64, "uniform", seed=3)
-    o = CSeren(False, "geometric")
+    o = CSeren(False, "geometric", kernel)
     o.setup(pos, mass, h, vel)
-    ref = np.load(os.path.join(GOLD, "kernel_tables.npz"))["tables"]
+    ref = np.load(os.path.join(GOLD, fixture + ".npz"))["tables"]
     assert np.array_equal(o.kernel(), ref)

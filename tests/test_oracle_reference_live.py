@@ -23,6 +23,8 @@ CASES = [
     (2000, "r2", 15, True, "gadget", 2),
     (2000, "plummer", 16, True, "eigen", 2),
     (1500, "uniform", 17, False, "geometric", 1, "m4tc"),
+    (2000, "r2", 18, False, "geometric", 2, "quintic", 1.0),
+    (1500, "plummer", 19, True, "gadget2", 2, "quintictc", 0.8),
 ]
 
 
@@ -32,11 +34,12 @@ def test_oracle_bit_exact_with_reference(case):
     from seren_amd import ics
     n, prof, seed, octu, mac, nev = case[:6]
     kernel = case[6] if len(case) > 6 else "m4"
+    h_fac = case[7] if len(case) > 7 else 1.2
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=1.5, ngrid=16)
     outs = []
     for cls in (RefSeren, CSeren):
         o = cls(octu, mac, kernel=kernel)
-        o.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+        o.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=0.25, abserror=1e-4)
         for e in range(nev):
             o.tree_update()
             if e == 0:
