@@ -93,6 +93,13 @@ enum {
  * (timestep_size.F90:117-122). */
 enum { SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1 };
 
+/* SPH option (Makefile SPH): GRAD_H (grad-h SPH: the rho-h iteration with
+ * Omega and zeta, all_sph_gradh.F90, hydro_gradh.F90, gravity_gradh.F90) or
+ * STANDARD with HFIND=NUMBER (h_gather.F90: h is half the distance of the
+ * pp_gather-th nearest neighbour; all_sph.F90, hydro.F90, gravity_sph.F90;
+ * Omega = 1, zeta = 0).  STANDARD is isothermal only here. */
+enum { SERENX_SPH_GRADH = 0, SERENX_SPH_STANDARD = 1 };
+
 typedef struct serenx_options {
     int32_t multipole;   /* SERENX_QUADRUPOLE or SERENX_OCTUPOLE */
     int32_t mac;         /* SERENX_MAC_* */
@@ -104,8 +111,12 @@ typedef struct serenx_options {
     double alpha;        /* AB viscosity alpha (beta := 2 alpha); params 'alpha' */
     double sound_speed;  /* isothermal c_s used by serenx_force_eval */
     int32_t thermal;     /* SERENX_THERMAL_* */
-    int32_t reserved;
+    int32_t sph;         /* SERENX_SPH_* */
     double gamma;        /* ratio of specific heats; params 'gamma' (1.4) */
+    int32_t pp_gather;   /* SERENX_SPH_STANDARD: neighbours per h (params
+                            'pp_gather'); 0 takes int((4 pi/3)(range h_fac)^3).
+                            Must be 0 for grad-h SPH, where h_fac sets it. */
+    int32_t reserved;
 } serenx_options;
 
 /* Fills SEREN's parameter defaults (default_parameters.F90:62-117,173):
@@ -118,8 +129,9 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out);
 int serenx_destroy(serenx_ctx *ctx);
 const char *serenx_last_error(void);
 
-/* pp_gather = int((4 pi/3)(2 h_fac)^3), pp_limit = 2 pp_gather
- * (initialize_seren_variables_1.F90:30-39) */
+/* pp_gather = int((4 pi/3)(range h_fac)^3) (initialize_seren_variables_1
+ * .F90:30-39; for STANDARD SPH the pp_gather option when set),
+ * pp_limit = 2 pp_gather */
 int serenx_neighbour_params(const serenx_ctx *ctx, int32_t *pp_gather, int32_t *pp_limit);
 
 /* ---------------------------------------------------------------------------
@@ -128,9 +140,10 @@ int serenx_neighbour_params(const serenx_ctx *ctx, int32_t *pp_gather, int32_t *
 
 /* Pointers to the caller's arrays, shaped as SEREN's module arrays
  * (particle_module, hydro_module, neighbour_module; modules.F90:292-542).
- * parray(6,ptot) = [x y z m h zeta/omega]; v(3,ptot); a(3,ptot);
- * pplist(pp_limit,ptot) holds 1-based ids.  agravmag may be NULL under the
- * geometric MAC.  accdo (Fortran default logical, 4 bytes) may be NULL: all
+ * parray(6,ptot) = [x y z m h zeta/omega] (with SERENX_SPH_STANDARD the zeta
+ * column is neither read nor written, and omega and rextent may be NULL);
+ * v(3,ptot); a(3,ptot); pplist(pp_limit,ptot) holds 1-based ids.
+ * agravmag may be NULL under the geometric MAC.  accdo (Fortran default logical, 4 bytes) may be NULL: all
  * particles active. */
 typedef struct serenx_host_state {
     int32_t ptot;

@@ -53,6 +53,9 @@ VARIANTS = {
     (False, "geometric", "m4", "fixed_2"): "quad_geo_tsf2",
     # THERMAL=ENERGY_EQN
     (False, "geometric", "m4", "energy"): "quad_geo_ee",
+    # SPH=STANDARD, HFIND=NUMBER
+    (False, "geometric", "m4", "standard"): "std_quad_geo",
+    (True, "gadget2", "m4", "standard"): "std_oct_gadget2",
 }
 
 
@@ -156,7 +159,7 @@ class RefSeren(_Common):
     _libs: Dict[str, C.CDLL] = {}
 
     def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4",
-                 timestep: str = "", thermal: str = "isothermal"):
+                 timestep: str = "", thermal: str = "isothermal", sph: str = "gradh"):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
         oracle/Makefile.ref dropin): the reference code as the host.
@@ -164,8 +167,11 @@ class RefSeren(_Common):
         CHECK_NEIB_TIMESTEP options ("restricted_1", "fixed_2"); the default
         builds have neither.  thermal="energy" selects the THERMAL=ENERGY_EQN
         build (set u with set_energy; thermal_iso then runs the reference's
-        update_thermal_properties)."""
-        if thermal == "energy":
+        update_thermal_properties).  sph="standard" selects the SPH=STANDARD
+        HFIND=NUMBER build (h_gather, all_sph, hydro, gravity_sph)."""
+        if sph == "standard":
+            key = (bool(octupole), mac, kernel, "standard")
+        elif thermal == "energy":
             key = (bool(octupole), mac, kernel, "energy")
         elif timestep:
             key = (bool(octupole), mac, kernel, timestep)
@@ -316,7 +322,7 @@ class CSeren(_Common):
             lib.oracle_create.restype = vp
             lib.oracle_create.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                                           C.c_double, C.c_double, C.c_double, C.c_double,
-                                          C.c_int]
+                                          C.c_int, C.c_int]
             lib.oracle_destroy.argtypes = [vp]
             lib.oracle_error.argtypes = [vp]
             for f in ("oracle_hguess", "oracle_sph_update", "oracle_hydro_forces",
@@ -339,13 +345,14 @@ class CSeren(_Common):
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, octupole, mac: str, kernel: str = "m4"):
+    def __init__(self, octupole, mac: str, kernel: str = "m4", sph: str = "gradh"):
         """octupole: False (quadrupole), True (the reference's octupole
         stocking) or "exact" (exact branch parallel-axis shift)."""
         self.lib = self._load()
         self.octupole = 2 if octupole == "exact" else int(bool(octupole))
         self.mac = {"geometric": 0, "gadget": 1, "gadget2": 2, "eigen": 3}[mac]
         self.kernel_id = KERNELS[kernel]
+        self.sph_std = {"gradh": 0, "standard": 1}[sph]
         self.ctx = None
 
     def __del__(self):
@@ -361,7 +368,7 @@ class CSeren(_Common):
         self.n = len(mass)
         self.ctx = self.lib.oracle_create(self.n, pos, mass, h, vel, int(self.octupole),
                                           self.mac, h_fac, thetamaxsqd, abserror, alpha,
-                                          self.kernel_id)
+                                          self.kernel_id, self.sph_std)
         self.pp_limit = self.params()["pp_limit"]
 
     def tree_update(self):
@@ -374,6 +381,8 @@ class CSeren(_Common):
 
     def sph_update(self):
         self.lib.oracle_sph_update(self.ctx)
+        if self.lib.oracle_error(self.ctx) == 2:
+            raise RuntimeError("Too few particles for pp_gather neighbours (h_gather)")
 
     def thermal_iso(self, cs):
         self.lib.oracle_thermal_iso(self.ctx, cs)

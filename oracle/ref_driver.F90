@@ -53,7 +53,10 @@ CONTAINS
     deallocate(agravmag)
 #endif
     deallocate(pptot, pplist)
-    deallocate(rho, temp, press, sound, div_v, rho_old, drhodt, omega)
+    deallocate(rho, temp, press, sound, div_v, rho_old, drhodt)
+#if defined(GRAD_H_SPH)
+    deallocate(omega)
+#endif
     deallocate(accdo, nlast, nlevel, laststep)
     deallocate(v_half)
 #if defined(CHECK_NEIGHBOUR_TIMESTEPS)
@@ -65,7 +68,10 @@ CONTAINS
     deallocate(first_cell_grav, last_cell_grav, BHgrav)
     deallocate(first_cell_hydro, last_cell_hydro, BHhydro)
     deallocate(cellof, BHnextptcl, whichchild, BHtemp, BHstock)
-    deallocate(w0, w1, w2, w3, w4, w5, w6, wh, wg)
+    deallocate(w0, w1, w2, w3, w4, w5, w6)
+#if defined(GRAD_H_SPH)
+    deallocate(wh, wg)
+#endif
     ref_allocated = .false.
   end subroutine ref_free_all
 
@@ -101,7 +107,10 @@ CONTAINS
 #endif
     allocate(pptot(1:pmax), pplist(1:pp_limit,1:pmax))
     allocate(rho(1:pmax), temp(1:pmax), press(1:pmax), sound(1:pmax))
-    allocate(div_v(1:pmax), rho_old(1:pmax), drhodt(1:pmax), omega(1:pmax))
+    allocate(div_v(1:pmax), rho_old(1:pmax), drhodt(1:pmax))
+#if defined(GRAD_H_SPH)
+    allocate(omega(1:pmax))
+#endif
     allocate(accdo(1:pmax), nlast(1:pmax), nlevel(1:pmax), laststep(1:pmax))
     allocate(v_half(1:VDIM,1:pmax))
 #if defined(CHECK_NEIGHBOUR_TIMESTEPS)
@@ -129,7 +138,9 @@ CONTAINS
     allocate(BHtemp(0:cmax_hydro), BHstock(0:cmax_hydro))
     allocate(w0(0:KERNTOT), w1(0:KERNTOT), w2(0:KERNTOT), w3(0:KERNTOT))
     allocate(w4(0:KERNTOT), w5(0:KERNTOT), w6(0:KERNTOT))
+#if defined(GRAD_H_SPH)
     allocate(wh(0:KERNTOT), wg(0:KERNTOT))
+#endif
     ref_allocated = .true.
 
     ! Tree node arrays start from a defined state so that unused plist slots
@@ -171,8 +182,10 @@ CONTAINS
        nlast(p) = 0
        laststep(p) = 0.0_DP
        if (rho(p) == 0.0_PR) rho(p) = 1.0_PR
+#if defined(GRAD_H_SPH)
        omega(p) = 1.0_PR
-       parray(ZETA,p) = 0.0_PR
+#endif
+       parray(ZETA,p) = 0.0_PR   ! DATATOT = ZETA with GRAVITY; unused without grad-h
     end do
     a(1:VDIM,1:ptot) = 0.0_PR
 #if !defined(GEOMETRIC_MAC)
@@ -299,7 +312,11 @@ CONTAINS
     ipar(6) = ltot_hydro
     ipar(7) = cmax_grav
     ipar(8) = cmax_hydro
+#if defined(GRAD_H_SPH)
     rpar(1) = rextent
+#else
+    rpar(1) = 0.0_DP
+#endif
     rpar(2) = thetamaxsqd
     rpar(3) = abserror
     rpar(4) = beta
@@ -326,7 +343,11 @@ CONTAINS
     a_o(1:3,1:n) = a(1:3,1:n)
     gpot_o(1:n) = gpot(1:n)
     rho_o(1:n) = rho(1:n)
+#if defined(GRAD_H_SPH)
     omega_o(1:n) = omega(1:n)
+#else
+    omega_o(1:n) = 1.0_DP
+#endif
     divv_o(1:n) = div_v(1:n)
     drhodt_o(1:n) = drhodt(1:n)
     press_o(1:n) = press(1:n)
@@ -396,7 +417,11 @@ CONTAINS
     real(c_double), intent(out) :: tab(0:KERNTOT,9)
     tab(:,1) = w0(:) ; tab(:,2) = w1(:) ; tab(:,3) = w2(:)
     tab(:,4) = w3(:) ; tab(:,5) = w4(:) ; tab(:,6) = w5(:)
+#if defined(GRAD_H_SPH)
     tab(:,7) = w6(:) ; tab(:,8) = wh(:) ; tab(:,9) = wg(:)
+#else
+    tab(:,7) = w6(:) ; tab(:,8) = 0.0_DP ; tab(:,9) = 0.0_DP
+#endif
   end subroutine ref_get_kernel
 
 ! ----------------------------------------------------------------------------

@@ -70,6 +70,7 @@ typedef struct {
 
 typedef struct oracle_ctx {
     int n, pp_gather, pp_limit, octupole, mac, kernel;
+    int sph_std; /* SPH=STANDARD with HFIND=NUMBER (else GRAD_H_SPH) */
     /* kernel range constants, macros.h:101-118 */
     double halfkerntot, kernrange, kernrangesqd, invkernrange;
     int energy;          /* THERMAL=ENERGY_EQN (INTERNAL_ENERGY) */
@@ -285,9 +286,10 @@ void oracle_destroy(oracle_ctx *c) {
 oracle_ctx *oracle_create(int n, const double *pos, const double *mass,
                           const double *h, const double *vel, int octupole,
                           int mac, double h_fac, double thetamaxsqd,
-                          double abserror, double alpha, int kernel) {
+                          double abserror, double alpha, int kernel, int sph) {
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     c->n = n;
+    c->sph_std = sph;
     c->kernel = kernel; /* 0 M4, 1 M4 TC, 2 QUINTIC, 3 QUINTIC TC */
     if (kernel >= 2) {
         c->halfkerntot = 3333.0;
@@ -1061,9 +1063,176 @@ static int get_neib(oracle_ctx *c, int p, double hp, int *potlist, int **bigbuf,
     return pp_tot;
 }
 
+/* h_gather.F90:13-203 (HGATHER, BH_TREE, no MINIMUM_H, insertion sort):
+ * brackets h between h_low and h_high = HMULT h_low until the pp_gather-th
+ * neighbour lies between them, then sorts the band's r^2. */
+static double h_gather(oracle_ctx *c, int p, double hguess) {
+    const double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
+    const double kr2 = c->kernrangesqd;
+    int pp_pot = 0, pp_max = LISTSIZE;
+    double h_old = hguess, h_high = HMULT * h_old, h_low = h_old / HMULT, h_search = 0.0;
+    double hrangesqd_high = kr2 * h_high * h_high, hrangesqd_low;
+    int *hl = (int *)malloc(sizeof(int) * pp_max), *pot = (int *)malloc(sizeof(int) * pp_max);
+    double *rs = (double *)malloc(sizeof(double) * pp_max);
+    int pp_low = 0, pp_high = 0, pp_tot;
+    if (c->pp_gather >= c->n) { /* the reference would search forever */
+        c->err = 2;
+        free(hl); free(pot); free(rs);
+        return hguess;
+    }
+    for (;;) {
+        for (;;) {
+            if (h_high > h_search) {
+                h_search = HMULT * h_high;
+                pp_pot = hydro_walk(c, rp, c->kernrange * h_search, pp_max, pot, 0);
+            }
+            if (pp_pot < 0) { /* list overflow: retry with room for every particle */
+                h_search = 0.0;
+                pp_pot = 0;
+                pp_max = c->n;
+                free(hl); free(pot); free(rs);
+                hl = (int *)malloc(sizeof(int) * pp_max);
+                pot = (int *)malloc(sizeof(int) * pp_max);
+                rs = (double *)malloc(sizeof(double) * pp_max);
+            } else if (pp_pot <= c->pp_gather) {
+                pp_pot = 0;
+                h_search = 0.0;
+                h_low = h_high;
+                h_high = HMULT * h_high;
+                hrangesqd_high = kr2 * h_high * h_high;
+            }
+            if (pp_pot > 0) break;
+        }
+        pp_high = 0;
+        pp_low = 0;
+        hrangesqd_high = kr2 * h_high * h_high;
+        hrangesqd_low = kr2 * h_low * h_low;
+        for (int i = 0; i < pp_pot; i++) {
+            int pp = pot[i];
+            if (pp == p) continue;
+            double dr0 = PX(pp, 0) - rp[0], dr1 = PX(pp, 1) - rp[1], dr2 = PX(pp, 2) - rp[2];
+            double drsqd = dr0 * dr0 + dr1 * dr1 + dr2 * dr2;
+            if (drsqd < hrangesqd_high) {
+                if (drsqd < hrangesqd_low) {
+                    pp_low++;
+                } else {
+                    hl[pp_high] = pp;
+                    rs[pp_high] = drsqd;
+                    pp_high++;
+                }
+            }
+            if (pp_high == pp_max) break;
+        }
+        pp_tot = pp_low + pp_high;
+        if (pp_tot < c->pp_gather) {
+            h_low = h_high;
+            h_high = HMULT * h_high;
+            hrangesqd_high = kr2 * h_high * h_high;
+        } else if (pp_low >= c->pp_gather) {
+            h_high = h_low;
+            h_low = h_low / HMULT;
+        } else {
+            break;
+        }
+    }
+    /* insertion_sort.F90:4-30 */
+    for (int j = 1; j < pp_high; j++) {
+        int iaux = hl[j];
+        double raux = rs[j];
+        int i = j - 1;
+        for (; i >= 0; i--) {
+            if (rs[i] <= raux) break;
+            rs[i + 1] = rs[i];
+            hl[i + 1] = hl[i];
+        }
+        rs[i + 1] = raux;
+        hl[i + 1] = iaux;
+    }
+    double h = c->invkernrange * sqrt(rs[c->pp_gather - pp_low - 1]);
+    free(hl); free(pot); free(rs);
+    return h;
+}
+
+/* all_sph.F90:11-167 (standard SPH): rho, div v and drho/dt over the gather
+ * neighbours in list order */
+static void all_sph(oracle_ctx *c, int p, int *potlist, int **big, int *onfly) {
+    double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
+    double hp = PX(p, SMOO), invhp = 1.0 / hp, hfactor = pow(invhp, 3.0);
+    double rhotemp = PX(p, MASS) * c->w0[0] * hfactor;
+    double vp[3] = {c->v[p][0], c->v[p][1], c->v[p][2]};
+    double div_v_p = 0.0;
+    int pp_numb = c->pptot[p];
+    const int *lst;
+    if (pp_numb <= c->pp_limit) {
+        lst = &c->pplist[(size_t)(p - 1) * c->pp_limit];
+    } else { /* gather_neib_on_fly: the gather subset comes in the same order */
+        pp_numb = get_neib(c, p, hp, potlist, big, onfly, c->n);
+        lst = onfly;
+    }
+    for (int i = 0; i < pp_numb; i++) {
+        int pp = lst[i];
+        double dv[3] = {c->v[pp][0] - vp[0], c->v[pp][1] - vp[1], c->v[pp][2] - vp[2]};
+        double mpp = PX(pp, MASS);
+        double dr[3] = {PX(pp, 0) - rp[0], PX(pp, 1) - rp[1], PX(pp, 2) - rp[2]};
+        double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        double dvdr = dv[0] * dr[0] + dv[1] * dr[1] + dv[2] * dr[2];
+        double drmag = sqrt(drsqd) + SMALL_NUMBER;
+        if (drmag >= c->kernrange * hp) continue;
+        double invdrmag = 1.0 / drmag;
+        int kern = (int)(c->halfkerntot * drmag * invhp);
+        if (kern > KERNTOT) kern = KERNTOT;
+        rhotemp = rhotemp + mpp * hfactor * c->w0[kern];
+        div_v_p = div_v_p - mpp * dvdr * c->w4[kern] * hfactor * invhp * invdrmag;
+    }
+    c->rho[p] = rhotemp;
+    div_v_p = div_v_p / rhotemp;
+    c->div_v[p] = div_v_p;
+    c->drhodt[p] = -rhotemp * div_v_p;
+}
+
+static void sph_update_standard(oracle_ctx *c) {
+    int n = c->n;
+    for (int p = 1; p <= n; p++) { /* each h from its own old value */
+        PX(p, SMOO) = h_gather(c, p, PX(p, SMOO));
+        if (c->err) return;
+    }
+    BHhydro_update_hmax(c);
+#pragma omp parallel
+    {
+        int *buf = (int *)malloc(sizeof(int) * (LISTSIZE + 1));
+        int *tmp = (int *)malloc(sizeof(int) * (c->pp_limit + 1));
+        int *big = NULL;
+#pragma omp for schedule(dynamic, 64)
+        for (int p = 1; p <= n; p++) {
+            int tot = get_neib(c, p, PX(p, SMOO), buf, &big, tmp, c->pp_limit);
+            c->pptot[p] = tot;
+            if (tot <= c->pp_limit)
+                memcpy(&c->pplist[(size_t)(p - 1) * c->pp_limit], tmp, sizeof(int) * tot);
+        }
+        free(buf);
+        free(tmp);
+        free(big);
+    }
+#pragma omp parallel
+    {
+        int *buf = (int *)malloc(sizeof(int) * (LISTSIZE + 1));
+        int *onfly = (int *)malloc(sizeof(int) * (n + 1));
+        int *big = NULL;
+#pragma omp for schedule(dynamic, 64)
+        for (int p = 1; p <= n; p++) all_sph(c, p, buf, &big, onfly);
+        free(buf);
+        free(onfly);
+        free(big);
+    }
+}
+
 /* sph_update.F90:10-158 (all particles active) */
 void oracle_sph_update(oracle_ctx *c) {
     int n = c->n;
+    if (c->sph_std) {
+        sph_update_standard(c);
+        return;
+    }
 #pragma omp parallel
     {
         int *buf = (int *)malloc(sizeof(int) * (LISTSIZE + 1));
@@ -1198,6 +1367,63 @@ static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **bi
     }
 }
 
+/* hydro.F90:11-323 (standard SPH, AB viscosity, isothermal): the kernel
+ * gradient averaged over h_p and h_pp (wmean) carries both the pressure and
+ * the viscous term */
+static void hydro_std(oracle_ctx *c, int p, int *onfly, int *potlist, int **big) {
+    double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
+    double hp = PX(p, SMOO), invhp = 1.0 / hp, hfactor_p = pow(invhp, 3.0);
+    double vp[3] = {c->v[p][0], c->v[p][1], c->v[p][2]};
+    double sound_p = c->sound[p], rho_p = c->rho[p];
+    double pfactor_p = c->press[p] / rho_p / rho_p;
+    double ah[3] = {0, 0, 0}, ad[3] = {0, 0, 0};
+    int pp_numb = c->pptot[p];
+    const int *lst;
+    if (pp_numb <= c->pp_limit) {
+        lst = &c->pplist[(size_t)(p - 1) * c->pp_limit];
+    } else {
+        pp_numb = get_neib(c, p, hp, potlist, big, onfly, c->n);
+        lst = onfly;
+    }
+    const double alpha = c->alpha;
+    for (int i = 0; i < pp_numb; i++) {
+        int pp = lst[i];
+        double wmean = 0.0, rho_pp = c->rho[pp], mpp = PX(pp, MASS), hpp = PX(pp, SMOO);
+        double dr[3] = {PX(pp, 0) - rp[0], PX(pp, 1) - rp[1], PX(pp, 2) - rp[2]};
+        double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        if (drsqd >= c->kernrangesqd * hp * hp && drsqd >= c->kernrangesqd * hpp * hpp) continue;
+        double drmag = sqrt(drsqd) + SMALL_NUMBER;
+        double invdrmag = 1.0 / drmag;
+        double du[3] = {dr[0] * invdrmag, dr[1] * invdrmag, dr[2] * invdrmag};
+        double dv[3] = {c->v[pp][0] - vp[0], c->v[pp][1] - vp[1], c->v[pp][2] - vp[2]};
+        double dvdr = dv[0] * dr[0] + dv[1] * dr[1] + dv[2] * dr[2];
+        if (drmag < c->kernrange * hp) {
+            int kern = (int)(c->halfkerntot * drmag * invhp);
+            if (kern > KERNTOT) kern = KERNTOT;
+            wmean = 0.5 * hfactor_p * invhp * c->w1[kern];
+        }
+        if (drmag < c->kernrange * hpp) {
+            double invhpp = 1.0 / hpp, hfactor_pp = pow(invhpp, 3.0);
+            int kern = (int)(c->halfkerntot * drmag * invhpp);
+            if (kern > KERNTOT) kern = KERNTOT;
+            wmean = wmean + 0.5 * hfactor_pp * invhpp * c->w1[kern];
+        }
+        double pf = pfactor_p + c->press[pp] / (rho_pp * rho_pp);
+        for (int k = 0; k < 3; k++) ah[k] = ah[k] + mpp * wmean * pf * du[k];
+        double rhomean = 0.5 * (rho_p + rho_pp);
+        if (dvdr < 0.0) {
+            double beta = 2 * alpha;
+            double alpha_mean = alpha;
+            double vsignal = 0.5 * (sound_p + c->sound[pp]);
+            double mu = 0.5 * (hp + hpp) * dvdr /
+                        (drsqd + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
+            double visc_factor = (-alpha_mean * vsignal * mu + beta * mu * mu) / rhomean;
+            for (int k = 0; k < 3; k++) ad[k] = ad[k] + mpp * visc_factor * wmean * du[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) c->a[p][k] = ah[k] + ad[k];
+}
+
 /* sph_hydro_forces.F90:9-120 */
 void oracle_hydro_forces(oracle_ctx *c) {
     int n = c->n;
@@ -1208,7 +1434,10 @@ void oracle_hydro_forces(oracle_ctx *c) {
 #pragma omp for schedule(dynamic, 64)
         for (int p = 1; p <= n; p++) {
             if (c->pptot[p] > c->pp_limit && !onfly) onfly = (int *)malloc(sizeof(int) * (n + 1));
-            hydro_gradh(c, p, onfly, potlist, &big);
+            if (c->sph_std)
+                hydro_std(c, p, onfly, potlist, &big);
+            else
+                hydro_gradh(c, p, onfly, potlist, &big);
         }
         free(onfly);
         free(potlist);
@@ -1244,6 +1473,37 @@ static inline void gravity_gradh(const oracle_ctx *c, double invhp, double hpp,
         dpot = dpot + invdrmag;
     }
     for (int k = 0; k < 3; k++) at[k] = 0.5 * mpp * invdrmag * grav * dr[k];
+    *dpotp = 0.5 * mpp * dpot;
+}
+
+/* gravity_sph.F90:11-92 (standard SPH near field, no zeta term) */
+static inline void gravity_sph(const oracle_ctx *c, double invhp, double hpp, double mpp,
+                               const double rp[3], const double rpp[3], double at[3],
+                               double *dpotp) {
+    double dr[3] = {rpp[0] - rp[0], rpp[1] - rp[1], rpp[2] - rp[2]};
+    double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    double drmag = sqrt(drsqd) + SMALL_NUMBER;
+    double invdrmag = 1.0 / drmag, grav, dpot;
+    if (invdrmag > c->invkernrange * invhp) {
+        int kern = (int)(c->halfkerntot * drmag * invhp);
+        if (kern > KERNTOT) kern = KERNTOT;
+        grav = invdrmag * invhp * invhp * c->w5[kern];
+        dpot = invhp * c->w6[kern];
+    } else {
+        grav = invdrmag * invdrmag * invdrmag;
+        dpot = invdrmag;
+    }
+    if (drmag < c->kernrange * hpp) {
+        double invhpp = 1.0 / hpp;
+        int kern = (int)(c->halfkerntot * drmag * invhpp);
+        if (kern > KERNTOT) kern = KERNTOT;
+        grav = grav + invdrmag * invhpp * invhpp * c->w5[kern];
+        dpot = dpot + invhpp * c->w6[kern];
+    } else {
+        grav = grav + invdrmag * invdrmag * invdrmag;
+        dpot = dpot + invdrmag;
+    }
+    for (int k = 0; k < 3; k++) at[k] = 0.5 * mpp * grav * dr[k];
     *dpotp = 0.5 * mpp * dpot;
 }
 
@@ -1347,8 +1607,11 @@ static void BHgrav_accel(const oracle_ctx *c, int p, double invhp, const double 
             for (int i = 0; i < nlist; i++) {
                 int pp = treelist[i];
                 double at[3], dpot, rpp[3] = {PX(pp, 0), PX(pp, 1), PX(pp, 2)};
-                gravity_gradh(c, invhp, PX(pp, SMOO), PX(pp, MASS), rp, rpp, zo_p,
-                              PX(pp, ZETA), at, &dpot);
+                if (c->sph_std)
+                    gravity_sph(c, invhp, PX(pp, SMOO), PX(pp, MASS), rp, rpp, at, &dpot);
+                else
+                    gravity_gradh(c, invhp, PX(pp, SMOO), PX(pp, MASS), rp, rpp, zo_p,
+                                  PX(pp, ZETA), at, &dpot);
                 for (int k = 0; k < 3; k++) agravp[k] = agravp[k] + at[k];
                 *potp = *potp + dpot;
             }

@@ -242,7 +242,7 @@ void free_all(Ctx &c) {
                     c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
-                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount, c.wstat};
+                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
@@ -272,6 +272,13 @@ int check_options(const serenx_options &o) {
                          "thermal must be SERENX_THERMAL_ISOTHERMAL or SERENX_THERMAL_ENERGY_EQN");
     if (o.thermal == SERENX_THERMAL_ENERGY_EQN && !(o.gamma > 1.0))
         return set_error(SERENX_EINVAL, "gamma must exceed 1 for the energy equation");
+    if (o.sph != SERENX_SPH_GRADH && o.sph != SERENX_SPH_STANDARD)
+        return set_error(SERENX_EINVAL, "sph must be SERENX_SPH_GRADH or SERENX_SPH_STANDARD");
+    if (o.sph == SERENX_SPH_STANDARD && o.thermal != SERENX_THERMAL_ISOTHERMAL)
+        return set_error(SERENX_EINVAL, "standard SPH is built for the isothermal option only");
+    if (o.pp_gather < 0 || (o.sph == SERENX_SPH_GRADH && o.pp_gather != 0))
+        return set_error(SERENX_EINVAL,
+                         "pp_gather must be 0 for grad-h SPH (h_fac sets it) and >= 0 otherwise");
     return SERENX_OK;
 }
 
@@ -286,6 +293,7 @@ size_t ctx_device_bytes(const Ctx &c) {
     b += cc * (sizeof(GNode) + sizeof(GNodeF) + sizeof(HNode) + 8 * 4 + 7 * 8);
     b += c.sorttmp_bytes + c.ovf_cap * 4 + sizeof(double) * KT_TOTAL;
     if (c.st.on) b += nn * (11 * 8 + 8 + 4);                 // step state
+    b += c.band_cap * BAND_MAX * 8;                         // standard SPH band
     return b;
 }
 
@@ -302,9 +310,16 @@ int run_tree_update(Ctx &c) {
 
 int run_sph_update(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "sph_update called before tree_update");
-    SX_TRY(sph_hsolve(c));
-    SX_TRY(update_hmax(c));
-    SX_TRY(sph_neighbours(c));
+    if (c.opt.sph == SERENX_SPH_STANDARD) {   // sph_update.F90:106-152
+        SX_TRY(sph_hgather(c));
+        SX_TRY(update_hmax(c));
+        SX_TRY(sph_neighbours(c));
+        SX_TRY(sph_density(c));
+    } else {                                  // sph_update.F90:55-101
+        SX_TRY(sph_hsolve(c));
+        SX_TRY(update_hmax(c));
+        SX_TRY(sph_neighbours(c));
+    }
     c.sph_ok = true;
     return SERENX_OK;
 }
@@ -326,13 +341,18 @@ int run_grav(Ctx &c) {
 namespace {
 
 // ---- host-state transfers -------------------------------------------------
-// parray(6,ptot) column-major: element (k, p) at parray[6 p + k]
+// parray(6,ptot) column-major (DATATOT = ZETA with GRAVITY, macros.h:53-57):
+// element (k, p) at parray[6 p + k].  Standard SPH has no zeta: column 5 is
+// neither read nor written, and the device keeps zeta = 0.
+bool skip_col(const Ctx &c, int k) { return k == 5 && c.opt.sph == SERENX_SPH_STANDARD; }
+
 int up_parray(Ctx &c, const int *cols, int ncols) {
     const int n = c.n;
     double *dst[6] = {c.x, c.y, c.z, c.m, c.h, c.zeta};
     c.hbuf.resize(n);
     for (int t = 0; t < ncols; t++) {
         const int k = cols[t];
+        if (skip_col(c, k)) continue;
         for (int p = 0; p < n; p++) c.hbuf[p] = c.hs.parray[6 * (size_t)p + k];
         SX_HIP(hipMemcpy(dst[k], c.hbuf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
@@ -345,6 +365,7 @@ int down_parray(Ctx &c, const int *cols, int ncols) {
     c.hbuf.resize(n);
     for (int t = 0; t < ncols; t++) {
         const int k = cols[t];
+        if (skip_col(c, k)) continue;
         SX_HIP(hipMemcpy(c.hbuf.data(), src[k], sizeof(double) * n, hipMemcpyDeviceToHost));
         for (int p = 0; p < n; p++) c.hs.parray[6 * (size_t)p + k] = c.hbuf[p];
     }
@@ -436,7 +457,9 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     c->opt = *opt;
     c->device = device;
     c->kg = kernel_geom(opt->kernel);
-    c->pp_gather = (int)((4.0 * PI_REF / 3.0) * ipow_seq(c->kg.range * opt->h_fac, 3));
+    c->pp_gather = (opt->sph == SERENX_SPH_STANDARD && opt->pp_gather > 0)
+                       ? opt->pp_gather
+                       : (int)((4.0 * PI_REF / 3.0) * ipow_seq(c->kg.range * opt->h_fac, 3));
     c->pp_limit = 2 * c->pp_gather;
     c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

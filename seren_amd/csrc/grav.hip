@@ -51,6 +51,13 @@ __host__ __device__ constexpr KGeom walk_geom() {
     return kernel_geom(QUINTIC ? SERENX_KERNEL_QUINTIC : SERENX_KERNEL_M4);
 }
 
+// dx^2 + dy^2 + dz^2 in the reference's operation order with no fused
+// multiply-add, also when inlined into code compiled with contraction on
+__device__ __forceinline__ double dist2_unfused(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+
 // 1/d for d > 0 to a few ulp: the hardware estimate refined by two Newton
 // steps y <- y + y (1 - d y) (each squares the relative error).
 __device__ __forceinline__ double rcp_fp64(double d) {
@@ -74,7 +81,10 @@ __device__ __forceinline__ unsigned pair_fp64(const KGeom &kg, const double4 *__
                                                double &ax, double &ay, double &az,
                                                double &pot) {
 #pragma clang fp contract(fast)
-    const double drsqd = dx * dx + dy * dy + dz * dz;
+    // r^2 unfused, as the reference forms it: standard SPH's h_gather puts
+    // the pp_gather-th neighbour at r = KERNRANGE h_p exactly, and a fused
+    // sum can move r across that boundary
+    const double drsqd = dist2_unfused(dx, dy, dz);
     const double drmag = sqrt(drsqd) + SMALL_NUMBER;
     const double invdrmag = rcp_fp64(drmag);
     const double id2 = invdrmag * invdrmag;

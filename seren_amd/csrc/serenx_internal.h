@@ -59,6 +59,9 @@ constexpr size_t KT_TOTAL = KT_H4_OFF + 4 * (size_t)KT;
 constexpr int WAVE = 64;
 // potential-neighbour list slots (opened leaves) per particle in the h solve
 constexpr int HL_SLOTS = 96;
+// standard SPH: r^2 values of the h_gather band kept per particle for the
+// selection of the pp_gather-th neighbour (more are narrowed by bisection)
+constexpr int BAND_MAX = 64;
 
 // Counters that every wave of a launch updates (walk statistics, interaction
 // counts, the step's reductions) are spread over CTR_ROWS rows of CTR_SLOTS
@@ -174,6 +177,8 @@ struct Ctx {
     uint8_t *active = nullptr;   // accdo
     double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
+    double *band = nullptr;      // standard SPH: BAND_MAX x 64 r^2 values per wave
+    size_t band_cap = 0;         // particles the band buffer holds
     double *tmpd = nullptr;      // permutation scratch (cap doubles)
     int *tmpi = nullptr;
 
@@ -291,6 +296,8 @@ int set_clouds(Ctx &c, int ncloud, const int *counts);
 // sph.hip
 int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles
 int sph_neighbours(Ctx &c);      // get_neib over active particles
+int sph_hgather(Ctx &c);         // standard SPH: h_gather over active particles
+int sph_density(Ctx &c);         // standard SPH: all_sph over active particles
 int sph_thermal_iso(Ctx &c, double cs);
 int sph_thermal(Ctx &c);          // thermal for the context's THERMAL option
 int hydro_forces(Ctx &c);        // hydro_gradh over active particles

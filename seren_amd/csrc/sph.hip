@@ -578,6 +578,330 @@ __global__ void k_thermal_energy(int n, double gamma, const double *__restrict__
     sound[i] = sqrt(press_p / rho_p);
 }
 
+// ---- standard SPH (SPH=STANDARD, HFIND=NUMBER) ---------------------------
+//
+// h_gather.F90:13-203 for every active particle.  The reference brackets h
+// between h_low and h_high = HMULT h_low until pp_low < pp_gather <= pp_tot
+// (the counts of other particles with r^2 below KERNRANGESQD h_low^2 and
+// h_high^2), sorts the r^2 of the band between them and takes element
+// pp_gather - pp_low: h = INVKERNRANGE sqrt(r^2 of the pp_gather-th nearest
+// other particle), which does not depend on the path the bracketing took.
+// Here each lane brackets the same way over its own potential-neighbour leaf
+// list (recorded by a group walk at radius KERNRANGE h_search whenever h_high
+// outgrows h_search, as the reference re-walks), narrows a band of more than
+// BAND_MAX values by bisection in r^2, copies the band to its slice of the
+// band buffer and sorts it there.  r^2 is formed as distance2.F90 forms it,
+// so h is the reference's bit for bit.  A lane that cannot bracket within
+// HG_ITER_MAX passes (fewer than pp_gather other particles) sets *err.
+constexpr int HG_ITER_MAX = 400;
+
+__global__ void __launch_bounds__(256) k_hgather(
+    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range, int *__restrict__ hlist,
+    double *__restrict__ band, const uint8_t *__restrict__ active, int pp_gather,
+    double *__restrict__ h, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = p < n && active[p];
+    const int lane = threadIdx.x & (WAVE - 1);
+    int *wl = hlist + (size_t)(p / WAVE) * (HL_SLOTS * WAVE) + lane;
+    double *bl = band + (size_t)(p / WAVE) * (BAND_MAX * WAVE) + lane;
+    double rpx = 0, rpy = 0, rpz = 0, h_old = 1.0;
+    int2 rr = make_int2(nc, nc);
+    if (valid) {
+        const double4 a = rec[p];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        h_old = h[p];
+        rr = root_range[cid[p]];
+    }
+    double h_high = HMULT * h_old, h_low = h_old / HMULT, h_search = 0.0;
+    bool iterating = valid;
+    int nslot = 0, cnt = 0, pp_low = 0, pp_tot = 0, passes = 0;
+    bool my_ovf = false, wave_ovf = false, have_list = false, failed = false;
+    unsigned vis = 0, lvis = 0;
+    auto dist2 = [&](const double4 &a) {   // distance2.F90, no contraction
+        const double dx = a.x - rpx, dy = a.y - rpy, dz = a.z - rpz;
+        return dx * dx + dy * dy + dz * dz;
+    };
+    // one pass over the lane's candidates: f(j, r^2) for each other particle
+    auto for_each = [&](bool on, auto &&f) {
+        if (!wave_ovf) {
+            list_loop(wl, nslot, on, p, [&](int j) { return rec[j]; },
+                      [&](int j, const double4 &a) { f(j, dist2(a)); });
+        } else {
+            const double R = kg.range * h_search;
+            vis += group_walk(hn, nc, on, rr.x, rr.y, lvis,
+                              [&](const HNode &nd, bool) {
+                                  return hgather_visit(nd, rpx, rpy, rpz, R);
+                              },
+                              [&](const HNode &nd, bool mine) {
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = nd.first + i;
+                    const double4 a = rec[j];
+                    if (mine && j != p) f(j, dist2(a));
+                }
+            });
+        }
+    };
+
+    // bracketing (h_gather.F90:57-175)
+    while (__any(iterating)) {
+        passes++;
+        bool newlist = false;
+        if (iterating && h_high > h_search) {
+            h_search = HMULT * h_high;
+            newlist = true;
+        }
+        if (!have_list || __any(newlist)) {
+            if (iterating) {
+                nslot = 0;
+                cnt = 0;
+            }
+            bool lovf = false;
+            const double R = kg.range * h_search;
+            vis += group_walk(hn, nc, iterating, rr.x, rr.y, lvis,
+                              [&](const HNode &nd, bool) {
+                                  return hgather_visit(nd, rpx, rpy, rpz, R);
+                              },
+                              [&](const HNode &nd, bool mine) {
+                if (mine) {
+                    cnt += nd.leaf;
+                    if (nslot < HL_SLOTS)
+                        wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
+                    else
+                        lovf = true;
+                    nslot++;
+                }
+            });
+            if (iterating) my_ovf = lovf;
+            have_list = true;
+            wave_ovf = __any(my_ovf);   // lanes that stopped keep their lists
+        }
+        // too few potential neighbours: grow and walk again (:93-99)
+        const bool grow = iterating && newlist && cnt <= pp_gather;
+        const bool count = iterating && !grow;
+        const double hr2_high = kg.rangesqd * h_high * h_high;
+        const double hr2_low = kg.rangesqd * h_low * h_low;
+        int lo = 0, tot = 0;
+        for_each(count, [&](int, double r2) {
+            if (r2 < hr2_high) {
+                tot++;
+                if (r2 < hr2_low) lo++;
+            }
+        });
+        if (grow) {
+            h_search = 0.0;
+            h_low = h_high;
+            h_high = HMULT * h_high;
+        } else if (count) {
+            pp_low = lo;
+            pp_tot = tot;
+            if (tot < pp_gather) {                    // :143-147
+                h_low = h_high;
+                h_high = HMULT * h_high;
+            } else if (lo >= pp_gather) {             // :150-153
+                h_high = h_low;
+                h_low = h_low / HMULT;
+            } else {                                  // :156-158
+                iterating = false;
+            }
+        }
+        if (iterating && passes >= HG_ITER_MAX) {
+            iterating = false;
+            failed = true;
+        }
+    }
+
+    // band [lo2, hi2) of r^2 with count(r^2 < lo2) = pp_low < pp_gather <=
+    // count(r^2 < hi2) = pp_tot; narrow it to BAND_MAX values at most
+    double lo2 = kg.rangesqd * h_low * h_low, hi2 = kg.rangesqd * h_high * h_high;
+    bool narrowing = valid && !failed && pp_tot - pp_low > BAND_MAX;
+    int nb = 0;
+    while (__any(narrowing)) {
+        const double mid = 0.5 * (lo2 + hi2);
+        int c = 0;
+        for_each(narrowing, [&](int, double r2) {
+            if (r2 < mid) c++;
+        });
+        if (narrowing) {
+            if (c >= pp_gather) {
+                hi2 = mid;
+                pp_tot = c;
+            } else {
+                lo2 = mid;
+                pp_low = c;
+            }
+            if (pp_tot - pp_low <= BAND_MAX) narrowing = false;
+            if (++nb >= HG_ITER_MAX) {
+                narrowing = false;
+                failed = true;
+            }
+        }
+    }
+    // copy the band and sort it (insertion_sort.F90:4-30)
+    const bool fill = valid && !failed;
+    int m = 0;
+    for_each(fill, [&](int, double r2) {
+        if (r2 < hi2 && !(r2 < lo2) && m < BAND_MAX) {
+            bl[m * WAVE] = r2;
+            m++;
+        }
+    });
+    if (fill) {
+        for (int j = 1; j < m; j++) {
+            const double raux = bl[j * WAVE];
+            int i = j - 1;
+            for (; i >= 0; i--) {
+                const double ri = bl[i * WAVE];
+                if (ri <= raux) break;
+                bl[(i + 1) * WAVE] = ri;
+            }
+            bl[(i + 1) * WAVE] = raux;
+        }
+        const int k = pp_gather - pp_low - 1;
+        if (k >= 0 && k < m)
+            h[p] = kg.invrange * sqrt(bl[k * WAVE]);
+        else
+            failed = true;
+    }
+    if (failed) atomicOr(err, 1);
+    wave_stats(wstat, 2, vis, lvis);
+}
+
+// all_sph.F90:11-167 (standard SPH) for every active particle: rho, div v
+// and drho/dt over the gather neighbours (r < KERNRANGE h_p) of its list, in
+// list order.  rec holds [2i] x y z m and [2i+1] vx vy vz 0.
+__global__ void __launch_bounds__(256) k_all_sph(
+    KGeom kg, int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
+    const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
+    const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab,
+    const double *__restrict__ h, double *__restrict__ rho, double *__restrict__ divv,
+    double *__restrict__ drhodt) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double *w0 = ktab + T_W0 * KT, *w4 = ktab + T_W4 * KT;
+    const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
+    const double rpx = a.x, rpy = a.y, rpz = a.z;
+    const double vpx = b.x, vpy = b.y, vpz = b.z;
+    const double hp = h[p], invhp = 1.0 / hp, hfactor = cr_pow3(invhp);
+    const double range_p = kg.range * hp;
+    double rhotemp = a.w * w0[0] * hfactor;
+    double div_v_p = 0.0;
+    const int nn = pptot[p];
+    const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
+    for (int i = 0; i < nn; i++) {
+        const int j = lst[i];
+        const double4 q = rec[2 * (size_t)j], qv = rec[2 * (size_t)j + 1];
+        const double dvx = qv.x - vpx, dvy = qv.y - vpy, dvz = qv.z - vpz;
+        const double mpp = q.w;
+        const double dx = q.x - rpx, dy = q.y - rpy, dz = q.z - rpz;
+        const double drsqd = dx * dx + dy * dy + dz * dz;
+        const double dvdr = dvx * dx + dvy * dy + dvz * dz;
+        const double drmag = sqrt(drsqd) + SMALL_NUMBER;
+        if (drmag >= range_p) continue;                        // all_sph.F90:103
+        const double invdrmag = 1.0 / drmag;
+        const int kern = min((int)(kg.half * drmag * invhp), KERNTOT);
+        rhotemp = rhotemp + mpp * hfactor * w0[kern];
+        div_v_p = div_v_p - mpp * dvdr * w4[kern] * hfactor * invhp * invdrmag;
+    }
+    rho[p] = rhotemp;
+    div_v_p = div_v_p / rhotemp;
+    divv[p] = div_v_p;
+    drhodt[p] = -rhotemp * div_v_p;
+}
+
+__global__ void k_pack_sph(int n, const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ z, const double *__restrict__ vx,
+                           const double *__restrict__ vy, const double *__restrict__ vz,
+                           const double *__restrict__ m, double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rec[2 * (size_t)i] = make_double4(x[i], y[i], z[i], m[i]);
+    rec[2 * (size_t)i + 1] = make_double4(vx[i], vy[i], vz[i], 0.0);
+}
+
+// Hydro record of standard SPH (hydro.F90:83-98, :180-188): [0] x y z h
+// [1] vx vy vz m  [2] rho pfactor sound invh  [3] hfactor P/(rho rho), with
+// pfactor = P / rho / rho, invh = 1 / h and hfactor = invh**3 formed as the
+// reference forms them.
+__global__ void k_pack_hydro_std(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ z, const double *__restrict__ h,
+                                 const double *__restrict__ vx, const double *__restrict__ vy,
+                                 const double *__restrict__ vz, const double *__restrict__ m,
+                                 const double *__restrict__ rho, const double *__restrict__ press,
+                                 const double *__restrict__ sound, double4 *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 *r = rec + 4 * (size_t)i;
+    const double hi = h[i], invh = 1.0 / hi, rho_i = rho[i], p_i = press[i];
+    r[0] = make_double4(x[i], y[i], z[i], hi);
+    r[1] = make_double4(vx[i], vy[i], vz[i], m[i]);
+    r[2] = make_double4(rho_i, p_i / rho_i / rho_i, sound[i], invh);
+    r[3] = make_double4(cr_pow3(invh), p_i / (rho_i * rho_i), 0.0, 0.0);
+}
+
+// hydro.F90:11-323 (standard SPH, AB viscosity, isothermal): the kernel
+// gradient averaged over h_p and h_pp (wmean) weighs both the pressure term
+// (P_p/rho_p^2 + P_pp/rho_pp^2) and the viscous term.
+__global__ void __launch_bounds__(256) k_hydro_std(
+    KGeom kg, int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
+    const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
+    const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab, double alpha,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double *w1 = ktab + T_W1 * KT;
+    const double4 *rp = rec + 4 * (size_t)p;
+    const double4 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3];
+    const double rpx = r0.x, rpy = r0.y, rpz = r0.z, hp = r0.w;
+    const double vpx = r1.x, vpy = r1.y, vpz = r1.z;
+    const double rho_p = r2.x, pfactor_p = r2.y, sound_p = r2.z, invhp = r2.w;
+    const double hfactor_p = r3.x;
+    double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0;
+    const int nn = pptot[p];
+    const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
+    for (int i = 0; i < nn; i++) {
+        const double4 *rq = rec + 4 * (size_t)lst[i];
+        const double4 q0 = rq[0], q1 = rq[1], q2 = rq[2];
+        const double hpp = q0.w, mpp = q1.w, rho_pp = q2.x;
+        const double dx = q0.x - rpx, dy = q0.y - rpy, dz = q0.z - rpz;
+        const double drsqd = dx * dx + dy * dy + dz * dz;
+        if (drsqd >= kg.rangesqd * hp * hp && drsqd >= kg.rangesqd * hpp * hpp) continue;
+        const double drmag = sqrt(drsqd) + SMALL_NUMBER;
+        const double invdrmag = 1.0 / drmag;
+        const double ux = dx * invdrmag, uy = dy * invdrmag, uz = dz * invdrmag;
+        const double dvx = q1.x - vpx, dvy = q1.y - vpy, dvz = q1.z - vpz;
+        const double dvdr = dvx * dx + dvy * dy + dvz * dz;
+        double wmean = 0.0;
+        if (drmag < kg.range * hp) {
+            const int kern = min((int)(kg.half * drmag * invhp), KERNTOT);
+            wmean = 0.5 * hfactor_p * invhp * w1[kern];
+        }
+        if (drmag < kg.range * hpp) {
+            const double invhpp = q2.w, hfactor_pp = rq[3].x;
+            const int kern = min((int)(kg.half * drmag * invhpp), KERNTOT);
+            wmean = wmean + 0.5 * hfactor_pp * invhpp * w1[kern];
+        }
+        const double pf = pfactor_p + rq[3].y;
+        ahx = ahx + mpp * wmean * pf * ux;
+        ahy = ahy + mpp * wmean * pf * uy;
+        ahz = ahz + mpp * wmean * pf * uz;
+        const double rhomean = 0.5 * (rho_p + rho_pp);
+        if (dvdr < 0.0) {                                      // hydro.F90:221-224
+            const double beta = 2 * alpha;
+            const double vsignal = 0.5 * (sound_p + q2.z);
+            const double mu = 0.5 * (hp + hpp) * dvdr /
+                              (drsqd + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
+            const double visc = (-alpha * vsignal * mu + beta * mu * mu) / rhomean;
+            adx = adx + mpp * visc * wmean * ux;
+            ady = ady + mpp * visc * wmean * uy;
+            adz = adz + mpp * visc * wmean * uz;
+        }
+    }
+    ax[p] = ahx + adx;
+    ay[p] = ahy + ady;
+    az[p] = ahz + adz;
+}
+
 __global__ void k_pow_third(int n, const double *__restrict__ x, double *__restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = ref_pow_third(x[i]);
@@ -610,6 +934,46 @@ int sph_hsolve(Ctx &c) {
                        c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    return SERENX_OK;
+}
+
+int sph_hgather(Ctx &c) {
+    if (c.n <= c.pp_gather)
+        return set_error(SERENX_EFEW, "h_gather needs more than pp_gather particles");
+    const size_t need = ((size_t)c.cap + TB - 1) / TB * TB;   // whole blocks, as hlist
+    if (c.band_cap < need) {
+        if (c.band) (void)hipFree(c.band);
+        c.band = nullptr;
+        c.band_cap = 0;
+        if (hipMalloc((void **)&c.band, sizeof(double) * need * BAND_MAX) != hipSuccess)
+            return set_error(SERENX_ENOMEM, "hipMalloc failed for the h_gather band buffer");
+        c.band_cap = need;
+    }
+    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.vx, c.vy, c.vz, c.m, rec);
+    hipLaunchKernelGGL(k_hgather, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
+                       rec, c.cid, c.root_range, c.hlist, c.band, c.active, c.pp_gather, c.h,
+                       c.flag, c.wstat);
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    if (c.hostpin[8] != 0)
+        return set_error(SERENX_EFEW, "h_gather could not bracket pp_gather neighbours "
+                                      "(a cloud with too few particles)");
+    c.hostpin[8] = 0;   // sph_neighbours reads it as the h-solve error flag
+    return SERENX_OK;
+}
+
+int sph_density(Ctx &c) {
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    hipLaunchKernelGGL(k_pack_sph, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
+                       c.vx, c.vy, c.vz, c.m, rec);
+    hipLaunchKernelGGL(k_all_sph, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, rec,
+                       c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
+                       c.h, c.rho, c.divv, c.drhodt);
+    SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
 
@@ -681,6 +1045,15 @@ int sph_thermal_iso(Ctx &c, double cs) {
 
 int hydro_forces(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
+    if (c.opt.sph == SERENX_SPH_STANDARD) {
+        hipLaunchKernelGGL(k_pack_hydro_std, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x,
+                           c.y, c.z, c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.press, c.sound, rec);
+        hipLaunchKernelGGL(k_hydro_std, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, rec,
+                           c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
+                           c.opt.alpha, c.ax, c.ay, c.az);
+        SX_HIP(hipGetLastError());
+        return SERENX_OK;
+    }
     hipLaunchKernelGGL(k_pack_hydro, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.omega, c.press, c.sound, rec);
     SX_HIP(hipGetLastError());

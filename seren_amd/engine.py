@@ -60,6 +60,7 @@ _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
 _LEVELS = {"adaptive": 0, "restricted": 1, "fixed": 2}
 _NEIB_CHECK = {0: 0, 1: 1, 2: 2, "off": 0, "end": 1, "immediate": 2}
 _THERMAL = {"isothermal": 0, "energy": 1}
+_SPH = {"gradh": 0, "standard": 1}
 
 
 class SerenxError(RuntimeError):
@@ -70,7 +71,8 @@ class Options(C.Structure):
     _fields_ = [("multipole", C.c_int32), ("mac", C.c_int32), ("precision", C.c_int32),
                 ("kernel", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
                 ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double),
-                ("thermal", C.c_int32), ("reserved", C.c_int32), ("gamma", C.c_double)]
+                ("thermal", C.c_int32), ("sph", C.c_int32), ("gamma", C.c_double),
+                ("pp_gather", C.c_int32), ("reserved", C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -175,9 +177,11 @@ class ForceEngine:
                  precision: str = "fp64", h_fac: float = 1.2, thetamaxsqd: float = 0.25,
                  abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
                  device: int = 0, kernel: str = "m4", thermal: str = "isothermal",
-                 gamma: float = 1.4):
+                 gamma: float = 1.4, sph: str = "gradh", pp_gather: int = 0):
         """thermal: "isothermal" (P = c_s^2 rho with sound_speed) or "energy"
-        (the energy equation: set u with set_energy)."""
+        (the energy equation: set u with set_energy).  sph: "gradh" or
+        "standard" (h from the pp_gather-th neighbour; pp_gather 0 takes the
+        h_fac value)."""
         self.lib = load_library()
         opt = Options()
         _check(self.lib.serenx_default_options(C.byref(opt)))
@@ -192,6 +196,8 @@ class ForceEngine:
         opt.sound_speed = sound_speed
         opt.thermal = _THERMAL[thermal]
         opt.gamma = gamma
+        opt.sph = _SPH[sph]
+        opt.pp_gather = pp_gather
         self.opt = opt
         self.ctx = C.c_void_p()
         _check(self.lib.serenx_create(C.byref(opt), device, C.byref(self.ctx)))

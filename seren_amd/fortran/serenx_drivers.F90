@@ -5,7 +5,8 @@
 ! src/sph_sim/sph_grav_forces.F90 (and adds BHhydro_hguess below in place of
 ! src/BHtree/BHhydro_hguess.F90) runs its force evaluation on the GPU through
 ! libserenx.so.  The module arrays are bound by address before every call,
-! so reallocation between steps is harmless.  Option set: GRAD_H_SPH, M4,
+! so reallocation between steps is harmless.  Option set: GRAD_H_SPH (or
+! SPH=STANDARD with HFIND=NUMBER, isothermal), M4 or QUINTIC (TC or not),
 ! ISOTHERMAL or ENERGY_EQN (INTERNAL_ENERGY: du_dt comes back from
 ! sph_hydro_forces; thermal and the advance stay on the host), AB viscosity,
 ! KS gravity, BH tree, REORDER_PARTICLES, NEIGHBOUR_LISTS, DOUBLE_PRECISION,
@@ -59,20 +60,36 @@ CONTAINS
        opt%thermal = SERENX_THERMAL_ENERGY_EQN
        opt%gamma = gamma
 #endif
+#if !defined(GRAD_H_SPH)
+       ! SPH=STANDARD, HFIND=NUMBER: h from the pp_gather-th neighbour
+       opt%sph = SERENX_SPH_STANDARD
+       opt%pp_gather = pp_gather
+#endif
        call serenx_f_check(serenx_create(opt, 0, serenx_ctx), "create")
     end if
     st%ptot = ptot
     st%pp_limit = pp_limit
     ! the module arrays are not TARGETs: take their addresses through
     ! assumed-size TARGET dummies (contiguous allocatables, passed by address)
+#if defined(GRAD_H_SPH)
     call addresses(parray, v, a, gpot, rho, omega, press, sound, div_v, drhodt, &
          & pptot, pplist, accdo)
+#else
+    ! no omega without grad-h: rho stands in for the unused pointer, then NULL
+    call addresses(parray, v, a, gpot, rho, rho, press, sound, div_v, drhodt, &
+         & pptot, pplist, accdo)
+    st%omega = c_null_ptr
+#endif
 #if !defined(GEOMETRIC_MAC)
     call address_of(agravmag, st%agravmag)
 #else
     st%agravmag = c_null_ptr
 #endif
+#if defined(GRAD_H_SPH)
     call address_of_scalar(rextent, st%rextent)
+#else
+    st%rextent = c_null_ptr
+#endif
 #if defined(INTERNAL_ENERGY)
     call address_of(du_dt, st%du_dt)
 #else

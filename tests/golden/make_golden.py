@@ -36,7 +36,7 @@ def energy0(n, seed):
 
 
 # name: (n, profile, seed, mach, octupole, mac, thetamaxsqd, abserror, evaluations, full
-#        [, kernel [, thermal [, h_fac]]])
+#        [, kernel [, thermal [, h_fac [, sph]]]])
 CASES = {
     "uniform1k_quad_geo": (1000, "uniform", 101, 1.0, False, "geometric", 0.25, 1e-4, 1, True),
     "plummer2k_quad_geo": (2048, "plummer", 202, 0.0, False, "geometric", 0.25, 1e-4, 1, True),
@@ -59,6 +59,11 @@ CASES = {
     # KERNEL=QUINTICTC, octupole + GADGET2: h_fac 0.8, pp_gather 57
     "plummer2k_oct_gadget2_quintictc_x2": (2048, "plummer", 110, 1.0, True, "gadget2", 0.25,
                                            1e-4, 2, True, "quintictc", "isothermal", 0.8),
+    # SPH=STANDARD, HFIND=NUMBER (h_gather, all_sph, hydro, gravity_sph)
+    "r2_2k_std_quad_geo_x2": (2048, "r2", 111, 2.0, False, "geometric", 0.25, 1e-4, 2, True,
+                              "m4", "isothermal", 1.2, "standard"),
+    "plummer2k_std_oct_gadget2_x3": (2048, "plummer", 112, 1.0, True, "gadget2", 0.25, 1e-4, 3,
+                                     True, "m4", "isothermal", 1.2, "standard"),
 }
 
 
@@ -71,8 +76,9 @@ def run_case(name, spec):
     kernel = spec[10] if len(spec) > 10 else "m4"
     thermal = spec[11] if len(spec) > 11 else "isothermal"
     h_fac = spec[12] if len(spec) > 12 else 1.2
+    sph = spec[13] if len(spec) > 13 else "gradh"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal)
+    ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal, sph=sph)
     ref.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=th2, abserror=ae)
     if thermal == "energy":
         ref.set_energy(energy0(n, seed), GAMMA)
@@ -113,6 +119,8 @@ def run_case(name, spec):
     )
     if h_fac != 1.2:
         out["h_fac"] = np.array(h_fac)
+    if sph != "gradh":
+        out["sph"] = np.array(sph)
     if thermal == "energy":
         out["thermal"] = np.array(thermal)
         out["gamma"] = np.array(GAMMA)

@@ -27,14 +27,17 @@ GOLD = os.path.join(HERE, "golden")
 @pytest.mark.parametrize("case", ["uniform1k_quad_geo", "plummer2k_quad_geo",
                                   "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
                                   "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc",
-                                  "r2_2k_quad_geo_energy_x2", "r2_2k_quad_geo_quintic_x2"])
+                                  "r2_2k_quad_geo_energy_x2", "r2_2k_quad_geo_quintic_x2",
+                                  "r2_2k_std_quad_geo_x2"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))
     n, seed, octu, nev = (int(v) for v in g["spec"])
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     thermal = str(g["thermal"]) if "thermal" in g.files else "isothermal"
-    host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel, thermal=thermal)
+    sph = str(g["sph"]) if "sph" in g.files else "gradh"
+    host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel, thermal=thermal,
+                    sph=sph)
     h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
     host.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], h_fac=h_fac,
                thetamaxsqd=float(g["thetamaxsqd"]),

@@ -45,11 +45,12 @@ def _ids(c):
     return f"n{c[0]}-{c[1]}-{_MULTI[c[3]]}-{c[4]}-x{c[7]}"
 
 
-def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal", h_fac=1.2):
+def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal", h_fac=1.2,
+            sph="gradh"):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole=_MULTI[octupole], mac=mac,
                        precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
-                       kernel=kernel, thermal=thermal, h_fac=h_fac)
+                       kernel=kernel, thermal=thermal, h_fac=h_fac, sph=sph)
 
 
 def _report(name, a, b):
@@ -186,8 +187,9 @@ def test_gpu_matches_reference_golden_vectors(case):
     energy = "thermal" in g.files and str(g["thermal"]) == "energy"
     print(f"golden {case}", flush=True)
     h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
+    sph = str(g["sph"]) if "sph" in g.files else "gradh"
     eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel,
-                  "energy" if energy else "isothermal", h_fac)
+                  "energy" if energy else "isothermal", h_fac, sph)
     eng.load(g["pos"], g["vel"], g["mass"], np.zeros(n))
     if energy:
         assert float(g["gamma"]) == eng.opt.gamma

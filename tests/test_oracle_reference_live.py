@@ -25,6 +25,9 @@ CASES = [
     (1500, "uniform", 17, False, "geometric", 1, "m4tc"),
     (2000, "r2", 18, False, "geometric", 2, "quintic", 1.0),
     (1500, "plummer", 19, True, "gadget2", 2, "quintictc", 0.8),
+    # SPH=STANDARD, HFIND=NUMBER
+    (2000, "r2", 20, False, "geometric", 2, "m4", 1.2, "standard"),
+    (1500, "plummer", 21, True, "gadget2", 3, "m4", 1.2, "standard"),
 ]
 
 
@@ -35,10 +38,11 @@ def test_oracle_bit_exact_with_reference(case):
     n, prof, seed, octu, mac, nev = case[:6]
     kernel = case[6] if len(case) > 6 else "m4"
     h_fac = case[7] if len(case) > 7 else 1.2
+    sph = case[8] if len(case) > 8 else "gradh"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=1.5, ngrid=16)
     outs = []
     for cls in (RefSeren, CSeren):
-        o = cls(octu, mac, kernel=kernel)
+        o = cls(octu, mac, kernel=kernel, sph=sph)
         o.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=0.25, abserror=1e-4)
         for e in range(nev):
             o.tree_update()
