@@ -242,7 +242,7 @@ void free_all(Ctx &c) {
                     c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
-                    c.hmaxs, c.gn, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band};
+                    c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
@@ -290,7 +290,7 @@ size_t ctx_device_bytes(const Ctx &c) {
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
-    b += cc * (sizeof(GNode) + sizeof(GNodeF) + sizeof(HNode) + 8 * 4 + 7 * 8);
+    b += cc * (sizeof(GNode) + 10 * 8 + sizeof(GNodeF) + sizeof(HNode) + 8 * 4 + 7 * 8);
     b += c.sorttmp_bytes + c.ovf_cap * 4 + sizeof(double) * KT_TOTAL;
     if (c.st.on) b += nn * (11 * 8 + 8 + 4);                 // step state
     b += c.band_cap * BAND_MAX * 8;                         // standard SPH band
@@ -930,6 +930,8 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int
     SX_HIP(hipMemcpy(d2b.data(), c.dfs2bfs, sizeof(int) * nc, hipMemcpyDeviceToHost));
     SX_HIP(hipMemcpy(cleaf.data(), c.c_leaf, sizeof(int) * nc, hipMemcpyDeviceToHost));
     SX_HIP(hipMemcpy(g.data(), c.gn, sizeof(GNode) * nc, hipMemcpyDeviceToHost));
+    std::vector<double> gs((size_t)nc * 10);
+    SX_HIP(hipMemcpy(gs.data(), c.gs, sizeof(double) * gs.size(), hipMemcpyDeviceToHost));
     SX_HIP(hipMemcpy(hh.data(), c.hn, sizeof(HNode) * nc, hipMemcpyDeviceToHost));
     const int sentinel = 2 * c.n + 1;   // cmax_grav + 1 (BHgrav_build.F90:75)
     for (int b = 0; b < nc; b++) {
@@ -953,7 +955,7 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int
         if (s)
             for (int k = 0; k < 10; k++)
                 s[10 * b + k] = (k == 9 && c.opt.multipole == SERENX_OCTUPOLE_EXACT)
-                                    ? 0.5 * gg.s[k] : gg.s[k];
+                                    ? 0.5 * gs[10 * (size_t)d + k] : gs[10 * (size_t)d + k];
         if (rmax) rmax[b] = hh[d].rmax;
         if (hrangemax) hrangemax[b] = hh[d].hrangemax;
     }

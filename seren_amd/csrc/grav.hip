@@ -132,43 +132,78 @@ __global__ void k_pack_grav(int n, double kernrange, const double *__restrict__ 
     r[3] = make_double4(h[i], 1.0 / h[i], zeta[i], kernrange * h[i]);
 }
 
-// 1/sqrt(a) for a > 0 to a few ulp: the hardware estimate refined by two
-// Newton steps y <- y + y (1 - a y^2) / 2 (each squares the relative error).
+// 1/sqrt(a) for a > 0 to about one ulp: the hardware estimate (relative
+// error below 2^-23) refined by one third-order step
+//   y <- y (1 + e/2 + 3e^2/8),  e = 1 - a y^2,
+// whose truncation error (5/16) e^3 is below 2^-66.
 __device__ __forceinline__ double rsqrt_fp64(double a) {
 #pragma clang fp contract(fast)
-    double y = __builtin_amdgcn_rsq(a);
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const double e = __fma_rn(-a * y, y, 1.0);
-        y = __fma_rn(0.5 * y, e, y);
-    }
-    return y;
+    const double y = __builtin_amdgcn_rsq(a);
+    const double e = __fma_rn(-a * y, y, 1.0);
+    return __fma_rn(y * e, __fma_rn(0.375, e, 0.5), y);
 }
 
 // Multipole terms of one accepted cell at offset (x, y, z) = r_p - r_cell:
 // the monopole, quadrupole and octupole terms of BHgrav_accel.F90:159-222,
 // evaluated in factored form.  With Q the traceless quadrupole,
-// S(r) = sum_k s_k m_k(r) the octupole cubic and G = grad S,
+// S(r) = sum_k s_k m_k(r) the octupole cubic and H = grad(S) / 2,
 //   a   = -m r / r^3 + Q r / r^5 - 5/2 (r.Q.r) r / r^7
-//         + G / (2 r^7) - 7/2 S r / r^9,
-//   phi = m / r + (r.Q.r) / (2 r^5) + S / (2 r^7),
+//         + H / r^7 - 7/3 (r.H) r / r^9,
+//   phi = m / r + (r.Q.r) / (2 r^5) + (r.H) / (3 r^7),
 // term for term the reference's expressions (its octupole force terms are
-// the components of G / 2); S is formed as (r.G) / 3 (Euler's identity for a
-// homogeneous cubic).  Coefficient arrays (t = scalar type):
-//   q = [xx xy yy zx zy], zz = -(xx + yy)
-//   s = [x^3 x^2y y^2x y^3 z^2x z^2y z^3 x^2z y^2z xyz]
+// the components of H); S is formed as 2 (r.H) / 3 (Euler's identity for a
+// homogeneous cubic).  q = [xx xy yy zx zy zz]; oc holds H's coefficients on
+// the monomials (xx xy xz yy zz yz), stocked per node.
 template <int OCT, class T>
-__device__ __forceinline__ void cell_terms(T x, T y, T z, T ir, T m, const T *q, const T *s,
+__device__ __forceinline__ void cell_terms(T x, T y, T z, T ir, T m, const T *q, const T *oc,
                                            T &ax, T &ay, T &az, T &pot) {
 #pragma clang fp contract(fast)
     const T ir2 = ir * ir;
     const T ir3 = ir2 * ir;
     const T ir5 = ir3 * ir2;
     const T ir7 = ir5 * ir2;
-    const T qzz = -(q[0] + q[2]);
     const T Qx = q[0] * x + q[1] * y + q[3] * z;
     const T Qy = q[1] * x + q[2] * y + q[4] * z;
-    const T Qz = q[3] * x + q[4] * y + qzz * z;
+    const T Qz = q[3] * x + q[4] * y + q[5] * z;
+    const T t5 = (x * Qx + y * Qy + z * Qz) * ir5;   // (r.Q.r) / r^5
+    T crad = -(m * ir3) - T(2.5) * t5 * ir2;
+    T phi = m * ir + T(0.5) * t5;
+    T gx = ax + Qx * ir5, gy = ay + Qy * ir5, gz = az + Qz * ir5;
+    if (OCT) {
+        const T xx = x * x, yy = y * y, zz = z * z;
+        const T xy = x * y, xz = x * z, yz = y * z;
+        const T Hx = oc[0] * xx + oc[1] * xy + oc[2] * xz + oc[3] * yy + oc[4] * zz + oc[5] * yz;
+        const T Hy = oc[6] * xx + oc[7] * xy + oc[8] * xz + oc[9] * yy + oc[10] * zz + oc[11] * yz;
+        const T Hz =
+            oc[12] * xx + oc[13] * xy + oc[14] * xz + oc[15] * yy + oc[16] * zz + oc[17] * yz;
+        const T rh = (x * Hx + y * Hy + z * Hz) * ir7;   // (r.H) / r^7
+        crad = crad - T(7.0 / 3.0) * rh * ir2;
+        phi = phi + T(1.0 / 3.0) * rh;
+        gx = gx + Hx * ir7;
+        gy = gy + Hy * ir7;
+        gz = gz + Hz * ir7;
+    }
+    ax = gx + crad * x;
+    ay = gy + crad * y;
+    az = gz + crad * z;
+    pot = pot + phi;
+}
+
+// The same terms from the moments themselves (the FP32 walk, whose nodes keep
+// 10 moments instead of 18 coefficients to stay at two cache lines):
+// G = grad S = 2 H, S formed as (r.G) / 3.
+//   s = [x^3 x^2y y^2x y^3 z^2x z^2y z^3 x^2z y^2z xyz]
+template <int OCT, class T>
+__device__ __forceinline__ void cell_terms_s(T x, T y, T z, T ir, T m, const T *q, const T *s,
+                                             T &ax, T &ay, T &az, T &pot) {
+#pragma clang fp contract(fast)
+    const T ir2 = ir * ir;
+    const T ir3 = ir2 * ir;
+    const T ir5 = ir3 * ir2;
+    const T ir7 = ir5 * ir2;
+    const T Qx = q[0] * x + q[1] * y + q[3] * z;
+    const T Qy = q[1] * x + q[2] * y + q[4] * z;
+    const T Qz = q[3] * x + q[4] * y + q[5] * z;
     const T qs = x * Qx + y * Qy + z * Qz;
     T crad = -(m * ir3) - T(2.5) * qs * ir7;
     T phi = m * ir + T(0.5) * qs * ir5;
@@ -199,7 +234,7 @@ template <int OCT>
 __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], double drsqd,
                                           double &ax, double &ay, double &az, double &pot) {
     const double ir = rsqrt_fp64(drsqd + SMALL_NUMBER);
-    cell_terms<OCT, double>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
+    cell_terms<OCT, double>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.oc, ax, ay, az, pot);
 }
 
 template <int OCT, int MACT, bool QUINTIC>
@@ -331,7 +366,7 @@ __device__ __forceinline__ void cell_fp32(const GNodeF &g, const float dr[3], fl
 #pragma clang fp contract(fast)
     const float drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
     const float ir = __builtin_amdgcn_rsqf(drsqd + 1.0e-30f);
-    cell_terms<OCT, float>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
+    cell_terms_s<OCT, float>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.s, ax, ay, az, pot);
 }
 
 // Cell terms are summed in FP32 over batches of CELL_BATCH cell visits of the

@@ -114,8 +114,11 @@ struct __attribute__((aligned(64))) GNode {
     int leaf;
     int first;
     int count;        // particles in the subtree
-    double q[5];      // quadrupole [xx xy yy zx zy]
-    double s[10];     // octupole
+    double q[6];      // quadrupole [xx xy yy zx zy zz], zz = -(xx + yy)
+    // octupole gradient coefficients of the walk: H = grad(S) / 2 with S the
+    // octupole cubic, H_a = sum_k oc[6a + k] mono_k over the monomials
+    // (xx, xy, xz, yy, zz, yz); the moments s themselves are in Ctx::gs
+    double oc[18];
 };
 
 // Hydro node (BHhydro_node, modules.F90:929-937), DFS order.
@@ -140,11 +143,11 @@ struct __attribute__((aligned(64))) GNodeF {
     int leaf;
     int first;
     int pad;
-    float q[5];
-    float s[10];
+    float q[6];       // [xx xy yy zx zy zz]
+    float s[10];      // octupole moments (the FP32 walk forms grad S from them)
 };
 
-static_assert(offsetof(GNode, q) == 64 && sizeof(GNode) == 192, "GNode layout");
+static_assert(offsetof(GNode, q) == 64 && sizeof(GNode) == 256, "GNode layout");
 static_assert(offsetof(GNodeF, q) <= 64 && sizeof(GNodeF) == 128, "GNodeF layout");
 static_assert(sizeof(HNode) == 64, "HNode layout");
 
@@ -212,6 +215,7 @@ struct Ctx {
         *c_level = nullptr, *bfs2dfs = nullptr, *dfs2bfs = nullptr, *dfs_parent = nullptr;
     double *bbmin = nullptr, *bbmax = nullptr, *hmaxs = nullptr;  // per DFS cell
     GNode *gn = nullptr;
+    double *gs = nullptr;        // octupole moments s[10] per node (stock, export)
     GNodeF *gnf = nullptr;
     HNode *hn = nullptr;
     int nc = 0, ltot = 0;

@@ -378,6 +378,7 @@ __device__ __forceinline__ void octupole_shift_exact(double *sc, const double *c
 // moments, COM and opening distances are bit-identical.
 __global__ void __launch_bounds__(256) k_stock(
     int nc, int base, int cnt, const int *__restrict__ bfs2dfs, GNode *__restrict__ gn,
+    double *__restrict__ gs,
     HNode *__restrict__ hn, double *__restrict__ bbmin, double *__restrict__ bbmax,
     double *__restrict__ hmaxs, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ h,
@@ -466,11 +467,12 @@ __global__ void __launch_bounds__(256) k_stock(
             qc[2] = qc[2] + mcc * (3.0 * dr[1] * dr[1] - drsqd);
             qc[3] = qc[3] + mcc * (3.0 * dr[2] * dr[0]);
             qc[4] = qc[4] + mcc * (3.0 * dr[2] * dr[1]);
+            const double *ks = gs + 10 * (size_t)cc;
             if (octupole == 2) {
-                octupole_shift_exact(sc, k.s, q, mcc, dr, drsqd);
+                octupole_shift_exact(sc, ks, q, mcc, dr, drsqd);
             } else if (octupole) {
                 // parallel-axis shift exactly as BHgrav_stock.F90:263-297
-                for (int d = 0; d < 10; d++) sc[d] = sc[d] + k.s[d];
+                for (int d = 0; d < 10; d++) sc[d] = sc[d] + ks[d];
                 sc[0] = sc[0] + mcc * dr[0] * (5.0 * dr[0] * dr[0] - 3.0 * drsqd) +
                         1.5 * dr[0] * q[0] - dr[1] * q[1] - dr[2] * q[3];
                 sc[1] = sc[1] + mcc * dr[1] * (15.0 * dr[0] * dr[0] - 3.0 * drsqd) +
@@ -534,8 +536,17 @@ __global__ void __launch_bounds__(256) k_stock(
     g.dminsqd = dminsqd;
     g.mac = macv;
     for (int k = 0; k < 5; k++) g.q[k] = qc[k];
-    for (int k = 0; k < 10; k++) g.s[k] = octupole ? sc[k] : 0.0;
-    if (octupole == 2) g.s[9] = 2.0 * sc[9];   // the walk's xyz coefficient is 1
+    g.q[5] = -(qc[0] + qc[2]);
+    double sv[10];
+    for (int k = 0; k < 10; k++) sv[k] = octupole ? sc[k] : 0.0;
+    if (octupole == 2) sv[9] = 2.0 * sc[9];   // the walk's xyz coefficient is 1
+    for (int k = 0; k < 10; k++) gs[10 * (size_t)c + k] = sv[k];
+    // H = grad(S) / 2 with S = s0 x^3 + s1 x^2y + s2 y^2x + s3 y^3 + s4 z^2x
+    //   + s5 z^2y + s6 z^3 + s7 x^2z + s8 y^2z + s9 xyz, on (xx xy xz yy zz yz)
+    const double oc[18] = {1.5 * sv[0], sv[1], sv[7], 0.5 * sv[2], 0.5 * sv[4], 0.5 * sv[9],
+                           0.5 * sv[1], sv[2], 0.5 * sv[9], 1.5 * sv[3], 0.5 * sv[5], sv[8],
+                           0.5 * sv[7], 0.5 * sv[9], sv[4], 0.5 * sv[8], 1.5 * sv[6], sv[5]};
+    for (int k = 0; k < 18; k++) g.oc[k] = oc[k];
     for (int k = 0; k < 3; k++) {
         bbmin[3 * c + k] = bmn[k];
         bbmax[3 * c + k] = bmx[k];
@@ -559,7 +570,8 @@ __global__ void __launch_bounds__(256) k_stock(
     }
 }
 
-__global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn, GNodeF *__restrict__ gf) {
+__global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn,
+                                 const double *__restrict__ gs, GNodeF *__restrict__ gf) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const GNode &g = gn[c];
@@ -568,8 +580,8 @@ __global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn, GNodeF *_
     f.dminsqd = g.dminsqd;
     f.mac = g.mac;
     f.m = (float)g.m;
-    for (int k = 0; k < 5; k++) f.q[k] = (float)g.q[k];
-    for (int k = 0; k < 10; k++) f.s[k] = (float)g.s[k];
+    for (int k = 0; k < 6; k++) f.q[k] = (float)g.q[k];
+    for (int k = 0; k < 10; k++) f.s[k] = (float)gs[10 * (size_t)c + k];
     f.next = g.next;
     f.leaf = g.leaf;
     f.first = g.first;
@@ -659,6 +671,7 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.bbmax, 3 * cc));
     SX_TRY(dalloc(&c.hmaxs, cc));
     SX_TRY(dalloc(&c.gn, cc));
+    SX_TRY(dalloc(&c.gs, cc * 10));
     SX_TRY(dalloc(&c.gnf, cc));
     SX_TRY(dalloc(&c.hn, cc));
     // temporary storage for sort and scans
@@ -862,12 +875,13 @@ int stock_tree(Ctx &c) {
         const int cnt = c.level_cnt[L];
         if (cnt == 0) continue;
         hipLaunchKernelGGL(k_stock, dim3(nblk(cnt)), dim3(TB), 0, s, c.nc, c.level_off[L], cnt,
-                           c.bfs2dfs, c.gn, c.hn, c.bbmin, c.bbmax, c.hmaxs, c.x, c.y, c.z, c.m,
+                           c.bfs2dfs, c.gn, c.gs, c.hn, c.bbmin, c.bbmax, c.hmaxs, c.x, c.y, c.z, c.m,
                            c.h, oct, c.opt.mac, c.opt.thetamaxsqd, c.opt.abserror, c.daux,
                            c.kg.range);
     }
     if (c.opt.precision == SERENX_MIXED)
-        hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gnf);
+        hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gs,
+                           c.gnf);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
@@ -875,7 +889,7 @@ int stock_tree(Ctx &c) {
 int refresh_f32_nodes(Ctx &c) {
     if (!c.tree_ok || c.nc <= 0) return SERENX_OK;
     hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, c.gn,
-                       c.gnf);
+                       c.gs, c.gnf);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
