@@ -225,6 +225,10 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.ovf_off, nn));
     SX_TRY(dalloc(&c.ovf_idx, nn));
     SX_TRY(dalloc(&c.hrec, nn * 16));
+    SX_TRY(dalloc(&c.grec, nn * 8));
+    SX_TRY(dalloc(&c.gax, nn));
+    SX_TRY(dalloc(&c.gay, nn));
+    SX_TRY(dalloc(&c.gaz, nn));
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     SX_TRY(ensure_tree_buffers(c));
     return SERENX_OK;
@@ -242,13 +246,18 @@ void free_all(Ctx &c) {
                     c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
-                    c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band};
+                    c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band,
+                    c.grec, c.gax, c.gay, c.gaz};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
     if (c.hostpin_d) (void)hipHostFree(c.hostpin_d);
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c.evg)
+        if (e) (void)hipEventDestroy(e);
+    if (c.evfork) (void)hipEventDestroy(c.evfork);
+    if (c.gstream) (void)hipStreamDestroy(c.gstream);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -286,7 +295,7 @@ int check_options(const serenx_options &o) {
 size_t ctx_device_bytes(const Ctx &c) {
     const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
     size_t b = 0;
-    b += nn * (24 * 8 + 16 * 8);                            // particle SoA, tmpd, records
+    b += nn * (24 * 8 + 16 * 8 + 11 * 8);                   // SoA, tmpd, hydro/gravity records
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
@@ -308,19 +317,29 @@ int run_tree_update(Ctx &c) {
     return SERENX_OK;
 }
 
-int run_sph_update(Ctx &c) {
+// sph_update.F90:55-101 (grad-h) and :106-152 (standard SPH), in two parts
+// so that the gravity walk, which needs only h, can start between them
+int run_sph_h(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "sph_update called before tree_update");
-    if (c.opt.sph == SERENX_SPH_STANDARD) {   // sph_update.F90:106-152
+    c.sph_ok = false;
+    if (c.opt.sph == SERENX_SPH_STANDARD)
         SX_TRY(sph_hgather(c));
-        SX_TRY(update_hmax(c));
-        SX_TRY(sph_neighbours(c));
-        SX_TRY(sph_density(c));
-    } else {                                  // sph_update.F90:55-101
+    else
         SX_TRY(sph_hsolve(c));
-        SX_TRY(update_hmax(c));
-        SX_TRY(sph_neighbours(c));
-    }
+    SX_TRY(update_hmax(c));
+    return SERENX_OK;
+}
+
+int run_sph_lists(Ctx &c) {
+    SX_TRY(sph_neighbours(c));
+    if (c.opt.sph == SERENX_SPH_STANDARD) SX_TRY(sph_density(c));
     c.sph_ok = true;
+    return SERENX_OK;
+}
+
+int run_sph_update(Ctx &c) {
+    SX_TRY(run_sph_h(c));
+    SX_TRY(run_sph_lists(c));
     return SERENX_OK;
 }
 
@@ -334,7 +353,8 @@ int run_hydro(Ctx &c) {
 
 int run_grav(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "gravity needs tree_update first");
-    SX_TRY(grav_forces(c));
+    SX_TRY(grav_launch(c));
+    SX_TRY(grav_join(c));
     return SERENX_OK;
 }
 
@@ -462,9 +482,14 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
                        : (int)((4.0 * PI_REF / 3.0) * ipow_seq(c->kg.range * opt->h_fac, 3));
     c->pp_limit = 2 * c->pp_gather;
     c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
+    // two streams at the default priority: the main one and the gravity walk's
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking);
     for (auto &ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
+    for (auto &ev : c->evg)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * KT_TOTAL);
     if (e == hipSuccess) e = hipMalloc((void **)&c->icount, CTR_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->icount, 0, CTR_WORDS * sizeof(unsigned long long));
@@ -739,7 +764,13 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     SX_TRY(run_tree_update(c));
     if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
     SX_HIP(hipEventRecord(c.ev[1], c.stream));
-    SX_TRY(run_sph_update(c));
+    // sph_update (h), then the gravity walk on its own stream beside the
+    // neighbour lists, thermal and hydro forces (sph_setup's order otherwise)
+    const bool grav = !(flags & SERENX_EVAL_NO_GRAV);
+    c.grav_timed = false;
+    SX_TRY(run_sph_h(c));
+    if (grav) SX_TRY(grav_launch(c));
+    SX_TRY(run_sph_lists(c));
     SX_TRY(sph_thermal(c));
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
     if (flags & SERENX_EVAL_NO_HYDRO) {
@@ -749,7 +780,7 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
         SX_TRY(run_hydro(c));
     }
     SX_HIP(hipEventRecord(c.ev[3], c.stream));
-    if (!(flags & SERENX_EVAL_NO_GRAV)) SX_TRY(run_grav(c));
+    if (grav) SX_TRY(grav_join(c));
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     return SERENX_OK;
 }
@@ -765,11 +796,16 @@ int serenx_last_timings(const serenx_ctx *ctx, double *ms4) {
     if (!ctx || !ms4) return set_error(SERENX_EINVAL, "null argument");
     const Ctx &c = *as_ctx(ctx);
     SX_HIP(hipEventSynchronize(c.ev[4]));
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 3; k++) {
         float ms = 0.0f;
         SX_HIP(hipEventElapsedTime(&ms, c.ev[k], c.ev[k + 1]));
         ms4[k] = ms;
     }
+    // gravity: the walk's own span on the gravity stream (it overlaps the
+    // SPH and hydro phases)
+    float gms = 0.0f;
+    if (c.grav_timed) SX_HIP(hipEventElapsedTime(&gms, c.evg[0], c.evg[1]));
+    ms4[3] = gms;
     return SERENX_OK;
 }
 

@@ -118,8 +118,9 @@ __device__ __forceinline__ unsigned pair_fp64(const KGeom &kg, const double4 *__
     return inside;
 }
 
-// Gravity source record of particle j, 64 bytes in the second half of the
-// 128-byte particle record: [2] x y z m  [3] h 1/h zeta KERNRANGE*h
+// Gravity source record of particle j, 64 bytes in a buffer of its own (the
+// walk runs beside the neighbour and hydro kernels, which use theirs):
+// [0] x y z m  [1] h 1/h zeta KERNRANGE*h
 __global__ void k_pack_grav(int n, double kernrange, const double *__restrict__ x,
                             const double *__restrict__ y,
                             const double *__restrict__ z, const double *__restrict__ m,
@@ -127,9 +128,22 @@ __global__ void k_pack_grav(int n, double kernrange, const double *__restrict__ 
                             double4 *__restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double4 *r = rec + 4 * (size_t)i;
-    r[2] = make_double4(x[i], y[i], z[i], m[i]);
-    r[3] = make_double4(h[i], 1.0 / h[i], zeta[i], kernrange * h[i]);
+    double4 *r = rec + 2 * (size_t)i;
+    r[0] = make_double4(x[i], y[i], z[i], m[i]);
+    r[1] = make_double4(h[i], 1.0 / h[i], zeta[i], kernrange * h[i]);
+}
+
+// a += a_grav for the active particles: the one addition sph_grav_forces.F90
+// makes (a holds the hydro acceleration)
+__global__ void k_add_grav(int n, const uint8_t *__restrict__ active,
+                           const double *__restrict__ gx, const double *__restrict__ gy,
+                           const double *__restrict__ gz, double *__restrict__ ax,
+                           double *__restrict__ ay, double *__restrict__ az) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !active[i]) return;
+    ax[i] = ax[i] + gx[i];
+    ay[i] = ay[i] + gy[i];
+    az[i] = az[i] + gz[i];
 }
 
 // 1/sqrt(a) for a > 0 to about one ulp: the hardware estimate (relative
@@ -250,7 +264,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     const bool valid = p < n && active[p];
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
     if (valid) {
-        const double4 a = rec[4 * (size_t)p + 2], b = rec[4 * (size_t)p + 3];
+        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
         rpx = a.x; rpy = a.y; rpz = a.z;
         invhp = b.y;
         zo_p = b.z;
@@ -280,7 +294,7 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
         [&](const GNode &g, bool mine) {
             for (int i = 0; i < g.leaf; i++) {
                 const int j = g.first + i;
-                const double4 a = rec[4 * (size_t)j + 2], b = rec[4 * (size_t)j + 3];
+                const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
                 const bool act = mine && j != p;
                 unsigned in_ = 0;
                 if (act)
@@ -292,9 +306,9 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
             }
         });
     if (valid) {
-        ax[p] = ax[p] + gx;
-        ay[p] = ay[p] + gy;
-        az[p] = az[p] + gz;
+        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+        ay[p] = gy;
+        az[p] = gz;
         gpot[p] = gp;
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
@@ -398,7 +412,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
     float invhp = 1.0f, zo_p = 0.0f;
     if (valid) {
-        const double4 a = rec[4 * (size_t)p + 2], b = rec[4 * (size_t)p + 3];
+        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
         rpx = a.x; rpy = a.y; rpz = a.z;
         invhp = (float)b.y;
         zo_p = (float)b.z;
@@ -442,7 +456,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         } else if (g.leaf > 0) {
             for (int i = 0; i < g.leaf; i++) {
                 const int j = g.first + i;
-                const double4 a = rec[4 * (size_t)j + 2], b = rec[4 * (size_t)j + 3];
+                const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
                 const bool act = mine && j != p;
                 unsigned in_ = 0;
                 if (act)
@@ -461,9 +475,9 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
     if (valid) {
         const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
-        ax[p] = ax[p] + gx;
-        ay[p] = ay[p] + gy;
-        az[p] = az[p] + gz;
+        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+        ay[p] = gy;
+        az[p] = gz;
         gpot[p] = cp + pq.value();
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
@@ -533,20 +547,20 @@ template <int OCT, int MACT, bool QUINTIC>
 void launch_walk(Ctx &c, const double4 *rec) {
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL((k_grav_mixed<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
-                           c.stream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
-                           reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.ax, c.ay,
-                           c.az, c.gpot, c.agm, c.icount, c.wstat);
+                           c.gstream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
+                           reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
+                           c.gaz, c.gpot, c.agm, c.icount, c.wstat);
     else
         hipLaunchKernelGGL((k_grav_fp64<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
-                           c.stream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
-                           reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.ax, c.ay,
-                           c.az, c.gpot, c.agm, c.icount, c.wstat);
+                           c.gstream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
+                           reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
+                           c.gaz, c.gpot, c.agm, c.icount, c.wstat);
 }
 
 template <int OCT, int MACT>
 void launch_grav(Ctx &c) {
-    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
-    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.kg.range, c.x,
+    double4 *rec = reinterpret_cast<double4 *>(c.grec);
+    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range, c.x,
                        c.y, c.z, c.m, c.h, c.zeta, rec);
     if (c.kg.range == walk_geom<true>().range)
         launch_walk<OCT, MACT, true>(c, rec);
@@ -556,8 +570,32 @@ void launch_grav(Ctx &c) {
 
 }  // namespace
 
+// The gravity walk on the context's gravity stream, after everything queued
+// on its main stream so far (the tree and h): it reads the tree, positions,
+// masses, h and zeta, and writes its own buffers (gravity records and
+// acceleration, gpot, agravmag, the interaction counts), so it can run beside
+// the neighbour lists, thermal and hydro forces.  grav_join adds its
+// acceleration to a on the main stream.
+int grav_launch(Ctx &c) {
+    SX_HIP(hipEventRecord(c.evfork, c.stream));
+    SX_HIP(hipStreamWaitEvent(c.gstream, c.evfork, 0));
+    SX_HIP(hipEventRecord(c.evg[0], c.gstream));
+    SX_TRY(grav_forces(c));
+    SX_HIP(hipEventRecord(c.evg[1], c.gstream));
+    c.grav_timed = true;
+    return SERENX_OK;
+}
+
+int grav_join(Ctx &c) {
+    SX_HIP(hipStreamWaitEvent(c.stream, c.evg[1], 0));
+    hipLaunchKernelGGL(k_add_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active, c.gax,
+                       c.gay, c.gaz, c.ax, c.ay, c.az);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
 int grav_forces(Ctx &c) {
-    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
+    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.gstream));
     const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
     switch (c.opt.mac) {
     case SERENX_MAC_GEOMETRIC:

@@ -160,6 +160,10 @@ struct Ctx {
     serenx_options opt{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t gstream = nullptr;   // gravity walk, beside the SPH kernels
+    hipEvent_t evfork = nullptr;     // fork of the gravity stream from the main one
+    hipEvent_t evg[2] = {};          // gravity walk start and end (timed)
+    bool grav_timed = false;         // evg holds a walk of the last evaluation
     hipEvent_t ev[8] = {};
     double last_ms[4] = {0, 0, 0, 0};
 
@@ -179,6 +183,8 @@ struct Ctx {
     int *porig = nullptr;
     uint8_t *active = nullptr;   // accdo
     double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
+    double *grec = nullptr;      // gravity source records, 8 doubles per particle
+    double *gax = nullptr, *gay = nullptr, *gaz = nullptr;   // gravity acceleration
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     double *band = nullptr;      // standard SPH: BAND_MAX x 64 r^2 values per wave
     size_t band_cap = 0;         // particles the band buffer holds
@@ -321,7 +327,11 @@ int run_hydro(Ctx &c);
 int run_grav(Ctx &c);
 
 // grav.hip
-int grav_forces(Ctx &c);         // BHgrav_accel over active particles
+int grav_forces(Ctx &c);         // BHgrav_accel over active particles (gravity stream)
+int grav_launch(Ctx &c);         // fork the gravity stream and run the walk on it
+int grav_join(Ctx &c);           // main stream waits for the walk and adds a_grav to a
+int run_sph_h(Ctx &c);           // sph_update, part 1: h (and hmax) of active particles
+int run_sph_lists(Ctx &c);       // sph_update, part 2: neighbour lists (+ standard density)
 int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot);
 
 }  // namespace serenx

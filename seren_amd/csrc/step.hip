@@ -564,8 +564,13 @@ int step_once(Ctx &c) {
     // (the first step, the half steps of a one-level leapfrog) they change
     // nothing but the tree's hrangemax, which the next sph_update recomputes
     // from the same h, and it is kept current here all the same
+    // the gravity walk runs on its own stream beside the neighbour lists,
+    // thermal and hydro forces, from the new h on
+    c.grav_timed = false;
     if (s.nactive > 0) {
-        SX_TRY(run_sph_update(c));
+        SX_TRY(run_sph_h(c));
+        SX_TRY(grav_launch(c));
+        SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));
     } else {
         SX_TRY(update_hmax(c));
@@ -573,7 +578,7 @@ int step_once(Ctx &c) {
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
     if (s.nactive > 0) SX_TRY(run_hydro(c));
     SX_HIP(hipEventRecord(c.ev[3], c.stream));
-    if (s.nactive > 0) SX_TRY(run_grav(c));
+    if (s.nactive > 0) SX_TRY(grav_join(c));
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     SX_TRY(step_timesteps(c));
     // every level 0 (and nminneib >= 0 without sinks): the check changes nothing
