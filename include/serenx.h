@@ -116,7 +116,10 @@ typedef struct serenx_options {
     int32_t pp_gather;   /* SERENX_SPH_STANDARD: neighbours per h (params
                             'pp_gather'); 0 takes int((4 pi/3)(range h_fac)^3).
                             Must be 0 for grad-h SPH, where h_fac sets it. */
-    int32_t reserved;
+    int32_t serial;      /* 0: the gravity walk runs on a second stream beside the
+                            neighbour and hydro kernels; 1: on the context's one
+                            stream (for callers that run many contexts at once,
+                            whose streams share the device's hardware queues) */
 } serenx_options;
 
 /* Fills SEREN's parameter defaults (default_parameters.F90:62-117,173):

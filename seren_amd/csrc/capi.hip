@@ -257,7 +257,7 @@ void free_all(Ctx &c) {
     for (auto &e : c.evg)
         if (e) (void)hipEventDestroy(e);
     if (c.evfork) (void)hipEventDestroy(c.evfork);
-    if (c.gstream) (void)hipStreamDestroy(c.gstream);
+    if (c.gstream && c.gstream != c.stream) (void)hipStreamDestroy(c.gstream);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -285,6 +285,7 @@ int check_options(const serenx_options &o) {
         return set_error(SERENX_EINVAL, "sph must be SERENX_SPH_GRADH or SERENX_SPH_STANDARD");
     if (o.sph == SERENX_SPH_STANDARD && o.thermal != SERENX_THERMAL_ISOTHERMAL)
         return set_error(SERENX_EINVAL, "standard SPH is built for the isothermal option only");
+    if (o.serial != 0 && o.serial != 1) return set_error(SERENX_EINVAL, "serial must be 0 or 1");
     if (o.pp_gather < 0 || (o.sph == SERENX_SPH_GRADH && o.pp_gather != 0))
         return set_error(SERENX_EINVAL,
                          "pp_gather must be 0 for grad-h SPH (h_fac sets it) and >= 0 otherwise");
@@ -484,7 +485,12 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
     // two streams at the default priority: the main one and the gravity walk's
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        if (opt->serial)
+            c->gstream = c->stream;
+        else
+            e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking);
+    }
     for (auto &ev : c->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     for (auto &ev : c->evg)

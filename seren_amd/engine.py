@@ -72,7 +72,7 @@ class Options(C.Structure):
                 ("kernel", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
                 ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double),
                 ("thermal", C.c_int32), ("sph", C.c_int32), ("gamma", C.c_double),
-                ("pp_gather", C.c_int32), ("reserved", C.c_int32)]
+                ("pp_gather", C.c_int32), ("serial", C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -177,11 +177,13 @@ class ForceEngine:
                  precision: str = "fp64", h_fac: float = 1.2, thetamaxsqd: float = 0.25,
                  abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
                  device: int = 0, kernel: str = "m4", thermal: str = "isothermal",
-                 gamma: float = 1.4, sph: str = "gradh", pp_gather: int = 0):
+                 gamma: float = 1.4, sph: str = "gradh", pp_gather: int = 0,
+                 serial: bool = False):
         """thermal: "isothermal" (P = c_s^2 rho with sound_speed) or "energy"
         (the energy equation: set u with set_energy).  sph: "gradh" or
         "standard" (h from the pp_gather-th neighbour; pp_gather 0 takes the
-        h_fac value)."""
+        h_fac value).  serial: keep the gravity walk on the context's one
+        stream (many concurrent contexts)."""
         self.lib = load_library()
         opt = Options()
         _check(self.lib.serenx_default_options(C.byref(opt)))
@@ -198,6 +200,7 @@ class ForceEngine:
         opt.gamma = gamma
         opt.sph = _SPH[sph]
         opt.pp_gather = pp_gather
+        opt.serial = int(serial)
         self.opt = opt
         self.ctx = C.c_void_p()
         _check(self.lib.serenx_create(C.byref(opt), device, C.byref(self.ctx)))

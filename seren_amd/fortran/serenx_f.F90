@@ -21,7 +21,7 @@ MODULE serenx_f
      real(c_double) :: h_fac, thetamaxsqd, abserror, alpha, sound_speed
      integer(c_int32_t) :: thermal, sph
      real(c_double) :: gamma
-     integer(c_int32_t) :: pp_gather, reserved
+     integer(c_int32_t) :: pp_gather, serial
   end type serenx_options
 
   type, bind(C) :: serenx_host_state

@@ -91,7 +91,8 @@ class CloudServer:
             self.used += need
         eng = None
         try:
-            eng = ForceEngine(precision=self.precision, **self.engine_kw)
+            # many clouds run at once: each walk stays on its cloud's one stream
+            eng = ForceEngine(precision=self.precision, serial=True, **self.engine_kw)
             eng.load(pos, vel, mass, h)
             eng.force_eval(hguess=True)
             eng.step_begin(**self.step_kw)
