@@ -161,8 +161,16 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
             act = WALK_SKIP;
         }
         if (__any(act == WALK_LEAF)) leaf(nd, act == WALK_LEAF);
+        // the next cell is the lowest resume pointer.  A lane descending makes
+        // it c + 1.  Otherwise no lane's pointer lies inside c's subtree
+        // (c, next): a lane reaches such a cell only through c itself, since
+        // subtrees are contiguous and nest, so a lane that was at c and moved
+        // on to c's next (not past its tree's end) holds the minimum; only
+        // when none did is the wave minimum reduced.
         if (__any(act == WALK_DESCEND))
             c = c + 1;
+        else if (__any(here && resume == nd.next))
+            c = nd.next;
         else
             c = uniform(wave_min_int(resume));
     }
