@@ -1,0 +1,61 @@
+"""GPU: the gravity walk's own stream changes no result.
+
+By default the walk runs on a second stream beside the neighbour-list,
+thermal and hydro kernels and its acceleration is added to a afterwards;
+with serial=True it runs on the context's one stream.  Both must give the
+same state bit for bit, for force evaluations and leapfrog steps, in FP64
+and in mixed precision, with the energy equation's du/dt too: any missing
+dependency between the two streams would show up as a difference.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from helpers import CS, cloud  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("porig", "pos", "v", "h", "zeta", "rho", "omega", "div_v", "a", "gpot", "agravmag",
+        "pptot")
+
+
+def _run(serial, precision, thermal, pos, vel, mass):
+    from seren_amd.engine import ForceEngine
+    eng = ForceEngine(multipole="octupole", mac="gadget2", precision=precision,
+                      thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS, thermal=thermal,
+                      serial=serial)
+    try:
+        n = len(mass)
+        eng.load(pos, vel, mass, np.zeros(n))
+        if thermal == "energy":
+            eng.set_energy(np.full(n, CS * CS / 0.4))
+        eng.force_eval(hguess=True)
+        eng.force_eval()
+        ev = eng.fetch()
+        eng.step_begin("lfkdk", nlevels=3, nbuildstep=4)
+        eng.step(6)
+        st = eng.fetch()
+        out = {"eval_" + k: ev[k] for k in KEYS}
+        out.update({"step_" + k: st[k] for k in KEYS})
+        if thermal == "energy":
+            out["u"], out["du_dt"] = eng.fetch_energy()
+        return out
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("precision,thermal", [("fp64", "isothermal"), ("mixed", "isothermal"),
+                                               ("fp64", "energy")])
+def test_overlapped_walk_matches_serial(precision, thermal):
+    print(f"case {precision} {thermal}", flush=True)
+    pos, vel, mass, h = cloud(30000, "r2", 41)
+    a = _run(False, precision, thermal, pos, vel, mass)
+    b = _run(True, precision, thermal, pos, vel, mass)
+    for k in a:
+        ne = int(np.sum(np.asarray(a[k]) != np.asarray(b[k])))
+        if ne:
+            print(f"  {k}: {ne} differ", flush=True)
+        assert ne == 0, k
