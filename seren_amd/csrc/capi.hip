@@ -635,6 +635,7 @@ int serenx_sph_update(serenx_ctx *ctx) {
     SX_HIP(hipSetDevice(c.device));
     const int cols[] = {0, 1, 2, 3, 4};
     SX_TRY(up_parray(c, cols, 5));
+    c.box_ok = false;   // the host's positions may have moved since its last stock
     SX_TRY(up_vec3(c, c.hs.v, c.vx, c.vy, c.vz));
     SX_TRY(up_accdo(c));
     SX_TRY(run_sph_update(c));

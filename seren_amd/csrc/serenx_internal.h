@@ -129,7 +129,8 @@ struct __attribute__((aligned(64))) HNode {
     int next;
     int leaf;
     int first;
-    int count;
+    float half[3];    // bounding-box half-extents, rounded up with a margin: every
+                      // particle of the subtree lies within r +- half
 };
 
 // Mixed-precision gravity node: FP64 COM (the cell offset is subtracted in
@@ -164,6 +165,8 @@ struct Ctx {
     hipEvent_t evfork = nullptr;     // fork of the gravity stream from the main one
     hipEvent_t evg[2] = {};          // gravity walk start and end (timed)
     bool grav_timed = false;         // evg holds a walk of the last evaluation
+    bool box_ok = false;             // HNode boxes hold the current positions (stocked
+                                     // since the positions last changed)
     hipEvent_t ev[8] = {};
     double last_ms[4] = {0, 0, 0, 0};
 

@@ -45,7 +45,19 @@ __global__ void k_pack_xyzh(int n, const double *__restrict__ x, const double *_
     if (i < n) rec[i] = make_double4(x[i], y[i], z[i], h[i]);
 }
 
-// HNode visit of BHhydrowalk_hgather.F90:40 for a search radius R
+// Squared distance from the point at offset (dx, dy, dz) from a node's box
+// centre to the node's (grown) bounding box: a lower bound of the distance to
+// any particle of the subtree.
+__device__ __forceinline__ double box_dist2(const HNode &nd, double dx, double dy, double dz) {
+    const double ex = fmax(fabs(dx) - (double)nd.half[0], 0.0);
+    const double ey = fmax(fabs(dy) - (double)nd.half[1], 0.0);
+    const double ez = fmax(fabs(dz) - (double)nd.half[2], 0.0);
+    return ex * ex + ey * ey + ez * ez;
+}
+
+// HNode visit of BHhydrowalk_hgather.F90:40 for a search radius R (the
+// reference's bounding-sphere test, so the leaves it yields are the
+// reference's potential list, whose size the h solve reads)
 __device__ __forceinline__ int hgather_visit(const HNode &nd, double rpx, double rpy, double rpz,
                                              double R) {
     const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
@@ -53,6 +65,16 @@ __device__ __forceinline__ int hgather_visit(const HNode &nd, double rpx, double
     const double lim = (R + nd.rmax) * (R + nd.rmax);
     if (!(drsqd <= lim)) return WALK_SKIP;
     return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
+}
+
+// Whether a leaf of that list can hold a particle within R: with fresh boxes
+// (box) a leaf whose box lies farther than R holds none, and its particles
+// are left out of the candidate lists (they cannot contribute, so the
+// candidates that remain come in the reference's order)
+__device__ __forceinline__ bool leaf_near(const HNode &nd, double rpx, double rpy, double rpz,
+                                          double R, bool box) {
+    if (!box) return true;
+    return box_dist2(nd, nd.r[0] - rpx, nd.r[1] - rpy, nd.r[2] - rpz) <= R * R * (1.0 + 1.0e-12);
 }
 
 // Loops every summing lane over its own potential-neighbour list, leaf by
@@ -106,7 +128,7 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
 // list length in nlist (-1: list incomplete) for k_hsolve_final; its list
 // stays in hlist (a lane joins no walk after it has converged).
 __global__ void __launch_bounds__(256) k_hsolve_iter(
-    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const double *__restrict__ roots, int *__restrict__ hlist,
     const uint8_t *__restrict__ active, const double *__restrict__ ktab,
@@ -175,12 +197,14 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
             vis += group_walk(hn, nc, iterating, rr.x, rr.y, lvis, visit,
                               [&](const HNode &nd, bool mine) {
                 if (mine) {
-                    cnt += nd.leaf;
-                    if (nslot < HL_SLOTS)
-                        wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
-                    else
-                        lovf = true;
-                    nslot++;
+                    cnt += nd.leaf;                       // the reference's pp_pot
+                    if (leaf_near(nd, rpx, rpy, rpz, R, box)) {
+                        if (nslot < HL_SLOTS)
+                            wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
+                        else
+                            lovf = true;
+                        nslot++;
+                    }
                 }
             });
             if (iterating) {
@@ -200,10 +224,11 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
         } else {
             vis += group_walk(hn, nc, sum, rr.x, rr.y, lvis, visit,
                               [&](const HNode &nd, bool mine) {
+                const bool use = mine && leaf_near(nd, rpx, rpy, rpz, R, box);
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
                     const double4 a = rec[j];
-                    if (mine && j != p) pair(a.x, a.y, a.z, a.w);
+                    if (use && j != p) pair(a.x, a.y, a.z, a.w);
                 }
             });
             fallback++;
@@ -270,7 +295,7 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
 // all_sph_gradh.F90:310-489: with h from the converged density, the gather
 // sums rho, Omega, div v and zeta over the last potential-neighbour list.
 __global__ void __launch_bounds__(256) k_hsolve_final(
-    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const int *__restrict__ hlist, const uint8_t *__restrict__ active,
     const double *__restrict__ ktab, double h_fac, const double *__restrict__ hhigh,
@@ -294,6 +319,9 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
         hp = h_fac * ref_pow_third(mp / rho[p]);                 // :317
         R = kg.range * hhigh[p];
         nslot = nlist[p];
+        // a final h beyond the list's radius can reach particles past R in
+        // leaves the box test left out: gather over the walk instead
+        if (box && kg.range * hp > R) nslot = -1;
     }
     const double invhp = 1.0 / hp;
     const double hfactor_p = cr_pow3(invhp);
@@ -331,10 +359,11 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
             hn, nc, walk, rr.x, rr.y, lvis,
             [&](const HNode &nd, bool) { return hgather_visit(nd, rpx, rpy, rpz, R); },
             [&](const HNode &nd, bool mine) {
+                const bool use = mine && leaf_near(nd, rpx, rpy, rpz, fmax(R, kg.range * hp), box);
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
                     const double4 a = rec[j];
-                    if (mine && j != p) pair(a.x, a.y, a.z, a.w, j);
+                    if (use && j != p) pair(a.x, a.y, a.z, a.w, j);
                 }
             });
         wave_stats(wstat, 2, vis, lvis);
@@ -360,7 +389,7 @@ constexpr int OVF_PER_WAVE = 1;
 // here (the reference does not store their list either, :115-118) and are
 // listed completely by the OVERFLOW_PASS instance.
 template <bool OVERFLOW_PASS>
-__global__ void __launch_bounds__(256) k_neib(KGeom kg, int n, const int *__restrict__ ovf_idx,
+__global__ void __launch_bounds__(256) k_neib(KGeom kg, bool box, int n, const int *__restrict__ ovf_idx,
                                               int nc,
                                               const HNode *__restrict__ hn,
                                               const double4 *__restrict__ rec,
@@ -405,6 +434,9 @@ __global__ void __launch_bounds__(256) k_neib(KGeom kg, int n, const int *__rest
             const double m1 = (R + nd.rmax) * (R + nd.rmax);
             const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
             if (!(drsqd <= m1 || drsqd <= m2)) return WALK_SKIP;   // BHhydro_walk.F90:43
+            // and the box test at the larger of the two radii (hgather_visit)
+            const double Rb = fmax(R, nd.hrangemax);
+            if (box && box_dist2(nd, dx, dy, dz) > Rb * Rb * (1.0 + 1.0e-12)) return WALK_SKIP;
             return nd.leaf == 0 ? WALK_DESCEND : WALK_LEAF;
         },
         [&](const HNode &nd, bool mine) {
@@ -596,7 +628,7 @@ __global__ void k_thermal_energy(int n, double gamma, const double *__restrict__
 constexpr int HG_ITER_MAX = 400;
 
 __global__ void __launch_bounds__(256) k_hgather(
-    KGeom kg, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
+    KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range, int *__restrict__ hlist,
     double *__restrict__ band, const uint8_t *__restrict__ active, int pp_gather,
     double *__restrict__ h, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
@@ -634,10 +666,11 @@ __global__ void __launch_bounds__(256) k_hgather(
                                   return hgather_visit(nd, rpx, rpy, rpz, R);
                               },
                               [&](const HNode &nd, bool mine) {
+                const bool use = mine && leaf_near(nd, rpx, rpy, rpz, R, box);
                 for (int i = 0; i < nd.leaf; i++) {
                     const int j = nd.first + i;
                     const double4 a = rec[j];
-                    if (mine && j != p) f(j, dist2(a));
+                    if (use && j != p) f(j, dist2(a));
                 }
             });
         }
@@ -664,12 +697,14 @@ __global__ void __launch_bounds__(256) k_hgather(
                               },
                               [&](const HNode &nd, bool mine) {
                 if (mine) {
-                    cnt += nd.leaf;
-                    if (nslot < HL_SLOTS)
-                        wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
-                    else
-                        lovf = true;
-                    nslot++;
+                    cnt += nd.leaf;                       // the reference's pp_pot
+                    if (leaf_near(nd, rpx, rpy, rpz, R, box)) {
+                        if (nslot < HL_SLOTS)
+                            wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
+                        else
+                            lovf = true;
+                        nslot++;
+                    }
                 }
             });
             if (iterating) my_ovf = lovf;
@@ -925,11 +960,11 @@ int sph_hsolve(Ctx &c) {
                        c.vx, c.vy, c.vz, c.m, rec);
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
-    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
+    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.root, c.hlist, c.active, c.ktab, c.opt.h_fac,
                        c.pp_gather, c.h, c.rho, c.tmpd, c.tmpi, c.wstat);
     SX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
+    hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.hlist, c.active, c.ktab, c.opt.h_fac, c.tmpd, c.tmpi, c.h, c.zeta,
                        c.rho, c.omega, c.divv, c.drhodt, c.flag, c.wstat);
     SX_HIP(hipGetLastError());
@@ -953,7 +988,7 @@ int sph_hgather(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.m, rec);
-    hipLaunchKernelGGL(k_hgather, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, c.nc, c.hn,
+    hipLaunchKernelGGL(k_hgather, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.hlist, c.band, c.active, c.pp_gather, c.h,
                        c.flag, c.wstat);
     SX_HIP(hipGetLastError());
@@ -983,7 +1018,7 @@ int sph_neighbours(Ctx &c) {
     // [0] x y z h with the solved h
     hipLaunchKernelGGL(k_pack_xyzh, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.x, c.y, c.z, c.h, rec);
     SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.kg, c.n, c.ovf_idx, c.nc, c.hn,
+    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.kg, c.box_ok, c.n, c.ovf_idx, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
                        c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
@@ -1018,7 +1053,7 @@ int sph_neighbours(Ctx &c) {
         }
         const int nov = c.hostpin[9];   // overflow particles, counted by k_neib<false>
         const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
-        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, nov, c.ovf_idx, c.nc,
+        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, c.box_ok, nov, c.ovf_idx, c.nc,
                            c.hn, rec, c.cid, c.root_range, c.active, c.pp_limit,
                            c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());

@@ -503,6 +503,7 @@ int step_neib_levels(Ctx &c) {
 int step_advance(Ctx &c) {
     Ctx::Step &s = c.st;
     const int n = c.n;
+    c.box_ok = false;   // positions move: the hydro boxes are stale until the next stock
     hipStream_t st = c.stream;
     const bool energy = c.opt.thermal == SERENX_THERMAL_ENERGY_EQN;
     double *u = energy ? c.u : nullptr, *u_old = energy ? s.u_old : nullptr;

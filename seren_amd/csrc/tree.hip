@@ -290,7 +290,6 @@ __global__ void k_dfs(int n, int nc, const int *__restrict__ c_start, const int 
     hn[d].next = nxt;
     hn[d].leaf = c_leaf[c];
     hn[d].first = s;
-    hn[d].count = e - s;
 }
 
 __global__ void k_dfs_parent(int nc, const int *__restrict__ c_parent, const int *__restrict__ bfs2dfs,
@@ -558,6 +557,7 @@ __global__ void __launch_bounds__(256) k_stock(
     if (leaf == 1) {
         for (int k = 0; k < 3; k++) hh.r[k] = bmn[k];
         hh.rmax = 0.0;
+        for (int k = 0; k < 3; k++) hh.half[k] = (float)(1.0e-9 * fabs(bmn[k]) + 1.0e-300);
     } else {
         double hc[3], half[3], dsq = 0.0;
         for (int k = 0; k < 3; k++) {
@@ -567,6 +567,11 @@ __global__ void __launch_bounds__(256) k_stock(
         }
         for (int k = 0; k < 3; k++) hh.r[k] = hc[k];
         hh.rmax = sqrt(dsq);
+        // the walks' box test: half-extents grown by 1e-9 of the coordinates
+        // and 1e-6 relative (covering the rounding of the centre, the
+        // subtraction and the FP32 conversion)
+        for (int k = 0; k < 3; k++)
+            hh.half[k] = (float)((half[k] + 1.0e-9 * (fabs(hc[k]) + half[k])) * (1.0 + 1.0e-6));
     }
 }
 
@@ -882,6 +887,7 @@ int stock_tree(Ctx &c) {
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gs,
                            c.gnf);
+    c.box_ok = true;   // the hydro boxes hold the current positions
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
