@@ -1,7 +1,6 @@
 """Benchmark: particles/s of SEREN integration steps on one MI355X.
 
-Workload (the GPU FP64 configuration of the reference's benchmark suite):
-N = 1,048,576 equal-mass particles, rho ~ r^-2 truncated at r = 1, with a
+Workload: N = 1,048,576 equal-mass particles, rho ~ r^-2 truncated at r = 1, with a
 divergence-free P(k) ~ k^-4 velocity field on a 128^3 grid at rms Mach 2
 (c_s = 0.2), octupole moments and the GADGET2 multipole MAC (abserror 1e-4),
 FP64 throughout.  One "step" (--mode step, the default) is one kick-drift-

@@ -46,8 +46,8 @@ CASES = {
     # M4 with the Thomas-Couchman gradient (TC_KERNEL)
     "uniform1k_quad_geo_tc": (1000, "uniform", 106, 1.0, False, "geometric", 0.25, 1e-4, 1, True,
                               "m4tc"),
-    # configuration 1 of the benchmark suite: N = 16384 uniform sphere, quadrupole,
-    # geometric MAC theta = 0.5, one force evaluation (digests for big arrays)
+    # the reference's CPU-sized case: N = 16384 uniform sphere, quadrupole, geometric
+    # MAC theta = 0.5, one force evaluation (digests for big arrays)
     "c1_uniform16k_quad_geo": (16384, "uniform", 1, 0.0, False, "geometric", 0.25, 1e-4, 1,
                                False),
     # THERMAL=ENERGY_EQN: ideal gas from u, du/dt
