@@ -260,6 +260,9 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     double *__restrict__ gpot, double *__restrict__ agm,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     constexpr KGeom kg = walk_geom<QUINTIC>();
+    // the walk is the critical path while the neighbour and hydro kernels run
+    // beside it: its waves take issue priority over theirs on shared SIMDs
+    __builtin_amdgcn_s_setprio(2);
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
@@ -407,6 +410,9 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
     constexpr KGeom kg64 = walk_geom<QUINTIC>();
     constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
+    // the walk is the critical path while the neighbour and hydro kernels run
+    // beside it: its waves take issue priority over theirs on shared SIMDs
+    __builtin_amdgcn_s_setprio(2);
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
