@@ -319,7 +319,8 @@ int run_tree_update(Ctx &c) {
 }
 
 // sph_update.F90:55-101 (grad-h) and :106-152 (standard SPH), in two parts
-// so that the gravity walk, which needs only h, can start between them
+// so that the gravity walk, which needs only h (not the hydro tree's
+// hrangemax), can start between them
 int run_sph_h(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "sph_update called before tree_update");
     c.sph_ok = false;
@@ -327,11 +328,11 @@ int run_sph_h(Ctx &c) {
         SX_TRY(sph_hgather(c));
     else
         SX_TRY(sph_hsolve(c));
-    SX_TRY(update_hmax(c));
     return SERENX_OK;
 }
 
 int run_sph_lists(Ctx &c) {
+    SX_TRY(update_hmax(c));
     SX_TRY(sph_neighbours(c));
     if (c.opt.sph == SERENX_SPH_STANDARD) SX_TRY(sph_density(c));
     c.sph_ok = true;

@@ -333,8 +333,8 @@ int run_grav(Ctx &c);
 int grav_forces(Ctx &c);         // BHgrav_accel over active particles (gravity stream)
 int grav_launch(Ctx &c);         // fork the gravity stream and run the walk on it
 int grav_join(Ctx &c);           // main stream waits for the walk and adds a_grav to a
-int run_sph_h(Ctx &c);           // sph_update, part 1: h (and hmax) of active particles
-int run_sph_lists(Ctx &c);       // sph_update, part 2: neighbour lists (+ standard density)
+int run_sph_h(Ctx &c);           // sph_update, part 1: h of the active particles
+int run_sph_lists(Ctx &c);       // sph_update, part 2: hmax, neighbour lists (+ standard density)
 int direct_gravity(Ctx &c, int nt, const int *dtargets, double *dacc, double *dpot);
 
 }  // namespace serenx
