@@ -95,9 +95,11 @@ enum { SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1 };
 
 /* SPH option (Makefile SPH): GRAD_H (grad-h SPH: the rho-h iteration with
  * Omega and zeta, all_sph_gradh.F90, hydro_gradh.F90, gravity_gradh.F90) or
- * STANDARD with HFIND=NUMBER (h_gather.F90: h is half the distance of the
- * pp_gather-th nearest neighbour; all_sph.F90, hydro.F90, gravity_sph.F90;
- * Omega = 1, zeta = 0).  STANDARD is isothermal only here. */
+ * STANDARD with HFIND=NUMBER (h_gather.F90: h is the distance of the
+ * pp_gather-th nearest neighbour over the kernel range; all_sph.F90,
+ * hydro.F90, gravity_sph.F90; Omega = 1, zeta = 0).  With
+ * THERMAL_ENERGY_EQN, STANDARD forms du/dt as hydro.F90 does: the pressure
+ * and viscous pair terms projected on dv.dr / |dr|, with no P div v term. */
 enum { SERENX_SPH_GRADH = 0, SERENX_SPH_STANDARD = 1 };
 
 typedef struct serenx_options {

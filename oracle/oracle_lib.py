@@ -56,6 +56,8 @@ VARIANTS = {
     # SPH=STANDARD, HFIND=NUMBER
     (False, "geometric", "m4", "standard"): "std_quad_geo",
     (True, "gadget2", "m4", "standard"): "std_oct_gadget2",
+    # SPH=STANDARD with THERMAL=ENERGY_EQN
+    (False, "geometric", "m4", "standard_energy"): "std_quad_geo_ee",
 }
 
 
@@ -170,7 +172,8 @@ class RefSeren(_Common):
         update_thermal_properties).  sph="standard" selects the SPH=STANDARD
         HFIND=NUMBER build (h_gather, all_sph, hydro, gravity_sph)."""
         if sph == "standard":
-            key = (bool(octupole), mac, kernel, "standard")
+            key = (bool(octupole), mac, kernel,
+                   "standard_energy" if thermal == "energy" else "standard")
         elif thermal == "energy":
             key = (bool(octupole), mac, kernel, "energy")
         elif timestep:

@@ -1367,16 +1367,17 @@ static void hydro_gradh(oracle_ctx *c, int p, int *onfly, int *potlist, int **bi
     }
 }
 
-/* hydro.F90:11-323 (standard SPH, AB viscosity, isothermal): the kernel
- * gradient averaged over h_p and h_pp (wmean) carries both the pressure and
- * the viscous term */
+/* hydro.F90:11-323 (standard SPH, AB viscosity): the kernel gradient
+ * averaged over h_p and h_pp (wmean) carries both the pressure and the
+ * viscous term; with the energy equation the same pair terms, projected on
+ * dv.dr / |dr|, give du/dt (hydro.F90:206-208, :262-264) */
 static void hydro_std(oracle_ctx *c, int p, int *onfly, int *potlist, int **big) {
     double rp[3] = {PX(p, 0), PX(p, 1), PX(p, 2)};
     double hp = PX(p, SMOO), invhp = 1.0 / hp, hfactor_p = pow(invhp, 3.0);
     double vp[3] = {c->v[p][0], c->v[p][1], c->v[p][2]};
     double sound_p = c->sound[p], rho_p = c->rho[p];
     double pfactor_p = c->press[p] / rho_p / rho_p;
-    double ah[3] = {0, 0, 0}, ad[3] = {0, 0, 0};
+    double ah[3] = {0, 0, 0}, ad[3] = {0, 0, 0}, dudt = 0.0;
     int pp_numb = c->pptot[p];
     const int *lst;
     if (pp_numb <= c->pp_limit) {
@@ -1410,6 +1411,7 @@ static void hydro_std(oracle_ctx *c, int p, int *onfly, int *potlist, int **big)
         }
         double pf = pfactor_p + c->press[pp] / (rho_pp * rho_pp);
         for (int k = 0; k < 3; k++) ah[k] = ah[k] + mpp * wmean * pf * du[k];
+        if (c->energy) dudt = dudt + 0.5 * mpp * wmean * dvdr * invdrmag * pf;
         double rhomean = 0.5 * (rho_p + rho_pp);
         if (dvdr < 0.0) {
             double beta = 2 * alpha;
@@ -1419,9 +1421,12 @@ static void hydro_std(oracle_ctx *c, int p, int *onfly, int *potlist, int **big)
                         (drsqd + 0.25 * ETA_SQD * (hp + hpp) * (hp + hpp));
             double visc_factor = (-alpha_mean * vsignal * mu + beta * mu * mu) / rhomean;
             for (int k = 0; k < 3; k++) ad[k] = ad[k] + mpp * visc_factor * wmean * du[k];
+            if (c->energy)
+                dudt = dudt + 0.5 * mpp * visc_factor * wmean * dvdr * invdrmag;
         }
     }
     for (int k = 0; k < 3; k++) c->a[p][k] = ah[k] + ad[k];
+    if (c->energy) c->du_dt[p] = dudt;
 }
 
 /* sph_hydro_forces.F90:9-120 */

@@ -283,8 +283,6 @@ int check_options(const serenx_options &o) {
         return set_error(SERENX_EINVAL, "gamma must exceed 1 for the energy equation");
     if (o.sph != SERENX_SPH_GRADH && o.sph != SERENX_SPH_STANDARD)
         return set_error(SERENX_EINVAL, "sph must be SERENX_SPH_GRADH or SERENX_SPH_STANDARD");
-    if (o.sph == SERENX_SPH_STANDARD && o.thermal != SERENX_THERMAL_ISOTHERMAL)
-        return set_error(SERENX_EINVAL, "standard SPH is built for the isothermal option only");
     if (o.serial != 0 && o.serial != 1) return set_error(SERENX_EINVAL, "serial must be 0 or 1");
     if (o.pp_gather < 0 || (o.sph == SERENX_SPH_GRADH && o.pp_gather != 0))
         return set_error(SERENX_EINVAL,

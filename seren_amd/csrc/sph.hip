@@ -874,14 +874,17 @@ __global__ void k_pack_hydro_std(int n, const double *__restrict__ x, const doub
     r[3] = make_double4(cr_pow3(invh), p_i / (rho_i * rho_i), 0.0, 0.0);
 }
 
-// hydro.F90:11-323 (standard SPH, AB viscosity, isothermal): the kernel
-// gradient averaged over h_p and h_pp (wmean) weighs both the pressure term
-// (P_p/rho_p^2 + P_pp/rho_pp^2) and the viscous term.
+// hydro.F90:11-323 (standard SPH, AB viscosity): the kernel gradient
+// averaged over h_p and h_pp (wmean) weighs both the pressure term
+// (P_p/rho_p^2 + P_pp/rho_pp^2) and the viscous term.  ENERGY adds du/dt,
+// the same pair terms projected on dv.dr / |dr| (hydro.F90:206-208, :262-264).
+template <bool ENERGY>
 __global__ void __launch_bounds__(256) k_hydro_std(
     KGeom kg, int n, const double4 *__restrict__ rec, const uint8_t *__restrict__ active,
     const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
     const int *__restrict__ ovf_list, int pp_limit, const double *__restrict__ ktab, double alpha,
-    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az) {
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    double *__restrict__ dudt_out) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= n || !active[p]) return;
     const double *w1 = ktab + T_W1 * KT;
@@ -891,7 +894,7 @@ __global__ void __launch_bounds__(256) k_hydro_std(
     const double vpx = r1.x, vpy = r1.y, vpz = r1.z;
     const double rho_p = r2.x, pfactor_p = r2.y, sound_p = r2.z, invhp = r2.w;
     const double hfactor_p = r3.x;
-    double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0;
+    double ahx = 0.0, ahy = 0.0, ahz = 0.0, adx = 0.0, ady = 0.0, adz = 0.0, dudt = 0.0;
     const int nn = pptot[p];
     const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
     for (int i = 0; i < nn; i++) {
@@ -920,6 +923,7 @@ __global__ void __launch_bounds__(256) k_hydro_std(
         ahx = ahx + mpp * wmean * pf * ux;
         ahy = ahy + mpp * wmean * pf * uy;
         ahz = ahz + mpp * wmean * pf * uz;
+        if (ENERGY) dudt = dudt + 0.5 * mpp * wmean * dvdr * invdrmag * pf;
         const double rhomean = 0.5 * (rho_p + rho_pp);
         if (dvdr < 0.0) {                                      // hydro.F90:221-224
             const double beta = 2 * alpha;
@@ -930,11 +934,13 @@ __global__ void __launch_bounds__(256) k_hydro_std(
             adx = adx + mpp * visc * wmean * ux;
             ady = ady + mpp * visc * wmean * uy;
             adz = adz + mpp * visc * wmean * uz;
+            if (ENERGY) dudt = dudt + 0.5 * mpp * visc * wmean * dvdr * invdrmag;
         }
     }
     ax[p] = ahx + adx;
     ay[p] = ahy + ady;
     az[p] = ahz + adz;
+    if (ENERGY) dudt_out[p] = dudt;
 }
 
 __global__ void k_pow_third(int n, const double *__restrict__ x, double *__restrict__ y) {
@@ -1083,9 +1089,14 @@ int hydro_forces(Ctx &c) {
     if (c.opt.sph == SERENX_SPH_STANDARD) {
         hipLaunchKernelGGL(k_pack_hydro_std, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x,
                            c.y, c.z, c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.press, c.sound, rec);
-        hipLaunchKernelGGL(k_hydro_std, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.n, rec,
-                           c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.pp_limit, c.ktab,
-                           c.opt.alpha, c.ax, c.ay, c.az);
+        if (c.opt.thermal == SERENX_THERMAL_ENERGY_EQN)
+            hipLaunchKernelGGL(k_hydro_std<true>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg,
+                               c.n, rec, c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
+                               c.pp_limit, c.ktab, c.opt.alpha, c.ax, c.ay, c.az, c.dudt);
+        else
+            hipLaunchKernelGGL(k_hydro_std<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg,
+                               c.n, rec, c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
+                               c.pp_limit, c.ktab, c.opt.alpha, c.ax, c.ay, c.az, c.dudt);
         SX_HIP(hipGetLastError());
         return SERENX_OK;
     }

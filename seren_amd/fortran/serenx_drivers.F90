@@ -6,7 +6,7 @@
 ! src/BHtree/BHhydro_hguess.F90) runs its force evaluation on the GPU through
 ! libserenx.so.  The module arrays are bound by address before every call,
 ! so reallocation between steps is harmless.  Option set: GRAD_H_SPH (or
-! SPH=STANDARD with HFIND=NUMBER, isothermal), M4 or QUINTIC (TC or not),
+! SPH=STANDARD with HFIND=NUMBER), M4 or QUINTIC (TC or not),
 ! ISOTHERMAL or ENERGY_EQN (INTERNAL_ENERGY: du_dt comes back from
 ! sph_hydro_forces; thermal and the advance stay on the host), AB viscosity,
 ! KS gravity, BH tree, REORDER_PARTICLES, NEIGHBOUR_LISTS, DOUBLE_PRECISION,

@@ -64,6 +64,9 @@ CASES = {
                               "m4", "isothermal", 1.2, "standard"),
     "plummer2k_std_oct_gadget2_x3": (2048, "plummer", 112, 1.0, True, "gadget2", 0.25, 1e-4, 3,
                                      True, "m4", "isothermal", 1.2, "standard"),
+    # SPH=STANDARD with THERMAL=ENERGY_EQN (hydro.F90's du/dt)
+    "r2_2k_std_quad_geo_energy_x2": (2048, "r2", 113, 2.0, False, "geometric", 0.25, 1e-4, 2,
+                                     True, "m4", "energy", 1.2, "standard"),
 }
 
 

@@ -28,7 +28,7 @@ GOLD = os.path.join(HERE, "golden")
                                   "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
                                   "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc",
                                   "r2_2k_quad_geo_energy_x2", "r2_2k_quad_geo_quintic_x2",
-                                  "r2_2k_std_quad_geo_x2"])
+                                  "r2_2k_std_quad_geo_x2", "r2_2k_std_quad_geo_energy_x2"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))

@@ -28,6 +28,8 @@ CASES = [
     # SPH=STANDARD, HFIND=NUMBER
     (2000, "r2", 20, False, "geometric", 2, "m4", 1.2, "standard"),
     (1500, "plummer", 21, True, "gadget2", 3, "m4", 1.2, "standard"),
+    # SPH=STANDARD with THERMAL=ENERGY_EQN
+    (2000, "plummer", 22, False, "geometric", 2, "m4", 1.2, "standard", "energy"),
 ]
 
 
@@ -39,11 +41,17 @@ def test_oracle_bit_exact_with_reference(case):
     kernel = case[6] if len(case) > 6 else "m4"
     h_fac = case[7] if len(case) > 7 else 1.2
     sph = case[8] if len(case) > 8 else "gradh"
+    thermal = case[9] if len(case) > 9 else "isothermal"
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=1.5, ngrid=16)
+    u0 = np.random.default_rng(seed).uniform(0.05, 0.15, n)
     outs = []
     for cls in (RefSeren, CSeren):
-        o = cls(octu, mac, kernel=kernel, sph=sph)
+        # the reference selects its THERMAL build; the oracle switches on set_energy
+        kw = {"thermal": thermal} if cls is RefSeren else {}
+        o = cls(octu, mac, kernel=kernel, sph=sph, **kw)
         o.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=0.25, abserror=1e-4)
+        if thermal == "energy":
+            o.set_energy(u0, 5.0 / 3.0)
         for e in range(nev):
             o.tree_update()
             if e == 0:
@@ -52,8 +60,12 @@ def test_oracle_bit_exact_with_reference(case):
             o.thermal_iso(0.2)
             o.hydro_forces()
             o.grav_forces()
-        outs.append((o.particles(), o.neib(), o.grav_tree()))
-    (pr, nr, tr), (pc, nc, tc) = outs
+        outs.append((o.particles(), o.neib(), o.grav_tree(),
+                     o.energy() if thermal == "energy" else None))
+    (pr, nr, tr, er), (pc, nc, tc, ec) = outs
+    if thermal == "energy":
+        assert np.array_equal(er[0], ec[0]) and np.array_equal(er[1], ec[1]), "u, du_dt"
+        assert np.any(er[1] != 0.0)
     for k in ("porig", "h", "rho", "omega", "zeta", "div_v", "a", "gpot", "agravmag"):
         assert np.array_equal(pr[k], pc[k]), k
     assert np.array_equal(nr[0], nc[0])
