@@ -102,6 +102,11 @@ enum { SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1 };
  * and viscous pair terms projected on dv.dr / |dr|, with no P div v term. */
 enum { SERENX_SPH_GRADH = 0, SERENX_SPH_STANDARD = 1 };
 
+/* HFIND option of SERENX_SPH_STANDARD (Makefile HFIND): NUMBER (h_gather.F90)
+ * or CONSTANT (sph_update.F90:107-109: every h is set to hmin before the
+ * neighbour search; sph_setup.F90:45-49 makes no h guess). */
+enum { SERENX_HFIND_NUMBER = 0, SERENX_HFIND_CONSTANT = 1 };
+
 typedef struct serenx_options {
     int32_t multipole;   /* SERENX_QUADRUPOLE or SERENX_OCTUPOLE */
     int32_t mac;         /* SERENX_MAC_* */
@@ -122,6 +127,9 @@ typedef struct serenx_options {
                             neighbour and hydro kernels; 1: on the context's one
                             stream (for callers that run many contexts at once,
                             whose streams share the device's hardware queues) */
+    int32_t hfind;       /* SERENX_HFIND_*; NUMBER unless sph is STANDARD */
+    double hmin;         /* HFIND_CONSTANT: the smoothing length of every particle
+                            (params hmin), > 0 */
 } serenx_options;
 
 /* Fills SEREN's parameter defaults (default_parameters.F90:62-117,173):

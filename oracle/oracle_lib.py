@@ -58,6 +58,8 @@ VARIANTS = {
     (True, "gadget2", "m4", "standard"): "std_oct_gadget2",
     # SPH=STANDARD with THERMAL=ENERGY_EQN
     (False, "geometric", "m4", "standard_energy"): "std_quad_geo_ee",
+    # SPH=STANDARD, HFIND=CONSTANT
+    (False, "geometric", "m4", "standard_consth"): "std_quad_geo_consth",
 }
 
 
@@ -161,7 +163,8 @@ class RefSeren(_Common):
     _libs: Dict[str, C.CDLL] = {}
 
     def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4",
-                 timestep: str = "", thermal: str = "isothermal", sph: str = "gradh"):
+                 timestep: str = "", thermal: str = "isothermal", sph: str = "gradh",
+                 hfind: str = "number"):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
         oracle/Makefile.ref dropin): the reference code as the host.
@@ -173,6 +176,7 @@ class RefSeren(_Common):
         HFIND=NUMBER build (h_gather, all_sph, hydro, gravity_sph)."""
         if sph == "standard":
             key = (bool(octupole), mac, kernel,
+                   "standard_consth" if hfind == "constant" else
                    "standard_energy" if thermal == "energy" else "standard")
         elif thermal == "energy":
             key = (bool(octupole), mac, kernel, "energy")
@@ -204,6 +208,7 @@ class RefSeren(_Common):
             lib.ref_step_setup.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.c_double]
             lib.ref_set_energy.argtypes = [C.c_int, _dp, C.c_double]
+            lib.ref_set_hmin.argtypes = [C.c_double]
             lib.ref_get_energy.argtypes = [C.c_int, _dp, _dp]
             lib.ref_integrate.argtypes = [C.c_double]
             lib.ref_get_step.argtypes = [_lp, _dp]
@@ -270,6 +275,10 @@ class RefSeren(_Common):
 
     def set_agravmag(self, am):
         self.lib.ref_set_agravmag(self.n, _f64(am))
+
+    def set_hmin(self, hmin):
+        """params hmin, the h of every particle under HFIND=CONSTANT."""
+        self.lib.ref_set_hmin(hmin)
 
     def set_energy(self, u, gamma=1.4):
         """Specific internal energy per particle (current order) and gamma."""
@@ -344,6 +353,7 @@ class CSeren(_Common):
             lib.oracle_get_hydro_tree.argtypes = [vp, C.c_int, _dp, _dp, _dp]
             lib.oracle_get_kernel.argtypes = [vp, _dp]
             lib.oracle_set_energy.argtypes = [vp, _dp, C.c_double]
+            lib.oracle_set_hmin.argtypes = [vp, C.c_double]
             lib.oracle_get_energy.argtypes = [vp, _dp, _dp]
             cls._lib = lib
         return cls._lib
@@ -398,6 +408,10 @@ class CSeren(_Common):
 
     def set_agravmag(self, am):
         self.lib.oracle_set_agravmag(self.ctx, _f64(am))
+
+    def set_hmin(self, hmin):
+        """HFIND=CONSTANT: every h is hmin from the next sph_update on."""
+        self.lib.oracle_set_hmin(self.ctx, hmin)
 
     def set_energy(self, u, gamma=1.4):
         """THERMAL=ENERGY_EQN: u per particle (current order) and gamma."""

@@ -202,6 +202,12 @@ CONTAINS
     nbuildstep = 1
   end subroutine ref_setup
 
+  ! params hmin: the smoothing length HFIND=CONSTANT sets (sph_update.F90:107-109)
+  subroutine ref_set_hmin(h) bind(C, name="ref_set_hmin")
+    real(c_double), value :: h
+    hmin = h
+  end subroutine ref_set_hmin
+
 ! ----------------------------------------------------------------------------
   subroutine ref_tree_update() bind(C, name="ref_tree_update")
     nbuild = nsteps

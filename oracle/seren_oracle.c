@@ -71,6 +71,7 @@ typedef struct {
 typedef struct oracle_ctx {
     int n, pp_gather, pp_limit, octupole, mac, kernel;
     int sph_std; /* SPH=STANDARD with HFIND=NUMBER (else GRAD_H_SPH) */
+    double hmin_const; /* > 0: HFIND=CONSTANT, every h is this value */
     /* kernel range constants, macros.h:101-118 */
     double halfkerntot, kernrange, kernrangesqd, invkernrange;
     int energy;          /* THERMAL=ENERGY_EQN (INTERNAL_ENERGY) */
@@ -1193,6 +1194,10 @@ static void all_sph(oracle_ctx *c, int p, int *potlist, int **big, int *onfly) {
 static void sph_update_standard(oracle_ctx *c) {
     int n = c->n;
     for (int p = 1; p <= n; p++) { /* each h from its own old value */
+        if (c->hmin_const > 0.0) { /* sph_update.F90:107-109, HFIND=CONSTANT */
+            PX(p, SMOO) = c->hmin_const;
+            continue;
+        }
         PX(p, SMOO) = h_gather(c, p, PX(p, SMOO));
         if (c->err) return;
     }
@@ -1290,6 +1295,9 @@ void oracle_set_energy(oracle_ctx *c, const double *u, double gamma) {
     c->gamma = gamma;
     for (int p = 1; p <= c->n; p++) c->u[p] = u[p - 1];
 }
+
+/* HFIND=CONSTANT with SPH=STANDARD: sph_update sets every h to hmin */
+void oracle_set_hmin(oracle_ctx *c, double hmin) { c->hmin_const = hmin; }
 
 void oracle_get_energy(const oracle_ctx *c, double *u, double *du_dt) {
     for (int p = 1; p <= c->n; p++) {

@@ -284,6 +284,12 @@ int check_options(const serenx_options &o) {
     if (o.sph != SERENX_SPH_GRADH && o.sph != SERENX_SPH_STANDARD)
         return set_error(SERENX_EINVAL, "sph must be SERENX_SPH_GRADH or SERENX_SPH_STANDARD");
     if (o.serial != 0 && o.serial != 1) return set_error(SERENX_EINVAL, "serial must be 0 or 1");
+    if (o.hfind != SERENX_HFIND_NUMBER && o.hfind != SERENX_HFIND_CONSTANT)
+        return set_error(SERENX_EINVAL, "hfind must be SERENX_HFIND_NUMBER or SERENX_HFIND_CONSTANT");
+    if (o.hfind == SERENX_HFIND_CONSTANT && o.sph != SERENX_SPH_STANDARD)
+        return set_error(SERENX_EINVAL, "SERENX_HFIND_CONSTANT needs SERENX_SPH_STANDARD");
+    if (o.hfind == SERENX_HFIND_CONSTANT && !(o.hmin > 0.0))
+        return set_error(SERENX_EINVAL, "SERENX_HFIND_CONSTANT needs hmin > 0");
     if (o.pp_gather < 0 || (o.sph == SERENX_SPH_GRADH && o.pp_gather != 0))
         return set_error(SERENX_EINVAL,
                          "pp_gather must be 0 for grad-h SPH (h_fac sets it) and >= 0 otherwise");

@@ -943,6 +943,12 @@ __global__ void __launch_bounds__(256) k_hydro_std(
     if (ENERGY) dudt_out[p] = dudt;
 }
 
+// sph_update.F90:107-109 (HFIND=CONSTANT): parray(SMOO,1:ptot) = hmin
+__global__ void k_fill_h(int n, double hmin, double *__restrict__ h) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = hmin;
+}
+
 __global__ void k_pow_third(int n, const double *__restrict__ x, double *__restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = ref_pow_third(x[i]);
@@ -979,6 +985,12 @@ int sph_hsolve(Ctx &c) {
 }
 
 int sph_hgather(Ctx &c) {
+    if (c.opt.hfind == SERENX_HFIND_CONSTANT) {
+        hipLaunchKernelGGL(k_fill_h, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.opt.hmin,
+                           c.h);
+        SX_HIP(hipGetLastError());
+        return SERENX_OK;
+    }
     if (c.n <= c.pp_gather)
         return set_error(SERENX_EFEW, "h_gather needs more than pp_gather particles");
     const size_t need = ((size_t)c.cap + TB - 1) / TB * TB;   // whole blocks, as hlist

@@ -72,7 +72,8 @@ class Options(C.Structure):
                 ("kernel", C.c_int32), ("h_fac", C.c_double), ("thetamaxsqd", C.c_double),
                 ("abserror", C.c_double), ("alpha", C.c_double), ("sound_speed", C.c_double),
                 ("thermal", C.c_int32), ("sph", C.c_int32), ("gamma", C.c_double),
-                ("pp_gather", C.c_int32), ("serial", C.c_int32)]
+                ("pp_gather", C.c_int32), ("serial", C.c_int32),
+                ("hfind", C.c_int32), ("hmin", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -178,11 +179,12 @@ class ForceEngine:
                  abserror: float = 1e-4, alpha: float = 1.0, sound_speed: float = 0.2,
                  device: int = 0, kernel: str = "m4", thermal: str = "isothermal",
                  gamma: float = 1.4, sph: str = "gradh", pp_gather: int = 0,
-                 serial: bool = False):
+                 serial: bool = False, hmin: float = 0.0):
         """thermal: "isothermal" (P = c_s^2 rho with sound_speed) or "energy"
         (the energy equation: set u with set_energy).  sph: "gradh" or
         "standard" (h from the pp_gather-th neighbour; pp_gather 0 takes the
-        h_fac value).  serial: keep the gravity walk on the context's one
+        h_fac value).  hmin > 0 with sph="standard" selects HFIND=CONSTANT:
+        every h is hmin.  serial: keep the gravity walk on the context's one
         stream (many concurrent contexts)."""
         self.lib = load_library()
         opt = Options()
@@ -201,6 +203,8 @@ class ForceEngine:
         opt.sph = _SPH[sph]
         opt.pp_gather = pp_gather
         opt.serial = int(serial)
+        opt.hfind = 1 if hmin > 0.0 else 0
+        opt.hmin = hmin
         self.opt = opt
         self.ctx = C.c_void_p()
         _check(self.lib.serenx_create(C.byref(opt), device, C.byref(self.ctx)))

@@ -65,6 +65,11 @@ CONTAINS
        opt%sph = SERENX_SPH_STANDARD
        opt%pp_gather = pp_gather
 #endif
+#if defined(CONSTANT_H)
+       ! HFIND=CONSTANT: every h is hmin (sph_update.F90:107-109)
+       opt%hfind = SERENX_HFIND_CONSTANT
+       opt%hmin = hmin
+#endif
        call serenx_f_check(serenx_create(opt, 0, serenx_ctx), "create")
     end if
     st%ptot = ptot

@@ -15,13 +15,15 @@ MODULE serenx_f
   integer(c_int32_t), parameter :: SERENX_FP64 = 0, SERENX_MIXED = 1
   integer(c_int32_t), parameter :: SERENX_THERMAL_ISOTHERMAL = 0, SERENX_THERMAL_ENERGY_EQN = 1
   integer(c_int32_t), parameter :: SERENX_SPH_GRADH = 0, SERENX_SPH_STANDARD = 1
+  integer(c_int32_t), parameter :: SERENX_HFIND_NUMBER = 0, SERENX_HFIND_CONSTANT = 1
 
   type, bind(C) :: serenx_options
      integer(c_int32_t) :: multipole, mac, precision, kernel
      real(c_double) :: h_fac, thetamaxsqd, abserror, alpha, sound_speed
      integer(c_int32_t) :: thermal, sph
      real(c_double) :: gamma
-     integer(c_int32_t) :: pp_gather, serial
+     integer(c_int32_t) :: pp_gather, serial, hfind
+     real(c_double) :: hmin
   end type serenx_options
 
   type, bind(C) :: serenx_host_state

@@ -64,6 +64,10 @@ CASES = {
                               "m4", "isothermal", 1.2, "standard"),
     "plummer2k_std_oct_gadget2_x3": (2048, "plummer", 112, 1.0, True, "gadget2", 0.25, 1e-4, 3,
                                      True, "m4", "isothermal", 1.2, "standard"),
+    # SPH=STANDARD with HFIND=CONSTANT: every h = hmin = 0.15 (about 55 neighbours)
+    "uniform2k_std_quad_geo_consth_x2": (2048, "uniform", 114, 1.0, False, "geometric", 0.25,
+                                         1e-4, 2, True, "m4", "isothermal", 1.2, "standard",
+                                         0.15),
     # SPH=STANDARD with THERMAL=ENERGY_EQN (hydro.F90's du/dt)
     "r2_2k_std_quad_geo_energy_x2": (2048, "r2", 113, 2.0, False, "geometric", 0.25, 1e-4, 2,
                                      True, "m4", "energy", 1.2, "standard"),
@@ -80,14 +84,18 @@ def run_case(name, spec):
     thermal = spec[11] if len(spec) > 11 else "isothermal"
     h_fac = spec[12] if len(spec) > 12 else 1.2
     sph = spec[13] if len(spec) > 13 else "gradh"
+    hmin = spec[14] if len(spec) > 14 else 0.0
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal, sph=sph)
+    ref = RefSeren(octu, mac, kernel=kernel, thermal=thermal, sph=sph,
+                   hfind="constant" if hmin > 0 else "number")
     ref.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=th2, abserror=ae)
+    if hmin > 0:
+        ref.set_hmin(hmin)
     if thermal == "energy":
         ref.set_energy(energy0(n, seed), GAMMA)
     for e in range(nev):
         ref.tree_update()
-        if e == 0:
+        if e == 0 and hmin == 0:   # HFIND=CONSTANT: no h guess (sph_setup.F90:45-49)
             ref.hguess()
         ref.sph_update()
         ref.thermal_iso(CS)
@@ -124,6 +132,8 @@ def run_case(name, spec):
         out["h_fac"] = np.array(h_fac)
     if sph != "gradh":
         out["sph"] = np.array(sph)
+    if hmin > 0:
+        out["hmin"] = np.array(hmin)
     if thermal == "energy":
         out["thermal"] = np.array(thermal)
         out["gamma"] = np.array(GAMMA)

@@ -28,7 +28,8 @@ GOLD = os.path.join(HERE, "golden")
                                   "r2_2k_oct_gadget2_x2", "r2_2k_oct_gadget_x2",
                                   "plummer2k_oct_eigen_x2", "uniform1k_quad_geo_tc",
                                   "r2_2k_quad_geo_energy_x2", "r2_2k_quad_geo_quintic_x2",
-                                  "r2_2k_std_quad_geo_x2", "r2_2k_std_quad_geo_energy_x2"])
+                                  "r2_2k_std_quad_geo_x2", "r2_2k_std_quad_geo_energy_x2",
+                                  "uniform2k_std_quad_geo_consth_x2"])
 def test_fortran_dropin_matches_reference(case):
     from oracle_lib import RefSeren
     g = np.load(os.path.join(GOLD, case + ".npz"))
@@ -36,17 +37,20 @@ def test_fortran_dropin_matches_reference(case):
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     thermal = str(g["thermal"]) if "thermal" in g.files else "isothermal"
     sph = str(g["sph"]) if "sph" in g.files else "gradh"
+    hmin = float(g["hmin"]) if "hmin" in g.files else 0.0
     host = RefSeren(bool(octu), str(g["mac"]), dropin=True, kernel=kernel, thermal=thermal,
-                    sph=sph)
+                    sph=sph, hfind="constant" if hmin > 0 else "number")
     h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
     host.setup(g["pos"], g["mass"], np.zeros(n), g["vel"], h_fac=h_fac,
                thetamaxsqd=float(g["thetamaxsqd"]),
                abserror=float(g["abserror"]))
     if thermal == "energy":
         host.set_energy(g["u0"], float(g["gamma"]))
+    if hmin > 0:
+        host.set_hmin(hmin)
     for e in range(nev):
         host.tree_update()
-        if e == 0:
+        if e == 0 and hmin == 0:
             host.hguess()
         host.sph_update()
         host.thermal_iso(CS)

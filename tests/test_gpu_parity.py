@@ -46,11 +46,11 @@ def _ids(c):
 
 
 def _engine(octupole, mac, theta2, abserr, kernel="m4", thermal="isothermal", h_fac=1.2,
-            sph="gradh"):
+            sph="gradh", hmin=0.0):
     from seren_amd.engine import ForceEngine
     return ForceEngine(multipole=_MULTI[octupole], mac=mac,
                        precision="fp64", thetamaxsqd=theta2, abserror=abserr, sound_speed=CS,
-                       kernel=kernel, thermal=thermal, h_fac=h_fac, sph=sph)
+                       kernel=kernel, thermal=thermal, h_fac=h_fac, sph=sph, hmin=hmin)
 
 
 def _report(name, a, b):
@@ -188,14 +188,15 @@ def test_gpu_matches_reference_golden_vectors(case):
     print(f"golden {case}", flush=True)
     h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
     sph = str(g["sph"]) if "sph" in g.files else "gradh"
+    hmin = float(g["hmin"]) if "hmin" in g.files else 0.0
     eng = _engine(bool(octu), mac, float(g["thetamaxsqd"]), float(g["abserror"]), kernel,
-                  "energy" if energy else "isothermal", h_fac, sph)
+                  "energy" if energy else "isothermal", h_fac, sph, hmin)
     eng.load(g["pos"], g["vel"], g["mass"], np.zeros(n))
     if energy:
         assert float(g["gamma"]) == eng.opt.gamma
         eng.set_energy(g["u0"])
     for e in range(nev):
-        eng.force_eval(hguess=(e == 0))
+        eng.force_eval(hguess=(e == 0 and hmin == 0))
     d = eng.fetch()
     t = eng.export_tree()
     assert np.array_equal(d["porig"], g["porig"])

@@ -39,9 +39,12 @@ def _run_oracle(g):
             abserror=float(g["abserror"]))
     if "thermal" in g.files and str(g["thermal"]) == "energy":
         o.set_energy(g["u0"], float(g["gamma"]))
+    hmin = float(g["hmin"]) if "hmin" in g.files else 0.0
+    if hmin > 0:
+        o.set_hmin(hmin)
     for e in range(nev):
         o.tree_update()
-        if e == 0:
+        if e == 0 and hmin == 0:   # HFIND=CONSTANT makes no h guess
             o.hguess()
         o.sph_update()
         o.thermal_iso(CS)
