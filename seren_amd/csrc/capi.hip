@@ -236,6 +236,7 @@ int ensure_capacity(Ctx &c, int n) {
 
 void free_all(Ctx &c) {
     step_free(c);
+    serve_free(c);
     void *ptrs[] = {c.x, c.y, c.z, c.m, c.h, c.zeta, c.vx, c.vy, c.vz, c.rho, c.omega,
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
                     c.u, c.dudt,
@@ -307,6 +308,7 @@ size_t ctx_device_bytes(const Ctx &c) {
     b += cc * (sizeof(GNode) + 10 * 8 + sizeof(GNodeF) + sizeof(HNode) + 8 * 4 + 7 * 8);
     b += c.sorttmp_bytes + c.ovf_cap * 4 + sizeof(double) * KT_TOTAL;
     if (c.st.on) b += nn * (11 * 8 + 8 + 4);                 // step state
+    if (c.sv.on) b += nn * 3 * 8 + (size_t)c.ncloud * 25;    // serving state
     b += c.band_cap * BAND_MAX * 8;                         // standard SPH band
     return b;
 }
@@ -359,8 +361,11 @@ int run_hydro(Ctx &c) {
 
 int run_grav(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "gravity needs tree_update first");
+    GravGuard guard(c.gstream);
+    guard.armed = true;
     SX_TRY(grav_launch(c));
     SX_TRY(grav_join(c));
+    guard.armed = false;
     return SERENX_OK;
 }
 
@@ -549,6 +554,7 @@ int serenx_destroy(serenx_ctx *ctx) {
     Ctx *c = as_ctx(ctx);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->gstream) (void)hipStreamSynchronize(c->gstream);
     free_all(*c);
     delete c;
     return SERENX_OK;
@@ -739,6 +745,7 @@ int serenx_load_forest(serenx_ctx *ctx, int32_t ncloud, const int32_t *counts,
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
     step_free(c);
+    serve_free(c);
     SX_TRY(ensure_capacity(c, n));
     c.n = n;
     SX_TRY(set_clouds(c, ncloud, counts));
@@ -771,6 +778,14 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     Ctx &c = *as_ctx(ctx);
     if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
     SX_HIP(hipSetDevice(c.device));
+    return run_force_eval(c, flags);
+}
+
+}  // extern "C"
+
+namespace serenx {
+
+int run_force_eval(Ctx &c, int flags) {
     SX_HIP(hipMemsetAsync(c.wstat, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     SX_TRY(run_tree_update(c));
@@ -781,7 +796,11 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     const bool grav = !(flags & SERENX_EVAL_NO_GRAV);
     c.grav_timed = false;
     SX_TRY(run_sph_h(c));
-    if (grav) SX_TRY(grav_launch(c));
+    GravGuard guard(c.gstream);
+    if (grav) {
+        guard.armed = true;
+        SX_TRY(grav_launch(c));
+    }
     SX_TRY(run_sph_lists(c));
     SX_TRY(sph_thermal(c));
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
@@ -793,9 +812,14 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
     }
     SX_HIP(hipEventRecord(c.ev[3], c.stream));
     if (grav) SX_TRY(grav_join(c));
+    guard.armed = false;
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     return SERENX_OK;
 }
+
+}  // namespace serenx
+
+extern "C" {
 
 int serenx_sync(serenx_ctx *ctx) {
     if (!ctx) return set_error(SERENX_EINVAL, "null context");
@@ -1088,6 +1112,77 @@ int serenx_set_precision(serenx_ctx *ctx, int32_t precision) {
     c.opt.precision = precision;
     if (to_mixed) SX_TRY(refresh_f32_nodes(c));
     return SERENX_OK;
+}
+
+int serenx_diagnostics(serenx_ctx *ctx, double *out) {
+    if (!ctx || !out) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
+    SX_HIP(hipSetDevice(c.device));
+    double *buf = nullptr;
+    const size_t count = (size_t)DIAG_MAX_BLOCKS * DIAG_SUMS + DIAG_VALUES;
+    SX_HIP(hipMalloc(&buf, sizeof(double) * count));
+    int rc = diagnostics(c, buf, buf + (size_t)DIAG_MAX_BLOCKS * DIAG_SUMS);
+    if (rc == SERENX_OK &&
+        hipMemcpyAsync(out, buf + (size_t)DIAG_MAX_BLOCKS * DIAG_SUMS,
+                       sizeof(double) * DIAG_VALUES, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "copy of the diagnostics failed");
+    if (rc == SERENX_OK && hipStreamSynchronize(c.stream) != hipSuccess)
+        rc = set_error(SERENX_EHIP, "diagnostics failed");
+    (void)hipFree(buf);
+    return rc;
+}
+
+int serenx_serve_begin(serenx_ctx *ctx, double accel_mult, double courant_mult) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return serve_begin(c, accel_mult, courant_mult);
+}
+
+int serenx_serve_step(serenx_ctx *ctx, int32_t ncloud, const uint8_t *select) {
+    if (!ctx || !select) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return serve_step(c, ncloud, select);
+}
+
+int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    SX_HIP(hipSetDevice(c.device));
+    return serve_info(c, ncloud, dt, time);
+}
+
+int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
+    if (!ctx) return set_error(SERENX_EINVAL, "null context");
+    Ctx &c = *as_ctx(ctx);
+    switch (key) {
+    case SERENX_TUNE_HSOLVE_SLOTS:
+        if (!(value >= 1.0 && value <= (double)HL_SLOTS) || value != (double)(int)value)
+            return set_error(SERENX_EINVAL, "h-solve list slots must be an integer in 1..96");
+        c.hl_cap = (int)value;
+        return SERENX_OK;
+    case SERENX_TUNE_MIXED_ABSERROR:
+        if (!(value >= 0.0) || !std::isfinite(value))
+            return set_error(SERENX_EINVAL, "mixed-walk MAC tolerance must be >= 0");
+        c.mixed_abserror = value;
+        return SERENX_OK;
+    default:
+        return set_error(SERENX_EINVAL, "unknown tuning key");
+    }
+}
+
+int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value) {
+    if (!ctx || !value) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    switch (key) {
+    case SERENX_TUNE_HSOLVE_SLOTS: *value = c.hl_cap; return SERENX_OK;
+    case SERENX_TUNE_MIXED_ABSERROR:
+        *value = c.mixed_abserror > 0.0 ? c.mixed_abserror : c.opt.abserror;
+        return SERENX_OK;
+    default: return set_error(SERENX_EINVAL, "unknown tuning key");
+    }
 }
 
 int serenx_memory(const serenx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes,

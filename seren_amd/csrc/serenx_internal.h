@@ -189,6 +189,8 @@ struct Ctx {
     double *grec = nullptr;      // gravity source records, 8 doubles per particle
     double *gax = nullptr, *gay = nullptr, *gaz = nullptr;   // gravity acceleration
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
+    int hl_cap = HL_SLOTS;       // slots used per lane (SERENX_TUNE_HSOLVE_SLOTS)
+    double mixed_abserror = 0.0; // mixed walk's MAC tolerance (0: opt.abserror)
     double *band = nullptr;      // standard SPH: BAND_MAX x 64 r^2 values per wave
     size_t band_cap = 0;         // particles the band buffer holds
     double *tmpd = nullptr;      // permutation scratch (cap doubles)
@@ -270,6 +272,22 @@ struct Ctx {
         double time = 0.0, timestep = 0.0, dt_max = 0.0;
     } st;
 
+    // batched serving steps (serenx_serve_*): one kick-drift-kick step of
+    // selected clouds of the forest, each with its own step size.  v_half per
+    // particle (tree order, permuted with the particles), per-cloud step size,
+    // time and selection on the device.
+    struct Serve {
+        bool on = false;
+        double accel_mult = 0.5, courant_mult = 0.2;
+        double *vhx = nullptr, *vhy = nullptr, *vhz = nullptr;
+        unsigned long long *dtbits = nullptr;   // per cloud: minimum step size (bits)
+        double *dt = nullptr, *time = nullptr;  // per cloud
+        uint8_t *sel = nullptr;                 // per cloud: 1 = stepped by the call
+        uint8_t *sel_host = nullptr;            // pinned staging of the selection
+        int ncloud = 0;
+        long long nsteps = 0;
+    } sv;
+
     // host-state binding
     serenx_host_state hs{};
     bool bound = false;
@@ -291,8 +309,28 @@ int hip_check(hipError_t e, const char *what);
         if (r_ != SERENX_OK) return r_;                                  \
     } while (0)
 
+// Waits for the gravity stream when it leaves scope armed: an error return
+// between grav_launch and grav_join must not leave the walk running while the
+// caller reuses the context (a rebuild rewrites the buffers it reads).
+struct GravGuard {
+    hipStream_t gs;
+    bool armed = false;
+    explicit GravGuard(hipStream_t s) : gs(s) {}
+    ~GravGuard() {
+        if (armed) (void)hipStreamSynchronize(gs);
+    }
+};
+
 // host-side kernel tables (capi.hip): kernel.F90:14-127, M4, NDIM=3
 void make_kernel_tables(double *tab, int kernel);
+
+// diag.hip: FP64 diagnostics.  partial holds DIAG_MAX_BLOCKS x DIAG_SUMS
+// doubles (block partial sums), dout DIAG_VALUES doubles, both on the device.
+constexpr int DIAG_MAX_BLOCKS = 1024;
+constexpr int DIAG_SUMS = 16;
+constexpr int DIAG_VALUES = 20;
+int diag_blocks(int n, int *per_block);
+int diagnostics(Ctx &c, double *partial, double *dout);
 
 // tree.hip
 int ensure_tree_buffers(Ctx &c);
@@ -322,12 +360,18 @@ int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_
                double *r_old, double *v_old, double *v_half, int32_t *nminneib);
 void step_free(Ctx &c);
 int permute_step(Ctx &c, const int *perm);
+int serve_begin(Ctx &c, double accel_mult, double courant_mult);
+int serve_step(Ctx &c, int ncloud, const uint8_t *select);
+int serve_info(Ctx &c, int ncloud, double *dt, double *time);
+void serve_free(Ctx &c);
+int permute_serve(Ctx &c, const int *perm);
 
 // capi.hip: the force path pieces the step driver reuses
 int run_tree_update(Ctx &c);
 int run_sph_update(Ctx &c);
 int run_hydro(Ctx &c);
 int run_grav(Ctx &c);
+int run_force_eval(Ctx &c, int flags);   // serenx_force_eval's body
 
 // grav.hip
 int grav_forces(Ctx &c);         // BHgrav_accel over active particles (gravity stream)

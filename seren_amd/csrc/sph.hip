@@ -130,7 +130,7 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
 __global__ void __launch_bounds__(256) k_hsolve_iter(
     KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
-    const double *__restrict__ roots, int *__restrict__ hlist,
+    const double *__restrict__ roots, int *__restrict__ hlist, int hl_cap,
     const uint8_t *__restrict__ active, const double *__restrict__ ktab,
     double h_fac, int pp_gather, double *__restrict__ h,
     double *__restrict__ rho, double *__restrict__ hhigh, int *__restrict__ nlist,
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) k_hsolve_iter(
                 if (mine) {
                     cnt += nd.leaf;                       // the reference's pp_pot
                     if (leaf_near(nd, rpx, rpy, rpz, R, box)) {
-                        if (nslot < HL_SLOTS)
+                        if (nslot < hl_cap)
                             wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
                         else
                             lovf = true;
@@ -630,7 +630,7 @@ constexpr int HG_ITER_MAX = 400;
 __global__ void __launch_bounds__(256) k_hgather(
     KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range, int *__restrict__ hlist,
-    double *__restrict__ band, const uint8_t *__restrict__ active, int pp_gather,
+    int hl_cap, double *__restrict__ band, const uint8_t *__restrict__ active, int pp_gather,
     double *__restrict__ h, int *__restrict__ err, unsigned long long *__restrict__ wstat) {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = p < n && active[p];
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(256) k_hgather(
     bool iterating = valid;
     int nslot = 0, cnt = 0, pp_low = 0, pp_tot = 0, passes = 0;
     bool my_ovf = false, wave_ovf = false, have_list = false, failed = false;
-    unsigned vis = 0, lvis = 0;
+    unsigned vis = 0, lvis = 0, walked = 0;   // walked: passes over the tree walk
     auto dist2 = [&](const double4 &a) {   // distance2.F90, no contraction
         const double dx = a.x - rpx, dy = a.y - rpy, dz = a.z - rpz;
         return dx * dx + dy * dy + dz * dz;
@@ -661,6 +661,7 @@ __global__ void __launch_bounds__(256) k_hgather(
                       [&](int j, const double4 &a) { f(j, dist2(a)); });
         } else {
             const double R = kg.range * h_search;
+            walked++;
             vis += group_walk(hn, nc, on, rr.x, rr.y, lvis,
                               [&](const HNode &nd, bool) {
                                   return hgather_visit(nd, rpx, rpy, rpz, R);
@@ -699,7 +700,7 @@ __global__ void __launch_bounds__(256) k_hgather(
                 if (mine) {
                     cnt += nd.leaf;                       // the reference's pp_pot
                     if (leaf_near(nd, rpx, rpy, rpz, R, box)) {
-                        if (nslot < HL_SLOTS)
+                        if (nslot < hl_cap)
                             wl[nslot * WAVE] = (nd.first << 4) | nd.leaf;
                         else
                             lovf = true;
@@ -750,14 +751,22 @@ __global__ void __launch_bounds__(256) k_hgather(
     // count(r^2 < hi2) = pp_tot; narrow it to BAND_MAX values at most
     double lo2 = kg.rangesqd * h_low * h_low, hi2 = kg.rangesqd * h_high * h_high;
     bool narrowing = valid && !failed && pp_tot - pp_low > BAND_MAX;
+    // tie: the band [lo2, hi2) holds one double only (lo2 and hi2 adjacent),
+    // so every band value is lo2 -- more than BAND_MAX particles at exactly
+    // that r^2, as in a lattice shell -- and the pp_gather-th is lo2 itself
+    bool tie = false;
     int nb = 0;
     while (__any(narrowing)) {
         const double mid = 0.5 * (lo2 + hi2);
+        const bool split = mid > lo2 && mid < hi2;
         int c = 0;
-        for_each(narrowing, [&](int, double r2) {
+        for_each(narrowing && split, [&](int, double r2) {
             if (r2 < mid) c++;
         });
-        if (narrowing) {
+        if (narrowing && !split) {
+            narrowing = false;
+            tie = true;
+        } else if (narrowing) {
             if (c >= pp_gather) {
                 hi2 = mid;
                 pp_tot = c;
@@ -773,7 +782,8 @@ __global__ void __launch_bounds__(256) k_hgather(
         }
     }
     // copy the band and sort it (insertion_sort.F90:4-30)
-    const bool fill = valid && !failed;
+    if (tie) h[p] = kg.invrange * sqrt(lo2);
+    const bool fill = valid && !failed && !tie;
     int m = 0;
     for_each(fill, [&](int, double r2) {
         if (r2 < hi2 && !(r2 < lo2) && m < BAND_MAX) {
@@ -800,6 +810,7 @@ __global__ void __launch_bounds__(256) k_hgather(
     }
     if (failed) atomicOr(err, 1);
     wave_stats(wstat, 2, vis, lvis);
+    if (walked && (threadIdx.x & 63) == 0) atomicAdd(ctr_row(wstat) + 11, (unsigned long long)walked);
 }
 
 // all_sph.F90:11-167 (standard SPH) for every active particle: rho, div v
@@ -973,7 +984,8 @@ int sph_hsolve(Ctx &c) {
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
     hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
-                       rec, c.cid, c.root_range, c.root, c.hlist, c.active, c.ktab, c.opt.h_fac,
+                       rec, c.cid, c.root_range, c.root, c.hlist, c.hl_cap, c.active, c.ktab,
+                       c.opt.h_fac,
                        c.pp_gather, c.h, c.rho, c.tmpd, c.tmpi, c.wstat);
     SX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_hsolve_final, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
@@ -989,6 +1001,7 @@ int sph_hgather(Ctx &c) {
         hipLaunchKernelGGL(k_fill_h, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.opt.hmin,
                            c.h);
         SX_HIP(hipGetLastError());
+        c.hostpin[8] = 0;   // no h solve ran: sph_neighbours reads slot 8 as its error flag
         return SERENX_OK;
     }
     if (c.n <= c.pp_gather)
@@ -1007,7 +1020,8 @@ int sph_hgather(Ctx &c) {
     hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.m, rec);
     hipLaunchKernelGGL(k_hgather, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
-                       rec, c.cid, c.root_range, c.hlist, c.band, c.active, c.pp_gather, c.h,
+                       rec, c.cid, c.root_range, c.hlist, c.hl_cap, c.band, c.active,
+                       c.pp_gather, c.h,
                        c.flag, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 8, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));

@@ -568,8 +568,10 @@ int step_once(Ctx &c) {
     // the gravity walk runs on its own stream beside the neighbour lists,
     // thermal and hydro forces, from the new h on
     c.grav_timed = false;
+    GravGuard guard(c.gstream);
     if (s.nactive > 0) {
         SX_TRY(run_sph_h(c));
+        guard.armed = true;
         SX_TRY(grav_launch(c));
         SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));
@@ -580,6 +582,7 @@ int step_once(Ctx &c) {
     if (s.nactive > 0) SX_TRY(run_hydro(c));
     SX_HIP(hipEventRecord(c.ev[3], c.stream));
     if (s.nactive > 0) SX_TRY(grav_join(c));
+    guard.armed = false;
     SX_HIP(hipEventRecord(c.ev[4], c.stream));
     SX_TRY(step_timesteps(c));
     // every level 0 (and nminneib >= 0 without sinks): the check changes nothing
@@ -654,6 +657,7 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
         return set_error(SERENX_EINVAL, "integration steps run on a single cloud (one context "
                                         "per cloud)");
     step_free(c);
+    serve_free(c);
     Ctx::Step &s = c.st;
     const size_t n = (size_t)c.cap + 1;
     double **arrs[] = {&s.rox, &s.roy, &s.roz, &s.vox, &s.voy, &s.voz, &s.vhx, &s.vhy, &s.vhz,
@@ -724,6 +728,247 @@ int step_fetch(Ctx &c, int32_t *nlevel, int64_t *nlast, double *laststep, int32_
             for (int p = 0; p < n; p++) dst[v][3 * (size_t)p + k] = t[p];
         }
     }
+    return SERENX_OK;
+}
+
+// ---- batched serving steps ------------------------------------------------
+//
+// One kick-drift-kick leapfrog step of every selected cloud of a forest, each
+// with its own step size: what sph_integrate.F90 does for one cloud with the
+// LFKDK integrator, one time-step level and adaptive steps (the step size is
+// the minimum of timestep_size over the cloud's particles, taken after each
+// force evaluation), in the operation order of the step wrapper above, so a
+// cloud stepped here follows the trajectory of serenx_step on that cloud
+// alone (tree rebuilt every step).  Clouds that are not selected are left
+// untouched: their particles are inactive in every walk, and walks never
+// leave their own cloud, so a cloud's results do not depend on which other
+// clouds share the call.
+
+namespace {
+
+__global__ void k_serve_select(int n, const int *__restrict__ cid,
+                               const uint8_t *__restrict__ sel, uint8_t *__restrict__ active) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) active[p] = sel[cid[p]];
+}
+
+// first kick and drift with the acceleration of the last evaluation (as
+// k_advance over the half step and the full step: v_half = v + a dt/2,
+// x = x + v_half dt); v is the predicted full-step velocity v_half + a dt/2
+// that the force evaluation sees (k_advance, dn > nhalf)
+__global__ void k_serve_kick_drift(int n, const int *__restrict__ cid,
+                                   const uint8_t *__restrict__ active,
+                                   const double *__restrict__ dtc, double *__restrict__ x,
+                                   double *__restrict__ y, double *__restrict__ z,
+                                   double *__restrict__ vx, double *__restrict__ vy,
+                                   double *__restrict__ vz, const double *__restrict__ ax,
+                                   const double *__restrict__ ay, const double *__restrict__ az,
+                                   double *__restrict__ vhx, double *__restrict__ vhy,
+                                   double *__restrict__ vhz) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double dt = dtc[cid[p]];
+    const double dth = dt / 2.0;                     // the wrapper's timestep
+    const double a[3] = {ax[p], ay[p], az[p]};
+    const double v[3] = {vx[p], vy[p], vz[p]};
+    double vh[3], r[3] = {x[p], y[p], z[p]};
+    for (int k = 0; k < 3; k++) {
+        vh[k] = v[k] + a[k] * dth;
+        r[k] = r[k] + vh[k] * (dth * 2.0);
+    }
+    x[p] = r[0]; y[p] = r[1]; z[p] = r[2];
+    vhx[p] = vh[0]; vhy[p] = vh[1]; vhz[p] = vh[2];
+    vx[p] = vh[0] + a[0] * dth;
+    vy[p] = vh[1] + a[1] * dth;
+    vz[p] = vh[2] + a[2] * dth;
+}
+
+// second kick with the new acceleration (advance_leapfrog_kdk.F90:34-38)
+__global__ void k_serve_kick(int n, const int *__restrict__ cid,
+                             const uint8_t *__restrict__ active, const double *__restrict__ dtc,
+                             double *__restrict__ vx, double *__restrict__ vy,
+                             double *__restrict__ vz, const double *__restrict__ ax,
+                             const double *__restrict__ ay, const double *__restrict__ az,
+                             const double *__restrict__ vhx, const double *__restrict__ vhy,
+                             const double *__restrict__ vhz) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double laststep = dtc[cid[p]];
+    vx[p] = vhx[p] + 0.5 * ax[p] * laststep;
+    vy[p] = vhy[p] + 0.5 * ay[p] * laststep;
+    vz[p] = vhz[p] + 0.5 * az[p] * laststep;
+}
+
+__global__ void k_serve_dt_init(int ncloud, const uint8_t *__restrict__ sel,
+                                unsigned long long *__restrict__ dtbits) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ncloud && sel[k]) dtbits[k] = (unsigned long long)__double_as_longlong(DT_INIT);
+}
+
+// timestep_size of every active particle, reduced to a minimum per cloud (a
+// positive double's bits order as the double).  The lanes of a wave mostly
+// belong to one cloud: then one atomic per wave.
+__global__ void k_serve_dt(int n, const int *__restrict__ cid, const uint8_t *__restrict__ active,
+                           const double *__restrict__ ax, const double *__restrict__ ay,
+                           const double *__restrict__ az, const double *__restrict__ h,
+                           const double *__restrict__ divv, const double *__restrict__ sound,
+                           double alpha, double accel_mult, double courant_mult,
+                           unsigned long long *__restrict__ dtbits) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = p < n && active[p];
+    double d = DT_INIT;
+    int k = -1;
+    if (on) {
+        d = timestep_size(ax[p], ay[p], az[p], h[p], divv[p], sound[p], alpha, 2.0 * alpha,
+                          accel_mult, courant_mult, false, 0.0, 0.0);
+        k = cid[p];
+    }
+    const unsigned long long m = __ballot(on);
+    if (m == 0) return;
+    unsigned long long b = (unsigned long long)__double_as_longlong(d);
+    const int k0 = __shfl(k, __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1), 64);
+    if (__all(!on || k == k0)) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long t = __shfl_xor(b, o, 64);
+            b = t < b ? t : b;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMin(dtbits + k0, b);
+    } else if (on) {
+        atomicMin(dtbits + k, b);
+    }
+}
+
+// the selected clouds advance their time by the step just taken and take
+// the new step size
+__global__ void k_serve_fold(int ncloud, const uint8_t *__restrict__ sel,
+                             const unsigned long long *__restrict__ dtbits,
+                             double *__restrict__ dtc, double *__restrict__ tcl, int advance) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncloud || !sel[k]) return;
+    if (advance) tcl[k] = tcl[k] + dtc[k];
+    dtc[k] = __longlong_as_double((long long)dtbits[k]);
+}
+
+__global__ void k_zero_doubles(int n, double *__restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+__global__ void k_fill_u8(int n, uint8_t v, uint8_t *__restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+}  // namespace
+
+void serve_free(Ctx &c) {
+    Ctx::Serve &v = c.sv;
+    void *ptrs[] = {v.vhx, v.vhy, v.vhz, v.dtbits, v.dt, v.time, v.sel};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (v.sel_host) (void)hipHostFree(v.sel_host);
+    v = Ctx::Serve{};
+}
+
+int permute_serve(Ctx &c, const int *perm) {
+    Ctx::Serve &v = c.sv;
+    if (!v.on) return SERENX_OK;
+    double **arrs[] = {&v.vhx, &v.vhy, &v.vhz};
+    for (double **a : arrs) {
+        hipLaunchKernelGGL(k_gather_step<double>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n,
+                           perm, (const double *)*a, c.tmpd);
+        std::swap(*a, c.tmpd);
+    }
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+// the step sizes of the selected clouds from the current accelerations
+static int serve_timesteps(Ctx &c, int advance) {
+    Ctx::Serve &v = c.sv;
+    hipLaunchKernelGGL(k_serve_dt_init, dim3(nblk(c.ncloud)), dim3(TB), 0, c.stream, c.ncloud,
+                       v.sel, v.dtbits);
+    hipLaunchKernelGGL(k_serve_dt, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.cid, c.active,
+                       c.ax, c.ay, c.az, c.h, c.divv, c.sound, c.opt.alpha, v.accel_mult,
+                       v.courant_mult, v.dtbits);
+    hipLaunchKernelGGL(k_serve_fold, dim3(nblk(c.ncloud)), dim3(TB), 0, c.stream, c.ncloud,
+                       v.sel, v.dtbits, v.dt, v.time, advance);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
+int serve_begin(Ctx &c, double accel_mult, double courant_mult) {
+    if (!(accel_mult > 0.0) || !(courant_mult > 0.0))
+        return set_error(SERENX_EINVAL, "accel_mult and courant_mult must be positive");
+    if (c.opt.thermal != SERENX_THERMAL_ISOTHERMAL)
+        return set_error(SERENX_EINVAL, "serving steps support the isothermal option only");
+    if (!c.tree_ok || !c.sph_ok)
+        return set_error(SERENX_ESTATE, "serve_begin needs the setup force evaluation first");
+    step_free(c);
+    serve_free(c);
+    Ctx::Serve &v = c.sv;
+    const size_t n = (size_t)c.cap + 1, nk = (size_t)c.ncloud;
+    SX_TRY(dmalloc(&v.vhx, n));
+    SX_TRY(dmalloc(&v.vhy, n));
+    SX_TRY(dmalloc(&v.vhz, n));
+    SX_TRY(dmalloc(&v.dtbits, nk));
+    SX_TRY(dmalloc(&v.dt, nk));
+    SX_TRY(dmalloc(&v.time, nk));
+    SX_TRY(dmalloc(&v.sel, nk));
+    SX_HIP(hipHostMalloc((void **)&v.sel_host, nk));
+    v.accel_mult = accel_mult;
+    v.courant_mult = courant_mult;
+    v.ncloud = c.ncloud;
+    v.nsteps = 0;
+    v.on = true;
+    // every cloud selected: the first step sizes from the setup evaluation
+    hipLaunchKernelGGL(k_fill_u8, dim3(nblk(c.ncloud)), dim3(TB), 0, c.stream, c.ncloud,
+                       (uint8_t)1, v.sel);
+    hipLaunchKernelGGL(k_fill_u8, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, (uint8_t)1,
+                       c.active);
+    hipLaunchKernelGGL(k_zero_doubles, dim3(nblk(c.ncloud)), dim3(TB), 0, c.stream, c.ncloud,
+                       v.time);
+    SX_TRY(serve_timesteps(c, 0));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return SERENX_OK;
+}
+
+int serve_step(Ctx &c, int ncloud, const uint8_t *select) {
+    Ctx::Serve &v = c.sv;
+    if (!v.on) return set_error(SERENX_ESTATE, "serenx_serve_step before serenx_serve_begin");
+    if (ncloud != c.ncloud || ncloud != v.ncloud)
+        return set_error(SERENX_EINVAL, "selection length differs from the number of clouds");
+    bool any = false;
+    for (int k = 0; k < ncloud; k++) {
+        v.sel_host[k] = select[k] ? 1 : 0;
+        any = any || select[k];
+    }
+    if (!any) return SERENX_OK;
+    SX_HIP(hipMemcpyAsync(v.sel, v.sel_host, ncloud, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_serve_select, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.cid, v.sel,
+                       c.active);
+    hipLaunchKernelGGL(k_serve_kick_drift, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.cid,
+                       c.active, v.dt, c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, v.vhx,
+                       v.vhy, v.vhz);
+    SX_HIP(hipGetLastError());
+    c.box_ok = false;   // positions moved: hydro boxes are stale until the stock
+    SX_TRY(run_force_eval(c, 0));
+    hipLaunchKernelGGL(k_serve_kick, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.cid,
+                       c.active, v.dt, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, v.vhx, v.vhy, v.vhz);
+    SX_TRY(serve_timesteps(c, 1));
+    // the selection buffer is reused by the next call: wait for this one
+    SX_HIP(hipStreamSynchronize(c.stream));
+    v.nsteps++;
+    return SERENX_OK;
+}
+
+int serve_info(Ctx &c, int ncloud, double *dt, double *time) {
+    Ctx::Serve &v = c.sv;
+    if (!v.on) return set_error(SERENX_ESTATE, "no serving state (serenx_serve_begin)");
+    if (ncloud != v.ncloud) return set_error(SERENX_EINVAL, "wrong number of clouds");
+    SX_HIP(hipStreamSynchronize(c.stream));
+    if (dt) SX_HIP(hipMemcpy(dt, v.dt, sizeof(double) * ncloud, hipMemcpyDeviceToHost));
+    if (time) SX_HIP(hipMemcpy(time, v.time, sizeof(double) * ncloud, hipMemcpyDeviceToHost));
     return SERENX_OK;
 }
 

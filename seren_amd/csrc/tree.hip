@@ -18,6 +18,7 @@
 //
 // Cells are stored in DFS pre-order for the walks (ifopen = c + 1, nextcell =
 // first cell after the subtree) and keep their BFS id for export.
+#include <cmath>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -575,15 +576,18 @@ __global__ void __launch_bounds__(256) k_stock(
     }
 }
 
+// macscale multiplies the multipole-MAC term for the mixed walk's own
+// tolerance (1 for the options' abserror)
 __global__ void k_fill_f32_nodes(int nc, const GNode *__restrict__ gn,
-                                 const double *__restrict__ gs, GNodeF *__restrict__ gf) {
+                                 const double *__restrict__ gs, double macscale,
+                                 GNodeF *__restrict__ gf) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const GNode &g = gn[c];
     GNodeF &f = gf[c];
     for (int k = 0; k < 3; k++) f.r[k] = g.r[k];
     f.dminsqd = g.dminsqd;
-    f.mac = g.mac;
+    f.mac = g.mac * macscale;
     f.m = (float)g.m;
     for (int k = 0; k < 6; k++) f.q[k] = (float)g.q[k];
     for (int k = 0; k < 10; k++) f.s[k] = (float)gs[10 * (size_t)c + k];
@@ -693,8 +697,16 @@ int ensure_tree_buffers(Ctx &c) {
     if (hipMalloc(&c.sorttmp, need + 256) != hipSuccess)
         return set_error(SERENX_ENOMEM, "hipMalloc failed for sort scratch");
     c.sorttmp_bytes = need + 256;
-    if (!c.hostpin) SX_HIP(hipHostMalloc((void **)&c.hostpin, 64 * sizeof(int)));
-    if (!c.hostpin_d) SX_HIP(hipHostMalloc((void **)&c.hostpin_d, 16 * sizeof(double)));
+    // pinned readback slots start defined: slot 8 is the h-solve error flag
+    // that every sph_update path writes before sph_neighbours reads it
+    if (!c.hostpin) {
+        SX_HIP(hipHostMalloc((void **)&c.hostpin, 64 * sizeof(int)));
+        std::memset(c.hostpin, 0, 64 * sizeof(int));
+    }
+    if (!c.hostpin_d) {
+        SX_HIP(hipHostMalloc((void **)&c.hostpin_d, 16 * sizeof(double)));
+        std::memset(c.hostpin_d, 0, 16 * sizeof(double));
+    }
     return SERENX_OK;
 }
 
@@ -716,6 +728,7 @@ int permute_particles(Ctx &c, const int *perm) {
                        (const uint8_t *)c.active, c.pact);
     std::swap(c.active, c.pact);
     SX_HIP(hipGetLastError());
+    SX_TRY(permute_serve(c, perm));
     return permute_step(c, perm);
 }
 
@@ -872,6 +885,22 @@ int build_tree(Ctx &c) {
     return SERENX_OK;
 }
 
+// Factor on the stocked multipole-MAC term (BHgrav_stock.F90:329-338) that
+// turns abserror into the mixed walk's tolerance: the term is proportional to
+// abserror^(-1/3) (GADGET), abserror^(-1/2) (GADGET2) or abserror^(-2/3)
+// (EIGEN).  The geometric MAC has no tolerance to trade.
+static double mixed_mac_scale(const Ctx &c) {
+    const double am = c.mixed_abserror, a0 = c.opt.abserror;
+    if (!(am > 0.0) || !(a0 > 0.0) || am == a0) return 1.0;
+    const double r = a0 / am;
+    switch (c.opt.mac) {
+    case SERENX_MAC_GADGET: return std::cbrt(r);
+    case SERENX_MAC_GADGET2: return std::sqrt(r);
+    case SERENX_MAC_EIGEN: return std::pow(r, 2.0 / 3.0);
+    default: return 1.0;
+    }
+}
+
 int stock_tree(Ctx &c) {
     hipStream_t s = c.stream;
     const int oct = c.opt.multipole == SERENX_OCTUPOLE_EXACT ? 2
@@ -886,7 +915,7 @@ int stock_tree(Ctx &c) {
     }
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, s, c.nc, c.gn, c.gs,
-                           c.gnf);
+                           mixed_mac_scale(c), c.gnf);
     c.box_ok = true;   // the hydro boxes hold the current positions
     SX_HIP(hipGetLastError());
     return SERENX_OK;
@@ -895,7 +924,7 @@ int stock_tree(Ctx &c) {
 int refresh_f32_nodes(Ctx &c) {
     if (!c.tree_ok || c.nc <= 0) return SERENX_OK;
     hipLaunchKernelGGL(k_fill_f32_nodes, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, c.gn,
-                       c.gs, c.gnf);
+                       c.gs, mixed_mac_scale(c), c.gnf);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }

@@ -175,6 +175,14 @@ STEP_CASES = {
     # KERNEL=QUINTIC, h_fac 1.0
     "step_rk_r2_2k_quad_geo_quintic_nlev3": (2048, "r2", 16, 2.0, False, "geometric", 3, 8, "",
                                              1.0, "isothermal", "quintic", 1.0),
+    # SPH=STANDARD with THERMAL=ENERGY_EQN: h_gather from the advanced h,
+    # hydro.F90's du/dt in the u advance, the energy time-step criterion
+    "step_rk_r2_2k_std_quad_geo_energy_nlev3": (2048, "r2", 17, 2.0, False, "geometric", 3, 8,
+                                                "", 1.0, "energy", "m4", 1.2, "standard"),
+    # SPH=STANDARD with HFIND=CONSTANT (every h = hmin = 0.15)
+    "step_rk_uniform2k_std_quad_geo_consth_nlev3": (2048, "uniform", 18, 1.0, False,
+                                                    "geometric", 3, 8, "", 1.0, "isothermal",
+                                                    "m4", 1.2, "standard", 0.15),
 }
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
@@ -185,12 +193,17 @@ def step_case_data(spec):
     timestep, dt_fixed = spec[8:10] if len(spec) > 8 else ("", 1.0)
     thermal = spec[10] if len(spec) > 10 else "isothermal"
     kernel, h_fac = spec[11:13] if len(spec) > 11 else ("m4", 1.2)
+    sph = spec[13] if len(spec) > 13 else "gradh"
+    hmin = spec[14] if len(spec) > 14 else 0.0
     pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
-    ref = RefSeren(octu, mac, timestep=timestep, thermal=thermal, kernel=kernel)
+    ref = RefSeren(octu, mac, timestep=timestep, thermal=thermal, kernel=kernel, sph=sph,
+                   hfind="constant" if hmin > 0 else "number")
     ref.setup(pos, mass, h, vel, h_fac=h_fac, thetamaxsqd=0.25, abserror=1e-4)
+    if hmin > 0:
+        ref.set_hmin(hmin)
     if thermal == "energy":
         ref.set_energy(energy0(n, seed), GAMMA)
-    ref.force_eval(CS, hguess=True)
+    ref.force_eval(CS, hguess=hmin == 0)
     ref.step_setup(nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
                    dt_fixed=dt_fixed)
     ints, reals, nact = [], [], []
@@ -208,6 +221,8 @@ def step_case_data(spec):
         timestep=np.array(timestep), dt_fixed=np.array(dt_fixed), nminneib=sp["nminneib"],
         thermal=np.array(thermal), **energy_out(ref, n, seed, thermal),
         **({} if kernel == "m4" else dict(kernel=np.array(kernel), h_fac=np.array(h_fac))),
+        **({} if sph == "gradh" else dict(sph=np.array(sph))),
+        **({} if hmin == 0 else dict(hmin=np.array(hmin))),
         step_ints=np.array(ints, np.int64), step_reals=np.array(reals),
         step_nactive=np.array(nact, np.int64),
         porig=p["porig"], pos=p["pos"], v=p["v"], a=p["a"], h=p["h"], rho=p["rho"],
@@ -226,6 +241,40 @@ def run_step_case(name, spec):
     d = step_case_data(spec)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
     print(name, "levels", np.bincount(d["nlevel"]), "time", d["step_reals"][-1][0])
+
+
+# diagnostics.F90 (DEBUG_DIAGNOSTICS builds) after the setup evaluation, and
+# after RK steps: name: (n, profile, seed, mach, thermal, steps)
+DIAG_CASES = {
+    "diag_r2_2k_quad_geo": (2048, "r2", 21, 2.0, "isothermal", 0),
+    "diag_uniform2k_quad_geo_steps": (2048, "uniform", 22, 1.0, "isothermal", 6),
+    "diag_r2_2k_quad_geo_energy": (2048, "r2", 23, 2.0, "energy", 0),
+}
+
+
+def run_diag_case(name, spec):
+    n, prof, seed, mach, thermal, nsteps = spec
+    pos, vel, mass, h = ics.make_cloud(n, prof, seed=seed, mach=mach, cs=CS, ngrid=32)
+    ref = RefSeren(False, "geometric", thermal=thermal, diag=True)
+    ref.setup(pos, mass, h, vel, thetamaxsqd=0.25, abserror=1e-4)
+    if thermal == "energy":
+        ref.set_energy(energy0(n, seed), GAMMA)
+    ref.force_eval(CS, hguess=True)
+    if nsteps:
+        ref.step_setup(nlevels=3, nbuildstep=8, accel_mult=0.5, courant_mult=0.2, dt_fixed=1.0)
+        for _ in range(nsteps):
+            ref.integrate(CS)
+    d = ref.diagnostics()
+    vec = np.concatenate([[d["mtot"]], d["rcom"], d["vcom"], d["mom"], d["angmom"], d["force"],
+                          [d["ketot"], d["gpetot"], d["utot"], d["etot"]]])
+    out = dict(spec=np.array([n, seed, nsteps], np.int64), profile=np.array(prof),
+               mach=np.array(mach), thermal=np.array(thermal), pos=pos, vel=vel, mass=mass,
+               diag=vec)
+    if thermal == "energy":
+        out["u0"] = energy0(n, seed)
+        out["gamma"] = np.array(GAMMA)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "etot", d["etot"], "mom", d["mom"])
 
 
 def kernel_tables(kernel):
@@ -249,6 +298,9 @@ def main(names=None):
     for name, spec in STEP_CASES.items():
         if not names or name in names:
             run_step_case(name, spec)
+    for name, spec in DIAG_CASES.items():
+        if not names or name in names:
+            run_diag_case(name, spec)
     for name, kernel in TABLES.items():
         if not names or name in names:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), tables=kernel_tables(kernel))

@@ -68,6 +68,33 @@ def test_h_is_the_pp_gather_th_neighbour(n, profile, pp_gather, kernel):
         eng.close()
 
 
+def test_h_in_a_lattice_shell_of_more_than_64_points():
+    """pp_gather inside a cubic-lattice shell of 72 points at one r^2 (26
+    spacings^2; 514 points lie closer): the band of r^2 values cannot be
+    narrowed below 64 values by bisection, and the pp_gather-th neighbour is
+    that shell, exactly as an insertion sort of the whole band finds it."""
+    g = np.arange(24, dtype=np.float64) / 16.0          # exact coordinates and r^2
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    n = len(pos)
+    mass = np.full(n, 1.0 / n)
+    eng = _engine(550)
+    try:
+        eng.load(pos, np.zeros_like(pos), mass, np.zeros(n))
+        eng.force_eval(hguess=True)
+        d = eng.fetch()
+        rng = np.random.default_rng(9)
+        idx = rng.choice(n, 1200, replace=False)
+        want = _kth_h(d["pos"], idx, 550, 0.5)
+        got = d["h"][idx]
+        print(f"  {int(np.sum(got != want))} of {len(idx)} differ", flush=True)
+        assert np.array_equal(got, want)
+        inner = np.all((d["pos"] > 6 / 16) & (d["pos"] < 17 / 16), axis=1)
+        assert inner.sum() > 500
+        assert np.all(d["h"][inner] == 0.5 * np.sqrt(26.0 / 256.0))
+    finally:
+        eng.close()
+
+
 def test_density_of_a_lattice_particle():
     """A particle in a cubic lattice with its neighbours at known distances:
     rho = sum m W(r, h) over r < 2h with the tabulated M4 kernel, within the

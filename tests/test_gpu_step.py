@@ -42,18 +42,22 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STEP_CASES = ["step_rk_uniform2k_quad_geo_nlev3", "step_rk_r2_2k_oct_gadget2_nlev1",
               "step_rk_r2_2k_quad_geo_nlev6_restricted_1", "step_rk_r2_2k_quad_geo_nlev6_fixed_2",
-              "step_rk_uniform2k_quad_geo_energy_nlev3", "step_rk_r2_2k_quad_geo_quintic_nlev3"]
+              "step_rk_uniform2k_quad_geo_energy_nlev3", "step_rk_r2_2k_quad_geo_quintic_nlev3",
+              "step_rk_r2_2k_std_quad_geo_energy_nlev3",
+              "step_rk_uniform2k_std_quad_geo_consth_nlev3"]
 # reference build -> (levels, neib_check)
 TIMESTEP_BUILDS = {"": ("adaptive", 0), "restricted_1": ("restricted", 1),
                    "fixed_2": ("fixed", 2)}
 STEP_KEYS = ["n", "nsteps", "nresync", "level_max", "level_step", "nbuild", "nstock"]
 
 
-def _engine(octupole, mac, thermal="isothermal", kernel="m4", h_fac=1.2):
+def _engine(octupole, mac, thermal="isothermal", kernel="m4", h_fac=1.2, sph="gradh",
+            hmin=0.0):
     from seren_amd.engine import ForceEngine
+    kw = dict(sph=sph, hmin=hmin)   # hmin > 0: HFIND=CONSTANT
     return ForceEngine(multipole="octupole" if octupole else "quadrupole", mac=mac,
                        precision="fp64", thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS,
-                       thermal=thermal, kernel=kernel, h_fac=h_fac)
+                       thermal=thermal, kernel=kernel, h_fac=h_fac, **kw)
 
 
 @pytest.mark.parametrize("name", STEP_CASES)
@@ -65,13 +69,16 @@ def test_rk_steps_match_reference_golden(name):
     energy = str(g["thermal"]) == "energy"
     kernel = str(g["kernel"]) if "kernel" in g.files else "m4"
     h_fac = float(g["h_fac"]) if "h_fac" in g.files else 1.2
-    eng = _engine(bool(octu), str(g["mac"]), "energy" if energy else "isothermal", kernel, h_fac)
+    sph = str(g["sph"]) if "sph" in g.files else "gradh"
+    hmin = float(g["hmin"]) if "hmin" in g.files else 0.0
+    eng = _engine(bool(octu), str(g["mac"]), "energy" if energy else "isothermal", kernel, h_fac,
+                  sph, hmin)
     try:
         h0 = np.zeros(n)
         eng.load(g["pos0"], g["vel0"], g["mass"], h0)
         if energy:
             eng.set_energy(g["u0"])
-        eng.force_eval(hguess=True)
+        eng.force_eval(hguess=hmin == 0)
         eng.step_begin("rk", nlevels=nlevels, nbuildstep=8, accel_mult=0.5, courant_mult=0.2,
                        levels=levels, dt_fixed=float(g["dt_fixed"]), neib_check=check)
         for k in range(nsteps):
