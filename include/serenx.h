@@ -247,6 +247,52 @@ int serenx_cloud_offsets(const serenx_ctx *ctx, int32_t *offsets);
  * loop interleave FP64 guard steps with mixed-precision steps. */
 int serenx_set_precision(serenx_ctx *ctx, int32_t precision);
 
+/* FP64 conservation diagnostics of the loaded particles, the sums of
+ * diagnostics.F90:85-134 (isothermal or energy-equation thermal energy; no
+ * sinks), reduced on the device in a fixed order (deterministic):
+ *   out[0] mtot, out[1..3] rcom, out[4..6] vcom, out[7..9] linear momentum,
+ *   out[10..12] angular momentum, out[13..15] net force sum m a,
+ *   out[16] ketot = sum m v^2 / 2, out[17] gpetot = sum m gpot / 2,
+ *   out[18] utot (3/2 sum m P/rho isothermal, sum m u with the energy
+ *   equation), out[19] etot = ketot - gpetot + utot.
+ * Allocates its partial-sum buffer on the device for the call. */
+int serenx_diagnostics(serenx_ctx *ctx, double *out);
+
+/* Batched serving steps (no single counterpart in the reference: each call
+ * does for every selected cloud of a forest what sph_integrate.F90 does for
+ * one cloud with the LFKDK integrator, one time-step level and adaptive
+ * steps, advance_leapfrog_kdk.F90:32-116 with timestep_size.F90:92-114).
+ * serenx_serve_begin, after the setup evaluation (serenx_force_eval with
+ * SERENX_EVAL_HGUESS) of a cloud or a forest (serenx_load_forest), takes
+ * every cloud's first step size: the minimum of timestep_size over its
+ * particles.  serenx_serve_step advances the clouds with select[k] != 0
+ * (ncloud entries) by one kick-drift-kick step of their own size: kick and
+ * drift with the last acceleration, one force evaluation (tree rebuilt) of
+ * their particles, the second kick, and their next step sizes.  Other
+ * clouds do not move, and no walk leaves its own cloud, so a cloud's state
+ * does not depend on which clouds share a call.  Isothermal option only.
+ * serenx_serve_info copies each cloud's next step size and its time. */
+int serenx_serve_begin(serenx_ctx *ctx, double accel_mult, double courant_mult);
+int serenx_serve_step(serenx_ctx *ctx, int32_t ncloud, const uint8_t *select);
+int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time);
+
+/* Tuning knobs of a context (no counterpart in the reference: they choose
+ * how the kernels run, and the autotuner in seren_amd/autotune.py sets them).
+ *  SERENX_TUNE_HSOLVE_SLOTS: potential-neighbour leaves recorded per particle
+ *    in the grad-h solve and h_gather lists, 1..96 (default 96).  A particle
+ *    whose list does not fit is summed over the tree walk at the same radius
+ *    (the same leaves in the same order): results do not change, only speed;
+ *    tests lower it to drive that path.
+ *  SERENX_TUNE_MIXED_ABSERROR: multipole-MAC tolerance of the mixed-precision
+ *    gravity walk (GADGET, GADGET2, EIGEN MACs; 0 = the options' abserror).
+ *    The FP64 walk always uses abserror (the reference's interaction sets);
+ *    the mixed walk may trade part of its accuracy margin for speed.  Takes
+ *    effect at the next tree stock.
+ * serenx_get_tuning reads the current value. */
+enum { SERENX_TUNE_HSOLVE_SLOTS = 1, SERENX_TUNE_MIXED_ABSERROR = 2 };
+int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value);
+int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value);
+
 /* Free and total device memory (hipMemGetInfo) of the context's device, and
  * the bytes this context holds. */
 int serenx_memory(const serenx_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes,

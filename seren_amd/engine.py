@@ -53,6 +53,8 @@ EXPORTED = [
     "serenx_default_step_options", "serenx_step_begin", "serenx_step", "serenx_step_get_info",
     "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets", "serenx_set_precision",
     "serenx_memory", "serenx_pow_third", "serenx_set_energy", "serenx_fetch_energy",
+    "serenx_set_tuning", "serenx_get_tuning", "serenx_serve_begin", "serenx_serve_step",
+    "serenx_serve_info", "serenx_diagnostics",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -64,7 +66,17 @@ _SPH = {"gradh": 0, "standard": 1}
 
 
 class SerenxError(RuntimeError):
-    pass
+    """A non-zero status of the library; code is the SERENX_E* value."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+# status codes (include/serenx.h)
+EINVAL, EHIP, ELMAX, ESTATE, ENOMEM, EHNEG, EFEW = 1, 2, 3, 4, 5, 6, 7
+# tuning keys (serenx_set_tuning)
+TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR = 1, 2
 
 
 class Options(C.Structure):
@@ -127,6 +139,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_load_forest.argtypes = [vp, C.c_int32, _ip, _dp, _dp, _dp, _dp]
     lib.serenx_cloud_offsets.argtypes = [vp, _ip]
     lib.serenx_set_precision.argtypes = [vp, C.c_int32]
+    lib.serenx_diagnostics.argtypes = [vp, _dp]
+    lib.serenx_serve_begin.argtypes = [vp, C.c_double, C.c_double]
+    lib.serenx_serve_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_uint8)]
+    lib.serenx_serve_info.argtypes = [vp, C.c_int32, _dp, _dp]
+    lib.serenx_set_tuning.argtypes = [vp, C.c_int32, C.c_double]
+    lib.serenx_get_tuning.argtypes = [vp, C.c_int32, _dp]
     lib.serenx_memory.argtypes = [vp] + [C.POINTER(C.c_int64)] * 3
     lib.serenx_set_agravmag.argtypes = [vp, _dp]
     lib.serenx_force_eval.argtypes = [vp, C.c_int32]
@@ -154,7 +172,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 def _check(rc: int) -> None:
     if rc != 0:
         msg = load_library().serenx_last_error().decode(errors="replace")
-        raise SerenxError(f"serenx error {rc}: {msg}")
+        raise SerenxError(f"serenx error {rc}: {msg}", rc)
 
 
 def _dptr(a: Optional[np.ndarray]):
@@ -256,6 +274,48 @@ class ForceEngine:
     def set_precision(self, precision: str) -> None:
         """'fp64' or 'mixed' for later evaluations (guard steps)."""
         _check(self.lib.serenx_set_precision(self.ctx, _PREC[precision]))
+
+    DIAG_KEYS = ("mtot", "rcom", "vcom", "mom", "angmom", "force", "ketot", "gpetot", "utot",
+                 "etot")
+
+    def diagnostics(self) -> Dict[str, np.ndarray]:
+        """FP64 conservation sums of the loaded particles (diagnostics.F90)."""
+        v = np.zeros(20)
+        _check(self.lib.serenx_diagnostics(self.ctx, _dptr(v)))
+        return self._diag_dict(v)
+
+    @classmethod
+    def _diag_dict(cls, v):
+        return {"mtot": v[0], "rcom": v[1:4].copy(), "vcom": v[4:7].copy(), "mom": v[7:10].copy(),
+                "angmom": v[10:13].copy(), "force": v[13:16].copy(), "ketot": v[16],
+                "gpetot": v[17], "utot": v[18], "etot": v[19]}
+
+    # ---- batched serving steps (serenx_serve_*) ---------------------------
+    def serve_begin(self, accel_mult: float = 0.5, courant_mult: float = 0.2) -> None:
+        """After the setup evaluation: every cloud's first step size."""
+        _check(self.lib.serenx_serve_begin(self.ctx, accel_mult, courant_mult))
+
+    def serve_step(self, select) -> None:
+        """One KDK step of the clouds with select[k] true, each with its own
+        step size (synchronous)."""
+        sel = np.ascontiguousarray(np.asarray(select, np.uint8))
+        _check(self.lib.serenx_serve_step(self.ctx, len(sel),
+                                          sel.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def serve_info(self, ncloud: int):
+        """Per-cloud next step size and time."""
+        dt, t = np.zeros(ncloud), np.zeros(ncloud)
+        _check(self.lib.serenx_serve_info(self.ctx, ncloud, _dptr(dt), _dptr(t)))
+        return dt, t
+
+    def set_tuning(self, key: int, value: float) -> None:
+        """serenx_set_tuning: TUNE_HSOLVE_SLOTS or TUNE_MIXED_ABSERROR."""
+        _check(self.lib.serenx_set_tuning(self.ctx, key, float(value)))
+
+    def get_tuning(self, key: int) -> float:
+        v = C.c_double()
+        _check(self.lib.serenx_get_tuning(self.ctx, key, C.byref(v)))
+        return v.value
 
     def memory(self) -> Dict[str, int]:
         """Free / total device bytes and the bytes this context holds."""

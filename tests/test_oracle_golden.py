@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz"))
                if not os.path.basename(p).startswith("kernel_tables")
-               and not os.path.basename(p).startswith("step_"))
+               and not os.path.basename(p).startswith("step_")
+               and not os.path.basename(p).startswith("diag_"))
 CS = 0.2
 
 
