@@ -51,11 +51,19 @@ def parse():
     ap.add_argument("--clouds", type=int, default=256, help="forest mode: number of clouds")
     ap.add_argument("--requests", type=int, default=2000, help="serve mode: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve mode: requests per second")
-    ap.add_argument("--slots", type=int, default=8, help="serve mode: concurrent requests")
     ap.add_argument("--pool", type=int, default=4, help="serve mode: resident clouds per size")
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--profile", default="r2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
+                    help="step/eval modes: time FP64, mixed, or both (the headline: mixed "
+                         "value beside the FP64 value)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false", default=True,
+                    help="mixed leg: keep the reference MAC tolerance (no autotuning)")
+    ap.add_argument("--mixed-abserror", type=float, default=0.0,
+                    help="MAC tolerance of the mixed walk (0: the autotuned one, or abserror)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="step/eval modes: requests of the bounded serving leg (0: none)")
+    ap.add_argument("--precision-serve", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--multipole", default="octupole")
     ap.add_argument("--mac", default="gadget2")
     ap.add_argument("--abserror", type=float, default=1e-4)
@@ -82,17 +90,17 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n}), flush=True)
 
 
-def measured_traffic(workload):
-    """HBM bytes per gravity-walk dispatch from the committed rocprofv3 counter
-    record of this workload (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel
-    calibration), or None when no record matches."""
+def measured_traffic(kernel, n):
+    """HBM bytes per dispatch of the walk kernel from the committed rocprofv3
+    counter record of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by
+    a copy-kernel calibration), or None when no record matches."""
     import glob
     for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
         try:
             rec = json.load(open(f))
         except (OSError, ValueError):
             continue
-        if rec.get("workload") == workload:
+        if rec.get("kernel") == kernel and rec.get("n") == n:
             return rec.get("traffic_bytes_per_dispatch")
     return None
 
@@ -135,9 +143,18 @@ def bench_forest(args):
     print(f"# generating {args.clouds} clouds, {ns.sum()} particles", file=sys.stderr, flush=True)
     clouds = [ics.make_cloud(int(n), "uniform", seed=args.seed + 1 + k, mach=float(m), cs=0.2,
                              ngrid=32) for k, (n, m) in enumerate(zip(ns, machs))]
-    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=args.precision,
+    prec = "mixed" if args.precision == "both" else args.precision
+    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=prec,
                       thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
     off = eng.load_forest(clouds)
+    tuned = None
+    if prec == "mixed" and args.tune:
+        from seren_amd.autotune import tune_mixed
+        eng.force_eval(hguess=True)
+        tuned = tune_mixed(eng, target_rms=1e-4, safety=0.5, seed=args.seed)
+        print(f"# autotuned mixed MAC tolerance {tuned['abserror']:g}", file=sys.stderr,
+              flush=True)
+        off = eng.load_forest(clouds)
     eng.force_eval(hguess=True)
     for _ in range(max(1, args.warmup)):
         eng.force_eval()
@@ -161,13 +178,14 @@ def bench_forest(args):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None,
-           "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
+           "dtype": "fp64" if prec == "fp64" else "fp32-mixed",
            "data": "synthetic (seeded uniform spheres, P(k)~k^-4 turbulent velocities)",
            "config": {"workload": f"forest of {args.clouds} clouds, N log-uniform "
                                   f"[4096, 262144] (total {ntot}), Mach U(1,5), "
                                   f"{args.multipole} + {args.mac} MAC, one force evaluation "
-                                  f"of the forest per step, {args.precision}",
-                      "clouds": args.clouds, "n_total": ntot, "precision": args.precision},
+                                  f"of the forest per step, {prec}",
+                      "clouds": args.clouds, "n_total": ntot, "precision": prec},
+           "autotune": tuned,
            "phases_ms": {k: float(v / args.steps) for k, v in
                          zip(("tree", "sph", "hydro", "grav"), phases)},
            "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
@@ -176,53 +194,144 @@ def bench_forest(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args):
-    """Serving: Poisson requests (one KDK leapfrog step each) on resident
-    clouds of N in {2k, 16k, 128k, 1M} drawn with p = {0.4, 0.3, 0.2, 0.1},
-    profiles uniform / r^-2 / Plummer, mixed precision with an FP64 guard
-    step every 4, admission by HBM budget; p50 / p99 latency and throughput."""
+def step_leg(args, cloud, precision, tune=False):
+    """Times K force-evaluating units (leapfrog steps, or bare evaluations)
+    of one precision on the cloud; returns the measured figures."""
+    from seren_amd.engine import ForceEngine
+    pos, vel, mass, h = cloud
+    n = len(mass)
+    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=precision,
+                      thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
+    out = {"precision": precision}
+    try:
+        eng.load(pos, vel, mass, h)
+        eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
+        eng.sync()
+        if tune:
+            from seren_amd.autotune import tune_mixed
+            t = tune_mixed(eng, target_rms=1e-4, safety=0.5, seed=args.seed)
+            out["autotune"] = t
+            print(f"# autotuned mixed MAC tolerance {t['abserror']:g}", file=sys.stderr,
+                  flush=True)
+            eng.load(pos, vel, mass, h)              # the timed run starts from the ICs
+            eng.force_eval(hguess=True)
+            eng.sync()
+        counts = np.zeros(4)
+        grav_ms = []
+        phases = np.zeros(4)
+        wstats = None
+
+        def unit():
+            """One timed unit: a leapfrog step (two sph_integrate calls) or a
+            bare force evaluation; returns per-phase ms and interaction counts."""
+            nonlocal wstats
+            ph, cnt, glaunch = np.zeros(4), np.zeros(4), []
+            for _ in range(2 if args.mode == "step" else 1):
+                if args.mode == "step":
+                    eng.step(1)
+                else:
+                    eng.force_eval()
+                t = eng.timings()
+                ph += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
+                c = np.array(eng.interaction_counts(), np.float64)
+                if c[0] > 0:                          # the call that evaluated forces
+                    wstats = eng.walk_stats()
+                    glaunch.append(t["grav"])
+                cnt += c
+            return ph, cnt, glaunch
+
+        if args.mode == "step":
+            eng.step_begin("lfkdk", nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
+        else:
+            eng.force_eval()                         # provides |a| for the multipole MAC
+        for _ in range(args.warmup):
+            unit()
+        eng.sync()
+        print(f"# {precision} warm", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ph, cnt, gl = unit()
+            phases += ph
+            counts += cnt
+            grav_ms.extend(gl)
+        eng.sync()
+        dt = time.perf_counter() - t0
+        out["step_info"] = eng.step_info() if args.mode == "step" else None
+        # accuracy of the tree gravity against the FP64 direct sum on a seeded
+        # subsample of targets (one more, gravity-only, evaluation; untimed)
+        eng.force_eval(hydro=False)
+        a_tree = eng.fetch(neighbours=False)["a"]
+        rng = np.random.default_rng(args.seed + 1)
+        tg = np.sort(rng.choice(n, min(n, args.direct_targets), replace=False))
+        a_dir, _ = eng.direct_gravity(tg.astype(np.int32))
+        err = np.linalg.norm(a_tree[tg] - a_dir, axis=1) / np.linalg.norm(a_dir, axis=1)
+    finally:
+        eng.close()
+    ncell, npp, npp_in_p, npp_in_j = counts
+    flops = ncell * F_CELL[args.multipole] + npp * F_PAIR + 8.0 * npp_in_p + 9.0 * npp_in_j
+    gms = float(np.sum(grav_ms))
+    achieved = flops / (gms * 1e-3) / 1e12
+    peak = MI355X_FP64_TFLOPS if precision == "fp64" else MI355X_FP32_TFLOPS
+    out.update({
+        "value": n * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+        "phases_ms": {k: float(v / args.steps) for k, v in
+                      zip(("tree", "sph", "hydro", "grav"), phases)},
+        "interactions_per_particle": {"cell": ncell / n / args.steps,
+                                      "pair": npp / n / args.steps},
+        "walk_stats": wstats,
+        "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
+                                         "rms_rel": float(np.sqrt(np.mean(err * err))),
+                                         "max_rel": float(err.max())},
+        "roofline": {"kernel": "gravity walk (" + ("k_grav_fp64" if precision == "fp64"
+                                                   else "k_grav_mixed") + ")",
+                     "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "walk_ms_per_eval": gms / max(len(grav_ms), 1)},
+    })
+    return out
+
+
+def serve_leg(args, nrequests, pool=2):
+    """Bounded serving run: Poisson requests at args.rate on resident clouds
+    of N in {2k, 16k, 128k, 1M} (p = 0.4, 0.3, 0.2, 0.1), mixed precision
+    with an FP64 guard step every 4 steps of a cloud, continuous batching in
+    pools; returns latency percentiles and throughput."""
     from seren_amd import ics
     from seren_amd.serve import CloudServer, poisson_run
     sizes, probs = [2048, 16384, 131072, 1048576], [0.4, 0.3, 0.2, 0.1]
     profiles = ["uniform", "r2", "plummer"]
-    srv = CloudServer(hbm_budget=200e9, slots=args.slots, precision=args.precision,
-                      guard_every=4, engine_kw=dict(multipole=args.multipole, mac=args.mac,
-                                                    abserror=args.abserror, sound_speed=0.2))
+    srv = CloudServer(hbm_budget=200e9, precision=args.precision_serve, guard_every=4,
+                      engine_kw=dict(multipole=args.multipole, mac=args.mac,
+                                     abserror=args.abserror, sound_speed=0.2))
     rng = np.random.default_rng(args.seed)
-    ids, p_cloud = [], []
-    print(f"# admitting {len(sizes) * args.pool} clouds", file=sys.stderr, flush=True)
+    ids, p_cloud, data = [], [], {}
     for si, (n, pr) in enumerate(zip(sizes, probs)):
-        for j in range(args.pool):
+        for j in range(pool):
             cid = len(ids)
             prof = profiles[int(rng.integers(3))]
-            pos, vel, mass, h = ics.make_cloud(n, prof, seed=args.seed + 100 + cid, mach=2.0,
-                                               cs=0.2, ngrid=32 if n < 200000 else 64)
-            srv.admit(cid, pos, vel, mass, h)
+            data[cid] = ics.make_cloud(n, prof, seed=args.seed + 100 + cid, mach=2.0, cs=0.2,
+                                       ngrid=32 if n < 200000 else 64)
             ids.append(cid)
-            p_cloud.append(pr / args.pool)
-    srv.start()
-    # warm every cloud once
-    for f in [srv.submit(c) for c in ids]:
-        f.result()
-    print("# warm", file=sys.stderr, flush=True)
-    res = poisson_run(srv, args.rate, args.requests, ids, p_cloud, seed=args.seed)
-    used = srv.used
-    srv.close()
-    out = {"metric": METRIC, "value": res["particles_per_s"], "unit": "particles/s",
-           "n_gpus": 1, "steps": args.requests, "warmup": len(ids),
-           "ms_per_step": res["mean_ms"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None,
-           "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
-           "data": "synthetic (seeded uniform / r^-2 / Plummer clouds, Mach 2)",
-           "config": {"workload": f"serving: Poisson {args.rate:g} req/s, {args.requests} "
-                                  f"requests (one KDK step each) on {len(ids)} resident "
-                                  f"clouds, N in {sizes} with p {probs}, {args.slots} slots, "
-                                  f"{args.precision} with FP64 guard step every 4",
-                      "precision": args.precision, "slots": args.slots},
-           "latency_ms": {"p50": res["p50_ms"], "p99": res["p99_ms"], "mean": res["mean_ms"]},
-           "offered_particles_per_s": res["offered_particles_per_s"],
-           "resident_bytes": used, "wall_s": res["wall_s"]}
-    print(json.dumps(out), flush=True)
+            p_cloud.append(pr / pool)
+    print(f"# admitting {len(ids)} clouds", file=sys.stderr, flush=True)
+    try:
+        srv.admit_many(data)
+        if args.mixed_abserror > 0:
+            from seren_amd.engine import TUNE_MIXED_ABSERROR
+            for p in srv.pools:
+                p.eng.set_tuning(TUNE_MIXED_ABSERROR, args.mixed_abserror)
+        srv.start()
+        for f in [srv.submit(c) for c in ids]:   # warm every cloud once
+            f.result()
+        print("# serving warm", file=sys.stderr, flush=True)
+        res = poisson_run(srv, args.rate, nrequests, ids, p_cloud, seed=args.seed)
+        res["pools"] = len(srv.pools)
+        res["resident_bytes"] = srv.used
+    finally:
+        srv.close()
+    res.update({"sizes": sizes, "probs": probs, "clouds": len(ids),
+                "precision": args.precision_serve, "guard_every": 4,
+                "mixed_abserror": args.mixed_abserror or args.abserror})
+    return res
 
 
 def main():
@@ -237,108 +346,59 @@ def main():
     if args.mode == "forest":
         return bench_forest(args)
     if args.mode == "serve":
-        return bench_serve(args)
+        res = serve_leg(args, args.requests, pool=args.pool)
+        out = {"metric": METRIC, "value": res["particles_per_s"], "unit": "particles/s",
+               "n_gpus": 1, "steps": args.requests, "warmup": 0, "ms_per_step": res["mean_ms"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "fp32-mixed", "data": "synthetic (seeded clouds, Mach 2)",
+               "config": {"workload": f"serving: Poisson {args.rate:g} req/s, {args.requests} "
+                                      "requests (one KDK step each), N in {2k,16k,128k,1M}"},
+               "serving": res}
+        print(json.dumps(out), flush=True)
+        return
     from seren_amd import ics
-    from seren_amd.engine import ForceEngine
     print(f"# generating {args.n} particles ({args.profile}, Mach {args.mach})", file=sys.stderr,
           flush=True)
-    pos, vel, mass, h = ics.make_cloud(args.n, args.profile, seed=args.seed, mach=args.mach,
-                                       cs=0.2, ngrid=128)
-    eng = ForceEngine(multipole=args.multipole, mac=args.mac, precision=args.precision,
-                      thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
-    eng.load(pos, vel, mass, h)
-    eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
-    eng.sync()
-    counts = np.zeros(4)
-    grav_ms = []
-    phases = np.zeros(4)
-    wstats = None
-
-    def unit():
-        """One timed unit: a leapfrog step (two sph_integrate calls) or a
-        bare force evaluation; returns per-phase ms and interaction counts."""
-        nonlocal wstats
-        ph, cnt, glaunch = np.zeros(4), np.zeros(4), []
-        for _ in range(2 if args.mode == "step" else 1):
-            if args.mode == "step":
-                eng.step(1)
-            else:
-                eng.force_eval()
-            t = eng.timings()
-            ph += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
-            c = np.array(eng.interaction_counts(), np.float64)
-            if c[0] > 0:                          # the call that evaluated forces
-                wstats = eng.walk_stats()
-                glaunch.append(t["grav"])
-            cnt += c
-        return ph, cnt, glaunch
-
-    if args.mode == "step":
-        eng.step_begin("lfkdk", nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
-    else:
-        eng.force_eval()                         # provides |a| for the multipole MAC
-    for _ in range(args.warmup):
-        unit()
-    eng.sync()
-    print("# warm", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ph, cnt, gl = unit()
-        phases += ph
-        counts += cnt
-        grav_ms.extend(gl)
-    eng.sync()
-    dt = time.perf_counter() - t0
-    step_info = eng.step_info() if args.mode == "step" else None
-    ncell, npp, npp_in_p, npp_in_j = counts
-    # accuracy of the tree gravity against the FP64 direct sum on a seeded
-    # subsample of targets (one more, gravity-only, evaluation; untimed)
-    eng.force_eval(hydro=False)
-    a_tree = eng.fetch(neighbours=False)["a"]
-    rng = np.random.default_rng(args.seed + 1)
-    tg = np.sort(rng.choice(args.n, min(args.n, args.direct_targets), replace=False))
-    a_dir, _ = eng.direct_gravity(tg.astype(np.int32))
-    err = np.linalg.norm(a_tree[tg] - a_dir, axis=1) / np.linalg.norm(a_dir, axis=1)
+    cloud = ics.make_cloud(args.n, args.profile, seed=args.seed, mach=args.mach, cs=0.2,
+                           ngrid=128)
     n = args.n
-    value = n * args.steps / dt
-    ms_step = 1e3 * dt / args.steps
-    # gravity-walk roofline: algorithmic flops of the force-evaluating
-    # launches / their measured duration (HIP events on the library's stream)
-    flops = ncell * F_CELL[args.multipole] + npp * F_PAIR + 8.0 * npp_in_p + 9.0 * npp_in_j
-    gms = float(np.sum(grav_ms))
-    achieved = flops / (gms * 1e-3) / 1e12
-    peak = MI355X_FP64_TFLOPS if args.precision == "fp64" else MI355X_FP32_TFLOPS
+    legs = {}
+    for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
+        legs[prec] = step_leg(args, cloud, prec, tune=(prec == "mixed" and args.tune))
+        if prec == "mixed" and "autotune" in legs[prec]:
+            args.mixed_abserror = legs[prec]["autotune"]["abserror"]
+    head = legs.get("mixed") or legs["fp64"]
     what = ("KDK leapfrog steps (timestep_size criteria, 1 level, tree build every 8)"
             if args.mode == "step" else "force evaluation")
     workload = (f"{what}, N={n} {args.profile} sphere, Mach {args.mach}, "
-                f"{args.multipole} + {args.mac} MAC (abserror {args.abserror}), {args.precision}")
-    traffic = measured_traffic(workload)
+                f"{args.multipole} + {args.mac} MAC (abserror {args.abserror}), "
+                f"{'+'.join(legs)}")
+    roof = dict(head["roofline"])
+    roof["traffic"] = measured_traffic(roof["kernel"], n)
     out = {
-        "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "metric": METRIC, "value": head["value"], "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp64" if args.precision == "fp64" else "fp32-mixed",
+        "dtype": "fp64" if head["precision"] == "fp64" else "fp32-mixed",
         "data": "synthetic (seeded r^-2 sphere, P(k)~k^-4 turbulent velocities)",
-        "config": {"workload": workload,
-                   "n": n, "profile": args.profile, "multipole": args.multipole,
-                   "mac": args.mac, "abserror": args.abserror, "precision": args.precision},
-        "phases_ms": {k: float(v / args.steps) for k, v in
-                      zip(("tree", "sph", "hydro", "grav"), phases)},
-        "interactions_per_particle": {"cell": ncell / n / args.steps,
-                                      "pair": npp / n / args.steps},
-        "walk_stats": wstats,
-        "step_state": step_info,
-        "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
-                                         "rms_rel": float(np.sqrt(np.mean(err * err))),
-                                         "max_rel": float(err.max())},
-        "roofline": {"kernel": "gravity walk", "bound": "valu", "achieved": achieved,
-                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": traffic},
+        "config": {"workload": workload, "n": n, "profile": args.profile,
+                   "multipole": args.multipole, "mac": args.mac, "abserror": args.abserror,
+                   "precision": head["precision"]},
+        "roofline": roof,
     }
-    eng.close()
+    if "fp64" in legs and "mixed" in legs:
+        out["fp64_value"] = legs["fp64"]["value"]
+        out["mixed_over_fp64"] = legs["mixed"]["value"] / legs["fp64"]["value"]
+        r64 = dict(legs["fp64"]["roofline"])
+        r64["traffic"] = measured_traffic(r64["kernel"], n)
+        out["roofline_fp64"] = r64
+    out["legs"] = legs
+    if args.serve_requests > 0:
+        print("# serving leg", file=sys.stderr, flush=True)
+        out["serving"] = serve_leg(args, args.serve_requests)
     if args.cpu:
         print("# cpu baseline (reference Fortran/OpenMP)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(pos, vel, mass, h, args)
+        out["cpu_baseline"] = cpu_baseline(*cloud, args)
     print(json.dumps(out), flush=True)
 
 

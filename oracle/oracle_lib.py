@@ -60,6 +60,9 @@ VARIANTS = {
     (False, "geometric", "m4", "standard_energy"): "std_quad_geo_ee",
     # SPH=STANDARD, HFIND=CONSTANT
     (False, "geometric", "m4", "standard_consth"): "std_quad_geo_consth",
+    # diagnostics.F90 builds (DEBUG_DIAGNOSTICS)
+    (False, "geometric", "m4", "diag"): "quad_geo_diag",
+    (False, "geometric", "m4", "energy_diag"): "quad_geo_ee_diag",
 }
 
 
@@ -164,7 +167,7 @@ class RefSeren(_Common):
 
     def __init__(self, octupole: bool, mac: str, dropin: bool = False, kernel: str = "m4",
                  timestep: str = "", thermal: str = "isothermal", sph: str = "gradh",
-                 hfind: str = "number"):
+                 hfind: str = "number", diag: bool = False):
         """dropin=True loads the reference force path whose five drivers are
         replaced by the Fortran binding of the GPU library (make -f
         oracle/Makefile.ref dropin): the reference code as the host.
@@ -173,8 +176,12 @@ class RefSeren(_Common):
         builds have neither.  thermal="energy" selects the THERMAL=ENERGY_EQN
         build (set u with set_energy; thermal_iso then runs the reference's
         update_thermal_properties).  sph="standard" selects the SPH=STANDARD
-        HFIND=NUMBER build (h_gather, all_sph, hydro, gravity_sph)."""
-        if sph == "standard":
+        HFIND=NUMBER build (h_gather, all_sph, hydro, gravity_sph).
+        diag=True selects the build with diagnostics.F90 (DEBUG_DIAGNOSTICS)
+        of the quad_geo or quad_geo_ee option set (see diagnostics())."""
+        if diag:
+            key = (bool(octupole), mac, kernel, "energy_diag" if thermal == "energy" else "diag")
+        elif sph == "standard":
             key = (bool(octupole), mac, kernel,
                    "standard_consth" if hfind == "constant" else
                    "standard_energy" if thermal == "energy" else "standard")
@@ -196,6 +203,8 @@ class RefSeren(_Common):
                                       C.c_double, C.c_double]
             lib.ref_thermal_iso.argtypes = [C.c_double]
             lib.ref_direct_gravity.argtypes = [C.c_int, _ip, _dp, _dp]
+            if diag:
+                lib.ref_diagnostics.argtypes = [C.c_int, C.c_char_p]
             lib.ref_set_agravmag.argtypes = [C.c_int, _dp]
             lib.ref_set_h.argtypes = [C.c_int, _dp]
             lib.ref_get_params.argtypes = [_ip, _dp]
@@ -217,6 +226,27 @@ class RefSeren(_Common):
             self._libs[variant] = lib
         self.lib = self._libs[variant]
         self.variant = variant
+
+    DIAG_FIELDS = ("nsteps", "time", "mtot", "rcom", "vcom", "mom", "angmom", "force", "utot",
+                   "gpetot", "ketot", "etot", "Atot")
+
+    def diagnostics(self) -> Dict[str, np.ndarray]:
+        """Runs the reference's diagnostics.F90 on the current state (diag
+        builds only) and reads back the line it writes: nsteps, time, mtot,
+        rcom, vcom, mom, angmom, force, utot, gpetot, ketot, etot, Atot, each
+        to the 11 significant digits of its E18.10 format."""
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = d.rstrip("/") + "/"
+            self.lib.ref_diagnostics(len(path), path.encode())
+            line = open(os.path.join(d, "ref.diag")).read().splitlines()[-1]
+        vals = [float(line[8 + 18 * k:8 + 18 * (k + 1)]) for k in range((len(line) - 8) // 18)]
+        out, k = {"nsteps": int(line[:8])}, 0
+        for name in self.DIAG_FIELDS[1:]:
+            w = 3 if name in ("rcom", "vcom", "mom", "angmom", "force") else 1
+            out[name] = np.array(vals[k:k + w]) if w == 3 else vals[k]
+            k += w
+        return out
 
     def setup(self, pos, mass, h, vel, h_fac=1.2, thetamaxsqd=0.25, abserror=1e-4,
               alpha=1.0):

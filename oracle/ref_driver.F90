@@ -265,6 +265,27 @@ CONTAINS
     !$OMP END PARALLEL DO
   end subroutine ref_direct_gravity
 
+#if defined(DEBUG_DIAGNOSTICS)
+! ----------------------------------------------------------------------------
+! diagnostics.F90 (DEBUG_DIAGNOSTICS): appends its sums to <dir>ref.diag.
+! The scales are unity under DIMENSIONLESS (units.F90:37-49).
+  subroutine ref_diagnostics(nc, dir) bind(C, name="ref_diagnostics")
+    use filename_module, only : run_dir, run_id
+    use scaling_module
+    integer(c_int), value :: nc
+    character(kind=c_char), intent(in) :: dir(nc)
+    integer :: i
+    run_dir = ""
+    do i=1,nc
+       run_dir(i:i) = dir(i)
+    end do
+    run_id = "ref"
+    rscale = 1.0_DP ; mscale = 1.0_DP ; vscale = 1.0_DP ; tscale = 1.0_DP
+    momscale = 1.0_DP ; angmomscale = 1.0_DP ; Escale = 1.0_DP
+    call diagnostics
+  end subroutine ref_diagnostics
+#endif
+
 ! ----------------------------------------------------------------------------
 ! Setters (state between calls, e.g. |a| from a previous evaluation)
 ! ----------------------------------------------------------------------------

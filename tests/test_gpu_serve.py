@@ -1,11 +1,20 @@
-"""GPU: the serving loop (seren_amd.serve) with continuous batching.
+"""GPU: the serving loop (seren_amd.serve) with continuous batching into
+shared device epochs.
 
-Several resident clouds receive interleaved force-evaluation-plus-step
-requests from concurrent worker slots, in mixed precision with FP64 guard
-steps.  Each cloud's state afterwards (positions, velocities, h, rho,
-accelerations, time-step levels) equals, bit for bit, the state after the
-same number of requests run on that cloud alone: results do not depend on
-what else is in flight.  Admission stays within the HBM budget.
+* Mix: clouds of 2k, 16k, 128k and 1M particles (the serving workload's
+  sizes) in pools receive interleaved force-evaluation-plus-step requests,
+  in mixed precision with FP64 guard steps; requests that arrive together
+  are stepped by one library call per pool (an epoch).  Each cloud's state
+  afterwards (positions, velocities, h, rho, accelerations, its step size
+  and time) equals, bit for bit, its state after the same number of
+  requests served with the cloud alone: results do not depend on what else
+  is in flight.
+* The serving step is the reference's step: a cloud advanced by
+  serenx_serve_step follows serenx_step with the LFKDK integrator, one
+  adaptive time-step level and the tree rebuilt every step
+  (sph_integrate.F90, advance_leapfrog_kdk.F90, timestep_size.F90) to
+  rounding.
+* Admission stays within the HBM budget and waits for released memory.
 """
 import os
 import sys
@@ -18,46 +27,88 @@ from helpers import cloud  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-CLOUDS = {0: (2048, "uniform", 41), 1: (4096, "r2", 42), 2: (3000, "plummer", 43),
-          3: (2048, "r2", 44)}
-REQUESTS = [0, 1, 2, 1, 3, 0, 2, 2, 1, 3, 0, 1, 3, 2, 0, 1]
-KEYS = ("pos", "v", "h", "rho", "a", "nlevel", "nlast", "v_half")
+CLOUDS = {0: (2048, "uniform", 41), 1: (2048, "r2", 42), 2: (2048, "plummer", 43),
+          3: (16384, "r2", 44), 4: (16384, "uniform", 46), 5: (131072, "plummer", 47),
+          6: (1048576, "r2", 48)}
+REQUESTS = [0, 1, 2, 1, 3, 0, 2, 5, 2, 1, 3, 6, 0, 1, 4, 3, 2, 0, 1, 4, 5, 6, 0]
+KEYS = ("porig", "pos", "v", "h", "rho", "a", "gpot")
 
 
-def _server(slots):
+def _server(budget=200e9):
     from seren_amd.serve import CloudServer
-    return CloudServer(hbm_budget=8e9, slots=slots, precision="mixed", guard_every=2)
+    return CloudServer(hbm_budget=budget, precision="mixed", guard_every=2)
 
 
 def test_results_same_alone_and_in_the_mix():
-    print("case serve 4 clouds, 16 requests, 3 slots", flush=True)
+    print(f"case serve {len(CLOUDS)} clouds (2k-1M), {len(REQUESTS)} requests", flush=True)
     data = {k: cloud(*v) for k, v in CLOUDS.items()}
-    mix = _server(3)
+    mix = _server()
     try:
-        for k, (pos, vel, mass, h) in data.items():
-            mix.admit(k, pos, vel, mass, h)
+        mix.admit_many(data)
         assert mix.used <= mix.budget
+        assert len(mix.pools) == 4          # 2k x3, 16k x2, 128k, 1M
         mix.start()
         futs = [mix.submit(k) for k in REQUESTS]
         for f in futs:
-            f.result(timeout=120)
+            f.result(timeout=300)
         got = {k: mix.state(k) for k in data}
-        assert sorted(mix.nsteps.values()) == sorted(REQUESTS.count(k) for k in data)
+        assert mix.nsteps == {k: REQUESTS.count(k) for k in data}
+        print(f"  {mix.epochs()} epochs for {len(REQUESTS)} requests", flush=True)
     finally:
         mix.close()
     assert mix.used == 0
     for k, (pos, vel, mass, h) in data.items():
-        alone = _server(1)
+        alone = _server()
         try:
             alone.admit(k, pos, vel, mass, h)
             alone.start()
             for _ in range(REQUESTS.count(k)):
-                alone.submit(k).result(timeout=120)
+                alone.submit(k).result(timeout=300)
             ref = alone.state(k)
         finally:
             alone.close()
+        print(f"  cloud {k} N={len(mass)}: t={ref['time']:.4e}", flush=True)
         for key in KEYS:
             assert np.array_equal(got[k][key], ref[key]), (k, key)
+        assert got[k]["dt"] == ref["dt"] and got[k]["time"] == ref["time"]
+
+
+def test_serving_step_is_the_kdk_step_of_the_reference():
+    from seren_amd.engine import ForceEngine
+    pos, vel, mass, h = cloud(4096, "r2", 49)
+    kw = dict(multipole="octupole", mac="gadget2", precision="fp64", abserror=1e-4,
+              sound_speed=0.2)
+    nsteps = 4
+    a = ForceEngine(**kw)
+    try:
+        a.load(pos, vel, mass, h)
+        a.force_eval(hguess=True)
+        a.serve_begin(0.5, 0.2)
+        for _ in range(nsteps):
+            a.serve_step([1])
+        da = a.fetch(neighbours=False)
+        dt, t = a.serve_info(1)
+    finally:
+        a.close()
+    b = ForceEngine(**kw)
+    try:
+        b.load(pos, vel, mass, h)
+        b.force_eval(hguess=True)
+        b.step_begin("lfkdk", nlevels=1, nbuildstep=1, accel_mult=0.5, courant_mult=0.2,
+                     levels="adaptive", neib_check=0)
+        b.step(2 * nsteps + 1)   # the last call evaluates forces and kicks
+        info = b.step_info()
+        st = b.fetch_step()
+        db = b.fetch(neighbours=False)
+    finally:
+        b.close()
+    assert np.array_equal(da["porig"], db["porig"])
+    scale = np.abs(st["r_old"]).max()
+    ex = np.abs(da["pos"] - st["r_old"]).max() / scale
+    ev = np.abs(da["v"] - st["v_old"]).max() / np.abs(st["v_old"]).max()
+    et = abs(t[0] - (info["time"] - info["dt_max"] / 2)) / t[0]
+    print(f"  |dx| {ex:.2e}  |dv| {ev:.2e}  time {et:.2e}", flush=True)
+    assert ex <= 1e-12 and ev <= 1e-12 and et <= 1e-12
 
 
 def test_admission_waits_for_memory():
@@ -65,13 +116,13 @@ def test_admission_waits_for_memory():
     print("case serve admission budget", flush=True)
     pos, vel, mass, h = cloud(2048, "uniform", 45)
     need = CloudServer.estimate_bytes(2048)
-    srv = CloudServer(hbm_budget=1.5 * need, slots=1)
+    srv = CloudServer(hbm_budget=1.5 * need)
     try:
         srv.admit(0, pos, vel, mass, h)
         with pytest.raises(MemoryError):
             srv.admit(1, pos, vel, mass, h, timeout=0.2)
         srv.release(0)
         srv.admit(1, pos, vel, mass, h, timeout=5)
-        assert 1 in srv.engines and 0 not in srv.engines
+        assert 1 in srv.pool_of and 0 not in srv.pool_of
     finally:
         srv.close()
