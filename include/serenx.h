@@ -288,8 +288,23 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *    The FP64 walk always uses abserror (the reference's interaction sets);
  *    the mixed walk may trade part of its accuracy margin for speed.  Takes
  *    effect at the next tree stock.
+ *  SERENX_TUNE_WALK_QUEUE: v >= 1 runs the gravity walk as a persistent
+ *    kernel of at most v blocks per CU (default 64: as many as fit) whose
+ *    waves take groups of 64 particles from per-XCD work queues, each ordered
+ *    by the cost of the group's previous walk (longest first); fewer blocks
+ *    leave room for the SPH kernels that run beside the walk.  0 launches one
+ *    wave per group in tree order.  Results do not change.
+ *  SERENX_TUNE_NEIB_GROUP: 1 (default) builds the neighbour lists with one
+ *    walk per wave that opens every cell any of its particles could need; 0
+ *    with the particles' own walks (BHhydro_walk.F90's test per particle).
+ *    The lists are the same.
  * serenx_get_tuning reads the current value. */
-enum { SERENX_TUNE_HSOLVE_SLOTS = 1, SERENX_TUNE_MIXED_ABSERROR = 2 };
+enum {
+    SERENX_TUNE_HSOLVE_SLOTS = 1,
+    SERENX_TUNE_MIXED_ABSERROR = 2,
+    SERENX_TUNE_WALK_QUEUE = 3,
+    SERENX_TUNE_NEIB_GROUP = 4
+};
 int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value);
 int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value);
 

@@ -12,6 +12,23 @@ namespace serenx {
 
 static thread_local std::string g_last_error;
 
+namespace {
+__global__ void k_zero_words(size_t nwords, uint32_t *__restrict__ p) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+}  // namespace
+
+int zero_words(void *p, size_t nwords, hipStream_t s) {
+    if (nwords == 0) return SERENX_OK;
+    const int blocks = (int)std::min<size_t>((nwords + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, s, nwords,
+                       static_cast<uint32_t *>(p));
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
+}
+
 int set_error(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
@@ -230,6 +247,25 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.gay, nn));
     SX_TRY(dalloc(&c.gaz, nn));
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
+    // work queue of the gravity walk: one entry per group of 64 particles
+    const size_t ng = ((size_t)n + WAVE - 1) / WAVE;
+    Ctx::WalkQueue &q = c.gq;
+    SX_TRY(dalloc(&q.ctr, (size_t)NXCD));
+    SX_TRY(dalloc(&q.cost, ng));
+    SX_TRY(dalloc(&q.order, ng));
+    SX_TRY(dalloc(&q.vals, ng));
+    SX_TRY(dalloc(&q.keys, ng));
+    SX_TRY(dalloc(&q.keys2, ng));
+    size_t tb = 0;
+    SX_HIP(rocprim::radix_sort_pairs(nullptr, tb, q.keys, q.keys2, q.vals, q.order, (unsigned)ng,
+                                     0, 35, c.stream));
+    if (q.tmp) (void)hipFree(q.tmp);
+    q.tmp = nullptr;
+    if (hipMalloc(&q.tmp, tb + 256) != hipSuccess)
+        return set_error(SERENX_ENOMEM, "hipMalloc failed for the walk queue's sort space");
+    q.tmp_bytes = tb + 256;
+    q.ngroups = 0;
+    q.have_cost = false;
     SX_TRY(ensure_tree_buffers(c));
     return SERENX_OK;
 }
@@ -248,7 +284,8 @@ void free_all(Ctx &c) {
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
                     c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band,
-                    c.grec, c.gax, c.gay, c.gaz};
+                    c.grec, c.gax, c.gay, c.gaz, c.gq.ctr, c.gq.cost, c.gq.order,
+                    c.gq.vals, c.gq.keys, c.gq.keys2, c.gq.tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
@@ -493,6 +530,10 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
                        : (int)((4.0 * PI_REF / 3.0) * ipow_seq(c->kg.range * opt->h_fac, 3));
     c->pp_limit = 2 * c->pp_gather;
     c->daux = 10.0 * std::pow(1.7976931348623157e308, -0.2);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+        c->ncu = ncu;
     // two streams at the default priority: the main one and the gravity walk's
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) {
@@ -786,7 +827,7 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
 namespace serenx {
 
 int run_force_eval(Ctx &c, int flags) {
-    SX_HIP(hipMemsetAsync(c.wstat, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
+    SX_TRY(zero_words(c.wstat, 2 * CTR_WORDS, c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     SX_TRY(run_tree_update(c));
     if (flags & SERENX_EVAL_HGUESS) SX_TRY(hguess(c));
@@ -1168,6 +1209,16 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
             return set_error(SERENX_EINVAL, "mixed-walk MAC tolerance must be >= 0");
         c.mixed_abserror = value;
         return SERENX_OK;
+    case SERENX_TUNE_NEIB_GROUP:
+        if (value != 0.0 && value != 1.0)
+            return set_error(SERENX_EINVAL, "neighbour walk mode must be 0 or 1");
+        c.neib_group = value != 0.0;
+        return SERENX_OK;
+    case SERENX_TUNE_WALK_QUEUE:
+        if (!(value >= 0.0 && value <= 64.0) || value != (double)(int)value)
+            return set_error(SERENX_EINVAL, "walk queue blocks per CU must be an integer in 0..64");
+        c.walk_queue = (int)value;
+        return SERENX_OK;
     default:
         return set_error(SERENX_EINVAL, "unknown tuning key");
     }
@@ -1181,6 +1232,8 @@ int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value) {
     case SERENX_TUNE_MIXED_ABSERROR:
         *value = c.mixed_abserror > 0.0 ? c.mixed_abserror : c.opt.abserror;
         return SERENX_OK;
+    case SERENX_TUNE_WALK_QUEUE: *value = c.walk_queue; return SERENX_OK;
+    case SERENX_TUNE_NEIB_GROUP: *value = c.neib_group ? 1.0 : 0.0; return SERENX_OK;
     default: return set_error(SERENX_EINVAL, "unknown tuning key");
     }
 }

@@ -18,6 +18,10 @@
 // cell-relative offset; cell terms are summed in FP32 batches added to FP64
 // sums, pair terms in Neumaier-compensated FP32 sums.  Its walk uses a group
 // MAC (see k_grav_mixed).
+#include <algorithm>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
 #include "walk.h"
 
 namespace serenx {
@@ -258,62 +262,69 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
     const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
-    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
+    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
+    GroupQueue queue, int *__restrict__ cost) {
     constexpr KGeom kg = walk_geom<QUINTIC>();
     // the walk is the critical path while the neighbour and hydro kernels run
     // beside it: its waves take issue priority over theirs on shared SIMDs
     __builtin_amdgcn_s_setprio(2);
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
-    const bool valid = p < n && active[p];
-    double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
-    if (valid) {
-        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
-        rpx = a.x; rpy = a.y; rpz = a.z;
-        invhp = b.y;
-        zo_p = b.z;
-        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
-    }
-    // one accumulator for cell and pair terms (the reference adds the pairs
-    // after the cells, :272-290; the order changes only the rounding)
-    double gx = 0, gy = 0, gz = 0, gp = 0;
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, lvis = 0;   // wave totals
-    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
-    const unsigned vis = group_walk(
-        gn, nc, valid, rr.x, rr.y, lvis,
-        [&](const GNode &g, bool here) -> int {
-            const double dr[3] = {rpx - g.r[0], rpy - g.r[1], rpz - g.r[2]};
-            const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-            bool accept;
-            if (MACT == SERENX_MAC_GEOMETRIC)
-                accept = drsqd > g.dminsqd;
-            else
-                accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-            accept = accept && g.leaf != 1;                    // :128-136, no MAC test
-            ncell += lanes(here && accept);
-            if (here && accept) cell_fp64<OCT>(g, dr, drsqd, gx, gy, gz, gp);
-            if (accept) return WALK_SKIP;
-            return g.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
-        },
-        [&](const GNode &g, bool mine) {
-            for (int i = 0; i < g.leaf; i++) {
-                const int j = g.first + i;
-                const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
-                const bool act = mine && j != p;
-                unsigned in_ = 0;
-                if (act)
-                    in_ = pair_fp64(kg, t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w, b.w,
-                                    b.y, b.z, gx, gy, gz, gp);
-                npp += lanes(act);
-                nin_p += lanes(in_ & 1);
-                nin_j += lanes(in_ & 2);
-            }
-        });
-    if (valid) {
-        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
-        ay[p] = gy;
-        az[p] = gz;
-        gpot[p] = gp;
-        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, vis = 0, lvis = 0;   // wave totals
+    const int home = blockIdx.x % NXCD;
+    int qdone = 0;
+    for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
+        const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
+        const bool valid = p < n && active[p];
+        double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
+        if (valid) {
+            const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
+            rpx = a.x; rpy = a.y; rpz = a.z;
+            invhp = b.y;
+            zo_p = b.z;
+            if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
+        }
+        // one accumulator for cell and pair terms (the reference adds the pairs
+        // after the cells, :272-290; the order changes only the rounding)
+        double gx = 0, gy = 0, gz = 0, gp = 0;
+        const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
+        const unsigned gv = group_walk(
+            gn, nc, valid, rr.x, rr.y, lvis,
+            [&](const GNode &nd, bool here) -> int {
+                const double dr[3] = {rpx - nd.r[0], rpy - nd.r[1], rpz - nd.r[2]};
+                const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                bool accept;
+                if (MACT == SERENX_MAC_GEOMETRIC)
+                    accept = drsqd > nd.dminsqd;
+                else
+                    accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
+                accept = accept && nd.leaf != 1;                   // :128-136, no MAC test
+                ncell += lanes(here && accept);
+                if (here && accept) cell_fp64<OCT>(nd, dr, drsqd, gx, gy, gz, gp);
+                if (accept) return WALK_SKIP;
+                return nd.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
+            },
+            [&](const GNode &nd, bool mine) {
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = nd.first + i;
+                    const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
+                    const bool act = mine && j != p;
+                    unsigned in_ = 0;
+                    if (act)
+                        in_ = pair_fp64(kg, t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w,
+                                        b.w, b.y, b.z, gx, gy, gz, gp);
+                    npp += lanes(act);
+                    nin_p += lanes(in_ & 1);
+                    nin_j += lanes(in_ & 2);
+                }
+            });
+        vis += gv;
+        if ((threadIdx.x & 63) == 0) cost[g] = (int)gv;
+        if (valid) {
+            ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+            ay[p] = gy;
+            az[p] = gz;
+            gpot[p] = gp;
+            if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+        }
     }
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
@@ -407,85 +418,94 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     double *__restrict__ gpot, double *__restrict__ agm,
-    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat) {
+    unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
+    GroupQueue queue, int *__restrict__ cost) {
     constexpr KGeom kg64 = walk_geom<QUINTIC>();
     constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
     // the walk is the critical path while the neighbour and hydro kernels run
     // beside it: its waves take issue priority over theirs on shared SIMDs
     __builtin_amdgcn_s_setprio(2);
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
-    const bool valid = p < n && active[p];
-    double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
-    float invhp = 1.0f, zo_p = 0.0f;
-    if (valid) {
-        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
-        rpx = a.x; rpy = a.y; rpz = a.z;
-        invhp = (float)b.y;
-        zo_p = (float)b.z;
-        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
-    }
-    double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
-    float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
-    int nbatch = 0;                                   // wave-uniform
-    CSum px, py, pz, pq;                              // pair terms
     unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, vis = 0, lvis = 0;   // wave totals
-    // the walk covers the lanes' trees (one DFS range, or a few adjacent ones
-    // when the wave straddles clouds); a lane takes part only inside its own
-    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
-    int c = uniform(wave_min_int(valid ? rr.x : nc));
-    const int cend = uniform(wave_max_int(valid ? rr.y : 0));
-    while (c < cend) {
-        const GNodeF g = gn[c];
-        vis++;
-        const bool mine = valid && c >= rr.x && c < rr.y;
-        lvis += lanes(mine);
-        const double d0 = rpx - g.r[0], d1 = rpy - g.r[1], d2 = rpz - g.r[2];
-        const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
-        bool accept;
-        if (MACT == SERENX_MAC_GEOMETRIC)
-            accept = drsqd > g.dminsqd;
-        else
-            accept = drsqd > g.mac * afactor && drsqd > g.dminsqd;
-        accept = accept && g.leaf != 1;                  // :128-136
-        if (__ballot(mine && !accept) == 0) {            // every lane accepts
-            ncell += lanes(mine);
-            if (mine) {
-                const float dr[3] = {(float)d0, (float)d1, (float)d2};
-                cell_fp32<OCT>(g, dr, bx, by, bz, bp);
-            }
-            if (++nbatch == CELL_BATCH) {
-                cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
-                bx = by = bz = bp = 0.0f;
-                nbatch = 0;
-            }
-            c = g.next;
-        } else if (g.leaf > 0) {
-            for (int i = 0; i < g.leaf; i++) {
-                const int j = g.first + i;
-                const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
-                const bool act = mine && j != p;
-                unsigned in_ = 0;
-                if (act)
-                    in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
-                                    (float)(a.z - rpz), (float)a.w, (float)b.w, (float)b.y,
-                                    (float)b.z, px, py, pz, pq);
-                npp += lanes(act);
-                nin_p += lanes(in_ & 1);
-                nin_j += lanes(in_ & 2);
-            }
-            c = g.next;
-        } else {
-            c = c + 1;
+    const int home = blockIdx.x % NXCD;
+    int qdone = 0;
+    for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
+        const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
+        const bool valid = p < n && active[p];
+        double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
+        float invhp = 1.0f, zo_p = 0.0f;
+        if (valid) {
+            const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
+            rpx = a.x; rpy = a.y; rpz = a.z;
+            invhp = (float)b.y;
+            zo_p = (float)b.z;
+            if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
         }
-    }
-    cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
-    if (valid) {
-        const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
-        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
-        ay[p] = gy;
-        az[p] = gz;
-        gpot[p] = cp + pq.value();
-        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+        double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
+        float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
+        int nbatch = 0;                                   // wave-uniform
+        CSum px, py, pz, pq;                              // pair terms
+        unsigned gv = 0;
+        // the walk covers the lanes' trees (one DFS range, or a few adjacent
+        // ones when the group straddles clouds); a lane takes part only inside
+        // its own
+        const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
+        int c = uniform(wave_min_int(valid ? rr.x : nc));
+        const int cend = uniform(wave_max_int(valid ? rr.y : 0));
+        while (c < cend) {
+            const GNodeF nd = gn[c];
+            gv++;
+            const bool mine = valid && c >= rr.x && c < rr.y;
+            lvis += lanes(mine);
+            const double d0 = rpx - nd.r[0], d1 = rpy - nd.r[1], d2 = rpz - nd.r[2];
+            const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
+            bool accept;
+            if (MACT == SERENX_MAC_GEOMETRIC)
+                accept = drsqd > nd.dminsqd;
+            else
+                accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
+            accept = accept && nd.leaf != 1;                 // :128-136
+            if (__ballot(mine && !accept) == 0) {            // every lane accepts
+                ncell += lanes(mine);
+                if (mine) {
+                    const float dr[3] = {(float)d0, (float)d1, (float)d2};
+                    cell_fp32<OCT>(nd, dr, bx, by, bz, bp);
+                }
+                if (++nbatch == CELL_BATCH) {
+                    cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
+                    bx = by = bz = bp = 0.0f;
+                    nbatch = 0;
+                }
+                c = nd.next;
+            } else if (nd.leaf > 0) {
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = nd.first + i;
+                    const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
+                    const bool act = mine && j != p;
+                    unsigned in_ = 0;
+                    if (act)
+                        in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx),
+                                        (float)(a.y - rpy), (float)(a.z - rpz), (float)a.w,
+                                        (float)b.w, (float)b.y, (float)b.z, px, py, pz, pq);
+                    npp += lanes(act);
+                    nin_p += lanes(in_ & 1);
+                    nin_j += lanes(in_ & 2);
+                }
+                c = nd.next;
+            } else {
+                c = c + 1;
+            }
+        }
+        cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
+        vis += gv;
+        if ((threadIdx.x & 63) == 0) cost[g] = (int)gv;
+        if (valid) {
+            const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
+            ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+            ay[p] = gy;
+            az[p] = gz;
+            gpot[p] = cp + pq.value();
+            if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+        }
     }
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
@@ -495,6 +515,25 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         atomicAdd(ic + 3, (unsigned long long)nin_j);
     }
     wave_stats(wstat, 0, vis, lvis);
+}
+
+// Work-queue order of the walk's groups: keys (range, ~cost) so that an
+// ascending sort lists every XCD range's groups by decreasing cost of their
+// previous walk; with no cost yet the keys are the ranges alone (a stable
+// sort keeps tree order within each)
+__global__ void k_queue_keys(int ngroups, const int *__restrict__ cost, int have_cost,
+                             uint64_t *__restrict__ keys, int *__restrict__ vals) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t range = (uint64_t)(((long long)NXCD * g) / ngroups);
+    const uint32_t c = have_cost ? (uint32_t)max(cost[g], 0) : 0u;
+    keys[g] = (range << 32) | (uint64_t)(0xFFFFFFFFu - c);
+    vals[g] = g;
+}
+
+__global__ void k_zero_ints(int n, int *__restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
 
 // direct_sph_gravity.F90:63-79: sum of gravity_gradh over all other
@@ -549,29 +588,67 @@ __global__ void __launch_bounds__(256) k_direct(KGeom kg, int n, int nt,
     }
 }
 
+// Blocks of the walk launch: with the work queue, as many as the device
+// holds at once (each wave then takes group after group); without it, one
+// wave per group.
+template <class K>
+int walk_blocks(Ctx &c, K kernel) {
+    const int per_wave = (c.gq.ngroups + 3) / 4;
+    if (c.walk_queue <= 0) return std::max(per_wave, 1);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TB, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    per_cu = std::min(per_cu, c.walk_queue);
+    return std::max(1, std::min(per_wave, per_cu * c.ncu));
+}
+
 template <int OCT, int MACT, bool QUINTIC>
 void launch_walk(Ctx &c, const double4 *rec) {
-    if (c.opt.precision == SERENX_MIXED)
-        hipLaunchKernelGGL((k_grav_mixed<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
-                           c.gstream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
+    GroupQueue q{c.gq.ctr, c.gq.order, c.gq.ngroups};
+    if (c.opt.precision == SERENX_MIXED) {
+        auto k = k_grav_mixed<OCT, MACT, QUINTIC>;
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
+                           rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
-                           c.gaz, c.gpot, c.agm, c.icount, c.wstat);
-    else
-        hipLaunchKernelGGL((k_grav_fp64<OCT, MACT, QUINTIC>), dim3(nblk(c.n)), dim3(TB), 0,
-                           c.gstream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
+                           c.gaz, c.gpot, c.agm, c.icount, c.wstat, q, c.gq.cost);
+    } else {
+        auto k = k_grav_fp64<OCT, MACT, QUINTIC>;
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
+                           rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
-                           c.gaz, c.gpot, c.agm, c.icount, c.wstat);
+                           c.gaz, c.gpot, c.agm, c.icount, c.wstat, q, c.gq.cost);
+    }
+}
+
+// the queue order of this walk (from the previous walk's costs) and zeroed
+// counters, on the gravity stream
+int prepare_queue(Ctx &c) {
+    Ctx::WalkQueue &q = c.gq;
+    const int ng = (c.n + WAVE - 1) / WAVE;
+    if (ng != q.ngroups) q.have_cost = false;   // another group layout
+    q.ngroups = ng;
+    hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,
+                       (int)(q.have_cost && c.walk_queue > 0), q.keys, q.vals);
+    size_t tb = q.tmp_bytes;
+    SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order, (unsigned)ng,
+                                     0, 35, c.gstream));
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.gstream, NXCD, q.ctr);
+    SX_HIP(hipGetLastError());
+    return SERENX_OK;
 }
 
 template <int OCT, int MACT>
-void launch_grav(Ctx &c) {
+int launch_grav(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.grec);
     hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range, c.x,
                        c.y, c.z, c.m, c.h, c.zeta, rec);
+    SX_TRY(prepare_queue(c));
     if (c.kg.range == walk_geom<true>().range)
         launch_walk<OCT, MACT, true>(c, rec);
     else
         launch_walk<OCT, MACT, false>(c, rec);
+    return SERENX_OK;
 }
 
 }  // namespace
@@ -601,23 +678,24 @@ int grav_join(Ctx &c) {
 }
 
 int grav_forces(Ctx &c) {
-    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.gstream));
+    SX_TRY(zero_words(c.icount, 2 * CTR_WORDS, c.gstream));
     const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
     switch (c.opt.mac) {
     case SERENX_MAC_GEOMETRIC:
-        oct ? launch_grav<1, SERENX_MAC_GEOMETRIC>(c) : launch_grav<0, SERENX_MAC_GEOMETRIC>(c);
+        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GEOMETRIC>(c) : launch_grav<0, SERENX_MAC_GEOMETRIC>(c)));
         break;
     case SERENX_MAC_GADGET:
-        oct ? launch_grav<1, SERENX_MAC_GADGET>(c) : launch_grav<0, SERENX_MAC_GADGET>(c);
+        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GADGET>(c) : launch_grav<0, SERENX_MAC_GADGET>(c)));
         break;
     case SERENX_MAC_EIGEN:
-        oct ? launch_grav<1, SERENX_MAC_EIGEN>(c) : launch_grav<0, SERENX_MAC_EIGEN>(c);
+        SX_TRY((oct ? launch_grav<1, SERENX_MAC_EIGEN>(c) : launch_grav<0, SERENX_MAC_EIGEN>(c)));
         break;
     default:
-        oct ? launch_grav<1, SERENX_MAC_GADGET2>(c) : launch_grav<0, SERENX_MAC_GADGET2>(c);
+        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GADGET2>(c) : launch_grav<0, SERENX_MAC_GADGET2>(c)));
         break;
     }
     SX_HIP(hipGetLastError());
+    c.gq.have_cost = true;
     return SERENX_OK;
 }
 

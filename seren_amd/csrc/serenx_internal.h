@@ -57,6 +57,9 @@ constexpr size_t KT_H4_OFF = (KT_G4F_OFF + 2 * (size_t)KT + 3) / 4 * 4;
 constexpr size_t KT_TOTAL = KT_H4_OFF + 4 * (size_t)KT;
 
 constexpr int WAVE = 64;
+// XCDs of an MI355X; the dispatcher deals a launch's workgroups to them
+// round-robin (workgroup i to XCD i % NXCD)
+constexpr int NXCD = 8;
 // potential-neighbour list slots (opened leaves) per particle in the h solve
 constexpr int HL_SLOTS = 96;
 // standard SPH: r^2 values of the h_gather band kept per particle for the
@@ -190,6 +193,7 @@ struct Ctx {
     double *gax = nullptr, *gay = nullptr, *gaz = nullptr;   // gravity acceleration
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     int hl_cap = HL_SLOTS;       // slots used per lane (SERENX_TUNE_HSOLVE_SLOTS)
+    bool neib_group = true;      // SERENX_TUNE_NEIB_GROUP: one neighbour walk per wave
     double mixed_abserror = 0.0; // mixed walk's MAC tolerance (0: opt.abserror)
     double *band = nullptr;      // standard SPH: BAND_MAX x 64 r^2 values per wave
     size_t band_cap = 0;         // particles the band buffer holds
@@ -246,6 +250,21 @@ struct Ctx {
     double *ktab = nullptr;
     double host_ktab[NTAB * KT];
 
+    // work queue of the persistent gravity walk (walk.h: GroupQueue)
+    struct WalkQueue {
+        int ngroups = 0;
+        int *ctr = nullptr;          // NXCD counters
+        int *cost = nullptr;         // per group: cells its last walk visited
+        int *order = nullptr;        // group ids by range, then decreasing cost
+        uint64_t *keys = nullptr, *keys2 = nullptr;
+        int *vals = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        bool have_cost = false;      // cost holds a walk of the current group layout
+    } gq;
+    int walk_queue = 64;             // SERENX_TUNE_WALK_QUEUE: blocks per CU (0: no queue)
+    int ncu = 1;                     // compute units of the device
+
     // interaction counters of the gravity walk
     unsigned long long *icount = nullptr;
     // walk statistics of the last force evaluation (serenx_walk_stats)
@@ -294,6 +313,11 @@ struct Ctx {
     std::vector<double> hbuf;   // staging for host-state transfers
     std::vector<int> hibuf;
 };
+
+// Zeroes nwords 32-bit words at p on stream s with a fill kernel (capi.hip).
+// The force path clears its flags and counters this way rather than with
+// hipMemsetAsync, whose nodes have given wrong results on graph replays.
+int zero_words(void *p, size_t nwords, hipStream_t s);
 
 // error reporting (capi.hip)
 int set_error(int code, const std::string &msg);

@@ -462,6 +462,132 @@ __global__ void __launch_bounds__(256) k_neib(KGeom kg, bool box, int n, const i
     if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
 }
 
+// wave-wide minimum / maximum of a double (every lane gets the result)
+__device__ __forceinline__ double wave_min_dbl(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max_dbl(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// get_neib.F90:46-118 with one walk per wave instead of one per particle.
+// Every particle j with r^2 <= KERNRANGESQD h_p^2 or r^2 <= KERNRANGESQD
+// h_j^2 lies in a leaf that the reference's BHhydro_walk opens for p: every
+// ancestor's bounding sphere holds j, so |r_p - r_c| <= |r_p - r_j| + rmax_c,
+// and hrangemax_c >= KERNRANGE h_j.  The reference keeps exactly those
+// particles, in DFS order.  Any walk that opens a superset of those leaves,
+// in DFS order, therefore yields the reference's lists once each candidate
+// passes the reference's pair test (formed as get_neib.F90:89-95 forms it).
+// This walk opens a cell for the whole wave when any lane could need it: the
+// distance from the bounding box of the wave's particles to the cell centre
+// is within max(R_max, hrangemax) + rmax (R_max the wave's largest KERNRANGE
+// h), with a relative margin for rounding, and with fresh boxes the two
+// boxes lie within max(R_max, hrangemax) of each other.  Visits are
+// wave-uniform: no lane keeps walk state of its own.
+template <bool OVERFLOW_PASS>
+__global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
+                                                    const int *__restrict__ ovf_idx, int nc,
+                                                    const HNode *__restrict__ hn,
+                                                    const double4 *__restrict__ rec,
+                                                    const int *__restrict__ cid,
+                                                    const int2 *__restrict__ root_range,
+                                                    const uint8_t *__restrict__ active,
+                                                    int pp_limit, int *__restrict__ pptot,
+                                                    int *__restrict__ pplist,
+                                                    const int *__restrict__ ovf_off,
+                                                    int *__restrict__ ovf_list,
+                                                    int *__restrict__ ovf_count,
+                                                    unsigned long long *__restrict__ wstat) {
+    constexpr double MARGIN = 1.0 + 1.0e-9;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int ot = (t / WAVE) * OVF_PER_WAVE + (t % WAVE);
+    const bool ovf_lane = (t % WAVE) < OVF_PER_WAVE && ot < n;
+    const int p = OVERFLOW_PASS ? (ovf_lane ? ovf_idx[ot] : 0) : t;
+    const bool valid = OVERFLOW_PASS ? ovf_lane : (p < n && active[p]);
+    double rpx = 0, rpy = 0, rpz = 0, hp = 0;
+    if (valid) {
+        const double4 a = rec[p];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        hp = a.w;
+    }
+    if (__ballot(valid) == 0) return;
+    const double hrangesqd_p = kg.rangesqd * hp * hp;
+    const double R = kg.range * hp;
+    // the wave's bounding box (centre g, half extents e) and largest radius
+    const double inf = __builtin_huge_val();
+    const double lox = wave_min_dbl(valid ? rpx : inf), hix = wave_max_dbl(valid ? rpx : -inf);
+    const double loy = wave_min_dbl(valid ? rpy : inf), hiy = wave_max_dbl(valid ? rpy : -inf);
+    const double loz = wave_min_dbl(valid ? rpz : inf), hiz = wave_max_dbl(valid ? rpz : -inf);
+    const double gx = 0.5 * (lox + hix), gy = 0.5 * (loy + hiy), gz = 0.5 * (loz + hiz);
+    const double ex = (hix - lox) * (0.5 * MARGIN), ey = (hiy - loy) * (0.5 * MARGIN),
+                 ez = (hiz - loz) * (0.5 * MARGIN);
+    const double Rmax = wave_max_dbl(valid ? R : 0.0);
+    int tot = 0;
+    int *row = nullptr;
+    int lim = pp_limit;   // entries this lane may store
+    if (valid) {
+        row = OVERFLOW_PASS ? ovf_list + ovf_off[p] : pplist + (size_t)p * pp_limit;
+        if (OVERFLOW_PASS) lim = pptot[p];
+    }
+    const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
+    int c = uniform(wave_min_int(rr.x));
+    const int cend = uniform(wave_max_int(valid ? rr.y : 0));
+    const unsigned nvalid = lanes(valid);
+    unsigned vis = 0, lvis = 0;
+    while (c < cend) {
+        const HNode nd = hn[c];
+        vis++;
+        lvis += nvalid;
+        const double reach = fmax(Rmax, nd.hrangemax);
+        // box of the wave to the cell centre
+        const double dx = fmax(fabs(nd.r[0] - gx) - ex, 0.0);
+        const double dy = fmax(fabs(nd.r[1] - gy) - ey, 0.0);
+        const double dz = fmax(fabs(nd.r[2] - gz) - ez, 0.0);
+        const double lim1 = (reach + nd.rmax) * MARGIN;
+        bool open = dx * dx + dy * dy + dz * dz <= lim1 * lim1;
+        if (open && box) {
+            // box of the wave to the cell's (grown) box
+            const double bx = fmax(fabs(nd.r[0] - gx) - ex - (double)nd.half[0], 0.0);
+            const double by = fmax(fabs(nd.r[1] - gy) - ey - (double)nd.half[1], 0.0);
+            const double bz = fmax(fabs(nd.r[2] - gz) - ez - (double)nd.half[2], 0.0);
+            const double lim2 = reach * MARGIN;
+            open = bx * bx + by * by + bz * bz <= lim2 * lim2;
+        }
+        if (!open) {
+            c = nd.next;
+            continue;
+        }
+        if (nd.leaf == 0) {
+            c = c + 1;
+            continue;
+        }
+        const bool mine = valid && c >= rr.x && c < rr.y;
+        const int k = nd.leaf, f0 = nd.first;
+        for (int i = 0; i < k; i++) {
+            const int j = f0 + i;
+            const double4 aj = rec[j];   // one scalar load per candidate
+            if (!mine || j == p) continue;
+            const double hrangesqd_pp = kg.rangesqd * aj.w * aj.w;
+            const double qx = aj.x - rpx, qy = aj.y - rpy, qz = aj.z - rpz;
+            const double dsq = qx * qx + qy * qy + qz * qz;
+            if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
+                if (tot < lim) row[tot] = j;
+                tot++;
+            }
+        }
+        c = nd.next;
+    }
+    if (valid && !OVERFLOW_PASS) {
+        pptot[p] = tot;
+        if (tot > pp_limit) atomicAdd(ovf_count, 1);
+    }
+    if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
+}
+
 __global__ void k_ovf_sizes(int n, int pp_limit, const int *__restrict__ pptot,
                             const uint8_t *__restrict__ active, int *__restrict__ sz,
                             int *__restrict__ is_ovf) {
@@ -977,7 +1103,7 @@ int sph_hsolve(Ctx &c) {
     if (c.n < c.pp_gather)
         return set_error(SERENX_EFEW, "fewer particles than pp_gather: the grad-h solve "
                                       "cannot reach its neighbour number");
-    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    SX_TRY(zero_words(c.flag, 1, c.stream));
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.m, rec);
@@ -1015,7 +1141,7 @@ int sph_hgather(Ctx &c) {
             return set_error(SERENX_ENOMEM, "hipMalloc failed for the h_gather band buffer");
         c.band_cap = need;
     }
-    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    SX_TRY(zero_words(c.flag, 1, c.stream));
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     hipLaunchKernelGGL(k_pack_hsolve, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.m, rec);
@@ -1049,8 +1175,9 @@ int sph_neighbours(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     // [0] x y z h with the solved h
     hipLaunchKernelGGL(k_pack_xyzh, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.x, c.y, c.z, c.h, rec);
-    SX_HIP(hipMemsetAsync(c.ovf_count, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_neib<false>, dim3(nblk(c.n)), dim3(TB), 0, s, c.kg, c.box_ok, c.n, c.ovf_idx, c.nc, c.hn,
+    SX_TRY(zero_words(c.ovf_count, 1, s));
+    hipLaunchKernelGGL(c.neib_group ? k_neib_group<false> : k_neib<false>, dim3(nblk(c.n)),
+                       dim3(TB), 0, s, c.kg, c.box_ok, c.n, c.ovf_idx, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
                        c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
@@ -1063,7 +1190,7 @@ int sph_neighbours(Ctx &c) {
         // sizes (flag) and overflow marks (incl; both tree-build scratch)
         hipLaunchKernelGGL(k_ovf_sizes, dim3(nblk(c.n)), dim3(TB), 0, s, c.n, c.pp_limit, c.pptot,
                            c.active, c.flag, c.incl);
-        SX_HIP(hipMemsetAsync(c.flag + c.n, 0, sizeof(int), s));
+        SX_TRY(zero_words(c.flag + c.n, 1, s));
         size_t tb = c.sorttmp_bytes;
         SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.flag, c.ovf_off, 0, (size_t)c.n + 1,
                                        rocprim::plus<int>(), s));
@@ -1085,7 +1212,7 @@ int sph_neighbours(Ctx &c) {
         }
         const int nov = c.hostpin[9];   // overflow particles, counted by k_neib<false>
         const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
-        hipLaunchKernelGGL(k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, c.box_ok, nov, c.ovf_idx, c.nc,
+        hipLaunchKernelGGL(c.neib_group ? k_neib_group<true> : k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, c.box_ok, nov, c.ovf_idx, c.nc,
                            c.hn, rec, c.cid, c.root_range, c.active, c.pp_limit,
                            c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
         SX_HIP(hipGetLastError());

@@ -548,8 +548,8 @@ int step_once(Ctx &c) {
     Ctx::Step &s = c.st;
     // phase markers as serenx_force_eval records them (serenx_last_timings)
     // and walk statistics of this step's force evaluation
-    SX_HIP(hipMemsetAsync(c.wstat, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
-    SX_HIP(hipMemsetAsync(c.icount, 0, CTR_WORDS * sizeof(unsigned long long), c.stream));
+    SX_TRY(zero_words(c.wstat, 2 * CTR_WORDS, c.stream));
+    SX_TRY(zero_words(c.icount, 2 * CTR_WORDS, c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     // tree_update.F90:24-88
     const bool build = s.nbuild == s.nsteps;
@@ -668,7 +668,7 @@ int step_begin(Ctx &c, const serenx_step_options &opt) {
     SX_TRY(dmalloc(&s.nminneib, n));
     SX_TRY(dmalloc(&s.red, CTR_WORDS));
     s.red_host.assign(CTR_WORDS, 0);
-    SX_HIP(hipMemsetAsync(s.nminneib, 0, sizeof(int) * n, c.stream));
+    SX_TRY(zero_words(s.nminneib, n, c.stream));
     s.opt = opt;
     s.on = true;
     hipLaunchKernelGGL(k_step_init, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x, c.y, c.z,

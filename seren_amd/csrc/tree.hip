@@ -809,7 +809,7 @@ int build_tree(Ctx &c) {
         }
     }
     SX_HIP(sort_keys(c.khi, c.khi2, c.idx, c.idx2, 63));
-    SX_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), s));
+    SX_TRY(zero_words(c.flag, 1, s));
     hipLaunchKernelGGL(k_deep_check, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.cid, c.flag);
     SX_HIP(hipMemcpyAsync(c.hostpin, c.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     SX_HIP(hipMemcpyAsync(c.hostpin_d, c.root, 11 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -867,7 +867,7 @@ int build_tree(Ctx &c) {
     c.nc = base;
     for (int L = ltot + 1; L <= LMAX + 1; L++) c.level_off[L] = base, c.level_cnt[L] = 0;
     // DFS layout
-    SX_HIP(hipMemsetAsync(c.ncs + n, 0, sizeof(int), s));
+    SX_TRY(zero_words(c.ncs + n, 1, s));
     tb = c.sorttmp_bytes;
     SX_HIP(rocprim::exclusive_scan(c.sorttmp, tb, c.ncs, c.off, 0, (size_t)n + 1,
                                    rocprim::plus<int>(), s));

@@ -22,7 +22,6 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // eighth of the blocks, so that spatially adjacent leaf groups -- which walk
 // the same cells and read the same particles -- share one L2.  A bijection of
 // [0, gridDim.x) for any grid size.
-constexpr int NXCD = 8;
 __device__ __forceinline__ int xcd_block() {
     const int nb = gridDim.x, i = blockIdx.x;
     const int q = nb / NXCD, r = nb % NXCD, xcd = i % NXCD;
@@ -175,6 +174,40 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
             c = uniform(wave_min_int(resume));
     }
     return visits;
+}
+
+// Work queue of a persistent walk.  A group is 64 consecutive particles in
+// tree order (one wave's targets, as in the one-wave-per-group launch).  The
+// groups are split into NXCD contiguous ranges of the tree order, one per XCD
+// (group g belongs to range 8 g / ngroups), and order[] lists each range's
+// groups by decreasing cost of their previous walk, so that the most
+// expensive groups start first (longest-processing-time order).  A wave of
+// block b takes groups from range b % NXCD -- the blocks of a launch are
+// dealt to the XCDs round-robin, so a range's groups, which are spatially
+// close, share one XCD's L2 -- and steals from the following ranges when its
+// own is empty.  Every call either returns a group or moves to the next
+// range, so every wave leaves the loop after at most ngroups + NXCD calls.
+struct GroupQueue {
+    int *ctr;            // NXCD counters, zero at the launch
+    const int *order;    // ngroups group ids
+    int ngroups;
+};
+
+__device__ __forceinline__ int queue_lo(int q, int ngroups) { return (q * ngroups + NXCD - 1) / NXCD; }
+
+// the next group of the calling wave (wave-uniform), -1 when none is left;
+// q counts the ranges this wave has exhausted
+__device__ __forceinline__ int next_group(const GroupQueue &Q, int home, int &q) {
+    while (q < NXCD) {
+        const int r = (home + q) % NXCD;
+        const int lo = queue_lo(r, Q.ngroups), hi = queue_lo(r + 1, Q.ngroups);
+        int t = 0;
+        if ((threadIdx.x & 63) == 0) t = atomicAdd(Q.ctr + r, 1);
+        t = uniform(__shfl(t, 0, 64));
+        if (lo + t < hi) return Q.order[lo + t];
+        q++;
+    }
+    return -1;
 }
 
 // Adds per-wave walk statistics into stats[k0], stats[k0 + 1] from lane 0:

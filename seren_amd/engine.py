@@ -76,7 +76,7 @@ class SerenxError(RuntimeError):
 # status codes (include/serenx.h)
 EINVAL, EHIP, ELMAX, ESTATE, ENOMEM, EHNEG, EFEW = 1, 2, 3, 4, 5, 6, 7
 # tuning keys (serenx_set_tuning)
-TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR = 1, 2
+TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR, TUNE_WALK_QUEUE, TUNE_NEIB_GROUP = 1, 2, 3, 4
 
 
 class Options(C.Structure):

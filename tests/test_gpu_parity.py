@@ -282,3 +282,34 @@ def _golden_check(case, slots=None):
             assert rel_err(d[k], g[k]).max() <= 1e-12, k
     eng.close()
     return stats
+
+
+@pytest.mark.parametrize("profile,octupole,mac", [("r2", True, "gadget2"),
+                                                   ("plummer", False, "geometric")])
+def test_neighbour_walk_modes_agree(profile, octupole, mac):
+    """The wave-uniform neighbour walk (any superset of the reference's
+    leaves gives the reference's lists after the pair test) against the
+    per-particle walks of BHhydro_walk.F90: identical pptot and pplist, and
+    so identical forces, on a 40k cloud (two evaluations: stale and fresh
+    boxes)."""
+    from seren_amd.engine import TUNE_NEIB_GROUP
+    pos, vel, mass, h = cloud(40000, profile, 77)
+    out = []
+    for mode in (0, 1):
+        eng = _engine(octupole, mac, 0.25, 1e-4)
+        try:
+            eng.set_tuning(TUNE_NEIB_GROUP, mode)
+            eng.load(pos, vel, mass, h)
+            eng.force_eval(hguess=True)
+            eng.force_eval()
+            out.append(eng.fetch())
+        finally:
+            eng.close()
+    a, b = out
+    assert np.array_equal(a["pptot"], b["pptot"])
+    lim = a["pplist"].shape[1]
+    ok = a["pptot"] <= lim
+    assert np.array_equal(a["pplist"][ok], b["pplist"][ok])
+    assert int(np.sum(~ok)) > 0 or profile == "plummer"
+    for k in ("a", "rho", "h"):
+        assert np.array_equal(a[k], b[k]), k
