@@ -369,6 +369,11 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist,
  * summed over particles: counts[0] cell terms, [1] particle-particle terms,
  * [2] pairs inside 2 h_p, [3] pairs inside 2 h_j (gravity_gradh.F90:61,72). */
 int serenx_last_timings(const serenx_ctx *ctx, double *ms4);
+
+/* Durations (ms, HIP events on the gravity stream) of the last evaluation's
+ * two walk phases: [0] traversal and cell terms, [1] near-field pair terms.
+ * Their sum is serenx_last_timings' gravity entry. */
+int serenx_walk_timings(const serenx_ctx *ctx, double *ms2);
 int serenx_interaction_counts(serenx_ctx *ctx, int64_t *counts4);
 
 /* Walk statistics of the last serenx_force_eval (performance diagnostics):

@@ -246,6 +246,13 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.gax, nn));
     SX_TRY(dalloc(&c.gay, nn));
     SX_TRY(dalloc(&c.gaz, nn));
+    SX_TRY(dalloc(&c.gpc, nn));
+    {
+        const size_t ngl = ((size_t)n + WAVE - 1) / WAVE;
+        SX_TRY(dalloc(&c.pl_leaf, ngl * PL_CAP));
+        SX_TRY(dalloc(&c.pl_mask, ngl * PL_CAP));
+        SX_TRY(dalloc(&c.pl_len, ngl));
+    }
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     // work queue of the gravity walk: one entry per group of 64 particles
     const size_t ng = ((size_t)n + WAVE - 1) / WAVE;
@@ -284,7 +291,7 @@ void free_all(Ctx &c) {
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
                     c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band,
-                    c.grec, c.gax, c.gay, c.gaz, c.gq.ctr, c.gq.cost, c.gq.order,
+                    c.grec, c.gax, c.gay, c.gaz, c.gpc, c.pl_leaf, c.pl_mask, c.pl_len, c.gq.ctr, c.gq.cost, c.gq.order,
                     c.gq.vals, c.gq.keys, c.gq.keys2, c.gq.tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -295,6 +302,7 @@ void free_all(Ctx &c) {
     for (auto &e : c.evg)
         if (e) (void)hipEventDestroy(e);
     if (c.evfork) (void)hipEventDestroy(c.evfork);
+    if (c.evfork2) (void)hipEventDestroy(c.evfork2);
     if (c.gstream && c.gstream != c.stream) (void)hipStreamDestroy(c.gstream);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
@@ -338,7 +346,8 @@ int check_options(const serenx_options &o) {
 size_t ctx_device_bytes(const Ctx &c) {
     const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
     size_t b = 0;
-    b += nn * (24 * 8 + 16 * 8 + 11 * 8);                   // SoA, tmpd, hydro/gravity records
+    b += nn * (24 * 8 + 16 * 8 + 12 * 8);                   // SoA, tmpd, hydro/gravity records
+    b += (nn + WAVE - 1) / WAVE * ((size_t)PL_CAP * 12 + 4);  // walk near-field lists
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
@@ -401,6 +410,7 @@ int run_grav(Ctx &c) {
     GravGuard guard(c.gstream);
     guard.armed = true;
     SX_TRY(grav_launch(c));
+    SX_TRY(grav_pairs(c));
     SX_TRY(grav_join(c));
     guard.armed = false;
     return SERENX_OK;
@@ -547,6 +557,7 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
     for (auto &ev : c->evg)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork2, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * KT_TOTAL);
     if (e == hipSuccess) e = hipMalloc((void **)&c->icount, CTR_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->icount, 0, CTR_WORDS * sizeof(unsigned long long));
@@ -836,12 +847,15 @@ int run_force_eval(Ctx &c, int flags) {
     // neighbour lists, thermal and hydro forces (sph_setup's order otherwise)
     const bool grav = !(flags & SERENX_EVAL_NO_GRAV);
     c.grav_timed = false;
-    SX_TRY(run_sph_h(c));
+    // the walk's cell phase beside the h solve, its pair phase beside the
+    // neighbour lists, thermal and hydro forces
     GravGuard guard(c.gstream);
     if (grav) {
         guard.armed = true;
         SX_TRY(grav_launch(c));
     }
+    SX_TRY(run_sph_h(c));
+    if (grav) SX_TRY(grav_pairs(c));
     SX_TRY(run_sph_lists(c));
     SX_TRY(sph_thermal(c));
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
@@ -880,9 +894,26 @@ int serenx_last_timings(const serenx_ctx *ctx, double *ms4) {
     }
     // gravity: the walk's own span on the gravity stream (it overlaps the
     // SPH and hydro phases)
-    float gms = 0.0f;
-    if (c.grav_timed) SX_HIP(hipEventElapsedTime(&gms, c.evg[0], c.evg[1]));
-    ms4[3] = gms;
+    float g1 = 0.0f, g2 = 0.0f;
+    if (c.grav_timed) {
+        SX_HIP(hipEventElapsedTime(&g1, c.evg[0], c.evg[1]));
+        SX_HIP(hipEventElapsedTime(&g2, c.evg[2], c.evg[3]));
+    }
+    ms4[3] = g1 + g2;
+    return SERENX_OK;
+}
+
+int serenx_walk_timings(const serenx_ctx *ctx, double *ms2) {
+    if (!ctx || !ms2) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    ms2[0] = ms2[1] = 0.0;
+    if (!c.grav_timed) return SERENX_OK;
+    SX_HIP(hipEventSynchronize(c.evg[3]));
+    float g1 = 0.0f, g2 = 0.0f;
+    SX_HIP(hipEventElapsedTime(&g1, c.evg[0], c.evg[1]));
+    SX_HIP(hipEventElapsedTime(&g2, c.evg[2], c.evg[3]));
+    ms2[0] = g1;
+    ms2[1] = g2;
     return SERENX_OK;
 }
 

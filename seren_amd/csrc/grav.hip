@@ -255,36 +255,60 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
     cell_terms<OCT, double>(dr[0], dr[1], dr[2], ir, g.m, g.q, g.oc, ax, ay, az, pot);
 }
 
-template <int OCT, int MACT, bool QUINTIC>
-__global__ void __launch_bounds__(256) k_grav_fp64(
-    int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
-    const int *__restrict__ cid, const int2 *__restrict__ root_range,
-    const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
-    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
-    double *__restrict__ gpot, double *__restrict__ agm,
+// Near-field list of a group: the leaves its walk opened, in DFS order, as
+// (first particle << 4) | particle count, with the mask of the lanes that
+// opened each.  Phase 1 of the walk writes it, phase 2 reads it.
+struct NearList {
+    int *leaf;                    // ngroups x PL_CAP entries
+    unsigned long long *mask;     // ngroups x PL_CAP lane masks
+    int *len;                     // per group: entries found (> PL_CAP: list incomplete)
+};
+
+__device__ __forceinline__ void near_append(const NearList &L, int g, int &len, int first,
+                                            int count, unsigned long long mask) {
+    if ((threadIdx.x & 63) == 0 && len < PL_CAP) {
+        L.leaf[(size_t)g * PL_CAP + len] = (first << 4) | count;
+        L.mask[(size_t)g * PL_CAP + len] = mask;
+    }
+    len++;
+}
+
+// The FP64 walk in two phases, so that the traversal and the cell terms --
+// which need only the positions, the tree stocked with the previous h (Q2:
+// BHgrav_stock runs before sph_update) and the previous |a| -- run beside
+// the h solve, and only the kernel-softened pair terms wait for the new h
+// and zeta.  The reference sums the cell terms of the walk first and the
+// near-field pairs of its list afterwards (BHgrav_accel.F90:225-226,
+// 272-290); so does this split, in the same orders.
+//
+// Phase 1: at each cell every active lane applies its own MAC
+// (BHgrav_accel.F90:154-158); accepted cells add a multipole term, opened
+// leaves go to the group's near-field list, opened branches descend.
+template <int OCT, int MACT>
+__global__ void __launch_bounds__(256) k_grav_cells_fp64(
+    int n, int nc, const GNode *__restrict__ gn, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ cid,
+    const int2 *__restrict__ root_range, const uint8_t *__restrict__ active,
+    const double *__restrict__ gpot, const double *__restrict__ agm, double *__restrict__ cx,
+    double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cp, NearList near,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
     GroupQueue queue, int *__restrict__ cost) {
-    constexpr KGeom kg = walk_geom<QUINTIC>();
-    // the walk is the critical path while the neighbour and hydro kernels run
-    // beside it: its waves take issue priority over theirs on shared SIMDs
+    // the walk is the critical path while the SPH kernels run beside it: its
+    // waves take issue priority over theirs on shared SIMDs
     __builtin_amdgcn_s_setprio(2);
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, vis = 0, lvis = 0;   // wave totals
+    unsigned ncell = 0, vis = 0, lvis = 0;   // wave totals
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
         const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
         const bool valid = p < n && active[p];
-        double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0, afactor = 0.0;
+        double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
         if (valid) {
-            const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
-            rpx = a.x; rpy = a.y; rpz = a.z;
-            invhp = b.y;
-            zo_p = b.z;
+            rpx = x[p]; rpy = y[p]; rpz = z[p];
             if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
         }
-        // one accumulator for cell and pair terms (the reference adds the pairs
-        // after the cells, :272-290; the order changes only the rounding)
         double gx = 0, gy = 0, gz = 0, gp = 0;
+        int len = 0;
         const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
         const unsigned gv = group_walk(
             gn, nc, valid, rr.x, rr.y, lvis,
@@ -303,37 +327,107 @@ __global__ void __launch_bounds__(256) k_grav_fp64(
                 return nd.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
             },
             [&](const GNode &nd, bool mine) {
-                for (int i = 0; i < nd.leaf; i++) {
-                    const int j = nd.first + i;
-                    const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
-                    const bool act = mine && j != p;
-                    unsigned in_ = 0;
-                    if (act)
-                        in_ = pair_fp64(kg, t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w,
-                                        b.w, b.y, b.z, gx, gy, gz, gp);
-                    npp += lanes(act);
-                    nin_p += lanes(in_ & 1);
-                    nin_j += lanes(in_ & 2);
-                }
+                near_append(near, g, len, nd.first, nd.leaf, __ballot(mine));
             });
         vis += gv;
-        if ((threadIdx.x & 63) == 0) cost[g] = (int)gv;
+        if ((threadIdx.x & 63) == 0) {
+            cost[g] = (int)gv;
+            near.len[g] = len;
+        }
         if (valid) {
-            ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
-            ay[p] = gy;
-            az[p] = gz;
-            gpot[p] = gp;
-            if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+            cx[p] = gx;
+            cy[p] = gy;
+            cz[p] = gz;
+            cp[p] = gp;
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(icount) + 0, (unsigned long long)ncell);
+    wave_stats(wstat, 0, vis, lvis);
+}
+
+// Phase 2: the pair terms of gravity_gradh.F90 over each lane's opened
+// leaves, in list order, added to the cell sums of phase 1.  A group whose
+// list did not fit repeats its walk (the same MAC decisions: they read the
+// stocked tree and the previous |a|, which phase 2 overwrites only after)
+// and takes the pairs from the leaves as it opens them.  One wave per group.
+template <int MACT, bool QUINTIC>
+__global__ void __launch_bounds__(256) k_grav_pairs_fp64(
+    int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
+    const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ cp, double *__restrict__ gpot, double *__restrict__ agm,
+    NearList near, unsigned long long *__restrict__ icount) {
+    constexpr KGeom kg = walk_geom<QUINTIC>();
+    __builtin_amdgcn_s_setprio(2);
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = p / WAVE, lane = (int)(threadIdx.x & (WAVE - 1));
+    const bool valid = p < n && active[p];
+    if (__ballot(valid) == 0) return;
+    double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0;
+    double gx = 0, gy = 0, gz = 0, gp = 0, afactor = 0.0;
+    if (valid) {
+        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        invhp = b.y;
+        zo_p = b.z;
+        gx = ax[p]; gy = ay[p]; gz = az[p]; gp = cp[p];
+        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
+    }
+    unsigned npp = 0, nin_p = 0, nin_j = 0;
+    auto leaf_pairs = [&](int first, int count, bool mine) {
+        for (int i = 0; i < count; i++) {
+            const int j = first + i;
+            const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
+            const bool act = mine && j != p;
+            unsigned in_ = 0;
+            if (act)
+                in_ = pair_fp64(kg, t4, invhp, zo_p, a.x - rpx, a.y - rpy, a.z - rpz, a.w, b.w,
+                                b.y, b.z, gx, gy, gz, gp);
+            npp += lanes(act);
+            nin_p += lanes(in_ & 1);
+            nin_j += lanes(in_ & 2);
+        }
+    };
+    const int len = uniform(near.len[g]);
+    if (len <= PL_CAP) {
+        for (int e = 0; e < len; e++) {
+            const int d = uniform(near.leaf[(size_t)g * PL_CAP + e]);
+            const unsigned long long m = near.mask[(size_t)g * PL_CAP + e];
+            leaf_pairs(d >> 4, d & 15, valid && ((m >> lane) & 1ull));
+        }
+    } else {
+        const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
+        unsigned lv = 0;
+        (void)group_walk(
+            gn, nc, valid, rr.x, rr.y, lv,
+            [&](const GNode &nd, bool) -> int {
+                const double dr[3] = {rpx - nd.r[0], rpy - nd.r[1], rpz - nd.r[2]};
+                const double drsqd = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                bool accept;
+                if (MACT == SERENX_MAC_GEOMETRIC)
+                    accept = drsqd > nd.dminsqd;
+                else
+                    accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
+                accept = accept && nd.leaf != 1;
+                if (accept) return WALK_SKIP;
+                return nd.leaf > 0 ? WALK_LEAF : WALK_DESCEND;
+            },
+            [&](const GNode &nd, bool mine) { leaf_pairs(nd.first, nd.leaf, mine); });
+    }
+    if (valid) {
+        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+        ay[p] = gy;
+        az[p] = gz;
+        gpot[p] = gp;
+        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+    }
+    if (lane == 0) {
         unsigned long long *ic = ctr_row(icount);
-        atomicAdd(ic + 0, (unsigned long long)ncell);
         atomicAdd(ic + 1, (unsigned long long)npp);
         atomicAdd(ic + 2, (unsigned long long)nin_p);
         atomicAdd(ic + 3, (unsigned long long)nin_j);
     }
-    wave_stats(wstat, 0, vis, lvis);
 }
 
 // ---- mixed precision ------------------------------------------------------
@@ -410,40 +504,49 @@ constexpr int CELL_BATCH = 8;
 // particle's own MAC, so the truncation error is at most the per-particle
 // walk's; the walk is wave-uniform (no per-lane resume state), and no cell
 // term is issued for a cell that only some lanes use.  The FP64 walk keeps
-// the reference's per-particle interaction sets.
-template <int OCT, int MACT, bool QUINTIC>
-__global__ void __launch_bounds__(256) k_grav_mixed(
-    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
-    const int *__restrict__ cid, const int2 *__restrict__ root_range,
-    const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
-    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
-    double *__restrict__ gpot, double *__restrict__ agm,
+// the reference's per-particle interaction sets.  As the FP64 walk, it runs
+// in two phases: cells (beside the h solve), then pairs.
+template <int MACT>
+__device__ __forceinline__ bool group_accepts(const GNodeF &nd, bool mine, double rpx, double rpy,
+                                              double rpz, double afactor, double &d0, double &d1,
+                                              double &d2) {
+    d0 = rpx - nd.r[0];
+    d1 = rpy - nd.r[1];
+    d2 = rpz - nd.r[2];
+    const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
+    bool accept;
+    if (MACT == SERENX_MAC_GEOMETRIC)
+        accept = drsqd > nd.dminsqd;
+    else
+        accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
+    accept = accept && nd.leaf != 1;                 // :128-136
+    return __ballot(mine && !accept) == 0;           // every lane accepts
+}
+
+template <int OCT, int MACT>
+__global__ void __launch_bounds__(256) k_grav_cells_mixed(
+    int n, int nc, const GNodeF *__restrict__ gn, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ cid,
+    const int2 *__restrict__ root_range, const uint8_t *__restrict__ active,
+    const double *__restrict__ gpot, const double *__restrict__ agm, double *__restrict__ ox,
+    double *__restrict__ oy, double *__restrict__ oz, double *__restrict__ op, NearList near,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
     GroupQueue queue, int *__restrict__ cost) {
-    constexpr KGeom kg64 = walk_geom<QUINTIC>();
-    constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
-    // the walk is the critical path while the neighbour and hydro kernels run
-    // beside it: its waves take issue priority over theirs on shared SIMDs
     __builtin_amdgcn_s_setprio(2);
-    unsigned ncell = 0, npp = 0, nin_p = 0, nin_j = 0, vis = 0, lvis = 0;   // wave totals
+    unsigned ncell = 0, vis = 0, lvis = 0;   // wave totals
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
         const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
         const bool valid = p < n && active[p];
         double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
-        float invhp = 1.0f, zo_p = 0.0f;
         if (valid) {
-            const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
-            rpx = a.x; rpy = a.y; rpz = a.z;
-            invhp = (float)b.y;
-            zo_p = (float)b.z;
+            rpx = x[p]; rpy = y[p]; rpz = z[p];
             if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
         }
         double cx = 0.0, cy = 0.0, cz = 0.0, cp = 0.0;   // cell terms
         float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
-        int nbatch = 0;                                   // wave-uniform
-        CSum px, py, pz, pq;                              // pair terms
+        int nbatch = 0, len = 0;                          // wave-uniform
         unsigned gv = 0;
         // the walk covers the lanes' trees (one DFS range, or a few adjacent
         // ones when the group straddles clouds); a lane takes part only inside
@@ -456,15 +559,8 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
             gv++;
             const bool mine = valid && c >= rr.x && c < rr.y;
             lvis += lanes(mine);
-            const double d0 = rpx - nd.r[0], d1 = rpy - nd.r[1], d2 = rpz - nd.r[2];
-            const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
-            bool accept;
-            if (MACT == SERENX_MAC_GEOMETRIC)
-                accept = drsqd > nd.dminsqd;
-            else
-                accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
-            accept = accept && nd.leaf != 1;                 // :128-136
-            if (__ballot(mine && !accept) == 0) {            // every lane accepts
+            double d0, d1, d2;
+            if (group_accepts<MACT>(nd, mine, rpx, rpy, rpz, afactor, d0, d1, d2)) {
                 ncell += lanes(mine);
                 if (mine) {
                     const float dr[3] = {(float)d0, (float)d1, (float)d2};
@@ -477,19 +573,7 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
                 }
                 c = nd.next;
             } else if (nd.leaf > 0) {
-                for (int i = 0; i < nd.leaf; i++) {
-                    const int j = nd.first + i;
-                    const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
-                    const bool act = mine && j != p;
-                    unsigned in_ = 0;
-                    if (act)
-                        in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx),
-                                        (float)(a.y - rpy), (float)(a.z - rpz), (float)a.w,
-                                        (float)b.w, (float)b.y, (float)b.z, px, py, pz, pq);
-                    npp += lanes(act);
-                    nin_p += lanes(in_ & 1);
-                    nin_j += lanes(in_ & 2);
-                }
+                near_append(near, g, len, nd.first, nd.leaf, __ballot(mine));
                 c = nd.next;
             } else {
                 c = c + 1;
@@ -497,24 +581,102 @@ __global__ void __launch_bounds__(256) k_grav_mixed(
         }
         cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
         vis += gv;
-        if ((threadIdx.x & 63) == 0) cost[g] = (int)gv;
+        if ((threadIdx.x & 63) == 0) {
+            cost[g] = (int)gv;
+            near.len[g] = len;
+        }
         if (valid) {
-            const double gx = cx + px.value(), gy = cy + py.value(), gz = cz + pz.value();
-            ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
-            ay[p] = gy;
-            az[p] = gz;
-            gpot[p] = cp + pq.value();
-            if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+            ox[p] = cx;
+            oy[p] = cy;
+            oz[p] = cz;
+            op[p] = cp;
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr_row(icount) + 0, (unsigned long long)ncell);
+    wave_stats(wstat, 0, vis, lvis);
+}
+
+template <int MACT, bool QUINTIC>
+__global__ void __launch_bounds__(256) k_grav_pairs_mixed(
+    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
+    const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ cpot, double *__restrict__ gpot, double *__restrict__ agm,
+    NearList near, unsigned long long *__restrict__ icount) {
+    constexpr KGeom kg64 = walk_geom<QUINTIC>();
+    constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
+    __builtin_amdgcn_s_setprio(2);
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    const int g = p / WAVE, lane = (int)(threadIdx.x & (WAVE - 1));
+    const bool valid = p < n && active[p];
+    if (__ballot(valid) == 0) return;
+    double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
+    float invhp = 1.0f, zo_p = 0.0f;
+    if (valid) {
+        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
+        rpx = a.x; rpy = a.y; rpz = a.z;
+        invhp = (float)b.y;
+        zo_p = (float)b.z;
+        if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
+    }
+    CSum px, py, pz, pq;   // pair terms
+    unsigned npp = 0, nin_p = 0, nin_j = 0;
+    auto leaf_pairs = [&](int first, int count, bool mine) {
+        for (int i = 0; i < count; i++) {
+            const int j = first + i;
+            const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
+            const bool act = mine && j != p;
+            unsigned in_ = 0;
+            if (act)
+                in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
+                                (float)(a.z - rpz), (float)a.w, (float)b.w, (float)b.y,
+                                (float)b.z, px, py, pz, pq);
+            npp += lanes(act);
+            nin_p += lanes(in_ & 1);
+            nin_j += lanes(in_ & 2);
+        }
+    };
+    const int len = uniform(near.len[g]);
+    if (len <= PL_CAP) {
+        for (int e = 0; e < len; e++) {
+            const int d = uniform(near.leaf[(size_t)g * PL_CAP + e]);
+            const unsigned long long m = near.mask[(size_t)g * PL_CAP + e];
+            leaf_pairs(d >> 4, d & 15, valid && ((m >> lane) & 1ull));
+        }
+    } else {
+        // the list did not fit: the same wave-uniform walk again, pairs only
+        const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
+        int c = uniform(wave_min_int(valid ? rr.x : nc));
+        const int cend = uniform(wave_max_int(valid ? rr.y : 0));
+        while (c < cend) {
+            const GNodeF nd = gn[c];
+            const bool mine = valid && c >= rr.x && c < rr.y;
+            double d0, d1, d2;
+            if (group_accepts<MACT>(nd, mine, rpx, rpy, rpz, afactor, d0, d1, d2)) {
+                c = nd.next;
+            } else if (nd.leaf > 0) {
+                leaf_pairs(nd.first, nd.leaf, mine);
+                c = nd.next;
+            } else {
+                c = c + 1;
+            }
+        }
+    }
+    if (valid) {
+        const double gx = ax[p] + px.value(), gy = ay[p] + py.value(), gz = az[p] + pz.value();
+        ax[p] = gx;   // the gravity acceleration; k_add_grav adds it to a
+        ay[p] = gy;
+        az[p] = gz;
+        gpot[p] = cpot[p] + pq.value();
+        if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
+    }
+    if (lane == 0) {
         unsigned long long *ic = ctr_row(icount);
-        atomicAdd(ic + 0, (unsigned long long)ncell);
         atomicAdd(ic + 1, (unsigned long long)npp);
         atomicAdd(ic + 2, (unsigned long long)nin_p);
         atomicAdd(ic + 3, (unsigned long long)nin_j);
     }
-    wave_stats(wstat, 0, vis, lvis);
 }
 
 // Work-queue order of the walk's groups: keys (range, ~cost) so that an
@@ -603,22 +765,39 @@ int walk_blocks(Ctx &c, K kernel) {
     return std::max(1, std::min(per_wave, per_cu * c.ncu));
 }
 
-template <int OCT, int MACT, bool QUINTIC>
-void launch_walk(Ctx &c, const double4 *rec) {
+NearList near_list(Ctx &c) { return NearList{c.pl_leaf, c.pl_mask, c.pl_len}; }
+
+// phase 1 of the walk (traversal and cell terms), persistent over the queue
+template <int OCT, int MACT>
+void launch_cells(Ctx &c) {
     GroupQueue q{c.gq.ctr, c.gq.order, c.gq.ngroups};
     if (c.opt.precision == SERENX_MIXED) {
-        auto k = k_grav_mixed<OCT, MACT, QUINTIC>;
+        auto k = k_grav_cells_mixed<OCT, MACT>;
         hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
-                           rec, c.cid, c.root_range, c.active,
-                           reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
-                           c.gaz, c.gpot, c.agm, c.icount, c.wstat, q, c.gq.cost);
+                           c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
+                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost);
     } else {
-        auto k = k_grav_fp64<OCT, MACT, QUINTIC>;
+        auto k = k_grav_cells_fp64<OCT, MACT>;
         hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
-                           rec, c.cid, c.root_range, c.active,
-                           reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
-                           c.gaz, c.gpot, c.agm, c.icount, c.wstat, q, c.gq.cost);
+                           c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
+                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost);
     }
+}
+
+// phase 2 (pair terms), one wave per group
+template <int MACT, bool QUINTIC>
+void launch_pairs(Ctx &c, const double4 *rec) {
+    const int blocks = nblk((long)c.gq.ngroups * WAVE);
+    if (c.opt.precision == SERENX_MIXED)
+        hipLaunchKernelGGL((k_grav_pairs_mixed<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
+                           c.gstream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
+                           reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount);
+    else
+        hipLaunchKernelGGL((k_grav_pairs_fp64<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
+                           c.gstream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
+                           reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount);
 }
 
 // the queue order of this walk (from the previous walk's costs) and zeroed
@@ -639,63 +818,86 @@ int prepare_queue(Ctx &c) {
 }
 
 template <int OCT, int MACT>
-int launch_grav(Ctx &c) {
+int launch_grav_cells(Ctx &c) {
+    SX_TRY(prepare_queue(c));
+    launch_cells<OCT, MACT>(c);
+    return SERENX_OK;
+}
+
+template <int MACT>
+int launch_grav_pairs(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.grec);
     hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range, c.x,
                        c.y, c.z, c.m, c.h, c.zeta, rec);
-    SX_TRY(prepare_queue(c));
     if (c.kg.range == walk_geom<true>().range)
-        launch_walk<OCT, MACT, true>(c, rec);
+        launch_pairs<MACT, true>(c, rec);
     else
-        launch_walk<OCT, MACT, false>(c, rec);
+        launch_pairs<MACT, false>(c, rec);
     return SERENX_OK;
 }
 
 }  // namespace
 
-// The gravity walk on the context's gravity stream, after everything queued
-// on its main stream so far (the tree and h): it reads the tree, positions,
-// masses, h and zeta, and writes its own buffers (gravity records and
-// acceleration, gpot, agravmag, the interaction counts), so it can run beside
-// the neighbour lists, thermal and hydro forces.  grav_join adds its
+// The gravity walk runs on the context's gravity stream in two phases.
+// grav_launch, queued after the tree is stocked, forks the gravity stream
+// and runs phase 1 (traversal and cell terms: they read the tree, the
+// positions and the previous |a| and potential) beside the h solve.
+// grav_pairs, queued after the h solve, makes the gravity stream wait for
+// it and runs phase 2 (pair terms, which need the new h and zeta), beside
+// the neighbour lists, thermal and hydro forces.  grav_join adds the walk's
 // acceleration to a on the main stream.
 int grav_launch(Ctx &c) {
     SX_HIP(hipEventRecord(c.evfork, c.stream));
     SX_HIP(hipStreamWaitEvent(c.gstream, c.evfork, 0));
     SX_HIP(hipEventRecord(c.evg[0], c.gstream));
-    SX_TRY(grav_forces(c));
+    SX_TRY(zero_words(c.icount, 2 * CTR_WORDS, c.gstream));
+    const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
+    switch (c.opt.mac) {
+    case SERENX_MAC_GEOMETRIC:
+        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GEOMETRIC>(c)
+                    : launch_grav_cells<0, SERENX_MAC_GEOMETRIC>(c)));
+        break;
+    case SERENX_MAC_GADGET:
+        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GADGET>(c)
+                    : launch_grav_cells<0, SERENX_MAC_GADGET>(c)));
+        break;
+    case SERENX_MAC_EIGEN:
+        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_EIGEN>(c)
+                    : launch_grav_cells<0, SERENX_MAC_EIGEN>(c)));
+        break;
+    default:
+        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GADGET2>(c)
+                    : launch_grav_cells<0, SERENX_MAC_GADGET2>(c)));
+        break;
+    }
+    SX_HIP(hipGetLastError());
     SX_HIP(hipEventRecord(c.evg[1], c.gstream));
+    c.grav_timed = false;
+    return SERENX_OK;
+}
+
+int grav_pairs(Ctx &c) {
+    SX_HIP(hipEventRecord(c.evfork2, c.stream));
+    SX_HIP(hipStreamWaitEvent(c.gstream, c.evfork2, 0));
+    SX_HIP(hipEventRecord(c.evg[2], c.gstream));
+    switch (c.opt.mac) {
+    case SERENX_MAC_GEOMETRIC: SX_TRY(launch_grav_pairs<SERENX_MAC_GEOMETRIC>(c)); break;
+    case SERENX_MAC_GADGET: SX_TRY(launch_grav_pairs<SERENX_MAC_GADGET>(c)); break;
+    case SERENX_MAC_EIGEN: SX_TRY(launch_grav_pairs<SERENX_MAC_EIGEN>(c)); break;
+    default: SX_TRY(launch_grav_pairs<SERENX_MAC_GADGET2>(c)); break;
+    }
+    SX_HIP(hipGetLastError());
+    SX_HIP(hipEventRecord(c.evg[3], c.gstream));
+    c.gq.have_cost = true;
     c.grav_timed = true;
     return SERENX_OK;
 }
 
 int grav_join(Ctx &c) {
-    SX_HIP(hipStreamWaitEvent(c.stream, c.evg[1], 0));
+    SX_HIP(hipStreamWaitEvent(c.stream, c.evg[3], 0));
     hipLaunchKernelGGL(k_add_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active, c.gax,
                        c.gay, c.gaz, c.ax, c.ay, c.az);
     SX_HIP(hipGetLastError());
-    return SERENX_OK;
-}
-
-int grav_forces(Ctx &c) {
-    SX_TRY(zero_words(c.icount, 2 * CTR_WORDS, c.gstream));
-    const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
-    switch (c.opt.mac) {
-    case SERENX_MAC_GEOMETRIC:
-        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GEOMETRIC>(c) : launch_grav<0, SERENX_MAC_GEOMETRIC>(c)));
-        break;
-    case SERENX_MAC_GADGET:
-        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GADGET>(c) : launch_grav<0, SERENX_MAC_GADGET>(c)));
-        break;
-    case SERENX_MAC_EIGEN:
-        SX_TRY((oct ? launch_grav<1, SERENX_MAC_EIGEN>(c) : launch_grav<0, SERENX_MAC_EIGEN>(c)));
-        break;
-    default:
-        SX_TRY((oct ? launch_grav<1, SERENX_MAC_GADGET2>(c) : launch_grav<0, SERENX_MAC_GADGET2>(c)));
-        break;
-    }
-    SX_HIP(hipGetLastError());
-    c.gq.have_cost = true;
     return SERENX_OK;
 }
 

@@ -65,6 +65,9 @@ constexpr int HL_SLOTS = 96;
 // standard SPH: r^2 values of the h_gather band kept per particle for the
 // selection of the pp_gather-th neighbour (more are narrowed by bisection)
 constexpr int BAND_MAX = 64;
+// near-field list entries per group of the gravity walk (a longer list makes
+// the group repeat its walk in the pair phase)
+constexpr int PL_CAP = 1024;
 
 // Counters that every wave of a launch updates (walk statistics, interaction
 // counts, the step's reductions) are spread over CTR_ROWS rows of CTR_SLOTS
@@ -166,7 +169,8 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipStream_t gstream = nullptr;   // gravity walk, beside the SPH kernels
     hipEvent_t evfork = nullptr;     // fork of the gravity stream from the main one
-    hipEvent_t evg[2] = {};          // gravity walk start and end (timed)
+    hipEvent_t evfork2 = nullptr;    // the h solve done: the walk's pair phase may start
+    hipEvent_t evg[4] = {};          // walk phase 1 start, end; phase 2 start, end (timed)
     bool grav_timed = false;         // evg holds a walk of the last evaluation
     bool box_ok = false;             // HNode boxes hold the current positions (stocked
                                      // since the positions last changed)
@@ -191,6 +195,11 @@ struct Ctx {
     double *hrec = nullptr;      // hydro pair records, 16 doubles per particle
     double *grec = nullptr;      // gravity source records, 8 doubles per particle
     double *gax = nullptr, *gay = nullptr, *gaz = nullptr;   // gravity acceleration
+    double *gpc = nullptr;       // potential of the walk's cell terms (phase 1)
+    // near-field lists of the walk: per group of 64 particles, PL_CAP leaves
+    // (first << 4 | count) and the masks of the lanes that opened them
+    int *pl_leaf = nullptr, *pl_len = nullptr;
+    unsigned long long *pl_mask = nullptr;
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     int hl_cap = HL_SLOTS;       // slots used per lane (SERENX_TUNE_HSOLVE_SLOTS)
     bool neib_group = true;      // SERENX_TUNE_NEIB_GROUP: one neighbour walk per wave
@@ -398,8 +407,8 @@ int run_grav(Ctx &c);
 int run_force_eval(Ctx &c, int flags);   // serenx_force_eval's body
 
 // grav.hip
-int grav_forces(Ctx &c);         // BHgrav_accel over active particles (gravity stream)
-int grav_launch(Ctx &c);         // fork the gravity stream and run the walk on it
+int grav_launch(Ctx &c);         // fork the gravity stream: walk phase 1 (cells), after the tree
+int grav_pairs(Ctx &c);          // walk phase 2 (pairs), after the h solve
 int grav_join(Ctx &c);           // main stream waits for the walk and adds a_grav to a
 int run_sph_h(Ctx &c);           // sph_update, part 1: h of the active particles
 int run_sph_lists(Ctx &c);       // sph_update, part 2: hmax, neighbour lists (+ standard density)

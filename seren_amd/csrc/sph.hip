@@ -462,18 +462,6 @@ __global__ void __launch_bounds__(256) k_neib(KGeom kg, bool box, int n, const i
     if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
 }
 
-// wave-wide minimum / maximum of a double (every lane gets the result)
-__device__ __forceinline__ double wave_min_dbl(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max_dbl(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 // get_neib.F90:46-118 with one walk per wave instead of one per particle.
 // Every particle j with r^2 <= KERNRANGESQD h_p^2 or r^2 <= KERNRANGESQD
 // h_j^2 lies in a leaf that the reference's BHhydro_walk opens for p: every
@@ -482,12 +470,11 @@ __device__ __forceinline__ double wave_max_dbl(double v) {
 // particles, in DFS order.  Any walk that opens a superset of those leaves,
 // in DFS order, therefore yields the reference's lists once each candidate
 // passes the reference's pair test (formed as get_neib.F90:89-95 forms it).
-// This walk opens a cell for the whole wave when any lane could need it: the
-// distance from the bounding box of the wave's particles to the cell centre
-// is within max(R_max, hrangemax) + rmax (R_max the wave's largest KERNRANGE
-// h), with a relative margin for rounding, and with fresh boxes the two
-// boxes lie within max(R_max, hrangemax) of each other.  Visits are
-// wave-uniform: no lane keeps walk state of its own.
+// This walk visits a cell's children when any lane's own test of
+// BHhydro_walk.F90:36-43 (and, with fresh boxes, the box test) passes the
+// cell, so every cell on any lane's own path is visited; a lane takes the
+// candidates of the leaves its own test passes.  Unlike group_walk it keeps
+// no per-lane resume state: the visit is one test per lane and a ballot.
 template <bool OVERFLOW_PASS>
 __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
                                                     const int *__restrict__ ovf_idx, int nc,
@@ -502,7 +489,6 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
                                                     int *__restrict__ ovf_list,
                                                     int *__restrict__ ovf_count,
                                                     unsigned long long *__restrict__ wstat) {
-    constexpr double MARGIN = 1.0 + 1.0e-9;
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const int ot = (t / WAVE) * OVF_PER_WAVE + (t % WAVE);
     const bool ovf_lane = (t % WAVE) < OVF_PER_WAVE && ot < n;
@@ -517,15 +503,6 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
     if (__ballot(valid) == 0) return;
     const double hrangesqd_p = kg.rangesqd * hp * hp;
     const double R = kg.range * hp;
-    // the wave's bounding box (centre g, half extents e) and largest radius
-    const double inf = __builtin_huge_val();
-    const double lox = wave_min_dbl(valid ? rpx : inf), hix = wave_max_dbl(valid ? rpx : -inf);
-    const double loy = wave_min_dbl(valid ? rpy : inf), hiy = wave_max_dbl(valid ? rpy : -inf);
-    const double loz = wave_min_dbl(valid ? rpz : inf), hiz = wave_max_dbl(valid ? rpz : -inf);
-    const double gx = 0.5 * (lox + hix), gy = 0.5 * (loy + hiy), gz = 0.5 * (loz + hiz);
-    const double ex = (hix - lox) * (0.5 * MARGIN), ey = (hiy - loy) * (0.5 * MARGIN),
-                 ez = (hiz - loz) * (0.5 * MARGIN);
-    const double Rmax = wave_max_dbl(valid ? R : 0.0);
     int tot = 0;
     int *row = nullptr;
     int lim = pp_limit;   // entries this lane may store
@@ -536,28 +513,24 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
     const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
     int c = uniform(wave_min_int(rr.x));
     const int cend = uniform(wave_max_int(valid ? rr.y : 0));
-    const unsigned nvalid = lanes(valid);
     unsigned vis = 0, lvis = 0;
     while (c < cend) {
         const HNode nd = hn[c];
         vis++;
-        lvis += nvalid;
-        const double reach = fmax(Rmax, nd.hrangemax);
-        // box of the wave to the cell centre
-        const double dx = fmax(fabs(nd.r[0] - gx) - ex, 0.0);
-        const double dy = fmax(fabs(nd.r[1] - gy) - ey, 0.0);
-        const double dz = fmax(fabs(nd.r[2] - gz) - ez, 0.0);
-        const double lim1 = (reach + nd.rmax) * MARGIN;
-        bool open = dx * dx + dy * dy + dz * dz <= lim1 * lim1;
-        if (open && box) {
-            // box of the wave to the cell's (grown) box
-            const double bx = fmax(fabs(nd.r[0] - gx) - ex - (double)nd.half[0], 0.0);
-            const double by = fmax(fabs(nd.r[1] - gy) - ey - (double)nd.half[1], 0.0);
-            const double bz = fmax(fabs(nd.r[2] - gz) - ez - (double)nd.half[2], 0.0);
-            const double lim2 = reach * MARGIN;
-            open = bx * bx + by * by + bz * bz <= lim2 * lim2;
+        const bool own = valid && c >= rr.x && c < rr.y;
+        // BHhydro_walk.F90:36-43 for this lane, and the box test at the
+        // larger of the two radii
+        const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+        const double drsqd = dx * dx + dy * dy + dz * dz;
+        const double m1 = (R + nd.rmax) * (R + nd.rmax);
+        const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
+        bool pass = own && (drsqd <= m1 || drsqd <= m2);
+        if (pass && box) {
+            const double Rb = fmax(R, nd.hrangemax);
+            pass = box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
         }
-        if (!open) {
+        lvis += lanes(pass);
+        if (__ballot(pass) == 0) {
             c = nd.next;
             continue;
         }
@@ -565,12 +538,11 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
             c = c + 1;
             continue;
         }
-        const bool mine = valid && c >= rr.x && c < rr.y;
         const int k = nd.leaf, f0 = nd.first;
         for (int i = 0; i < k; i++) {
             const int j = f0 + i;
             const double4 aj = rec[j];   // one scalar load per candidate
-            if (!mine || j == p) continue;
+            if (!pass || j == p) continue;
             const double hrangesqd_pp = kg.rangesqd * aj.w * aj.w;
             const double qx = aj.x - rpx, qy = aj.y - rpy, qz = aj.z - rpz;
             const double dsq = qx * qx + qy * qy + qz * qz;

@@ -570,9 +570,10 @@ int step_once(Ctx &c) {
     c.grav_timed = false;
     GravGuard guard(c.gstream);
     if (s.nactive > 0) {
-        SX_TRY(run_sph_h(c));
         guard.armed = true;
-        SX_TRY(grav_launch(c));
+        SX_TRY(grav_launch(c));   // cell phase beside the h solve
+        SX_TRY(run_sph_h(c));
+        SX_TRY(grav_pairs(c));    // pair phase beside the lists, thermal and hydro
         SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));
     } else {

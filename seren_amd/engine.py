@@ -54,7 +54,7 @@ EXPORTED = [
     "serenx_fetch_step", "serenx_load_forest", "serenx_cloud_offsets", "serenx_set_precision",
     "serenx_memory", "serenx_pow_third", "serenx_set_energy", "serenx_fetch_energy",
     "serenx_set_tuning", "serenx_get_tuning", "serenx_serve_begin", "serenx_serve_step",
-    "serenx_serve_info", "serenx_diagnostics",
+    "serenx_serve_info", "serenx_diagnostics", "serenx_walk_timings",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_cloud_offsets.argtypes = [vp, _ip]
     lib.serenx_set_precision.argtypes = [vp, C.c_int32]
     lib.serenx_diagnostics.argtypes = [vp, _dp]
+    lib.serenx_walk_timings.argtypes = [vp, _dp]
     lib.serenx_serve_begin.argtypes = [vp, C.c_double, C.c_double]
     lib.serenx_serve_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_uint8)]
     lib.serenx_serve_info.argtypes = [vp, C.c_int32, _dp, _dp]
@@ -385,6 +386,12 @@ class ForceEngine:
         ms = np.zeros(4)
         _check(self.lib.serenx_last_timings(self.ctx, _dptr(ms)))
         return dict(tree=ms[0], sph=ms[1], hydro=ms[2], grav=ms[3])
+
+    def walk_timings(self):
+        """ms of the last walk's cell phase and pair phase."""
+        ms = np.zeros(2)
+        _check(self.lib.serenx_walk_timings(self.ctx, _dptr(ms)))
+        return dict(cells=ms[0], pairs=ms[1])
 
     def walk_stats(self) -> Dict[str, float]:
         """Per-walk SIMD utilisation and h-solve pass counts of the last evaluation."""

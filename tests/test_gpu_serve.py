@@ -102,10 +102,14 @@ def test_serving_step_is_the_kdk_step_of_the_reference():
         db = b.fetch(neighbours=False)
     finally:
         b.close()
-    assert np.array_equal(da["porig"], db["porig"])
-    scale = np.abs(st["r_old"]).max()
-    ex = np.abs(da["pos"] - st["r_old"]).max() / scale
-    ev = np.abs(da["v"] - st["v_old"]).max() / np.abs(st["v_old"]).max()
+    # compare particle by particle (original ids): the wrapper also builds a
+    # tree at its half steps, so near-equal keys may leave another tree order
+    ia, ib = np.argsort(da["porig"]), np.argsort(db["porig"])
+    xa, va = da["pos"][ia], da["v"][ia]
+    xb, vb = st["r_old"][ib], st["v_old"][ib]
+    scale = np.abs(xb).max()
+    ex = np.abs(xa - xb).max() / scale
+    ev = np.abs(va - vb).max() / np.abs(vb).max()
     et = abs(t[0] - (info["time"] - info["dt_max"] / 2)) / t[0]
     print(f"  |dx| {ex:.2e}  |dv| {ev:.2e}  time {et:.2e}", flush=True)
     assert ex <= 1e-12 and ev <= 1e-12 and et <= 1e-12
