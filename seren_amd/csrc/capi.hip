@@ -247,15 +247,14 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.gay, nn));
     SX_TRY(dalloc(&c.gaz, nn));
     SX_TRY(dalloc(&c.gpc, nn));
-    {
-        const size_t ngl = ((size_t)n + WAVE - 1) / WAVE;
-        SX_TRY(dalloc(&c.pl_leaf, ngl * PL_CAP));
-        SX_TRY(dalloc(&c.pl_mask, ngl * PL_CAP));
-        SX_TRY(dalloc(&c.pl_len, ngl));
-    }
+    // per group of the gravity walk (at most one partial group per cloud)
+    const size_t ng = walk_max_groups(n, c.pp_gather);
+    SX_TRY(dalloc(&c.grp, ng));
+    SX_TRY(dalloc(&c.pl_leaf, ng * PL_CAP));
+    SX_TRY(dalloc(&c.pl_mask, ng * PL_CAP));
+    SX_TRY(dalloc(&c.pl_len, ng));
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
-    // work queue of the gravity walk: one entry per group of 64 particles
-    const size_t ng = ((size_t)n + WAVE - 1) / WAVE;
+    // work queue of the gravity walk: one entry per group
     Ctx::WalkQueue &q = c.gq;
     SX_TRY(dalloc(&q.ctr, (size_t)NXCD));
     SX_TRY(dalloc(&q.cost, ng));
@@ -291,7 +290,7 @@ void free_all(Ctx &c) {
                     c.root_range, c.sorttmp, c.c_start, c.c_end, c.c_parent, c.c_leaf,
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
                     c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band,
-                    c.grec, c.gax, c.gay, c.gaz, c.gpc, c.pl_leaf, c.pl_mask, c.pl_len, c.gq.ctr, c.gq.cost, c.gq.order,
+                    c.grec, c.gax, c.gay, c.gaz, c.gpc, c.grp, c.pl_leaf, c.pl_mask, c.pl_len, c.gq.ctr, c.gq.cost, c.gq.order,
                     c.gq.vals, c.gq.keys, c.gq.keys2, c.gq.tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -347,7 +346,7 @@ size_t ctx_device_bytes(const Ctx &c) {
     const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
     size_t b = 0;
     b += nn * (24 * 8 + 16 * 8 + 12 * 8);                   // SoA, tmpd, hydro/gravity records
-    b += (nn + WAVE - 1) / WAVE * ((size_t)PL_CAP * 12 + 4);  // walk near-field lists
+    b += walk_max_groups(c.cap, c.pp_gather) * ((size_t)PL_CAP * 12 + 4 + 8 + 6 * 8);  // walk groups
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch

@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_grav_cells_fp64(
     const double *__restrict__ gpot, const double *__restrict__ agm, double *__restrict__ cx,
     double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ cp, NearList near,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
-    GroupQueue queue, int *__restrict__ cost) {
+    GroupQueue queue, int *__restrict__ cost, const int2 *__restrict__ grp) {
     // the walk is the critical path while the SPH kernels run beside it: its
     // waves take issue priority over theirs on shared SIMDs
     __builtin_amdgcn_s_setprio(2);
@@ -300,8 +300,9 @@ __global__ void __launch_bounds__(256) k_grav_cells_fp64(
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
-        const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
-        const bool valid = p < n && active[p];
+        const int2 gr = grp[g];
+        const int p = gr.x + (int)(threadIdx.x & (WAVE - 1));
+        const bool valid = p < gr.y && active[p];
         double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
         if (valid) {
             rpx = x[p]; rpy = y[p]; rpz = z[p];
@@ -357,12 +358,16 @@ __global__ void __launch_bounds__(256) k_grav_pairs_fp64(
     const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     const double *__restrict__ cp, double *__restrict__ gpot, double *__restrict__ agm,
-    NearList near, unsigned long long *__restrict__ icount) {
+    NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
+    int ngrp) {
     constexpr KGeom kg = walk_geom<QUINTIC>();
     __builtin_amdgcn_s_setprio(2);
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
-    const int g = p / WAVE, lane = (int)(threadIdx.x & (WAVE - 1));
-    const bool valid = p < n && active[p];
+    const int g = (int)((xcd_block() * blockDim.x + threadIdx.x) / WAVE);
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    if (g >= ngrp) return;   // wave-uniform
+    const int2 gr = grp[g];
+    const int p = gr.x + lane;
+    const bool valid = p < gr.y && active[p];
     if (__ballot(valid) == 0) return;
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0;
     double gx = 0, gy = 0, gz = 0, gp = 0, afactor = 0.0;
@@ -531,14 +536,15 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
     const double *__restrict__ gpot, const double *__restrict__ agm, double *__restrict__ ox,
     double *__restrict__ oy, double *__restrict__ oz, double *__restrict__ op, NearList near,
     unsigned long long *__restrict__ icount, unsigned long long *__restrict__ wstat,
-    GroupQueue queue, int *__restrict__ cost) {
+    GroupQueue queue, int *__restrict__ cost, const int2 *__restrict__ grp) {
     __builtin_amdgcn_s_setprio(2);
     unsigned ncell = 0, vis = 0, lvis = 0;   // wave totals
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone)) {
-        const int p = g * WAVE + (int)(threadIdx.x & (WAVE - 1));
-        const bool valid = p < n && active[p];
+        const int2 gr = grp[g];
+        const int p = gr.x + (int)(threadIdx.x & (WAVE - 1));
+        const bool valid = p < gr.y && active[p];
         double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
         if (valid) {
             rpx = x[p]; rpy = y[p]; rpz = z[p];
@@ -603,13 +609,17 @@ __global__ void __launch_bounds__(256) k_grav_pairs_mixed(
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     const double *__restrict__ cpot, double *__restrict__ gpot, double *__restrict__ agm,
-    NearList near, unsigned long long *__restrict__ icount) {
+    NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
+    int ngrp) {
     constexpr KGeom kg64 = walk_geom<QUINTIC>();
     constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
     __builtin_amdgcn_s_setprio(2);
-    const int p = xcd_block() * blockDim.x + threadIdx.x;
-    const int g = p / WAVE, lane = (int)(threadIdx.x & (WAVE - 1));
-    const bool valid = p < n && active[p];
+    const int g = (int)((xcd_block() * blockDim.x + threadIdx.x) / WAVE);
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    if (g >= ngrp) return;   // wave-uniform
+    const int2 gr = grp[g];
+    const int p = gr.x + lane;
+    const bool valid = p < gr.y && active[p];
     if (__ballot(valid) == 0) return;
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
     float invhp = 1.0f, zo_p = 0.0f;
@@ -775,36 +785,40 @@ void launch_cells(Ctx &c) {
         auto k = k_grav_cells_mixed<OCT, MACT>;
         hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
                            c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
-                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost);
+                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost,
+                           c.grp);
     } else {
         auto k = k_grav_cells_fp64<OCT, MACT>;
         hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
                            c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
-                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost);
+                           c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost,
+                           c.grp);
     }
 }
 
 // phase 2 (pair terms), one wave per group
 template <int MACT, bool QUINTIC>
 void launch_pairs(Ctx &c, const double4 *rec) {
-    const int blocks = nblk((long)c.gq.ngroups * WAVE);
+    const int blocks = nblk((long)c.ngrp * WAVE);
     if (c.opt.precision == SERENX_MIXED)
         hipLaunchKernelGGL((k_grav_pairs_mixed<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
                            c.gstream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
-                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount);
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp,
+                           c.ngrp);
     else
         hipLaunchKernelGGL((k_grav_pairs_fp64<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
                            c.gstream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
-                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount);
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp,
+                           c.ngrp);
 }
 
 // the queue order of this walk (from the previous walk's costs) and zeroed
 // counters, on the gravity stream
 int prepare_queue(Ctx &c) {
     Ctx::WalkQueue &q = c.gq;
-    const int ng = (c.n + WAVE - 1) / WAVE;
+    const int ng = c.ngrp;
     if (ng != q.ngroups) q.have_cost = false;   // another group layout
     q.ngroups = ng;
     hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,

@@ -200,6 +200,11 @@ struct Ctx {
     // (first << 4 | count) and the masks of the lanes that opened them
     int *pl_leaf = nullptr, *pl_len = nullptr;
     unsigned long long *pl_mask = nullptr;
+    // groups of the gravity walk: [start, end) of up to 64 consecutive
+    // particles of one cloud (each cloud's groups start at its first
+    // particle, so a cloud is grouped the same alone and in a forest)
+    int2 *grp = nullptr;
+    int ngrp = 0;
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     int hl_cap = HL_SLOTS;       // slots used per lane (SERENX_TUNE_HSOLVE_SLOTS)
     bool neib_group = true;      // SERENX_TUNE_NEIB_GROUP: one neighbour walk per wave
@@ -219,7 +224,7 @@ struct Ctx {
     // clouds: a forest of independent trees, one root per cloud at level 0;
     // cloud k owns the particles [cloud_off[k], cloud_off[k+1]) in input and
     // in tree order (the build permutes within clouds only)
-    int ncloud = 1, max_clouds = 1, max_chunks = 1, nchunk = 0;
+    int ncloud = 1, max_clouds = 1, max_chunks = 1, nchunk = 0;   // forest_max_clouds
     std::vector<int> cloud_off_h;
     int *cloud_off = nullptr;    // device, ncloud + 1
     int *cid = nullptr;          // cloud of each particle
@@ -362,6 +367,14 @@ void make_kernel_tables(double *tab, int kernel);
 constexpr int DIAG_MAX_BLOCKS = 1024;
 constexpr int DIAG_SUMS = 16;
 constexpr int DIAG_VALUES = 20;
+// Most clouds a forest of n particles may hold: one per pp_gather particles
+// (a cloud needs pp_gather particles for its h solve).  The gravity walk has
+// at most (n + 63) / 64 + this many groups.
+inline int forest_max_clouds(int n, int pp_gather) { return n / (pp_gather > 1 ? pp_gather : 1) + 1; }
+inline size_t walk_max_groups(int n, int pp_gather) {
+    return ((size_t)n + WAVE - 1) / WAVE + (size_t)forest_max_clouds(n, pp_gather);
+}
+
 int diag_blocks(int n, int *per_block);
 int diagnostics(Ctx &c, double *partial, double *dout);
 

@@ -657,7 +657,7 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.off, nn));
     SX_TRY(dalloc(&c.firstlev, nn));
     // a forest holds at most one cloud per pp_gather particles (set_clouds)
-    c.max_clouds = n / std::max(c.pp_gather, 1) + 1;
+    c.max_clouds = forest_max_clouds(n, c.pp_gather);
     c.max_chunks = n / BB_CHUNK + c.max_clouds + 1;
     SX_TRY(dalloc(&c.root, 11 * (size_t)c.max_clouds + 16));
     SX_TRY(dalloc(&c.root_range, (size_t)c.max_clouds));
@@ -762,6 +762,16 @@ int set_clouds(Ctx &c, int ncloud, const int *counts) {
                      hipMemcpyHostToDevice));
     SX_HIP(hipMemcpy(c.chunks, chunks.data(), sizeof(int) * chunks.size(),
                      hipMemcpyHostToDevice));
+    // the gravity walk's groups: 64 particles at a time from each cloud's start
+    std::vector<int2> groups;
+    for (int k = 0; k < ncloud; k++)
+        for (int b = c.cloud_off_h[k]; b < c.cloud_off_h[k + 1]; b += WAVE)
+            groups.push_back(make_int2(b, std::min(b + WAVE, c.cloud_off_h[k + 1])));
+    if (groups.size() > walk_max_groups(c.cap, c.pp_gather))
+        return set_error(SERENX_EINVAL, "walk groups beyond capacity");
+    SX_HIP(hipMemcpy(c.grp, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice));
+    c.ngrp = (int)groups.size();
+    c.gq.have_cost = false;   // another group layout
     hipLaunchKernelGGL(k_fill_cid, dim3(64, ncloud), dim3(TB), 0, c.stream, ncloud, c.cloud_off,
                        c.cid);
     SX_HIP(hipGetLastError());

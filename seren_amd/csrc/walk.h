@@ -176,9 +176,10 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
     return visits;
 }
 
-// Work queue of a persistent walk.  A group is 64 consecutive particles in
-// tree order (one wave's targets, as in the one-wave-per-group launch).  The
-// groups are split into NXCD contiguous ranges of the tree order, one per XCD
+// Work queue of a persistent walk.  A group is up to 64 consecutive particles
+// of one cloud in tree order (one wave's targets, as in the one-wave-per-group
+// launch).  The groups are split into NXCD contiguous ranges of the tree
+// order, one per XCD
 // (group g belongs to range 8 g / ngroups), and order[] lists each range's
 // groups by decreasing cost of their previous walk, so that the most
 // expensive groups start first (longest-processing-time order).  A wave of
