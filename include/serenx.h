@@ -258,6 +258,19 @@ int serenx_set_precision(serenx_ctx *ctx, int32_t precision);
  * Allocates its partial-sum buffer on the device for the call. */
 int serenx_diagnostics(serenx_ctx *ctx, double *out);
 
+/* serenx_diagnostics without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Writes the same DIAG_VALUES = 20 values,
+ * bit for bit, to out_dev -- device memory of at least 20 doubles -- on the
+ * context's stream and returns without waiting: they are there once
+ * serenx_sync(ctx) returns.  The block partial sums live in a fixed
+ * 128 KiB device array of the library (1024 rows of 16 sums, one copy per
+ * device, held until the process ends; every particle count fits, because
+ * the rows take ever longer slices of particles), so calls on one device
+ * that overlap in time (two contexts on different streams) race on it.
+ * Calls no hipMalloc, hipFree or other allocating routine. */
+int serenx_diagnostics_noalloc(serenx_ctx *ctx, double *out_dev);
+
 /* Batched serving steps (no single counterpart in the reference: each call
  * does for every selected cloud of a forest what sph_integrate.F90 does for
  * one cloud with the LFKDK integrator, one time-step level and adaptive

@@ -530,6 +530,8 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         return set_error(SERENX_EHIP, "no HIP device available");
     if (device < 0 || device >= ndev) return set_error(SERENX_EINVAL, "bad device index");
     SX_HIP(hipSetDevice(device));
+    // the allocation-free diagnostics' scratch address on this device (once)
+    SX_TRY(diag_scratch_setup(device));
     Ctx *c = new Ctx();
     c->opt = *opt;
     c->device = device;

@@ -9,6 +9,7 @@
 // k_diag_final adds the block partials in block order.  The partial sums are
 // the scratch buffer of the reduction.
 #include <algorithm>
+#include <mutex>
 
 #include "serenx_internal.h"
 
@@ -111,6 +112,39 @@ int diag_blocks(int n, int *per_block) {
     return nb;
 }
 
+// Scratch of serenx_diagnostics_noalloc: the block partial sums of
+// k_diag_partial, DIAG_MAX_BLOCKS x DIAG_SUMS doubles (128 KiB), one copy
+// per device for the life of the process.  In host code its name is not a
+// device address: kernels receive the address hipGetSymbolAddress gives for
+// the device (diag_scratch).
+static __device__ __align__(256) double scratch[(size_t)DIAG_MAX_BLOCKS * DIAG_SUMS];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[SCRATCH_DEVICES] = {};
+}  // namespace
+
+int diag_scratch_setup(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES)
+        return set_error(SERENX_EINVAL, "device index beyond the diagnostics scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return SERENX_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(SERENX_EHIP, std::string("hipGetSymbolAddress of the diagnostics "
+                                                  "scratch failed: ") + hipGetErrorString(e));
+    scratch_ptr[device] = static_cast<double *>(p);
+    return SERENX_OK;
+}
+
+double *diag_scratch(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return scratch_ptr[device];
+}
+
 int diagnostics(Ctx &c, double *partial, double *dout) {
     int per_block = 0;
     const int nb = diag_blocks(c.n, &per_block);
@@ -124,3 +158,23 @@ int diagnostics(Ctx &c, double *partial, double *dout) {
 }
 
 }  // namespace serenx
+
+extern "C" int serenx_diagnostics_noalloc(serenx_ctx *ctx, double *out_dev) {
+    using namespace serenx;
+    if (!ctx || !out_dev) return set_error(SERENX_EINVAL, "null argument");
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx);
+    if (c.n <= 0) return set_error(SERENX_ESTATE, "no particles loaded");
+    SX_HIP(hipSetDevice(c.device));
+    double *part = diag_scratch(c.device);
+    if (!part)
+        return set_error(SERENX_ESTATE, "diagnostics scratch not set up on this device "
+                                        "(serenx_create sets it up)");
+    int per_block = 0;
+    const int nb = diag_blocks(c.n, &per_block);
+    if (nb > DIAG_MAX_BLOCKS || per_block <= 0)
+        return set_error(SERENX_EINVAL, "particle count beyond the diagnostics scratch "
+                                        "(1024 partial-sum rows)");
+    // every block writes all of its row before k_diag_final reads the rows
+    // [0, nb), so no row needs clearing between calls
+    return diagnostics(c, part, out_dev);
+}

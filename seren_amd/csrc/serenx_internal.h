@@ -377,6 +377,11 @@ inline size_t walk_max_groups(int n, int pp_gather) {
 
 int diag_blocks(int n, int *per_block);
 int diagnostics(Ctx &c, double *partial, double *dout);
+// the device address of the allocation-free diagnostics scratch on a device:
+// resolved once per device by diag_scratch_setup (with that device current),
+// null before
+int diag_scratch_setup(int device);
+double *diag_scratch(int device);
 
 // tree.hip
 int ensure_tree_buffers(Ctx &c);

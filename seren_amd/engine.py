@@ -55,6 +55,7 @@ EXPORTED = [
     "serenx_memory", "serenx_pow_third", "serenx_set_energy", "serenx_fetch_energy",
     "serenx_set_tuning", "serenx_get_tuning", "serenx_serve_begin", "serenx_serve_step",
     "serenx_serve_info", "serenx_diagnostics", "serenx_walk_timings",
+    "serenx_diagnostics_noalloc",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -140,6 +141,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_cloud_offsets.argtypes = [vp, _ip]
     lib.serenx_set_precision.argtypes = [vp, C.c_int32]
     lib.serenx_diagnostics.argtypes = [vp, _dp]
+    lib.serenx_diagnostics_noalloc.argtypes = [vp, vp]   # (context, device pointer)
     lib.serenx_walk_timings.argtypes = [vp, _dp]
     lib.serenx_serve_begin.argtypes = [vp, C.c_double, C.c_double]
     lib.serenx_serve_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_uint8)]
