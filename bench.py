@@ -204,6 +204,10 @@ def step_leg(args, cloud, precision, tune=False):
                       thetamaxsqd=0.25, abserror=args.abserror, sound_speed=0.2)
     out = {"precision": precision}
     try:
+        if precision == "mixed" and not tune and args.mixed_abserror > 0:
+            from seren_amd.engine import TUNE_MIXED_ABSERROR
+            eng.set_tuning(TUNE_MIXED_ABSERROR, args.mixed_abserror)
+            out["mixed_abserror"] = args.mixed_abserror
         eng.load(pos, vel, mass, h)
         eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
         eng.sync()
