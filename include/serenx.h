@@ -311,12 +311,18 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *    walk per wave that opens every cell any of its particles could need; 0
  *    with the particles' own walks (BHhydro_walk.F90's test per particle).
  *    The lists are the same.
+ *  SERENX_TUNE_HSOLVE_WAVES: waves per SIMD of the grad-h iteration kernel:
+ *    8 (registers capped at 64, a few spills), 0 (the compiler's budget,
+ *    5 waves), -1 (default: 8 in mixed precision, where the SPH kernels are
+ *    the critical path, 0 in FP64, where the h solve runs beside the
+ *    issue-bound FP64 walk).  Results do not change.
  * serenx_get_tuning reads the current value. */
 enum {
     SERENX_TUNE_HSOLVE_SLOTS = 1,
     SERENX_TUNE_MIXED_ABSERROR = 2,
     SERENX_TUNE_WALK_QUEUE = 3,
-    SERENX_TUNE_NEIB_GROUP = 4
+    SERENX_TUNE_NEIB_GROUP = 4,
+    SERENX_TUNE_HSOLVE_WAVES = 5
 };
 int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value);
 int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value);

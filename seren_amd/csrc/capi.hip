@@ -1251,6 +1251,11 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
             return set_error(SERENX_EINVAL, "walk queue blocks per CU must be an integer in 0..64");
         c.walk_queue = (int)value;
         return SERENX_OK;
+    case SERENX_TUNE_HSOLVE_WAVES:
+        if (value != -1.0 && value != 0.0 && value != 8.0)
+            return set_error(SERENX_EINVAL, "h-solve waves per SIMD must be -1, 0 or 8");
+        c.hsolve_waves = (int)value;
+        return SERENX_OK;
     default:
         return set_error(SERENX_EINVAL, "unknown tuning key");
     }
@@ -1266,6 +1271,7 @@ int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value) {
         return SERENX_OK;
     case SERENX_TUNE_WALK_QUEUE: *value = c.walk_queue; return SERENX_OK;
     case SERENX_TUNE_NEIB_GROUP: *value = c.neib_group ? 1.0 : 0.0; return SERENX_OK;
+    case SERENX_TUNE_HSOLVE_WAVES: *value = c.hsolve_waves; return SERENX_OK;
     default: return set_error(SERENX_EINVAL, "unknown tuning key");
     }
 }

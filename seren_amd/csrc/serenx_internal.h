@@ -208,6 +208,7 @@ struct Ctx {
     int *hlist = nullptr;        // h-solve leaf lists, HL_SLOTS x 64 per wave
     int hl_cap = HL_SLOTS;       // slots used per lane (SERENX_TUNE_HSOLVE_SLOTS)
     bool neib_group = true;      // SERENX_TUNE_NEIB_GROUP: one neighbour walk per wave
+    int hsolve_waves = -1;       // SERENX_TUNE_HSOLVE_WAVES: -1 auto, 0, 8
     double mixed_abserror = 0.0; // mixed walk's MAC tolerance (0: opt.abserror)
     double *band = nullptr;      // standard SPH: BAND_MAX x 64 r^2 values per wave
     size_t band_cap = 0;         // particles the band buffer holds

@@ -127,7 +127,15 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
 // order).  On exit every lane leaves rhotemp in rho, h_high in hhigh and its
 // list length in nlist (-1: list incomplete) for k_hsolve_final; its list
 // stays in hlist (a lane joins no walk after it has converged).
-__global__ void __launch_bounds__(256) k_hsolve_iter(
+//
+// WAVES > 0 holds the kernel to WAVES waves per SIMD (64 VGPRs at 8, with a
+// few spills outside the candidate loops): more waves in flight hide more of
+// the walks' memory latency, at the cost of issue slots taken from kernels
+// running beside it.  0 leaves the register budget to the compiler
+// (92 VGPRs, 5 waves).
+template <int WAVES>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES > 0 ? WAVES : 8))) k_hsolve_iter(
     KGeom kg, bool box, int n, int nc, const HNode *__restrict__ hn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const double *__restrict__ roots, int *__restrict__ hlist, int hl_cap,
@@ -1081,7 +1089,10 @@ int sph_hsolve(Ctx &c) {
                        c.vx, c.vy, c.vz, c.m, rec);
     SX_HIP(hipGetLastError());
     // h_high per particle in the permutation scratch, list lengths in tmpi
-    hipLaunchKernelGGL(k_hsolve_iter, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
+    // the h solve at 8 waves per SIMD in mixed precision, where it is on the
+    // critical path; beside the FP64 walk (issue-bound) at the compiler's 5
+    const bool w8 = c.hsolve_waves == 8 || (c.hsolve_waves < 0 && c.opt.precision == SERENX_MIXED);
+    hipLaunchKernelGGL(w8 ? k_hsolve_iter<8> : k_hsolve_iter<0>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg, c.box_ok, c.n, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.root, c.hlist, c.hl_cap, c.active, c.ktab,
                        c.opt.h_fac,
                        c.pp_gather, c.h, c.rho, c.tmpd, c.tmpi, c.wstat);

@@ -78,6 +78,7 @@ class SerenxError(RuntimeError):
 EINVAL, EHIP, ELMAX, ESTATE, ENOMEM, EHNEG, EFEW = 1, 2, 3, 4, 5, 6, 7
 # tuning keys (serenx_set_tuning)
 TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR, TUNE_WALK_QUEUE, TUNE_NEIB_GROUP = 1, 2, 3, 4
+TUNE_HSOLVE_WAVES = 5
 
 
 class Options(C.Structure):

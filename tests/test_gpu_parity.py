@@ -313,3 +313,29 @@ def test_neighbour_walk_modes_agree(profile, octupole, mac):
     assert int(np.sum(~ok)) > 0 or profile == "plummer"
     for k in ("a", "rho", "h"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_hsolve_occupancy_modes_agree():
+    """The grad-h iteration kernel held to 8 waves per SIMD (registers capped,
+    a few spills) and at the compiler's register budget: the same h, rho,
+    Omega and neighbour lists bit for bit, in both precisions (the default
+    picks by precision)."""
+    from seren_amd.engine import TUNE_HSOLVE_WAVES
+    pos, vel, mass, h = cloud(30000, "r2", 78)
+    for precision in ("fp64", "mixed"):
+        out = []
+        for waves in (0, 8, -1):
+            eng = _engine(True, "gadget2", 0.25, 1e-4)
+            try:
+                eng.set_precision(precision)
+                eng.set_tuning(TUNE_HSOLVE_WAVES, waves)
+                assert eng.get_tuning(TUNE_HSOLVE_WAVES) == waves
+                eng.load(pos, vel, mass, h)
+                eng.force_eval(hguess=True)
+                eng.force_eval()
+                out.append(eng.fetch())
+            finally:
+                eng.close()
+        for d in out[1:]:
+            for k in ("h", "rho", "omega", "zeta", "pptot", "a", "gpot"):
+                assert np.array_equal(out[0][k], d[k]), (precision, k)
