@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="step", choices=["step", "eval", "forest", "serve"])
     ap.add_argument("--clouds", type=int, default=256, help="forest mode: number of clouds")
+    ap.add_argument("--nbuild", type=int, default=8,
+                    help="forest mode: tree built every nbuild evaluations, restocked between "
+                         "(tree_update.F90's nbuildstep)")
+    ap.add_argument("--no-graph", dest="graph", action="store_false", default=True,
+                    help="forest mode: run the restock evaluations eagerly, not as HIP graphs")
     ap.add_argument("--requests", type=int, default=2000, help="serve mode: requests")
     ap.add_argument("--rate", type=float, default=200.0, help="serve mode: requests per second")
     ap.add_argument("--pool", type=int, default=4, help="serve mode: resident clouds per size")
@@ -134,7 +139,9 @@ def bench_forest(args):
     """Batched ensemble: independent uniform clouds, N log-uniform in
     [4096, 262144], each with its own P(k) ~ k^-4 turbulent seed at Mach
     U(1, 5), loaded as one forest; a step is one force evaluation of the
-    whole forest (every phase one launch over all clouds)."""
+    whole forest (every phase one launch over all clouds): the tree is built
+    every nbuild steps and restocked in between, the restock evaluations
+    replayed from recorded HIP graphs (recorded again after each build)."""
     from seren_amd import ics
     from seren_amd.engine import ForceEngine
     rng = np.random.default_rng(args.seed)
@@ -156,17 +163,25 @@ def bench_forest(args):
               flush=True)
         off = eng.load_forest(clouds)
     eng.force_eval(hguess=True)
-    for _ in range(max(1, args.warmup)):
-        eng.force_eval()
+
+    def one(k):
+        if k % max(args.nbuild, 1) == 0:
+            eng.force_eval()
+        else:
+            eng.force_eval(restock=True, graph=args.graph)
+
+    for k in range(max(1, args.warmup)):
+        one(k)
     eng.sync()
     phases = np.zeros(4)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.force_eval()
+    for k in range(args.steps):
+        one(k)
         t = eng.timings()
         phases += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
     eng.sync()
     dt = time.perf_counter() - t0
+    ginfo = eng.graph_info()
     ntot = int(off[-1])
     tg = np.sort(rng.choice(ntot, min(ntot, args.direct_targets), replace=False))
     eng.force_eval(hydro=False)
@@ -183,8 +198,12 @@ def bench_forest(args):
            "config": {"workload": f"forest of {args.clouds} clouds, N log-uniform "
                                   f"[4096, 262144] (total {ntot}), Mach U(1,5), "
                                   f"{args.multipole} + {args.mac} MAC, one force evaluation "
-                                  f"of the forest per step, {prec}",
-                      "clouds": args.clouds, "n_total": ntot, "precision": prec},
+                                  f"of the forest per step (tree built every {args.nbuild}, "
+                                  f"restocked between{', as HIP graphs' if args.graph else ''}), "
+                                  f"{prec}",
+                      "clouds": args.clouds, "n_total": ntot, "precision": prec,
+                      "nbuild": args.nbuild, "graph": bool(args.graph)},
+           "graphs": ginfo,
            "autotune": tuned,
            "phases_ms": {k: float(v / args.steps) for k, v in
                          zip(("tree", "sph", "hydro", "grav"), phases)},

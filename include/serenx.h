@@ -69,7 +69,11 @@ enum { SERENX_FP64 = 0, SERENX_MIXED = 1 };
 enum {
     SERENX_EVAL_HGUESS = 1,   /* run BHhydro_hguess before the h solve (setup) */
     SERENX_EVAL_NO_GRAV = 2,  /* skip the gravity walk */
-    SERENX_EVAL_NO_HYDRO = 4  /* skip the hydro pair forces */
+    SERENX_EVAL_NO_HYDRO = 4, /* skip the hydro pair forces */
+    SERENX_EVAL_RESTOCK = 8,  /* no build: the tree of the last build, restocked
+                                 with the current positions (tree_update.F90
+                                 between rebuilds) */
+    SERENX_EVAL_GRAPH = 16    /* with RESTOCK: run as recorded HIP graphs */
 };
 
 /* kernel (Makefile KERNEL, kernel.F90:14-236, macros.h:101-118): M4 cubic
@@ -339,8 +343,21 @@ int serenx_set_agravmag(serenx_ctx *ctx, const double *agravmag);
 /* One force evaluation in the sph_setup.F90:42-66 order: tree_update,
  * [hguess], sph_update, isothermal thermal (P = c_s^2 rho, c = c_s),
  * sph_hydro_forces, sph_grav_forces.  Enqueued on the context's stream;
- * returns after the work is queued (serenx_sync waits). */
+ * returns after the work is queued (serenx_sync waits).
+ * SERENX_EVAL_RESTOCK evaluates on the tree of the last build, restocked
+ * with the current positions, as tree_update.F90 does between rebuilds
+ * (tree_update.F90:24-88; the particle order does not change).  It waits
+ * once, after the neighbour rows, for the host to see whether any particle
+ * has more than pp_limit neighbours.  Adding SERENX_EVAL_GRAPH runs it as
+ * two recorded HIP graphs (grad-h SPH; other options run it eagerly):
+ * recorded at the first such call and replayed until a build, a load, a
+ * precision or tuning change, or a reallocation makes a new recording;
+ * results are identical to the eager restock evaluation. */
 int serenx_force_eval(serenx_ctx *ctx, int32_t flags);
+
+/* Recordings and replays of the restock evaluation's graphs so far (no
+ * counterpart in the reference). */
+int serenx_graph_info(const serenx_ctx *ctx, int64_t *captures, int64_t *launches);
 int serenx_sync(serenx_ctx *ctx);
 
 /* Copies the current state to the host, in tree order.  Any pointer may be

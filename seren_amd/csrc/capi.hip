@@ -222,6 +222,7 @@ int dalloc(T **p, size_t count) {
 }
 
 int ensure_capacity(Ctx &c, int n) {
+    c.graph_epoch++;   // buffers may move
     // leaf descriptors of the h solve pack a particle index in 27 bits
     if (n >= (1 << 27))
         return set_error(SERENX_EINVAL, "more than 2^27 - 1 particles on one device");
@@ -291,7 +292,8 @@ void free_all(Ctx &c) {
                     c.c_level, c.bfs2dfs, c.dfs2bfs, c.dfs_parent, c.bbmin, c.bbmax,
                     c.hmaxs, c.gn, c.gs, c.gnf, c.hn, c.ktab, c.icount, c.wstat, c.band,
                     c.grec, c.gax, c.gay, c.gaz, c.gpc, c.grp, c.pl_leaf, c.pl_mask, c.pl_len, c.gq.ctr, c.gq.cost, c.gq.order,
-                    c.gq.vals, c.gq.keys, c.gq.keys2, c.gq.tmp};
+                    c.gq.vals, c.gq.keys, c.gq.keys2, c.gq.tmp, c.fs_ck, c.fs_ck2, c.fs_pos,
+                    c.fs_perm, c.fs_v, c.fs_k};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.hostpin) (void)hipHostFree(c.hostpin);
@@ -302,6 +304,8 @@ void free_all(Ctx &c) {
         if (e) (void)hipEventDestroy(e);
     if (c.evfork) (void)hipEventDestroy(c.evfork);
     if (c.evfork2) (void)hipEventDestroy(c.evfork2);
+    if (c.evjoin) (void)hipEventDestroy(c.evjoin);
+    graphs_free(c);
     if (c.gstream && c.gstream != c.stream) (void)hipStreamDestroy(c.gstream);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
@@ -559,6 +563,7 @@ int serenx_create(const serenx_options *opt, int device, serenx_ctx **out) {
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork2, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void **)&c->ktab, sizeof(double) * KT_TOTAL);
     if (e == hipSuccess) e = hipMalloc((void **)&c->icount, CTR_WORDS * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->icount, 0, CTR_WORDS * sizeof(unsigned long long));
@@ -838,7 +843,140 @@ int serenx_force_eval(serenx_ctx *ctx, int32_t flags) {
 
 namespace serenx {
 
+namespace {
+
+// Part A of a restock evaluation: stock the tree of the last build with the
+// current positions (tree_update.F90's steps between rebuilds), the h
+// solve, hmax and the neighbour rows; the walk's cell phase beside them on
+// the gravity stream, joined at the end.  Launches only.
+int eval_part_a(Ctx &c, int flags) {
+    const bool grav = !(flags & SERENX_EVAL_NO_GRAV);
+    SX_TRY(zero_words(c.wstat, 2 * CTR_WORDS, c.stream));
+    SX_HIP(hipEventRecord(c.ev[0], c.stream));
+    SX_TRY(stock_tree(c));
+    SX_HIP(hipEventRecord(c.ev[1], c.stream));
+    c.grav_timed = false;
+    if (grav) SX_TRY(grav_launch(c));
+    SX_TRY(run_sph_h(c));
+    SX_TRY(update_hmax(c));
+    SX_TRY(sph_neib_rows(c));
+    if (grav) {
+        SX_HIP(hipEventRecord(c.evjoin, c.gstream));
+        SX_HIP(hipStreamWaitEvent(c.stream, c.evjoin, 0));
+    }
+    return SERENX_OK;
+}
+
+// Part B: thermal and hydro forces, the walk's pair phase beside them, and
+// a_grav added to a.  Launches only.
+int eval_part_b(Ctx &c, int flags) {
+    const bool grav = !(flags & SERENX_EVAL_NO_GRAV);
+    if (c.opt.sph == SERENX_SPH_STANDARD) SX_TRY(sph_density(c));
+    c.sph_ok = true;
+    SX_TRY(sph_thermal(c));
+    SX_HIP(hipEventRecord(c.ev[2], c.stream));
+    if (grav) SX_TRY(grav_pairs(c));
+    if (flags & SERENX_EVAL_NO_HYDRO) {
+        hipLaunchKernelGGL(k_zero_active, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active,
+                           c.ax, c.ay, c.az);
+    } else {
+        SX_TRY(run_hydro(c));
+    }
+    SX_HIP(hipEventRecord(c.ev[3], c.stream));
+    if (grav) SX_TRY(grav_join(c));
+    SX_HIP(hipEventRecord(c.ev[4], c.stream));
+    return SERENX_OK;
+}
+
+// Records part (0: A, 1: B) into *exec.  Host state the launches set is set
+// by the recording as by an eager run; a failed recording leaves no graph.
+int capture_part(Ctx &c, int part, int flags, hipGraphExec_t *exec) {
+    if (*exec) (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+    SX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    c.capturing = true;
+    int rc = part == 0 ? eval_part_a(c, flags) : eval_part_b(c, flags);
+    c.capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc == SERENX_OK && e != hipSuccess)
+        rc = set_error(SERENX_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    if (rc == SERENX_OK) {
+        const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            *exec = nullptr;
+            rc = set_error(SERENX_EHIP, std::string("graph instantiation failed: ") +
+                                            hipGetErrorString(ei));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+// A restock evaluation from two graphs: A (recorded once per graph epoch
+// and flags), the host's look at the h-solve status and the rows above
+// pp_limit (whose eager pass may reallocate their list, a new epoch for B),
+// then B.  The graphs run the same kernels with the same arguments as the
+// eager path, except the walk's queue order (tree order; the order does not
+// change results).
+int run_eval_graph(Ctx &c, int flags) {
+    Ctx::Graphs &G = c.gr;
+    if (!G.a || G.epoch_a != c.graph_epoch || G.flags_a != flags) {
+        SX_TRY(capture_part(c, 0, flags, &G.a));
+        G.epoch_a = c.graph_epoch;
+        G.flags_a = flags;
+        G.captures++;
+    }
+    SX_HIP(hipGraphLaunch(G.a, c.stream));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    SX_TRY(sph_neib_finish(c));
+    if (!G.b || G.epoch_b != c.graph_epoch || G.flags_b != flags) {
+        SX_TRY(capture_part(c, 1, flags, &G.b));
+        G.epoch_b = c.graph_epoch;
+        G.flags_b = flags;
+        G.captures++;
+    }
+    SX_HIP(hipGraphLaunch(G.b, c.stream));
+    G.launches++;
+    c.grav_timed = !(flags & SERENX_EVAL_NO_GRAV);
+    c.box_ok = true;
+    c.sph_ok = true;
+    return SERENX_OK;
+}
+
+}  // namespace
+
+void graphs_free(Ctx &c) {
+    if (c.gr.a) (void)hipGraphExecDestroy(c.gr.a);
+    if (c.gr.b) (void)hipGraphExecDestroy(c.gr.b);
+    c.gr.a = c.gr.b = nullptr;
+    c.gr.epoch_a = c.gr.epoch_b = -1;
+}
+
 int run_force_eval(Ctx &c, int flags) {
+    if (flags & (SERENX_EVAL_RESTOCK | SERENX_EVAL_GRAPH)) {
+        if (!c.tree_ok)
+            return set_error(SERENX_ESTATE, "a restock evaluation needs a tree built by an "
+                                            "earlier evaluation");
+        if (flags & SERENX_EVAL_HGUESS)
+            return set_error(SERENX_EINVAL, "SERENX_EVAL_HGUESS needs a tree build");
+        if ((flags & SERENX_EVAL_GRAPH) && c.opt.sph == SERENX_SPH_GRADH &&
+            c.opt.hfind != SERENX_HFIND_CONSTANT) {
+            GravGuard guard(c.gstream);
+            guard.armed = true;
+            SX_TRY(run_eval_graph(c, flags & ~SERENX_EVAL_GRAPH));
+            guard.armed = false;
+            return SERENX_OK;
+        }
+        GravGuard guard(c.gstream);
+        guard.armed = true;
+        SX_TRY(eval_part_a(c, flags));
+        SX_HIP(hipStreamSynchronize(c.stream));
+        SX_TRY(sph_neib_finish(c));
+        SX_TRY(eval_part_b(c, flags));
+        guard.armed = false;
+        return SERENX_OK;
+    }
     SX_TRY(zero_words(c.wstat, 2 * CTR_WORDS, c.stream));
     SX_HIP(hipEventRecord(c.ev[0], c.stream));
     SX_TRY(run_tree_update(c));
@@ -876,6 +1014,14 @@ int run_force_eval(Ctx &c, int flags) {
 }  // namespace serenx
 
 extern "C" {
+
+int serenx_graph_info(const serenx_ctx *ctx, int64_t *captures, int64_t *launches) {
+    if (!ctx || !captures || !launches) return set_error(SERENX_EINVAL, "null argument");
+    const Ctx &c = *as_ctx(ctx);
+    *captures = c.gr.captures;
+    *launches = c.gr.launches;
+    return SERENX_OK;
+}
 
 int serenx_sync(serenx_ctx *ctx) {
     if (!ctx) return set_error(SERENX_EINVAL, "null context");
@@ -1182,6 +1328,7 @@ int serenx_set_precision(serenx_ctx *ctx, int32_t precision) {
     Ctx &c = *as_ctx(ctx);
     SX_HIP(hipSetDevice(c.device));
     const bool to_mixed = precision == SERENX_MIXED && c.opt.precision != SERENX_MIXED;
+    if (precision != c.opt.precision) c.graph_epoch++;
     c.opt.precision = precision;
     if (to_mixed) SX_TRY(refresh_f32_nodes(c));
     return SERENX_OK;
@@ -1230,6 +1377,7 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
 int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
     if (!ctx) return set_error(SERENX_EINVAL, "null context");
     Ctx &c = *as_ctx(ctx);
+    c.graph_epoch++;   // recorded graphs hold launch shapes the knobs choose
     switch (key) {
     case SERENX_TUNE_HSOLVE_SLOTS:
         if (!(value >= 1.0 && value <= (double)HL_SLOTS) || value != (double)(int)value)

@@ -703,6 +703,12 @@ __global__ void k_queue_keys(int ngroups, const int *__restrict__ cost, int have
     vals[g] = g;
 }
 
+// the queue in tree order (each XCD range's groups in order)
+__global__ void k_queue_identity(int ngroups, int *__restrict__ order) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups) order[g] = g;
+}
+
 __global__ void k_zero_ints(int n, int *__restrict__ a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0;
@@ -821,11 +827,18 @@ int prepare_queue(Ctx &c) {
     const int ng = c.ngrp;
     if (ng != q.ngroups) q.have_cost = false;   // another group layout
     q.ngroups = ng;
-    hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,
-                       (int)(q.have_cost && c.walk_queue > 0), q.keys, q.vals);
-    size_t tb = q.tmp_bytes;
-    SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order, (unsigned)ng,
-                                     0, 35, c.gstream));
+    if (c.capturing) {
+        // a walk recorded into a graph takes its groups in tree order: the
+        // sort library clears its buffers with memset nodes, which graphs
+        // here must not hold (they have given wrong results on replays)
+        hipLaunchKernelGGL(k_queue_identity, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.order);
+    } else {
+        hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,
+                           (int)(q.have_cost && c.walk_queue > 0), q.keys, q.vals);
+        size_t tb = q.tmp_bytes;
+        SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order,
+                                         (unsigned)ng, 0, 35, c.gstream));
+    }
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.gstream, NXCD, q.ctr);
     SX_HIP(hipGetLastError());
     return SERENX_OK;

@@ -175,7 +175,23 @@ struct Ctx {
     bool box_ok = false;             // HNode boxes hold the current positions (stocked
                                      // since the positions last changed)
     hipEvent_t ev[8] = {};
+    hipEvent_t evjoin = nullptr;     // the gravity stream's cell phase joined (graph part A)
     double last_ms[4] = {0, 0, 0, 0};
+
+    // Captured restock evaluations (SERENX_EVAL_GRAPH): part A (stock, h
+    // solve, neighbour rows, walk cell phase) and part B (thermal, hydro,
+    // walk pair phase, join), each valid while graph_epoch and the flags it
+    // was captured with are unchanged.  graph_epoch changes whenever a
+    // device address or a launch shape a captured graph holds may change (a
+    // build, a load, a reallocation, a precision or tuning change).
+    struct Graphs {
+        hipGraphExec_t a = nullptr, b = nullptr;
+        long epoch_a = -1, epoch_b = -1;
+        int flags_a = 0, flags_b = 0;
+        long long captures = 0, launches = 0;
+    } gr;
+    long graph_epoch = 0;
+    bool capturing = false;          // host code is being recorded into a graph
 
     int n = 0;          // particles loaded
     int cap = 0;        // particle capacity of the device arrays
@@ -217,6 +233,10 @@ struct Ctx {
 
     // tree build scratch
     uint64_t *khi = nullptr, *klo = nullptr, *khi2 = nullptr, *klo2 = nullptr;
+    // forest sort: cloud keys, positions, permutation and gathered keys/values
+    int *fs_ck = nullptr, *fs_ck2 = nullptr, *fs_pos = nullptr, *fs_perm = nullptr,
+        *fs_v = nullptr;
+    uint64_t *fs_k = nullptr;
     int *idx = nullptr, *idx2 = nullptr;
     uint8_t *cpl = nullptr, *pact = nullptr;
     int *flag = nullptr, *incl = nullptr, *cellof = nullptr;
@@ -399,6 +419,8 @@ int set_clouds(Ctx &c, int ncloud, const int *counts);
 // sph.hip
 int sph_hsolve(Ctx &c);          // all_sph_gradh over active particles
 int sph_neighbours(Ctx &c);      // get_neib over active particles
+int sph_neib_rows(Ctx &c);       // sph_neighbours, launch part: rows and overflow count
+int sph_neib_finish(Ctx &c);     // sph_neighbours, host part: status, rows above pp_limit
 int sph_hgather(Ctx &c);         // standard SPH: h_gather over active particles
 int sph_density(Ctx &c);         // standard SPH: all_sph over active particles
 int sph_thermal_iso(Ctx &c, double cs);
@@ -424,6 +446,7 @@ int run_sph_update(Ctx &c);
 int run_hydro(Ctx &c);
 int run_grav(Ctx &c);
 int run_force_eval(Ctx &c, int flags);   // serenx_force_eval's body
+void graphs_free(Ctx &c);
 
 // grav.hip
 int grav_launch(Ctx &c);         // fork the gravity stream: walk phase 1 (cells), after the tree

@@ -1117,6 +1117,7 @@ int sph_hgather(Ctx &c) {
         return set_error(SERENX_EFEW, "h_gather needs more than pp_gather particles");
     const size_t need = ((size_t)c.cap + TB - 1) / TB * TB;   // whole blocks, as hlist
     if (c.band_cap < need) {
+        c.graph_epoch++;
         if (c.band) (void)hipFree(c.band);
         c.band = nullptr;
         c.band_cap = 0;
@@ -1153,7 +1154,11 @@ int sph_density(Ctx &c) {
     return SERENX_OK;
 }
 
-int sph_neighbours(Ctx &c) {
+// The neighbour rows of sph_neighbours, launches only: rows of up to
+// pp_limit ids, every particle's count, and the number of particles above
+// pp_limit copied to pinned slot 9 (read by sph_neib_finish once the stream
+// has got there).
+int sph_neib_rows(Ctx &c) {
     hipStream_t s = c.stream;
     double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     // [0] x y z h with the solved h
@@ -1165,7 +1170,21 @@ int sph_neighbours(Ctx &c) {
                        c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    SX_HIP(hipStreamSynchronize(s));
+    return SERENX_OK;
+}
+
+int sph_neighbours(Ctx &c) {
+    SX_TRY(sph_neib_rows(c));
+    SX_HIP(hipStreamSynchronize(c.stream));
+    return sph_neib_finish(c);
+}
+
+// The host part of sph_neighbours, after the stream has run sph_neib_rows:
+// the h solve's status (slot 8) and the complete lists of the particles
+// above pp_limit (the rows get_neib_on_fly.F90 would walk for).
+int sph_neib_finish(Ctx &c) {
+    hipStream_t s = c.stream;
+    double4 *rec = reinterpret_cast<double4 *>(c.hrec);
     if (c.hostpin[8] != 0)
         return set_error(SERENX_EHNEG, "Negative smoothing lengths (all_sph_gradh)");
     if (c.hostpin[9] > 0) {
@@ -1187,6 +1206,7 @@ int sph_neighbours(Ctx &c) {
         SX_HIP(hipStreamSynchronize(s));
         const size_t need = (size_t)c.hostpin[10];
         if (need > c.ovf_cap) {
+            c.graph_epoch++;   // a captured hydro pass holds the old address
             if (c.ovf_list) (void)hipFree(c.ovf_list);
             c.ovf_list = nullptr;
             c.ovf_cap = need + need / 4 + 1024;

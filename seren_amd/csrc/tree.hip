@@ -162,6 +162,35 @@ __global__ void k_gather_u64(int n, const int *__restrict__ idx, const uint64_t 
     if (i < n) dst[i] = src[idx[i]];
 }
 
+// the cloud of every element of a key-sorted order (vals: particle indices)
+// and its position, keys of the stable sort by cloud that follows
+__global__ void k_cloud_keys(int n, const int *__restrict__ vals, const int *__restrict__ cid,
+                             int *__restrict__ ck, int *__restrict__ pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ck[i] = cid[vals[i]];
+    pos[i] = i;
+}
+
+// keys and values reordered by perm, into k2/v2
+__global__ void k_permute_kv(int n, const int *__restrict__ perm, const uint64_t *__restrict__ k,
+                             const int *__restrict__ v, uint64_t *__restrict__ k2,
+                             int *__restrict__ v2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = perm[i];
+    k2[i] = k[j];
+    v2[i] = v[j];
+}
+
+__global__ void k_copy_kv(int n, const uint64_t *__restrict__ k2, const int *__restrict__ v2,
+                          uint64_t *__restrict__ k, int *__restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    k[i] = k2[i];
+    v[i] = v2[i];
+}
+
 // any run of > LEAFMAX equal 21-level keys in one cloud needs the deeper 9
 // levels sorted
 __global__ void k_deep_check(int n, const uint64_t *__restrict__ khi, const int *__restrict__ cid,
@@ -666,6 +695,12 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.chunks, 3 * (size_t)c.max_chunks));
     SX_TRY(dalloc(&c.bbpart, 6 * (size_t)c.max_chunks));
     SX_TRY(dalloc(&c.cid, nn));
+    SX_TRY(dalloc(&c.fs_ck, nn));
+    SX_TRY(dalloc(&c.fs_ck2, nn));
+    SX_TRY(dalloc(&c.fs_pos, nn));
+    SX_TRY(dalloc(&c.fs_perm, nn));
+    SX_TRY(dalloc(&c.fs_v, nn));
+    SX_TRY(dalloc(&c.fs_k, nn));
     c.ccap = 2 * n + 2 + c.max_clouds;
     const size_t cc = (size_t)c.ccap;
     SX_TRY(dalloc(&c.c_start, cc));
@@ -684,14 +719,16 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.gnf, cc));
     SX_TRY(dalloc(&c.hn, cc));
     // temporary storage for sort and scans
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     rocprim::radix_sort_pairs(nullptr, b1, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0, 63,
                               c.stream);
+    rocprim::radix_sort_pairs(nullptr, b4, c.fs_ck, c.fs_ck2, c.fs_pos, c.fs_perm, (unsigned)n,
+                              0, 16, c.stream);
     rocprim::inclusive_scan(nullptr, b2, c.flag, c.incl, (size_t)n, rocprim::plus<int>(),
                             c.stream);
     rocprim::exclusive_scan(nullptr, b3, c.ncs, c.off, 0, (size_t)n + 1, rocprim::plus<int>(),
                             c.stream);
-    size_t need = std::max(b1, std::max(b2, b3));
+    size_t need = std::max(std::max(b1, b4), std::max(b2, b3));
     if (c.sorttmp) (void)hipFree(c.sorttmp);
     c.sorttmp = nullptr;
     if (hipMalloc(&c.sorttmp, need + 256) != hipSuccess)
@@ -756,6 +793,7 @@ int set_clouds(Ctx &c, int ncloud, const int *counts) {
     if (first[ncloud] > c.max_chunks) return set_error(SERENX_EINVAL, "too many clouds");
     c.ncloud = ncloud;
     c.nchunk = first[ncloud];
+    c.graph_epoch++;
     SX_HIP(hipMemcpy(c.cloud_off, c.cloud_off_h.data(), sizeof(int) * (ncloud + 1),
                      hipMemcpyHostToDevice));
     SX_HIP(hipMemcpy(c.chunk_first, first.data(), sizeof(int) * (ncloud + 1),
@@ -782,6 +820,7 @@ int set_clouds(Ctx &c, int ncloud, const int *counts) {
 int build_tree(Ctx &c) {
     const int n = c.n;
     hipStream_t s = c.stream;
+    c.graph_epoch++;   // a new topology, and the particle arrays trade places
     c.tree_ok = false;
     c.sph_ok = false;
     // bounding box and root cell of every cloud
@@ -793,31 +832,30 @@ int build_tree(Ctx &c) {
                        c.khi, c.klo, c.idx);
     SX_HIP(hipGetLastError());
     // sort by the 21-level key within each cloud; redo with all 30 levels
-    // only when needed
+    // only when needed.  A forest sorts all particles by key, then by cloud:
+    // both radix sorts are stable, so every cloud comes out in the order of
+    // sorting it alone (ties in input order), in the cloud's own place.  (A
+    // segmented sort gives each cloud to one workgroup: tens of ms for a
+    // cloud of a million particles.)
+    const int cbits = std::max(1, 32 - __builtin_clz((unsigned)c.ncloud));
     auto sort_keys = [&](uint64_t *kin, uint64_t *kout, int *vin, int *vout,
                          int bits) -> hipError_t {
         size_t bytes = c.sorttmp_bytes;
-        if (c.ncloud == 1)
-            return rocprim::radix_sort_pairs(c.sorttmp, bytes, kin, kout, vin, vout, (unsigned)n,
-                                              0, bits, s);
-        return rocprim::segmented_radix_sort_pairs(c.sorttmp, bytes, kin, kout, vin, vout,
-                                                   (unsigned)n, (unsigned)c.ncloud, c.cloud_off,
-                                                   c.cloud_off + 1, 0, bits, s);
+        hipError_t e = rocprim::radix_sort_pairs(c.sorttmp, bytes, kin, kout, vin, vout,
+                                                 (unsigned)n, 0, bits, s);
+        if (e != hipSuccess || c.ncloud == 1) return e;
+        hipLaunchKernelGGL(k_cloud_keys, dim3(nblk(n)), dim3(TB), 0, s, n, vout, c.cid, c.fs_ck,
+                           c.fs_pos);
+        bytes = c.sorttmp_bytes;
+        e = rocprim::radix_sort_pairs(c.sorttmp, bytes, c.fs_ck, c.fs_ck2, c.fs_pos, c.fs_perm,
+                                      (unsigned)n, 0, cbits, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_permute_kv, dim3(nblk(n)), dim3(TB), 0, s, n, c.fs_perm, kout, vout,
+                           c.fs_k, c.fs_v);
+        hipLaunchKernelGGL(k_copy_kv, dim3(nblk(n)), dim3(TB), 0, s, n, c.fs_k, c.fs_v, kout,
+                           vout);
+        return hipGetLastError();
     };
-    if (c.ncloud > 1) {
-        size_t need = 0;
-        SX_HIP(rocprim::segmented_radix_sort_pairs(nullptr, need, c.khi, c.khi2, c.idx, c.idx2,
-                                                   (unsigned)n, (unsigned)c.ncloud, c.cloud_off,
-                                                   c.cloud_off + 1, 0, 63, s));
-        if (need > c.sorttmp_bytes) {
-            SX_HIP(hipStreamSynchronize(s));
-            if (c.sorttmp) (void)hipFree(c.sorttmp);
-            c.sorttmp = nullptr;
-            if (hipMalloc(&c.sorttmp, need + 256) != hipSuccess)
-                return set_error(SERENX_ENOMEM, "hipMalloc failed for sort scratch");
-            c.sorttmp_bytes = need + 256;
-        }
-    }
     SX_HIP(sort_keys(c.khi, c.khi2, c.idx, c.idx2, 63));
     SX_TRY(zero_words(c.flag, 1, s));
     hipLaunchKernelGGL(k_deep_check, dim3(nblk(n)), dim3(TB), 0, s, n, c.khi2, c.cid, c.flag);

@@ -32,7 +32,7 @@ LIB_PATH = os.environ.get("SERENX_LIB") or os.path.join(HERE, "libserenx.so")
 QUADRUPOLE, OCTUPOLE, OCTUPOLE_EXACT = 2, 3, 4
 MAC_GEOMETRIC, MAC_GADGET, MAC_GADGET2, MAC_EIGEN = 0, 1, 2, 3
 FP64, MIXED = 0, 1
-EVAL_HGUESS, EVAL_NO_GRAV, EVAL_NO_HYDRO = 1, 2, 4
+EVAL_HGUESS, EVAL_NO_GRAV, EVAL_NO_HYDRO, EVAL_RESTOCK, EVAL_GRAPH = 1, 2, 4, 8, 16
 
 _MULTIPOLE = {"quadrupole": QUADRUPOLE, "octupole": OCTUPOLE, "octupole_exact": OCTUPOLE_EXACT}
 _MAC = {"geometric": MAC_GEOMETRIC, "gadget": MAC_GADGET, "gadget2": MAC_GADGET2,
@@ -55,7 +55,7 @@ EXPORTED = [
     "serenx_memory", "serenx_pow_third", "serenx_set_energy", "serenx_fetch_energy",
     "serenx_set_tuning", "serenx_get_tuning", "serenx_serve_begin", "serenx_serve_step",
     "serenx_serve_info", "serenx_diagnostics", "serenx_walk_timings",
-    "serenx_diagnostics_noalloc",
+    "serenx_diagnostics_noalloc", "serenx_graph_info",
 ]
 INT_RK, INT_LFKDK = 0, 1
 _INTEGRATOR = {"rk": INT_RK, "lfkdk": INT_LFKDK}
@@ -143,6 +143,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.serenx_set_precision.argtypes = [vp, C.c_int32]
     lib.serenx_diagnostics.argtypes = [vp, _dp]
     lib.serenx_diagnostics_noalloc.argtypes = [vp, vp]   # (context, device pointer)
+    lib.serenx_graph_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.serenx_walk_timings.argtypes = [vp, _dp]
     lib.serenx_serve_begin.argtypes = [vp, C.c_double, C.c_double]
     lib.serenx_serve_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_uint8)]
@@ -331,10 +332,21 @@ class ForceEngine:
         am = np.ascontiguousarray(am, np.float64)
         _check(self.lib.serenx_set_agravmag(self.ctx, _dptr(am)))
 
-    def force_eval(self, hguess: bool = False, grav: bool = True, hydro: bool = True) -> None:
+    def force_eval(self, hguess: bool = False, grav: bool = True, hydro: bool = True,
+                   restock: bool = False, graph: bool = False) -> None:
+        """One force evaluation.  restock: on the tree of the last build,
+        restocked with the current positions (tree_update.F90 between
+        rebuilds); graph (with restock): as recorded HIP graphs."""
         flags = (EVAL_HGUESS if hguess else 0) | (0 if grav else EVAL_NO_GRAV) | \
-            (0 if hydro else EVAL_NO_HYDRO)
+            (0 if hydro else EVAL_NO_HYDRO) | (EVAL_RESTOCK if restock or graph else 0) | \
+            (EVAL_GRAPH if graph else 0)
         _check(self.lib.serenx_force_eval(self.ctx, flags))
+
+    def graph_info(self) -> Dict[str, int]:
+        """Recordings and replays of the restock evaluation's graphs."""
+        cap, lau = C.c_int64(), C.c_int64()
+        _check(self.lib.serenx_graph_info(self.ctx, C.byref(cap), C.byref(lau)))
+        return {"captures": int(cap.value), "launches": int(lau.value)}
 
     def sync(self) -> None:
         _check(self.lib.serenx_sync(self.ctx))
