@@ -257,7 +257,7 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     // work queue of the gravity walk: one entry per group
     Ctx::WalkQueue &q = c.gq;
-    SX_TRY(dalloc(&q.ctr, (size_t)NXCD));
+    SX_TRY(dalloc(&q.ctr, (size_t)(2 * NXCD)));   // phase 1 and phase 2 counters
     SX_TRY(dalloc(&q.cost, ng));
     SX_TRY(dalloc(&q.order, ng));
     SX_TRY(dalloc(&q.vals, ng));

@@ -350,21 +350,20 @@ __global__ void __launch_bounds__(256) k_grav_cells_fp64(
 // leaves, in list order, added to the cell sums of phase 1.  A group whose
 // list did not fit repeats its walk (the same MAC decisions: they read the
 // stocked tree and the previous |a|, which phase 2 overwrites only after)
-// and takes the pairs from the leaves as it opens them.  One wave per group.
+// and takes the pairs from the leaves as it opens them.  One wave per group
+// at a time; the kernels below take the groups from the walk's queue.
 template <int MACT, bool QUINTIC>
-__global__ void __launch_bounds__(256) k_grav_pairs_fp64(
-    int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
+__device__ __forceinline__ void pairs_group_fp64(
+    int g, int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     const double *__restrict__ cp, double *__restrict__ gpot, double *__restrict__ agm,
     NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
-    int ngrp) {
+    int ngrp,
+    unsigned &npp_out, unsigned &nin_p_out, unsigned &nin_j_out) {
     constexpr KGeom kg = walk_geom<QUINTIC>();
-    __builtin_amdgcn_s_setprio(2);
-    const int g = (int)((xcd_block() * blockDim.x + threadIdx.x) / WAVE);
     const int lane = (int)(threadIdx.x & (WAVE - 1));
-    if (g >= ngrp) return;   // wave-uniform
     const int2 gr = grp[g];
     const int p = gr.x + lane;
     const bool valid = p < gr.y && active[p];
@@ -427,7 +426,32 @@ __global__ void __launch_bounds__(256) k_grav_pairs_fp64(
         gpot[p] = gp;
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
-    if (lane == 0) {
+    npp_out += npp;
+    nin_p_out += nin_p;
+    nin_j_out += nin_j;
+}
+
+template <int MACT, bool QUINTIC>
+__global__ void __launch_bounds__(256) k_grav_pairs_fp64(
+    int n, int nc, const GNode *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
+    const uint8_t *__restrict__ active, const double4 *__restrict__ t4,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ cp, double *__restrict__ gpot, double *__restrict__ agm,
+    NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
+    int ngrp,
+    GroupQueue queue) {
+    // persistent over the walk's queue: the groups of each XCD range
+    // longest-first (by the previous walk's cost), as phase 1 takes them
+    __builtin_amdgcn_s_setprio(2);
+    unsigned npp = 0, nin_p = 0, nin_j = 0;   // wave totals
+    const int home = blockIdx.x % NXCD;
+    int qdone = 0;
+    for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone))
+        pairs_group_fp64<MACT, QUINTIC>(g, n, nc, gn, rec, cid, root_range, active, t4, ax, ay,
+                                        az, cp, gpot, agm, near, icount, grp, ngrp, npp, nin_p,
+                                        nin_j);
+    if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
         atomicAdd(ic + 1, (unsigned long long)npp);
         atomicAdd(ic + 2, (unsigned long long)nin_p);
@@ -603,20 +627,18 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
 }
 
 template <int MACT, bool QUINTIC>
-__global__ void __launch_bounds__(256) k_grav_pairs_mixed(
-    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+__device__ __forceinline__ void pairs_group_mixed(
+    int g, int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
     const double *__restrict__ cpot, double *__restrict__ gpot, double *__restrict__ agm,
     NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
-    int ngrp) {
+    int ngrp,
+    unsigned &npp_out, unsigned &nin_p_out, unsigned &nin_j_out) {
     constexpr KGeom kg64 = walk_geom<QUINTIC>();
     constexpr KGeomF kg{(float)kg64.half, (float)kg64.range, (float)kg64.invrange};
-    __builtin_amdgcn_s_setprio(2);
-    const int g = (int)((xcd_block() * blockDim.x + threadIdx.x) / WAVE);
     const int lane = (int)(threadIdx.x & (WAVE - 1));
-    if (g >= ngrp) return;   // wave-uniform
     const int2 gr = grp[g];
     const int p = gr.x + lane;
     const bool valid = p < gr.y && active[p];
@@ -681,7 +703,32 @@ __global__ void __launch_bounds__(256) k_grav_pairs_mixed(
         gpot[p] = cpot[p] + pq.value();
         if (MACT != SERENX_MAC_GEOMETRIC) agm[p] = sqrt(gx * gx + gy * gy + gz * gz);
     }
-    if (lane == 0) {
+    npp_out += npp;
+    nin_p_out += nin_p;
+    nin_j_out += nin_j;
+}
+
+template <int MACT, bool QUINTIC>
+__global__ void __launch_bounds__(256) k_grav_pairs_mixed(
+    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    const int *__restrict__ cid, const int2 *__restrict__ root_range,
+    const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ cpot, double *__restrict__ gpot, double *__restrict__ agm,
+    NearList near, unsigned long long *__restrict__ icount, const int2 *__restrict__ grp,
+    int ngrp,
+    GroupQueue queue) {
+    // persistent over the walk's queue: the groups of each XCD range
+    // longest-first (by the previous walk's cost), as phase 1 takes them
+    __builtin_amdgcn_s_setprio(2);
+    unsigned npp = 0, nin_p = 0, nin_j = 0;   // wave totals
+    const int home = blockIdx.x % NXCD;
+    int qdone = 0;
+    for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone))
+        pairs_group_mixed<MACT, QUINTIC>(g, n, nc, gn, rec, cid, root_range, active, t4, ax,
+                                         ay, az, cpot, gpot, agm, near, icount, grp, ngrp, npp,
+                                         nin_p, nin_j);
+    if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
         atomicAdd(ic + 1, (unsigned long long)npp);
         atomicAdd(ic + 2, (unsigned long long)nin_p);
@@ -802,22 +849,24 @@ void launch_cells(Ctx &c) {
     }
 }
 
-// phase 2 (pair terms), one wave per group
+// phase 2 (pair terms), persistent over the same queue order (its second
+// set of counters)
 template <int MACT, bool QUINTIC>
 void launch_pairs(Ctx &c, const double4 *rec) {
-    const int blocks = nblk((long)c.ngrp * WAVE);
-    if (c.opt.precision == SERENX_MIXED)
-        hipLaunchKernelGGL((k_grav_pairs_mixed<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
-                           c.gstream, c.n, c.nc, c.gnf, rec, c.cid, c.root_range, c.active,
+    GroupQueue q{c.gq.ctr + NXCD, c.gq.order, c.gq.ngroups};
+    if (c.opt.precision == SERENX_MIXED) {
+        auto k = k_grav_pairs_mixed<MACT, QUINTIC>;
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
+                           rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
-                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp,
-                           c.ngrp);
-    else
-        hipLaunchKernelGGL((k_grav_pairs_fp64<MACT, QUINTIC>), dim3(blocks), dim3(TB), 0,
-                           c.gstream, c.n, c.nc, c.gn, rec, c.cid, c.root_range, c.active,
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp, c.ngrp, q);
+    } else {
+        auto k = k_grav_pairs_fp64<MACT, QUINTIC>;
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
+                           rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
-                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp,
-                           c.ngrp);
+                           c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp, c.ngrp, q);
+    }
 }
 
 // the queue order of this walk (from the previous walk's costs) and zeroed
@@ -839,7 +888,7 @@ int prepare_queue(Ctx &c) {
         SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order,
                                          (unsigned)ng, 0, 35, c.gstream));
     }
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.gstream, NXCD, q.ctr);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.gstream, 2 * NXCD, q.ctr);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }

@@ -720,14 +720,14 @@ int ensure_tree_buffers(Ctx &c) {
     SX_TRY(dalloc(&c.hn, cc));
     // temporary storage for sort and scans
     size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-    rocprim::radix_sort_pairs(nullptr, b1, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0, 63,
-                              c.stream);
-    rocprim::radix_sort_pairs(nullptr, b4, c.fs_ck, c.fs_ck2, c.fs_pos, c.fs_perm, (unsigned)n,
-                              0, 16, c.stream);
-    rocprim::inclusive_scan(nullptr, b2, c.flag, c.incl, (size_t)n, rocprim::plus<int>(),
-                            c.stream);
-    rocprim::exclusive_scan(nullptr, b3, c.ncs, c.off, 0, (size_t)n + 1, rocprim::plus<int>(),
-                            c.stream);
+    SX_HIP(rocprim::radix_sort_pairs(nullptr, b1, c.khi, c.khi2, c.idx, c.idx2, (unsigned)n, 0,
+                                     63, c.stream));
+    SX_HIP(rocprim::radix_sort_pairs(nullptr, b4, c.fs_ck, c.fs_ck2, c.fs_pos, c.fs_perm,
+                                     (unsigned)n, 0, 16, c.stream));
+    SX_HIP(rocprim::inclusive_scan(nullptr, b2, c.flag, c.incl, (size_t)n, rocprim::plus<int>(),
+                                   c.stream));
+    SX_HIP(rocprim::exclusive_scan(nullptr, b3, c.ncs, c.off, 0, (size_t)n + 1,
+                                   rocprim::plus<int>(), c.stream));
     size_t need = std::max(std::max(b1, b4), std::max(b2, b3));
     if (c.sorttmp) (void)hipFree(c.sorttmp);
     c.sorttmp = nullptr;

@@ -189,7 +189,7 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
 // own is empty.  Every call either returns a group or moves to the next
 // range, so every wave leaves the loop after at most ngroups + NXCD calls.
 struct GroupQueue {
-    int *ctr;            // NXCD counters, zero at the launch
+    int *ctr;            // NXCD counters (one set per walk phase), zero at the launch
     const int *order;    // ngroups group ids
     int ngroups;
 };
