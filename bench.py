@@ -66,6 +66,11 @@ def parse():
                     help="mixed leg: keep the reference MAC tolerance (no autotuning)")
     ap.add_argument("--mixed-abserror", type=float, default=0.0,
                     help="MAC tolerance of the mixed walk (0: the autotuned one, or abserror)")
+    ap.add_argument("--walk-queue", type=int, default=-1,
+                    help="gravity walk blocks per compute unit (-1: autotuned, or the library's "
+                         "default with --no-tune-walk)")
+    ap.add_argument("--no-tune-walk", dest="tune_walk", action="store_false", default=True,
+                    help="keep the library's walk blocks per CU (no timing of candidates)")
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="step/eval modes: requests of the bounded serving leg (0: none)")
     ap.add_argument("--precision-serve", default="mixed", choices=["fp64", "mixed"])
@@ -163,6 +168,15 @@ def bench_forest(args):
               flush=True)
         off = eng.load_forest(clouds)
     eng.force_eval(hguess=True)
+    walk_tuned = None
+    if args.walk_queue >= 0:
+        from seren_amd.engine import TUNE_WALK_QUEUE
+        eng.set_tuning(TUNE_WALK_QUEUE, args.walk_queue)
+    elif args.tune_walk:
+        from seren_amd.autotune import tune_walk_blocks
+        walk_tuned = tune_walk_blocks(
+            eng, lambda: eng.force_eval(restock=True, graph=args.graph))
+        print(f"# walk blocks per CU {walk_tuned['blocks_per_cu']}", file=sys.stderr, flush=True)
 
     def one(k):
         if k % max(args.nbuild, 1) == 0:
@@ -205,6 +219,7 @@ def bench_forest(args):
                       "nbuild": args.nbuild, "graph": bool(args.graph)},
            "graphs": ginfo,
            "autotune": tuned,
+           "walk_blocks": walk_tuned,
            "phases_ms": {k: float(v / args.steps) for k, v in
                          zip(("tree", "sph", "hydro", "grav"), phases)},
            "accuracy_vs_fp64_direct_sum": {"targets": int(len(tg)),
@@ -227,6 +242,10 @@ def step_leg(args, cloud, precision, tune=False):
             from seren_amd.engine import TUNE_MIXED_ABSERROR
             eng.set_tuning(TUNE_MIXED_ABSERROR, args.mixed_abserror)
             out["mixed_abserror"] = args.mixed_abserror
+        if args.walk_queue >= 0:
+            from seren_amd.engine import TUNE_WALK_QUEUE
+            eng.set_tuning(TUNE_WALK_QUEUE, args.walk_queue)
+            out["walk_queue"] = args.walk_queue
         eng.load(pos, vel, mass, h)
         eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
         eng.sync()
@@ -269,6 +288,12 @@ def step_leg(args, cloud, precision, tune=False):
             eng.step_begin("lfkdk", nlevels=1, nbuildstep=8, accel_mult=0.5, courant_mult=0.2)
         else:
             eng.force_eval()                         # provides |a| for the multipole MAC
+        if args.walk_queue < 0 and args.tune_walk:
+            # the walk's blocks per CU, timed on the unit itself (untimed here)
+            from seren_amd.autotune import tune_walk_blocks
+            out["walk_blocks"] = tune_walk_blocks(eng, unit)
+            print(f"# walk blocks per CU {out['walk_blocks']['blocks_per_cu']}",
+                  file=sys.stderr, flush=True)
         for _ in range(args.warmup):
             unit()
         eng.sync()
@@ -327,7 +352,7 @@ def step_leg(args, cloud, precision, tune=False):
     return out
 
 
-def serve_leg(args, nrequests, pool=2):
+def serve_leg(args, nrequests, pool=4):
     """Bounded serving run: Poisson requests at args.rate on resident clouds
     of N in {2k, 16k, 128k, 1M} (p = 0.4, 0.3, 0.2, 0.1), mixed precision
     with an FP64 guard step every 4 steps of a cloud, continuous batching in

@@ -308,9 +308,12 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *  SERENX_TUNE_WALK_QUEUE: v >= 1 runs the gravity walk as a persistent
  *    kernel of at most v blocks per CU (default 64: as many as fit) whose
  *    waves take groups of 64 particles from per-XCD work queues, each ordered
- *    by the cost of the group's previous walk (longest first); fewer blocks
- *    leave room for the SPH kernels that run beside the walk.  0 launches one
- *    wave per group in tree order.  Results do not change.
+ *    by the cost of the group's previous walk (longest first).  Fewer blocks
+ *    leave wave slots to the SPH kernels that run beside the walk in force
+ *    evaluations and steps; the best value depends on the workload (the
+ *    autotuner in seren_amd/autotune.py times a few).  A walk with nothing
+ *    beside it (serenx_grav_forces) takes all that fit.  0 launches one wave
+ *    per group in tree order.  Results do not change.
  *  SERENX_TUNE_NEIB_GROUP: 1 (default) builds the neighbour lists with one
  *    walk per wave that opens every cell any of its particles could need; 0
  *    with the particles' own walks (BHhydro_walk.F90's test per particle).

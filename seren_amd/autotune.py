@@ -11,20 +11,28 @@ walks without changing any result.
 tune_mixed() measures, on the cloud loaded in an engine, the evaluation time
 and the error against the direct sum on a seeded sample of targets for each
 candidate tolerance, and keeps the fastest whose rms error stays within
-safety * target (a margin for clouds and steps it has not seen).  Everything
-runs on the device through the library; nothing here is a CPU fallback.
+safety * target (a margin for clouds and steps it has not seen).
+
+tune_walk_blocks() chooses the gravity walk's blocks per CU
+(SERENX_TUNE_WALK_QUEUE) by timing the caller's unit of work (a force
+evaluation or a step) for each candidate: the walk shares the device's
+vector ALUs with the SPH kernels beside it, and the split that finishes the
+unit first depends on the cloud and the precision.  Results do not depend
+on it.  Everything runs on the device through the library; nothing here is
+a CPU fallback.
 """
 from __future__ import annotations
 
 import time
-from typing import Dict, List, Sequence
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 
-from .engine import TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR, ForceEngine
+from .engine import TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR, TUNE_WALK_QUEUE, ForceEngine
 
 DEFAULT_TOLERANCES = (1e-4, 2e-4, 4e-4, 8e-4, 1.6e-3, 3.2e-3)
 DEFAULT_SLOTS = (32, 48, 64, 96)
+DEFAULT_WALK_BLOCKS = (3, 4, 6, 64)
 
 
 def _grav_error(eng: ForceEngine, targets: np.ndarray) -> Dict[str, float]:
@@ -90,3 +98,28 @@ def tune_slots(eng: ForceEngine, slots: Sequence[int] = DEFAULT_SLOTS,
     best = min(table, key=lambda r: r["ms"])
     eng.set_tuning(TUNE_HSOLVE_SLOTS, best["slots"])
     return {"slots": best["slots"], "table": table}
+
+
+def tune_walk_blocks(eng: ForceEngine, unit: Optional[Callable[[], None]] = None,
+                     blocks: Sequence[int] = DEFAULT_WALK_BLOCKS,
+                     reps: int = 3) -> Dict[str, object]:
+    """Chooses the walk's blocks per CU by the median time of reps calls of
+    unit (default: a force evaluation) after one untimed call per candidate,
+    and sets it.  A step as the unit advances the engine's state."""
+    if unit is None:
+        unit = eng.force_eval
+    table = []
+    for b in blocks:
+        eng.set_tuning(TUNE_WALK_QUEUE, b)
+        unit()
+        ts = []
+        for _ in range(reps):
+            eng.sync()
+            t0 = time.perf_counter()
+            unit()
+            eng.sync()
+            ts.append(time.perf_counter() - t0)
+        table.append({"blocks_per_cu": int(b), "ms": 1e3 * float(np.median(ts))})
+    best = min(table, key=lambda r: r["ms"])
+    eng.set_tuning(TUNE_WALK_QUEUE, best["blocks_per_cu"])
+    return {"blocks_per_cu": best["blocks_per_cu"], "table": table}

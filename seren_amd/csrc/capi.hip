@@ -386,8 +386,8 @@ int run_sph_h(Ctx &c) {
     return SERENX_OK;
 }
 
+// the neighbour lists (after update_hmax)
 int run_sph_lists(Ctx &c) {
-    SX_TRY(update_hmax(c));
     SX_TRY(sph_neighbours(c));
     if (c.opt.sph == SERENX_SPH_STANDARD) SX_TRY(sph_density(c));
     c.sph_ok = true;
@@ -396,6 +396,7 @@ int run_sph_lists(Ctx &c) {
 
 int run_sph_update(Ctx &c) {
     SX_TRY(run_sph_h(c));
+    SX_TRY(update_hmax(c));
     SX_TRY(run_sph_lists(c));
     return SERENX_OK;
 }
@@ -412,8 +413,11 @@ int run_grav(Ctx &c) {
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "gravity needs tree_update first");
     GravGuard guard(c.gstream);
     guard.armed = true;
-    SX_TRY(grav_launch(c));
-    SX_TRY(grav_pairs(c));
+    c.walk_alone = true;   // nothing runs beside this walk
+    const int rc1 = grav_launch(c);
+    const int rc2 = rc1 == SERENX_OK ? grav_pairs(c) : rc1;
+    c.walk_alone = false;
+    SX_TRY(rc2);
     SX_TRY(grav_join(c));
     guard.armed = false;
     return SERENX_OK;
@@ -928,6 +932,7 @@ int run_eval_graph(Ctx &c, int flags) {
         G.captures++;
     }
     SX_HIP(hipGraphLaunch(G.a, c.stream));
+    c.gq.order_ready = false;   // graph A leaves the queue in tree order
     SX_HIP(hipStreamSynchronize(c.stream));
     SX_TRY(sph_neib_finish(c));
     if (!G.b || G.epoch_b != c.graph_epoch || G.flags_b != flags) {
@@ -994,6 +999,9 @@ int run_force_eval(Ctx &c, int flags) {
         SX_TRY(grav_launch(c));
     }
     SX_TRY(run_sph_h(c));
+    // hmax before the pair phase: the persistent pair kernel fills the
+    // device, and the small per-level hmax launches would wait behind it
+    SX_TRY(update_hmax(c));
     if (grav) SX_TRY(grav_pairs(c));
     SX_TRY(run_sph_lists(c));
     SX_TRY(sph_thermal(c));
@@ -1398,6 +1406,7 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
         if (!(value >= 0.0 && value <= 64.0) || value != (double)(int)value)
             return set_error(SERENX_EINVAL, "walk queue blocks per CU must be an integer in 0..64");
         c.walk_queue = (int)value;
+        c.gq.order_ready = false;   // its keys depend on the queue's use
         return SERENX_OK;
     case SERENX_TUNE_HSOLVE_WAVES:
         if (value != -1.0 && value != 0.0 && value != 8.0)

@@ -16,8 +16,9 @@
 // Mixed precision (SERENX_MIXED): the cell offset rp - rc is formed in FP64
 // and the multipole and pair terms are evaluated in FP32 on that
 // cell-relative offset; cell terms are summed in FP32 batches added to FP64
-// sums, pair terms in Neumaier-compensated FP32 sums.  Its walk uses a group
-// MAC (see k_grav_mixed).
+// sums, pair terms in FP32 per leaf, each leaf's sum added to
+// Neumaier-compensated FP32 sums.  Its walk uses a group MAC (see
+// group_accepts).
 #include <algorithm>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
@@ -135,6 +136,33 @@ __global__ void k_pack_grav(int n, double kernrange, const double *__restrict__ 
     double4 *r = rec + 2 * (size_t)i;
     r[0] = make_double4(x[i], y[i], z[i], m[i]);
     r[1] = make_double4(h[i], 1.0 / h[i], zeta[i], kernrange * h[i]);
+}
+
+// Gravity source records of the mixed walk, 32 bytes per particle, written
+// per leaf of the gravity tree (one thread per node; leaves partition the
+// particles):
+//   [0] x - x_f, y - y_f, z - z_f (FP32), m / 2   [1] KERNRANGE h, 1/h, 1/h^2, zeta/h^2
+// where (x_f, y_f, z_f) is the leaf's first particle.  A pair term then
+// takes the offset r_j - r_p as (r_j - r_f) + (r_f - r_p): the second term is
+// formed in FP64 once per leaf and lane, so no FP64 work remains per pair.
+__global__ void k_pack_grav_mixed(int nc, double kernrange, const GNodeF *__restrict__ gn,
+                                  const double *__restrict__ x, const double *__restrict__ y,
+                                  const double *__restrict__ z, const double *__restrict__ m,
+                                  const double *__restrict__ h, const double *__restrict__ zeta,
+                                  float4 *__restrict__ rec) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int k = gn[c].leaf, f = gn[c].first;
+    if (k <= 0) return;
+    const double xf = x[f], yf = y[f], zf = z[f];
+    for (int i = 0; i < k; i++) {
+        const int j = f + i;
+        const double hj = h[j], ih = 1.0 / hj;
+        rec[2 * (size_t)j] = make_float4((float)(x[j] - xf), (float)(y[j] - yf),
+                                         (float)(z[j] - zf), (float)(0.5 * m[j]));
+        rec[2 * (size_t)j + 1] = make_float4((float)(kernrange * hj), (float)ih,
+                                             (float)(ih * ih), (float)(zeta[j] * ih * ih));
+    }
 }
 
 // a += a_grav for the active particles: the one addition sph_grav_forces.F90
@@ -471,43 +499,61 @@ struct CSum {
     __device__ __forceinline__ double value() const { return (double)s + (double)c; }
 };
 
-__device__ __forceinline__ unsigned pair_fp32(const KGeomF &kg, const float4 *__restrict__ t4,
-                                               float invhp,
-                                               float zo_p, float dx, float dy, float dz, float mj,
-                                               float rangehj, float invhj, float zj, CSum &ax,
-                                               CSum &ay, CSum &az, CSum &pot) {
+// The target's side of a mixed pair term, per lane: 1/h_p, 1/h_p^2,
+// zeta_p/h_p^2, the kernel-range limit on 1/r (KERNRANGE h_p)^-1 and the
+// table scale KERNTOT / (KERNRANGE h_p)
+struct PairTarget {
+    float ih, ih2, zi2, lim, hk;
+};
+
+// gravity_gradh.F90:42-86 for one pair in FP32 on the offset (dx, dy, dz) =
+// r_j - r_p, with the source record of k_pack_grav_mixed (hm = m_j / 2, rh =
+// KERNRANGE h_j, ih = 1/h_j, ih2 = 1/h_j^2, zi2 = zeta_j/h_j^2).  |r| and 1/|r|
+// come from one reciprocal square root.  The softened terms are selected per
+// lane under wave-uniform branches (taken when any lane is inside the
+// range), so that the wave runs no divergent branches; a lane with act
+// unset adds nothing.  The terms
+// are added to FP32 sums (the caller's per-leaf partial sums).  mact is the
+// wave's mask of the lanes with act set; the wave counts the pairs inside
+// the kernel range of h_p (nin_p) and of h_j (nin_j) from the masks of the
+// two comparisons themselves.
+__device__ __forceinline__ void pair_f32(const KGeomF &kg, const float4 *__restrict__ t4,
+                                         const PairTarget &tp, bool act,
+                                         unsigned long long mact, float dx, float dy,
+                                         float dz, float hm, float rh, float ih, float ih2,
+                                         float zi2, float &ax, float &ay, float &az, float &pot,
+                                         unsigned &nin_p, unsigned &nin_j) {
 #pragma clang fp contract(fast)
-    const float drsqd = dx * dx + dy * dy + dz * dz;
-    const float drmag = sqrtf(drsqd) + 1.0e-20f;
-    const float invdrmag = __builtin_amdgcn_rcpf(drmag);
-    float grav, dpot;
-    unsigned inside = 0;
-    if (invdrmag > kg.invrange * invhp) {
-        inside = 1;
-        const int kern = min((int)(kg.half * drmag * invhp), KERNTOT);
-        const float4 w = t4[kern];
-        grav = invhp * invhp * (w.y + zo_p * w.x * invhp * invhp);
-        dpot = invhp * w.z;
-    } else {
-        grav = invdrmag * invdrmag;
-        dpot = invdrmag;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float ir = __builtin_amdgcn_rsqf(d2 + 1.0e-30f);
+    const float dr = d2 * ir;
+    const float ir2 = ir * ir;
+    const bool in_p = act && ir > tp.lim;
+    const bool in_j = act && dr < rh;
+    const unsigned long long mp = ballot(ir > tp.lim) & mact, mj = ballot(dr < rh) & mact;
+    nin_p += (unsigned)__popcll(mp);
+    nin_j += (unsigned)__popcll(mj);
+    float grav = ir2, dpot = ir;
+    // the kernel-softened parts only when a lane needs them (wave-uniform
+    // branches); unsigned table indices (at most KERNTOT) give 32-bit offsets
+    if (mp) {
+        const float4 wp = t4[(unsigned)min((int)(tp.hk * dr), KERNTOT)];
+        grav = in_p ? tp.ih2 * (wp.y + tp.zi2 * wp.x) : grav;
+        dpot = in_p ? tp.ih * wp.z : dpot;
     }
-    if (drmag < rangehj) {
-        inside |= 2;
-        const int kern = min((int)(kg.half * drmag * invhj), KERNTOT);
-        const float4 w = t4[kern];
-        grav = grav + invhj * invhj * (w.y + zj * w.x * invhj * invhj);
-        dpot = dpot + invhj * w.z;
-    } else {
-        grav = grav + invdrmag * invdrmag;
-        dpot = dpot + invdrmag;
+    float gj = ir2, pj = ir;
+    if (mj) {
+        const float4 wj = t4[(unsigned)min((int)(kg.half * dr * ih), KERNTOT)];
+        gj = in_j ? ih2 * (wj.y + zi2 * wj.x) : gj;
+        pj = in_j ? ih * wj.z : pj;
     }
-    const float f = 0.5f * mj * invdrmag * grav;
-    ax.add(f * dx);
-    ay.add(f * dy);
-    az.add(f * dz);
-    pot.add(0.5f * mj * dpot);
-    return inside;
+    grav = grav + gj;
+    dpot = dpot + pj;
+    const float f = act ? hm * ir * grav : 0.0f;
+    ax = ax + f * dx;
+    ay = ay + f * dy;
+    az = az + f * dz;
+    pot = pot + (act ? hm * dpot : 0.0f);
 }
 
 // FP32 multipole terms on the FP32 cell offset, added to an FP32 batch sum
@@ -549,7 +595,7 @@ __device__ __forceinline__ bool group_accepts(const GNodeF &nd, bool mine, doubl
     else
         accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
     accept = accept && nd.leaf != 1;                 // :128-136
-    return __ballot(mine && !accept) == 0;           // every lane accepts
+    return ballot(mine && !accept) == 0;             // every lane accepts
 }
 
 template <int OCT, int MACT>
@@ -628,7 +674,9 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
 
 template <int MACT, bool QUINTIC>
 __device__ __forceinline__ void pairs_group_mixed(
-    int g, int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    int g, int n, int nc, const GNodeF *__restrict__ gn, const float4 *__restrict__ rec,
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ h, const double *__restrict__ zeta,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
@@ -644,29 +692,47 @@ __device__ __forceinline__ void pairs_group_mixed(
     const bool valid = p < gr.y && active[p];
     if (__ballot(valid) == 0) return;
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
-    float invhp = 1.0f, zo_p = 0.0f;
+    PairTarget tp{1.0f, 1.0f, 0.0f, 0.0f, 0.0f};
     if (valid) {
-        const double4 a = rec[2 * (size_t)p], b = rec[2 * (size_t)p + 1];
-        rpx = a.x; rpy = a.y; rpz = a.z;
-        invhp = (float)b.y;
-        zo_p = (float)b.z;
+        rpx = x[p]; rpy = y[p]; rpz = z[p];
+        const double ih = 1.0 / h[p];
+        tp.ih = (float)ih;
+        tp.ih2 = (float)(ih * ih);
+        tp.zi2 = (float)(zeta[p] * ih * ih);
+        tp.lim = (float)(kg64.invrange * ih);
+        tp.hk = (float)(kg64.half * ih);
         if (MACT != SERENX_MAC_GEOMETRIC) afactor = mac_afactor<MACT>(agm[p], gpot[p]);
     }
-    CSum px, py, pz, pq;   // pair terms
+    CSum px, py, pz, pq;   // pair terms, one compensated addition per leaf
     unsigned npp = 0, nin_p = 0, nin_j = 0;
     auto leaf_pairs = [&](int first, int count, bool mine) {
+        // r_f - r_p in FP64, once per leaf (r_f: the leaf's first particle)
+        const float ox = (float)(x[first] - rpx), oy = (float)(y[first] - rpy),
+                    oz = (float)(z[first] - rpz);
+        float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f;
+        const unsigned long long mm = ballot(mine);
+        // the next source's record is loaded (scalar loads) before the
+        // current pair is formed; after the leaf's last particle that is
+        // particle first + count <= n, within the buffer's n + 1 records
+        float4 a = rec[2 * (size_t)first], b = rec[2 * (size_t)first + 1];
         for (int i = 0; i < count; i++) {
             const int j = first + i;
-            const double4 a = rec[2 * (size_t)j], b = rec[2 * (size_t)j + 1];
-            const bool act = mine && j != p;
-            unsigned in_ = 0;
-            if (act)
-                in_ = pair_fp32(kg, t4, invhp, zo_p, (float)(a.x - rpx), (float)(a.y - rpy),
-                                (float)(a.z - rpz), (float)a.w, (float)b.w, (float)b.y,
-                                (float)b.z, px, py, pz, pq);
-            npp += lanes(act);
-            nin_p += lanes(in_ & 1);
-            nin_j += lanes(in_ & 2);
+            const float4 an = rec[2 * (size_t)(j + 1)], bn = rec[2 * (size_t)(j + 1) + 1];
+            // the lanes taking the pair: those that opened the leaf, less the
+            // one whose particle is j
+            const unsigned sl = (unsigned)(j - gr.x);
+            const unsigned long long mact = mm & ~(sl < WAVE ? 1ull << sl : 0ull);
+            npp += (unsigned)__popcll(mact);
+            pair_f32(kg, t4, tp, mine && j != p, mact, a.x + ox, a.y + oy, a.z + oz, a.w, b.x,
+                     b.y, b.z, b.w, bx, by, bz, bp, nin_p, nin_j);
+            a = an;
+            b = bn;
+        }
+        if (mine) {
+            px.add(bx);
+            py.add(by);
+            pz.add(bz);
+            pq.add(bp);
         }
     };
     const int len = uniform(near.len[g]);
@@ -710,7 +776,9 @@ __device__ __forceinline__ void pairs_group_mixed(
 
 template <int MACT, bool QUINTIC>
 __global__ void __launch_bounds__(256) k_grav_pairs_mixed(
-    int n, int nc, const GNodeF *__restrict__ gn, const double4 *__restrict__ rec,
+    int n, int nc, const GNodeF *__restrict__ gn, const float4 *__restrict__ rec,
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ h, const double *__restrict__ zeta,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
     const uint8_t *__restrict__ active, const float4 *__restrict__ t4,
     double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
@@ -725,9 +793,9 @@ __global__ void __launch_bounds__(256) k_grav_pairs_mixed(
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone))
-        pairs_group_mixed<MACT, QUINTIC>(g, n, nc, gn, rec, cid, root_range, active, t4, ax,
-                                         ay, az, cpot, gpot, agm, near, icount, grp, ngrp, npp,
-                                         nin_p, nin_j);
+        pairs_group_mixed<MACT, QUINTIC>(g, n, nc, gn, rec, x, y, z, h, zeta, cid, root_range,
+                                         active, t4, ax, ay, az, cpot, gpot, agm, near, icount,
+                                         grp, ngrp, npp, nin_p, nin_j);
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
         atomicAdd(ic + 1, (unsigned long long)npp);
@@ -813,36 +881,44 @@ __global__ void __launch_bounds__(256) k_direct(KGeom kg, int n, int nt,
     }
 }
 
-// Blocks of the walk launch: with the work queue, as many as the device
-// holds at once (each wave then takes group after group); without it, one
-// wave per group.
+// Blocks per CU of the walk launch: the knob's value (default: as many as
+// fit).  Fewer blocks leave wave slots to the SPH kernels that run beside
+// the walk; the device's vector ALUs are the bound both share, and the best
+// split depends on the workload (the bench's autotuner picks it per run).
+// A walk with nothing beside it (serenx_grav_forces) takes all that fit.
+int walk_cap(const Ctx &c) { return c.walk_alone ? 64 : c.walk_queue; }
+
+// Blocks of the walk launch: with the work queue, at most cap per CU and as
+// many as the device holds at once (each wave then takes group after
+// group); without it, one wave per group.
 template <class K>
-int walk_blocks(Ctx &c, K kernel) {
+int walk_blocks(Ctx &c, K kernel, int cap) {
     const int per_wave = (c.gq.ngroups + 3) / 4;
-    if (c.walk_queue <= 0) return std::max(per_wave, 1);
+    if (c.walk_queue == 0) return std::max(per_wave, 1);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TB, 0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
-    per_cu = std::min(per_cu, c.walk_queue);
+    per_cu = std::min(per_cu, cap);
     return std::max(1, std::min(per_wave, per_cu * c.ncu));
 }
 
 NearList near_list(Ctx &c) { return NearList{c.pl_leaf, c.pl_mask, c.pl_len}; }
 
-// phase 1 of the walk (traversal and cell terms), persistent over the queue
+// phase 1 of the walk (traversal and cell terms), persistent over the
+// queue, on stream s with at most cap blocks per CU
 template <int OCT, int MACT>
-void launch_cells(Ctx &c) {
+void launch_cells(Ctx &c, hipStream_t s, int cap) {
     GroupQueue q{c.gq.ctr, c.gq.order, c.gq.ngroups};
     if (c.opt.precision == SERENX_MIXED) {
         auto k = k_grav_cells_mixed<OCT, MACT>;
-        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k, cap)), dim3(TB), 0, s, c.n, c.nc, c.gnf,
                            c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
                            c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost,
                            c.grp);
     } else {
         auto k = k_grav_cells_fp64<OCT, MACT>;
-        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k, cap)), dim3(TB), 0, s, c.n, c.nc, c.gn,
                            c.x, c.y, c.z, c.cid, c.root_range, c.active, c.gpot, c.agm, c.gax,
                            c.gay, c.gaz, c.gpc, near_list(c), c.icount, c.wstat, q, c.gq.cost,
                            c.grp);
@@ -850,66 +926,117 @@ void launch_cells(Ctx &c) {
 }
 
 // phase 2 (pair terms), persistent over the same queue order (its second
-// set of counters)
+// set of counters), on stream s with at most cap blocks per CU
 template <int MACT, bool QUINTIC>
-void launch_pairs(Ctx &c, const double4 *rec) {
+void launch_pairs(Ctx &c, const double4 *rec, hipStream_t s, int cap) {
     GroupQueue q{c.gq.ctr + NXCD, c.gq.order, c.gq.ngroups};
     if (c.opt.precision == SERENX_MIXED) {
         auto k = k_grav_pairs_mixed<MACT, QUINTIC>;
-        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gnf,
-                           rec, c.cid, c.root_range, c.active,
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k, cap)), dim3(TB), 0, s, c.n, c.nc, c.gnf,
+                           reinterpret_cast<const float4 *>(rec), c.x, c.y, c.z, c.h, c.zeta,
+                           c.cid, c.root_range, c.active,
                            reinterpret_cast<const float4 *>(c.ktab + KT_G4F_OFF), c.gax, c.gay,
                            c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp, c.ngrp, q);
     } else {
         auto k = k_grav_pairs_fp64<MACT, QUINTIC>;
-        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k)), dim3(TB), 0, c.gstream, c.n, c.nc, c.gn,
+        hipLaunchKernelGGL(k, dim3(walk_blocks(c, k, cap)), dim3(TB), 0, s, c.n, c.nc, c.gn,
                            rec, c.cid, c.root_range, c.active,
                            reinterpret_cast<const double4 *>(c.ktab + KT_G4_OFF), c.gax, c.gay,
                            c.gaz, c.gpc, c.gpot, c.agm, near_list(c), c.icount, c.grp, c.ngrp, q);
     }
 }
 
+// sorts the queue order from the costs of the last walk, on the gravity
+// stream
+int sort_queue(Ctx &c) {
+    Ctx::WalkQueue &q = c.gq;
+    const int ng = q.ngroups;
+    hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,
+                       (int)(q.have_cost && c.walk_queue != 0), q.keys, q.vals);
+    size_t tb = q.tmp_bytes;
+    SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order, (unsigned)ng,
+                                     0, 35, c.gstream));
+    return SERENX_OK;
+}
+
 // the queue order of this walk (from the previous walk's costs) and zeroed
-// counters, on the gravity stream
+// counters, on the gravity stream.  The order is normally sorted at the end
+// of the previous walk (grav_pairs), where the sort runs beside the hydro
+// kernels; here it is sorted only when that order is missing or stale.
 int prepare_queue(Ctx &c) {
     Ctx::WalkQueue &q = c.gq;
     const int ng = c.ngrp;
-    if (ng != q.ngroups) q.have_cost = false;   // another group layout
+    if (ng != q.ngroups) {   // another group layout
+        q.have_cost = false;
+        q.order_ready = false;
+    }
     q.ngroups = ng;
     if (c.capturing) {
         // a walk recorded into a graph takes its groups in tree order: the
         // sort library clears its buffers with memset nodes, which graphs
         // here must not hold (they have given wrong results on replays)
         hipLaunchKernelGGL(k_queue_identity, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.order);
-    } else {
-        hipLaunchKernelGGL(k_queue_keys, dim3(nblk(ng)), dim3(TB), 0, c.gstream, ng, q.cost,
-                           (int)(q.have_cost && c.walk_queue > 0), q.keys, q.vals);
-        size_t tb = q.tmp_bytes;
-        SX_HIP(rocprim::radix_sort_pairs(q.tmp, tb, q.keys, q.keys2, q.vals, q.order,
-                                         (unsigned)ng, 0, 35, c.gstream));
+    } else if (!q.order_ready) {
+        SX_TRY(sort_queue(c));
     }
+    q.order_ready = false;   // used by this walk
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c.gstream, 2 * NXCD, q.ctr);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
 
+// the cell phase on the gravity stream: queue prepared, then the walk
 template <int OCT, int MACT>
 int launch_grav_cells(Ctx &c) {
     SX_TRY(prepare_queue(c));
-    launch_cells<OCT, MACT>(c);
+    launch_cells<OCT, MACT>(c, c.gstream, walk_cap(c));
     return SERENX_OK;
 }
 
+int cells_dispatch(Ctx &c) {
+    const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
+    switch (c.opt.mac) {
+    case SERENX_MAC_GEOMETRIC:
+        return oct ? launch_grav_cells<1, SERENX_MAC_GEOMETRIC>(c)
+                   : launch_grav_cells<0, SERENX_MAC_GEOMETRIC>(c);
+    case SERENX_MAC_GADGET:
+        return oct ? launch_grav_cells<1, SERENX_MAC_GADGET>(c)
+                   : launch_grav_cells<0, SERENX_MAC_GADGET>(c);
+    case SERENX_MAC_EIGEN:
+        return oct ? launch_grav_cells<1, SERENX_MAC_EIGEN>(c)
+                   : launch_grav_cells<0, SERENX_MAC_EIGEN>(c);
+    default:
+        return oct ? launch_grav_cells<1, SERENX_MAC_GADGET2>(c)
+                   : launch_grav_cells<0, SERENX_MAC_GADGET2>(c);
+    }
+}
+
+// the pair phase on the gravity stream: source records packed, then the
+// walk
 template <int MACT>
 int launch_grav_pairs(Ctx &c) {
     double4 *rec = reinterpret_cast<double4 *>(c.grec);
-    hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range, c.x,
-                       c.y, c.z, c.m, c.h, c.zeta, rec);
-    if (c.kg.range == walk_geom<true>().range)
-        launch_pairs<MACT, true>(c, rec);
+    if (c.opt.precision == SERENX_MIXED)
+        hipLaunchKernelGGL(k_pack_grav_mixed, dim3(nblk(c.nc)), dim3(TB), 0, c.gstream, c.nc,
+                           c.kg.range, c.gnf, c.x, c.y, c.z, c.m, c.h, c.zeta,
+                           reinterpret_cast<float4 *>(c.grec));
     else
-        launch_pairs<MACT, false>(c, rec);
+        hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range,
+                           c.x, c.y, c.z, c.m, c.h, c.zeta, rec);
+    if (c.kg.range == walk_geom<true>().range)
+        launch_pairs<MACT, true>(c, rec, c.gstream, walk_cap(c));
+    else
+        launch_pairs<MACT, false>(c, rec, c.gstream, walk_cap(c));
     return SERENX_OK;
+}
+
+int pairs_dispatch(Ctx &c) {
+    switch (c.opt.mac) {
+    case SERENX_MAC_GEOMETRIC: return launch_grav_pairs<SERENX_MAC_GEOMETRIC>(c);
+    case SERENX_MAC_GADGET: return launch_grav_pairs<SERENX_MAC_GADGET>(c);
+    case SERENX_MAC_EIGEN: return launch_grav_pairs<SERENX_MAC_EIGEN>(c);
+    default: return launch_grav_pairs<SERENX_MAC_GADGET2>(c);
+    }
 }
 
 }  // namespace
@@ -927,25 +1054,7 @@ int grav_launch(Ctx &c) {
     SX_HIP(hipStreamWaitEvent(c.gstream, c.evfork, 0));
     SX_HIP(hipEventRecord(c.evg[0], c.gstream));
     SX_TRY(zero_words(c.icount, 2 * CTR_WORDS, c.gstream));
-    const bool oct = c.opt.multipole != SERENX_QUADRUPOLE;
-    switch (c.opt.mac) {
-    case SERENX_MAC_GEOMETRIC:
-        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GEOMETRIC>(c)
-                    : launch_grav_cells<0, SERENX_MAC_GEOMETRIC>(c)));
-        break;
-    case SERENX_MAC_GADGET:
-        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GADGET>(c)
-                    : launch_grav_cells<0, SERENX_MAC_GADGET>(c)));
-        break;
-    case SERENX_MAC_EIGEN:
-        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_EIGEN>(c)
-                    : launch_grav_cells<0, SERENX_MAC_EIGEN>(c)));
-        break;
-    default:
-        SX_TRY((oct ? launch_grav_cells<1, SERENX_MAC_GADGET2>(c)
-                    : launch_grav_cells<0, SERENX_MAC_GADGET2>(c)));
-        break;
-    }
+    SX_TRY(cells_dispatch(c));
     SX_HIP(hipGetLastError());
     SX_HIP(hipEventRecord(c.evg[1], c.gstream));
     c.grav_timed = false;
@@ -956,21 +1065,25 @@ int grav_pairs(Ctx &c) {
     SX_HIP(hipEventRecord(c.evfork2, c.stream));
     SX_HIP(hipStreamWaitEvent(c.gstream, c.evfork2, 0));
     SX_HIP(hipEventRecord(c.evg[2], c.gstream));
-    switch (c.opt.mac) {
-    case SERENX_MAC_GEOMETRIC: SX_TRY(launch_grav_pairs<SERENX_MAC_GEOMETRIC>(c)); break;
-    case SERENX_MAC_GADGET: SX_TRY(launch_grav_pairs<SERENX_MAC_GADGET>(c)); break;
-    case SERENX_MAC_EIGEN: SX_TRY(launch_grav_pairs<SERENX_MAC_EIGEN>(c)); break;
-    default: SX_TRY(launch_grav_pairs<SERENX_MAC_GADGET2>(c)); break;
-    }
+    SX_TRY(pairs_dispatch(c));
     SX_HIP(hipGetLastError());
     SX_HIP(hipEventRecord(c.evg[3], c.gstream));
     c.gq.have_cost = true;
     c.grav_timed = true;
+    // the next walk's order, from this walk's costs: it runs beside the
+    // main stream's hydro kernels, and the join waits for it, so that no
+    // work of the walk outlives the evaluation
+    if (!c.capturing && c.gq.ngroups == c.ngrp) {
+        SX_TRY(sort_queue(c));
+        SX_HIP(hipGetLastError());
+        c.gq.order_ready = true;
+    }
+    SX_HIP(hipEventRecord(c.evjoin, c.gstream));
     return SERENX_OK;
 }
 
 int grav_join(Ctx &c) {
-    SX_HIP(hipStreamWaitEvent(c.stream, c.evg[3], 0));
+    SX_HIP(hipStreamWaitEvent(c.stream, c.evjoin, 0));
     hipLaunchKernelGGL(k_add_grav, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.active, c.gax,
                        c.gay, c.gaz, c.ax, c.ay, c.az);
     SX_HIP(hipGetLastError());

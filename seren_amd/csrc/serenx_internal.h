@@ -175,7 +175,7 @@ struct Ctx {
     bool box_ok = false;             // HNode boxes hold the current positions (stocked
                                      // since the positions last changed)
     hipEvent_t ev[8] = {};
-    hipEvent_t evjoin = nullptr;     // the gravity stream's cell phase joined (graph part A)
+    hipEvent_t evjoin = nullptr;     // the gravity stream's work done (cell phase in graph part A)
     double last_ms[4] = {0, 0, 0, 0};
 
     // Captured restock evaluations (SERENX_EVAL_GRAPH): part A (stock, h
@@ -288,7 +288,7 @@ struct Ctx {
     // work queue of the persistent gravity walk (walk.h: GroupQueue)
     struct WalkQueue {
         int ngroups = 0;
-        int *ctr = nullptr;          // NXCD counters
+        int *ctr = nullptr;          // 2 NXCD counters (one set per walk phase)
         int *cost = nullptr;         // per group: cells its last walk visited
         int *order = nullptr;        // group ids by range, then decreasing cost
         uint64_t *keys = nullptr, *keys2 = nullptr;
@@ -296,8 +296,10 @@ struct Ctx {
         void *tmp = nullptr;
         size_t tmp_bytes = 0;
         bool have_cost = false;      // cost holds a walk of the current group layout
+        bool order_ready = false;    // order was sorted, after the last walk, for the next
     } gq;
     int walk_queue = 64;             // SERENX_TUNE_WALK_QUEUE: blocks per CU (0: no queue)
+    bool walk_alone = false;         // no SPH kernels run beside the walk (gravity only)
     int ncu = 1;                     // compute units of the device
 
     // interaction counters of the gravity walk

@@ -573,6 +573,9 @@ int step_once(Ctx &c) {
         guard.armed = true;
         SX_TRY(grav_launch(c));   // cell phase beside the h solve
         SX_TRY(run_sph_h(c));
+        // hmax before the pair phase: the persistent pair kernel fills the
+        // device, and the small per-level hmax launches would wait behind it
+        SX_TRY(update_hmax(c));
         SX_TRY(grav_pairs(c));    // pair phase beside the lists, thermal and hydro
         SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));

@@ -810,6 +810,7 @@ int set_clouds(Ctx &c, int ncloud, const int *counts) {
     SX_HIP(hipMemcpy(c.grp, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice));
     c.ngrp = (int)groups.size();
     c.gq.have_cost = false;   // another group layout
+    c.gq.order_ready = false;
     hipLaunchKernelGGL(k_fill_cid, dim3(64, ncloud), dim3(TB), 0, c.stream, ncloud, c.cloud_off,
                        c.cid);
     SX_HIP(hipGetLastError());

@@ -222,7 +222,14 @@ __device__ __forceinline__ void wave_stats(unsigned long long *stats, int k0, un
     }
 }
 
+// mask of the lanes of the wave for which pred holds, from the condition's
+// own lane mask (HIP's __ballot takes an int, which costs a select and a
+// compare per call)
+__device__ __forceinline__ unsigned long long ballot(bool pred) {
+    return __builtin_amdgcn_ballot_w64(pred);
+}
+
 // number of lanes of the wave for which pred holds (wave-uniform)
-__device__ __forceinline__ unsigned lanes(bool pred) { return (unsigned)__popcll(__ballot(pred)); }
+__device__ __forceinline__ unsigned lanes(bool pred) { return (unsigned)__popcll(ballot(pred)); }
 
 }  // namespace serenx

@@ -22,12 +22,14 @@ KEYS = ("porig", "pos", "v", "h", "zeta", "rho", "omega", "div_v", "a", "gpot", 
         "pptot")
 
 
-def _run(serial, precision, thermal, pos, vel, mass):
-    from seren_amd.engine import ForceEngine
+def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None):
+    from seren_amd.engine import TUNE_WALK_QUEUE, ForceEngine
     eng = ForceEngine(multipole="octupole", mac="gadget2", precision=precision,
                       thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS, thermal=thermal,
                       serial=serial)
     try:
+        if walk_queue is not None:
+            eng.set_tuning(TUNE_WALK_QUEUE, walk_queue)
         n = len(mass)
         eng.load(pos, vel, mass, np.zeros(n))
         if thermal == "energy":
@@ -59,3 +61,21 @@ def test_overlapped_walk_matches_serial(precision, thermal):
         if ne:
             print(f"  {k}: {ne} differ", flush=True)
         assert ne == 0, k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_walk_occupancy_and_queue_change_nothing(precision):
+    """The walk's blocks per CU (all that fit by default, 1, 3) and its queue
+    (off: one wave per group in tree order) only change which wave takes
+    which group; the queue order sorted at the end of one walk for the next
+    is used from the second evaluation on."""
+    print(f"case walk queue settings {precision}", flush=True)
+    pos, vel, mass, h = cloud(30000, "r2", 42)
+    ref = _run(False, precision, "isothermal", pos, vel, mass)
+    for wq in (0, 1, 3):
+        b = _run(False, precision, "isothermal", pos, vel, mass, walk_queue=wq)
+        for k in ref:
+            ne = int(np.sum(np.asarray(ref[k]) != np.asarray(b[k])))
+            if ne:
+                print(f"  walk queue {wq} {k}: {ne} differ", flush=True)
+            assert ne == 0, (wq, k)
