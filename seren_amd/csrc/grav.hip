@@ -581,21 +581,22 @@ constexpr int CELL_BATCH = 8;
 // term is issued for a cell that only some lanes use.  The FP64 walk keeps
 // the reference's per-particle interaction sets.  As the FP64 walk, it runs
 // in two phases: cells (beside the h solve), then pairs.
+//
+// A group lies in one cloud (its walk covers that cloud's DFS range), so
+// the lanes taking part are the same at every cell: vmask, the valid lanes.
+// The test forms each comparison's lane mask directly and combines the
+// masks in scalar registers.
 template <int MACT>
-__device__ __forceinline__ bool group_accepts(const GNodeF &nd, bool mine, double rpx, double rpy,
-                                              double rpz, double afactor, double &d0, double &d1,
-                                              double &d2) {
+__device__ __forceinline__ bool group_accepts(const GNodeF &nd, unsigned long long vmask,
+                                              double rpx, double rpy, double rpz, double afactor,
+                                              double &d0, double &d1, double &d2) {
     d0 = rpx - nd.r[0];
     d1 = rpy - nd.r[1];
     d2 = rpz - nd.r[2];
     const double drsqd = d0 * d0 + d1 * d1 + d2 * d2;
-    bool accept;
-    if (MACT == SERENX_MAC_GEOMETRIC)
-        accept = drsqd > nd.dminsqd;
-    else
-        accept = drsqd > nd.mac * afactor && drsqd > nd.dminsqd;
-    accept = accept && nd.leaf != 1;                 // :128-136
-    return ballot(mine && !accept) == 0;             // every lane accepts
+    unsigned long long ok = ballot(drsqd > nd.dminsqd);
+    if (MACT != SERENX_MAC_GEOMETRIC) ok &= ballot(drsqd > nd.mac * afactor);
+    return nd.leaf != 1 && (vmask & ~ok) == 0;       // :128-136; every lane accepts
 }
 
 template <int OCT, int MACT>
@@ -624,24 +625,23 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
         float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f; // current batch of cell terms
         int nbatch = 0, len = 0;                          // wave-uniform
         unsigned gv = 0;
-        // the walk covers the lanes' trees (one DFS range, or a few adjacent
-        // ones when the group straddles clouds); a lane takes part only inside
-        // its own
+        // the walk covers the tree of the group's cloud (groups never straddle
+        // clouds); every valid lane takes part at every cell, and the cell
+        // terms of the other lanes are formed and never stored
+        const unsigned long long vmask = ballot(valid);
+        const unsigned nvalid = (unsigned)__popcll(vmask);
         const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
         int c = uniform(wave_min_int(valid ? rr.x : nc));
         const int cend = uniform(wave_max_int(valid ? rr.y : 0));
         while (c < cend) {
             const GNodeF nd = gn[c];
             gv++;
-            const bool mine = valid && c >= rr.x && c < rr.y;
-            lvis += lanes(mine);
+            lvis += nvalid;
             double d0, d1, d2;
-            if (group_accepts<MACT>(nd, mine, rpx, rpy, rpz, afactor, d0, d1, d2)) {
-                ncell += lanes(mine);
-                if (mine) {
-                    const float dr[3] = {(float)d0, (float)d1, (float)d2};
-                    cell_fp32<OCT>(nd, dr, bx, by, bz, bp);
-                }
+            if (group_accepts<MACT>(nd, vmask, rpx, rpy, rpz, afactor, d0, d1, d2)) {
+                ncell += nvalid;
+                const float dr[3] = {(float)d0, (float)d1, (float)d2};
+                cell_fp32<OCT>(nd, dr, bx, by, bz, bp);
                 if (++nbatch == CELL_BATCH) {
                     cx += (double)bx; cy += (double)by; cz += (double)bz; cp += (double)bp;
                     bx = by = bz = bp = 0.0f;
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
                 }
                 c = nd.next;
             } else if (nd.leaf > 0) {
-                near_append(near, g, len, nd.first, nd.leaf, __ballot(mine));
+                near_append(near, g, len, nd.first, nd.leaf, vmask);
                 c = nd.next;
             } else {
                 c = c + 1;
@@ -747,14 +747,14 @@ __device__ __forceinline__ void pairs_group_mixed(
         const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
         int c = uniform(wave_min_int(valid ? rr.x : nc));
         const int cend = uniform(wave_max_int(valid ? rr.y : 0));
+        const unsigned long long vmask = ballot(valid);
         while (c < cend) {
             const GNodeF nd = gn[c];
-            const bool mine = valid && c >= rr.x && c < rr.y;
             double d0, d1, d2;
-            if (group_accepts<MACT>(nd, mine, rpx, rpy, rpz, afactor, d0, d1, d2)) {
+            if (group_accepts<MACT>(nd, vmask, rpx, rpy, rpz, afactor, d0, d1, d2)) {
                 c = nd.next;
             } else if (nd.leaf > 0) {
-                leaf_pairs(nd.first, nd.leaf, mine);
+                leaf_pairs(nd.first, nd.leaf, valid);
                 c = nd.next;
             } else {
                 c = c + 1;
