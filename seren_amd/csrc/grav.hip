@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) k_grav_cells_fp64(
                 return nd.leaf > 0 ? WALK_LEAF : WALK_DESCEND;     // :242-258
             },
             [&](const GNode &nd, bool mine) {
-                near_append(near, g, len, nd.first, nd.leaf, __ballot(mine));
+                near_append(near, g, len, nd.first, nd.leaf, ballot(mine));
             });
         vis += gv;
         if ((threadIdx.x & 63) == 0) {
@@ -395,7 +395,7 @@ __device__ __forceinline__ void pairs_group_fp64(
     const int2 gr = grp[g];
     const int p = gr.x + lane;
     const bool valid = p < gr.y && active[p];
-    if (__ballot(valid) == 0) return;
+    if (ballot(valid) == 0) return;
     double rpx = 0, rpy = 0, rpz = 0, invhp = 1.0, zo_p = 0.0;
     double gx = 0, gy = 0, gz = 0, gp = 0, afactor = 0.0;
     if (valid) {
@@ -690,7 +690,7 @@ __device__ __forceinline__ void pairs_group_mixed(
     const int2 gr = grp[g];
     const int p = gr.x + lane;
     const bool valid = p < gr.y && active[p];
-    if (__ballot(valid) == 0) return;
+    if (ballot(valid) == 0) return;
     double rpx = 0, rpy = 0, rpz = 0, afactor = 0.0;
     PairTarget tp{1.0f, 1.0f, 0.0f, 0.0f, 0.0f};
     if (valid) {

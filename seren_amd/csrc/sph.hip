@@ -101,7 +101,7 @@ __device__ __forceinline__ void list_loop(const int *__restrict__ wl, int nslot,
     bool have = sum && advance();
     decltype(load(0)) next{};
     if (have) next = load(jcur);
-    while (__any(have)) {
+    while (any_lane(have)) {
         const auto cur = next;
         const int jc = jcur;
         const bool hc = have;
@@ -167,7 +167,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES > 0 ? WAVES : 8)
     int nslot = 0, cnt = 0, slot_max = 0;
     bool have_list = false, wave_ovf = false, my_ovf = false;
 
-    while (__any(iterating)) {
+    while (any_lane(iterating)) {
         passes++;
         bool newlist = false;
         if (iterating && hp > h_high) {
@@ -194,7 +194,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES > 0 ? WAVES : 8)
             som = som + mj * hfactor_p * invhp * w.y;
         };
 
-        if (!have_list || __any(newlist)) {
+        if (!have_list || any_lane(newlist)) {
             // record every iterating lane's list (unchanged radii give the same
             // lists); converged lanes keep theirs
             if (iterating) {
@@ -220,7 +220,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES > 0 ? WAVES : 8)
                 my_ovf = lovf;
             }
             have_list = true;
-            wave_ovf = __any(lovf);
+            wave_ovf = any_lane(lovf);
             walks++;
         }
         // too few potential neighbours: grow h and gather again (:185-189)
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) k_hsolve_final(
               [&](int j, const double4 &a) { pair(a.x, a.y, a.z, a.w, j); });
     unsigned vis = 0, lvis = 0;
     const bool walk = valid && nslot < 0;
-    if (__any(walk)) {
+    if (any_lane(walk)) {
         // lists that did not fit: gather over the tree walk at the same radius
         const int2 rr = valid ? root_range[cid[p]] : make_int2(nc, nc);
         vis = group_walk(
@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
         rpx = a.x; rpy = a.y; rpz = a.z;
         hp = a.w;
     }
-    if (__ballot(valid) == 0) return;
+    if (ballot(valid) == 0) return;
     const double hrangesqd_p = kg.rangesqd * hp * hp;
     const double R = kg.range * hp;
     int tot = 0;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
             pass = box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
         }
         lvis += lanes(pass);
-        if (__ballot(pass) == 0) {
+        if (ballot(pass) == 0) {
             c = nd.next;
             continue;
         }
@@ -784,14 +784,14 @@ __global__ void __launch_bounds__(256) k_hgather(
     };
 
     // bracketing (h_gather.F90:57-175)
-    while (__any(iterating)) {
+    while (any_lane(iterating)) {
         passes++;
         bool newlist = false;
         if (iterating && h_high > h_search) {
             h_search = HMULT * h_high;
             newlist = true;
         }
-        if (!have_list || __any(newlist)) {
+        if (!have_list || any_lane(newlist)) {
             if (iterating) {
                 nslot = 0;
                 cnt = 0;
@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(256) k_hgather(
             });
             if (iterating) my_ovf = lovf;
             have_list = true;
-            wave_ovf = __any(my_ovf);   // lanes that stopped keep their lists
+            wave_ovf = any_lane(my_ovf);   // lanes that stopped keep their lists
         }
         // too few potential neighbours: grow and walk again (:93-99)
         const bool grow = iterating && newlist && cnt <= pp_gather;
@@ -862,7 +862,7 @@ __global__ void __launch_bounds__(256) k_hgather(
     // that r^2, as in a lattice shell -- and the pp_gather-th is lo2 itself
     bool tie = false;
     int nb = 0;
-    while (__any(narrowing)) {
+    while (any_lane(narrowing)) {
         const double mid = 0.5 * (lo2 + hi2);
         const bool split = mid > lo2 && mid < hi2;
         int c = 0;

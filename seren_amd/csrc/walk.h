@@ -16,6 +16,16 @@ __device__ __forceinline__ int wave_max_int(int v) { return __ockl_wfred_max_i32
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// whether pred holds on any lane of the wave
+__device__ __forceinline__ bool any_lane(bool pred) { return __builtin_amdgcn_ballot_w64(pred) != 0; }
+
+// mask of the lanes of the wave for which pred holds, from the condition's
+// own lane mask (HIP's __ballot takes an int, which costs a select and a
+// compare per call)
+__device__ __forceinline__ unsigned long long ballot(bool pred) {
+    return __builtin_amdgcn_ballot_w64(pred);
+}
+
 // The dispatcher deals the workgroups of a launch to the 8 XCDs round-robin
 // (workgroup i runs on XCD i % 8), and each XCD has its own L2.  This returns
 // the logical block index under which XCD x receives the x-th contiguous
@@ -152,23 +162,23 @@ __device__ __forceinline__ unsigned group_walk(const Node *__restrict__ nodes, i
         visits++;
         const bool here = resume == c;
         int act = visit(nd, here);
-        lane_visits += (unsigned)__popcll(__ballot(here));
+        lane_visits += (unsigned)__popcll(ballot(here));
         if (here) {
             resume = (act == WALK_DESCEND) ? c + 1 : nd.next;
             if (resume >= end) resume = nc;   // past the end of the lane's own tree
         } else {
             act = WALK_SKIP;
         }
-        if (__any(act == WALK_LEAF)) leaf(nd, act == WALK_LEAF);
+        if (ballot(act == WALK_LEAF)) leaf(nd, act == WALK_LEAF);
         // the next cell is the lowest resume pointer.  A lane descending makes
         // it c + 1.  Otherwise no lane's pointer lies inside c's subtree
         // (c, next): a lane reaches such a cell only through c itself, since
         // subtrees are contiguous and nest, so a lane that was at c and moved
         // on to c's next (not past its tree's end) holds the minimum; only
         // when none did is the wave minimum reduced.
-        if (__any(act == WALK_DESCEND))
+        if (ballot(act == WALK_DESCEND))
             c = c + 1;
-        else if (__any(here && resume == nd.next))
+        else if (ballot(here) & ballot(resume == nd.next))
             c = nd.next;
         else
             c = uniform(wave_min_int(resume));
@@ -220,13 +230,6 @@ __device__ __forceinline__ void wave_stats(unsigned long long *stats, int k0, un
         atomicAdd(row + k0, (unsigned long long)visits);
         atomicAdd(row + k0 + 1, (unsigned long long)lane_visits);
     }
-}
-
-// mask of the lanes of the wave for which pred holds, from the condition's
-// own lane mask (HIP's __ballot takes an int, which costs a select and a
-// compare per call)
-__device__ __forceinline__ unsigned long long ballot(bool pred) {
-    return __builtin_amdgcn_ballot_w64(pred);
 }
 
 // number of lanes of the wave for which pred holds (wave-uniform)
