@@ -700,6 +700,127 @@ __global__ void __launch_bounds__(256) k_hydro(
     if (ENERGY) dudt[p] = (0.0 + dudt_diss) - press[p] * divv[p] / omega[p] / rho_p;
 }
 
+// Mixed-precision hydro record, 64 bytes per particle: the position in FP64
+// (the offsets are formed in FP64, then rounded), everything else in FP32,
+// with the per-particle factors of k_pack_hydro formed in FP64 first.
+struct __attribute__((aligned(64))) HydroRecF {
+    double x, y, z;
+    float h, vx, vy, vz, m, rho, pf, sound, invh, hfac;
+};
+static_assert(sizeof(HydroRecF) == 64, "HydroRecF layout");
+
+__global__ void k_pack_hydro_mixed(int n, const double *__restrict__ x,
+                                   const double *__restrict__ y, const double *__restrict__ z,
+                                   const double *__restrict__ h, const double *__restrict__ vx,
+                                   const double *__restrict__ vy, const double *__restrict__ vz,
+                                   const double *__restrict__ m, const double *__restrict__ rho,
+                                   const double *__restrict__ omega,
+                                   const double *__restrict__ press,
+                                   const double *__restrict__ sound,
+                                   HydroRecF *__restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hi = h[i], rh = rho[i];
+    const double invh = 1.0 / hi;
+    HydroRecF r;
+    r.x = x[i];
+    r.y = y[i];
+    r.z = z[i];
+    r.h = (float)hi;
+    r.vx = (float)vx[i];
+    r.vy = (float)vy[i];
+    r.vz = (float)vz[i];
+    r.m = (float)m[i];
+    r.rho = (float)rh;
+    r.pf = (float)(press[i] / rh / rh / omega[i]);
+    r.sound = (float)sound[i];
+    r.invh = (float)invh;
+    r.hfac = (float)cr_pow4(invh);
+    rec[i] = r;
+}
+
+// hydro_gradh.F90:87-297 as k_hydro computes it, in FP32 on the
+// FP64-formed offsets (mixed precision): half the record bytes of the FP64
+// kernel, whose neighbour gathers miss L2 and bound it.  |r| and 1/|r| come
+// from one reciprocal square root, the divisions from reciprocals; the
+// accelerations are summed in FP32 over the particle's neighbours (tens of
+// terms) and stored in FP64.  ENERGY: du/dt as k_hydro, the P div v term in
+// FP64.
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) k_hydro_mixed(
+    KGeom kg, int n, const HydroRecF *__restrict__ rec, const uint8_t *__restrict__ active,
+    const int *__restrict__ pptot, const int *__restrict__ pplist, const int *__restrict__ ovf_off,
+    const int *__restrict__ ovf_list, int pp_limit, const float *__restrict__ w1, float alpha,
+    double *__restrict__ ax, double *__restrict__ ay, double *__restrict__ az,
+    const double *__restrict__ press, const double *__restrict__ divv,
+    const double *__restrict__ omega, const double *__restrict__ rho,
+    double *__restrict__ dudt) {
+#pragma clang fp contract(fast)
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const HydroRecF s = rec[p];
+    const double rpx = s.x, rpy = s.y, rpz = s.z;
+    const float hp = s.h, vpx = s.vx, vpy = s.vy, vpz = s.vz;
+    const float rho_p = s.rho, pfactor_p = s.pf, sound_p = s.sound, invhp = s.invh,
+                hfactor_p = s.hfac;
+    const float rangesqd = (float)kg.rangesqd, range = (float)kg.range, half = (float)kg.half;
+    const float beta = 2.0f * alpha;
+    const float lim_p = rangesqd * hp * hp, rng_p = range * hp;
+    float ahx = 0.0f, ahy = 0.0f, ahz = 0.0f, adx = 0.0f, ady = 0.0f, adz = 0.0f;
+    float dudt_diss = 0.0f;
+    const int nn = pptot[p];
+    const int *lst = (nn <= pp_limit) ? pplist + (size_t)p * pp_limit : ovf_list + ovf_off[p];
+    for (int i = 0; i < nn; i++) {
+        const HydroRecF q = rec[lst[i]];
+        const float hpp = q.h, mpp = q.m;
+        float ux = (float)(q.x - rpx), uy = (float)(q.y - rpy), uz = (float)(q.z - rpz);
+        const float dr2 = ux * ux + uy * uy + uz * uz;
+        if (dr2 >= lim_p && dr2 >= rangesqd * hpp * hpp) continue;
+        const float ir = __builtin_amdgcn_rsqf(dr2 + 1.0e-30f);
+        const float drmag = dr2 * ir;
+        const float dvx = q.vx - vpx, dvy = q.vy - vpy, dvz = q.vz - vpz;
+        const float dvdr = dvx * ux + dvy * uy + dvz * uz;
+        ux = ux * ir;
+        uy = uy * ir;
+        uz = uz * ir;
+        float wmean = 0.0f;
+        if (drmag < rng_p) {
+            const float wk = w1[min((int)(half * drmag * invhp), KERNTOT)];
+            wmean = 0.5f * hfactor_p * wk;
+            const float f = mpp * pfactor_p * hfactor_p * wk;
+            ahx = ahx + f * ux;
+            ahy = ahy + f * uy;
+            ahz = ahz + f * uz;
+        }
+        if (drmag < range * hpp) {
+            const float wk = w1[min((int)(half * drmag * q.invh), KERNTOT)];
+            wmean = wmean + 0.5f * q.hfac * wk;
+            const float f = mpp * q.pf * q.hfac * wk;
+            ahx = ahx + f * ux;
+            ahy = ahy + f * uy;
+            ahz = ahz + f * uz;
+        }
+        if (dvdr < 0.0f) {
+            const float hs = hp + hpp;
+            const float vsignal = 0.5f * (sound_p + q.sound);
+            const float mu = 0.5f * hs * dvdr *
+                             __builtin_amdgcn_rcpf(drmag * drmag + 0.25f * (float)ETA_SQD * hs * hs);
+            const float visc = (-alpha * vsignal * mu + beta * mu * mu) *
+                               __builtin_amdgcn_rcpf(0.5f * (rho_p + q.rho));
+            const float f = mpp * visc * wmean;
+            adx = adx + f * ux;
+            ady = ady + f * uy;
+            adz = adz + f * uz;
+            if (ENERGY) dudt_diss = dudt_diss + 0.5f * mpp * visc * wmean * dvdr * ir;
+        }
+    }
+    ax[p] = (double)(ahx + adx);
+    ay[p] = (double)(ahy + ady);
+    az[p] = (double)(ahz + adz);
+    if (ENERGY)
+        dudt[p] = (0.0 + (double)dudt_diss) - press[p] * divv[p] / omega[p] / rho[p];
+}
+
 // thermal.F90:108-113 (INTERNAL_ENERGY, ideal gas): P = (gamma - 1) rho u,
 // c = sqrt(P / rho), for every particle: update_thermal_properties.F90:76-81
 // loops over all particles once any is active (its active-list loop is
@@ -1253,6 +1374,26 @@ int hydro_forces(Ctx &c) {
             hipLaunchKernelGGL(k_hydro_std<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg,
                                c.n, rec, c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
                                c.pp_limit, c.ktab, c.opt.alpha, c.ax, c.ay, c.az, c.dudt);
+        SX_HIP(hipGetLastError());
+        return SERENX_OK;
+    }
+    if (c.opt.precision == SERENX_MIXED) {
+        // mixed precision: FP32 pair terms on 64-byte records
+        HydroRecF *rf = reinterpret_cast<HydroRecF *>(c.hrec);
+        const float *w1f = reinterpret_cast<const float *>(c.ktab + KT_F32_OFF) + T_W1 * KT;
+        hipLaunchKernelGGL(k_pack_hydro_mixed, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.n, c.x,
+                           c.y, c.z, c.h, c.vx, c.vy, c.vz, c.m, c.rho, c.omega, c.press, c.sound,
+                           rf);
+        if (c.opt.thermal == SERENX_THERMAL_ENERGY_EQN)
+            hipLaunchKernelGGL(k_hydro_mixed<true>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg,
+                               c.n, rf, c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
+                               c.pp_limit, w1f, (float)c.opt.alpha, c.ax, c.ay, c.az, c.press,
+                               c.divv, c.omega, c.rho, c.dudt);
+        else
+            hipLaunchKernelGGL(k_hydro_mixed<false>, dim3(nblk(c.n)), dim3(TB), 0, c.stream, c.kg,
+                               c.n, rf, c.active, c.pptot, c.pplist, c.ovf_off, c.ovf_list,
+                               c.pp_limit, w1f, (float)c.opt.alpha, c.ax, c.ay, c.az, c.press,
+                               c.divv, c.omega, c.rho, c.dudt);
         SX_HIP(hipGetLastError());
         return SERENX_OK;
     }
