@@ -884,8 +884,9 @@ __global__ void __launch_bounds__(256) k_direct(KGeom kg, int n, int nt,
 // Blocks per CU of the walk launch: the knob's value (default: as many as
 // fit).  Fewer blocks leave wave slots to the SPH kernels that run beside
 // the walk; the device's vector ALUs are the bound both share, and the best
-// split depends on the workload (the bench's autotuner picks it per run).
-// A walk with nothing beside it (serenx_grav_forces) takes all that fit.
+// split depends on the workload (seren_amd/autotune.py times a few
+// candidates on it).  A walk with nothing beside it (serenx_grav_forces)
+// takes all that fit.
 int walk_cap(const Ctx &c) { return c.walk_alone ? 64 : c.walk_queue; }
 
 // Blocks of the walk launch: with the work queue, at most cap per CU and as
