@@ -672,9 +672,13 @@ __global__ void __launch_bounds__(256) k_grav_cells_mixed(
     wave_stats(wstat, 0, vis, lvis);
 }
 
+// leaves of a near list staged in LDS at once by the mixed pair phase
+constexpr int PL_STAGE = 32;
+
 template <int MACT, bool QUINTIC>
 __device__ __forceinline__ void pairs_group_mixed(
     int g, int n, int nc, const GNodeF *__restrict__ gn, const float4 *__restrict__ rec,
+    float4 *__restrict__ stage,
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const double *__restrict__ h, const double *__restrict__ zeta,
     const int *__restrict__ cid, const int2 *__restrict__ root_range,
@@ -735,12 +739,62 @@ __device__ __forceinline__ void pairs_group_mixed(
             pq.add(bp);
         }
     };
+    // the same terms with the leaf's records read from the wave's LDS stage
+    // (src: the leaf's LEAFMAX slots of two float4)
+    auto leaf_pairs_staged = [&](const float4 *src, int first, int count, bool mine) {
+        const float ox = (float)(x[first] - rpx), oy = (float)(y[first] - rpy),
+                    oz = (float)(z[first] - rpz);
+        float bx = 0.0f, by = 0.0f, bz = 0.0f, bp = 0.0f;
+        const unsigned long long mm = ballot(mine);
+        for (int i = 0; i < count; i++) {
+            const int j = first + i;
+            const float4 a = src[2 * i], b = src[2 * i + 1];
+            const unsigned sl = (unsigned)(j - gr.x);
+            const unsigned long long mact = mm & ~(sl < WAVE ? 1ull << sl : 0ull);
+            npp += (unsigned)__popcll(mact);
+            pair_f32(kg, t4, tp, mine && j != p, mact, a.x + ox, a.y + oy, a.z + oz, a.w, b.x,
+                     b.y, b.z, b.w, bx, by, bz, bp, nin_p, nin_j);
+        }
+        if (mine) {
+            px.add(bx);
+            py.add(by);
+            pz.add(bz);
+            pq.add(bp);
+        }
+    };
     const int len = uniform(near.len[g]);
     if (len <= PL_CAP) {
-        for (int e = 0; e < len; e++) {
-            const int d = uniform(near.leaf[(size_t)g * PL_CAP + e]);
-            const unsigned long long m = near.mask[(size_t)g * PL_CAP + e];
-            leaf_pairs(d >> 4, d & 15, valid && ((m >> lane) & 1ull));
+        // the near list in batches of PL_STAGE leaves: lane l copies the
+        // records of the batch's leaf l into the wave's LDS stage (all the
+        // batch's loads in flight at once), then the wave forms the pairs
+        // leaf by leaf in list order from LDS
+        for (int e0 = 0; e0 < len; e0 += PL_STAGE) {
+            const int nb = min(PL_STAGE, len - e0);
+            int dl = 0;
+            if (lane < nb) {
+                dl = near.leaf[(size_t)g * PL_CAP + e0 + lane];
+                const int f = dl >> 4, k = dl & 15;
+                float4 *dst = stage + 2 * LEAFMAX * lane;
+                for (int i = 0; i < k; i++) {
+                    dst[2 * i] = rec[2 * (size_t)(f + i)];
+                    dst[2 * i + 1] = rec[2 * (size_t)(f + i) + 1];
+                }
+            }
+            // the wave's LDS writes before its reads (lanes read other
+            // lanes' slots)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int e = 0; e < nb; e++) {
+                const int d = __builtin_amdgcn_readlane(dl, e);
+                const unsigned long long m = near.mask[(size_t)g * PL_CAP + e0 + e];
+                leaf_pairs_staged(stage + 2 * LEAFMAX * e, d >> 4, d & 15,
+                                  valid && ((m >> lane) & 1ull));
+            }
+            // every read of this batch before the next batch's writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     } else {
         // the list did not fit: the same wave-uniform walk again, pairs only
@@ -789,13 +843,16 @@ __global__ void __launch_bounds__(256) k_grav_pairs_mixed(
     // persistent over the walk's queue: the groups of each XCD range
     // longest-first (by the previous walk's cost), as phase 1 takes them
     __builtin_amdgcn_s_setprio(2);
+    // each wave's LDS stage: PL_STAGE leaves of LEAFMAX two-float4 records
+    __shared__ float4 stage_all[(256 / WAVE) * PL_STAGE * LEAFMAX * 2];
+    float4 *stage = stage_all + (threadIdx.x / WAVE) * (PL_STAGE * LEAFMAX * 2);
     unsigned npp = 0, nin_p = 0, nin_j = 0;   // wave totals
     const int home = blockIdx.x % NXCD;
     int qdone = 0;
     for (int g = next_group(queue, home, qdone); g >= 0; g = next_group(queue, home, qdone))
-        pairs_group_mixed<MACT, QUINTIC>(g, n, nc, gn, rec, x, y, z, h, zeta, cid, root_range,
-                                         active, t4, ax, ay, az, cpot, gpot, agm, near, icount,
-                                         grp, ngrp, npp, nin_p, nin_j);
+        pairs_group_mixed<MACT, QUINTIC>(g, n, nc, gn, rec, stage, x, y, z, h, zeta, cid,
+                                         root_range, active, t4, ax, ay, az, cpot, gpot, agm,
+                                         near, icount, grp, ngrp, npp, nin_p, nin_j);
     if ((threadIdx.x & 63) == 0) {
         unsigned long long *ic = ctr_row(icount);
         atomicAdd(ic + 1, (unsigned long long)npp);
