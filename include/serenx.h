@@ -323,13 +323,20 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *    5 waves), -1 (default: 8 in mixed precision, where the SPH kernels are
  *    the critical path, 0 in FP64, where the h solve runs beside the
  *    issue-bound FP64 walk).  Results do not change.
+ *  SERENX_TUNE_NEAR_CAP: near-field leaves the gravity walk's cell phase
+ *    records per group of 64 particles for the pair phase, 1..1024 (default
+ *    1024).  A group whose list does not fit walks the tree again in the
+ *    pair phase with the same MAC decisions (the same leaves in the same
+ *    order): results do not change, only speed; tests lower it to drive
+ *    that path.
  * serenx_get_tuning reads the current value. */
 enum {
     SERENX_TUNE_HSOLVE_SLOTS = 1,
     SERENX_TUNE_MIXED_ABSERROR = 2,
     SERENX_TUNE_WALK_QUEUE = 3,
     SERENX_TUNE_NEIB_GROUP = 4,
-    SERENX_TUNE_HSOLVE_WAVES = 5
+    SERENX_TUNE_HSOLVE_WAVES = 5,
+    SERENX_TUNE_NEAR_CAP = 6
 };
 int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value);
 int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value);

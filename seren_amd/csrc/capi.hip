@@ -1413,6 +1413,11 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
             return set_error(SERENX_EINVAL, "h-solve waves per SIMD must be -1, 0 or 8");
         c.hsolve_waves = (int)value;
         return SERENX_OK;
+    case SERENX_TUNE_NEAR_CAP:
+        if (!(value >= 1.0 && value <= (double)PL_CAP) || value != (double)(int)value)
+            return set_error(SERENX_EINVAL, "near-list capacity must be an integer in 1..1024");
+        c.near_cap = (int)value;
+        return SERENX_OK;
     default:
         return set_error(SERENX_EINVAL, "unknown tuning key");
     }
@@ -1429,6 +1434,7 @@ int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value) {
     case SERENX_TUNE_WALK_QUEUE: *value = c.walk_queue; return SERENX_OK;
     case SERENX_TUNE_NEIB_GROUP: *value = c.neib_group ? 1.0 : 0.0; return SERENX_OK;
     case SERENX_TUNE_HSOLVE_WAVES: *value = c.hsolve_waves; return SERENX_OK;
+    case SERENX_TUNE_NEAR_CAP: *value = c.near_cap; return SERENX_OK;
     default: return set_error(SERENX_EINVAL, "unknown tuning key");
     }
 }

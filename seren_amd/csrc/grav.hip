@@ -289,12 +289,13 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
 struct NearList {
     int *leaf;                    // ngroups x PL_CAP entries
     unsigned long long *mask;     // ngroups x PL_CAP lane masks
-    int *len;                     // per group: entries found (> PL_CAP: list incomplete)
+    int *len;                     // per group: entries found (> cap: list incomplete)
+    int cap;                      // entries a group may store (<= PL_CAP)
 };
 
 __device__ __forceinline__ void near_append(const NearList &L, int g, int &len, int first,
                                             int count, unsigned long long mask) {
-    if ((threadIdx.x & 63) == 0 && len < PL_CAP) {
+    if ((threadIdx.x & 63) == 0 && len < L.cap) {
         L.leaf[(size_t)g * PL_CAP + len] = (first << 4) | count;
         L.mask[(size_t)g * PL_CAP + len] = mask;
     }
@@ -422,7 +423,7 @@ __device__ __forceinline__ void pairs_group_fp64(
         }
     };
     const int len = uniform(near.len[g]);
-    if (len <= PL_CAP) {
+    if (len <= near.cap) {
         for (int e = 0; e < len; e++) {
             const int d = uniform(near.leaf[(size_t)g * PL_CAP + e]);
             const unsigned long long m = near.mask[(size_t)g * PL_CAP + e];
@@ -763,7 +764,7 @@ __device__ __forceinline__ void pairs_group_mixed(
         }
     };
     const int len = uniform(near.len[g]);
-    if (len <= PL_CAP) {
+    if (len <= near.cap) {
         // the near list in batches of PL_STAGE leaves: lane l copies the
         // records of the batch's leaf l into the wave's LDS stage (all the
         // batch's loads in flight at once), then the wave forms the pairs
@@ -961,7 +962,7 @@ int walk_blocks(Ctx &c, K kernel, int cap) {
     return std::max(1, std::min(per_wave, per_cu * c.ncu));
 }
 
-NearList near_list(Ctx &c) { return NearList{c.pl_leaf, c.pl_mask, c.pl_len}; }
+NearList near_list(Ctx &c) { return NearList{c.pl_leaf, c.pl_mask, c.pl_len, c.near_cap}; }
 
 // phase 1 of the walk (traversal and cell terms), persistent over the
 // queue, on stream s with at most cap blocks per CU

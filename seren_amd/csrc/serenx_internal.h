@@ -299,6 +299,7 @@ struct Ctx {
         bool order_ready = false;    // order was sorted, after the last walk, for the next
     } gq;
     int walk_queue = 64;             // SERENX_TUNE_WALK_QUEUE: blocks per CU (0: no queue)
+    int near_cap = PL_CAP;           // SERENX_TUNE_NEAR_CAP: near-list leaves a group may store
     bool walk_alone = false;         // no SPH kernels run beside the walk (gravity only)
     int ncu = 1;                     // compute units of the device
 

@@ -22,14 +22,17 @@ KEYS = ("porig", "pos", "v", "h", "zeta", "rho", "omega", "div_v", "a", "gpot", 
         "pptot")
 
 
-def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None):
-    from seren_amd.engine import TUNE_WALK_QUEUE, ForceEngine
+def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None, near_cap=None):
+    from seren_amd.engine import TUNE_NEAR_CAP, TUNE_WALK_QUEUE, ForceEngine
     eng = ForceEngine(multipole="octupole", mac="gadget2", precision=precision,
                       thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS, thermal=thermal,
                       serial=serial)
     try:
         if walk_queue is not None:
             eng.set_tuning(TUNE_WALK_QUEUE, walk_queue)
+        if near_cap is not None:
+            eng.set_tuning(TUNE_NEAR_CAP, near_cap)
+            assert eng.get_tuning(TUNE_NEAR_CAP) == near_cap
         n = len(mass)
         eng.load(pos, vel, mass, np.zeros(n))
         if thermal == "energy":
@@ -79,3 +82,20 @@ def test_walk_occupancy_and_queue_change_nothing(precision):
             if ne:
                 print(f"  walk queue {wq} {k}: {ne} differ", flush=True)
             assert ne == 0, (wq, k)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_near_list_fallback_changes_nothing(precision):
+    """Groups whose near-field list does not fit walk the tree again in the
+    pair phase with the same MAC decisions: with room for 4 leaves per group
+    (nearly every group falls back) the state is the default's bit for bit,
+    through evaluations and steps."""
+    print(f"case near-list fallback {precision}", flush=True)
+    pos, vel, mass, h = cloud(30000, "uniform", 43)
+    ref = _run(False, precision, "isothermal", pos, vel, mass)
+    b = _run(False, precision, "isothermal", pos, vel, mass, near_cap=4)
+    for k in ref:
+        ne = int(np.sum(np.asarray(ref[k]) != np.asarray(b[k])))
+        if ne:
+            print(f"  near cap 4 {k}: {ne} differ", flush=True)
+        assert ne == 0, k
