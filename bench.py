@@ -3,7 +3,8 @@
 Workload: N = 1,048,576 equal-mass particles, rho ~ r^-2 truncated at r = 1, with a
 divergence-free P(k) ~ k^-4 velocity field on a 128^3 grid at rms Mach 2
 (c_s = 0.2), octupole moments and the GADGET2 multipole MAC (abserror 1e-4),
-FP64 throughout.  One "step" (--mode step, the default) is one kick-drift-
+an FP64 leg and a mixed-precision leg (the headline value; FP64 value and
+ratio beside it), then a bounded serving leg.  One "step" (--mode step, the default) is one kick-drift-
 kick leapfrog step with SEREN's timestep criteria (timestep_size.F90,
 sph_timesteps.F90, one time-step level): two device-resident sph_integrate
 calls, the second holding the force evaluation of every particle -- tree
@@ -28,6 +29,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# The serving leg steps up to 7 pools at once, each on its own stream; HIP
+# maps a process's streams onto at most GPU_MAX_HW_QUEUES hardware queues (4
+# unless set), and pools that share a queue run one after the other (at 200
+# requests/s: p99 1.6-1.8 s with 4 queues, 0.5-0.7 s with 16).  Set before
+# the runtime initialises; a value the caller set is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 METRIC = "particles/s (tree grav+grad-h hydro), FP32-mixed vs FP64 @1e-4 rms; p99 lat"
 
@@ -74,6 +82,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="step/eval modes: requests of the bounded serving leg (0: none)")
     ap.add_argument("--precision-serve", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--serve-walk-stream", action="store_true",
+                    help="serving: each pool's gravity walk on a second stream beside its SPH "
+                         "kernels (default: one stream per pool)")
     ap.add_argument("--multipole", default="octupole")
     ap.add_argument("--mac", default="gadget2")
     ap.add_argument("--abserror", type=float, default=1e-4)
@@ -363,7 +374,8 @@ def serve_leg(args, nrequests, pool=4):
     profiles = ["uniform", "r2", "plummer"]
     srv = CloudServer(hbm_budget=200e9, precision=args.precision_serve, guard_every=4,
                       engine_kw=dict(multipole=args.multipole, mac=args.mac,
-                                     abserror=args.abserror, sound_speed=0.2))
+                                     abserror=args.abserror, sound_speed=0.2),
+                      serial=not args.serve_walk_stream)
     rng = np.random.default_rng(args.seed)
     ids, p_cloud, data = [], [], {}
     for si, (n, pr) in enumerate(zip(sizes, probs)):

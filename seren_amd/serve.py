@@ -131,7 +131,11 @@ class CloudServer:
                  pool_particles: int = 1_500_000, engine_kw: Optional[dict] = None,
                  step_kw: Optional[dict] = None, serial: bool = True, slots: int = 0):
         """slots is accepted for compatibility and ignored: concurrency is one
-        epoch worker per pool."""
+        epoch worker per pool, each pool on its own stream.  HIP maps a
+        process's streams onto at most GPU_MAX_HW_QUEUES hardware queues (4
+        unless the environment sets it before the runtime starts); pools
+        that share a queue run one after the other, so a serving process
+        should set it to at least the number of pools (bench.py sets 16)."""
         self.budget = float(hbm_budget)
         self.precision = precision
         self.guard_every = int(guard_every)
