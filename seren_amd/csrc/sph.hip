@@ -526,16 +526,21 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
         const HNode nd = hn[c];
         vis++;
         const bool own = valid && c >= rr.x && c < rr.y;
-        // BHhydro_walk.F90:36-43 for this lane, and the box test at the
-        // larger of the two radii
+        // With fresh boxes, the box test at the larger of the two radii
+        // alone: every neighbour j of p in the cell's subtree lies in the
+        // box and within max(KERNRANGE h_p, hrangemax) of p, so the cells it
+        // passes are a superset of those holding neighbours.  With stale
+        // boxes, BHhydro_walk.F90:36-43 for this lane.
         const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
-        const double drsqd = dx * dx + dy * dy + dz * dz;
-        const double m1 = (R + nd.rmax) * (R + nd.rmax);
-        const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
-        bool pass = own && (drsqd <= m1 || drsqd <= m2);
-        if (pass && box) {
+        bool pass;
+        if (box) {
             const double Rb = fmax(R, nd.hrangemax);
-            pass = box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
+            pass = own && box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
+        } else {
+            const double drsqd = dx * dx + dy * dy + dz * dz;
+            const double m1 = (R + nd.rmax) * (R + nd.rmax);
+            const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
+            pass = own && (drsqd <= m1 || drsqd <= m2);
         }
         lvis += lanes(pass);
         if (ballot(pass) == 0) {
