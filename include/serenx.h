@@ -324,8 +324,8 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *    the critical path, 0 in FP64, where the h solve runs beside the
  *    issue-bound FP64 walk).  Results do not change.
  *  SERENX_TUNE_NEAR_CAP: near-field leaves the gravity walk's cell phase
- *    records per group of 64 particles for the pair phase, 1..1024 (default
- *    1024).  A group whose list does not fit walks the tree again in the
+ *    records per group of 64 particles for the pair phase, 1..4096 (default
+ *    4096).  A group whose list does not fit walks the tree again in the
  *    pair phase with the same MAC decisions (the same leaves in the same
  *    order): results do not change, only speed; tests lower it to drive
  *    that path.

@@ -251,8 +251,9 @@ int ensure_capacity(Ctx &c, int n) {
     // per group of the gravity walk (at most one partial group per cloud)
     const size_t ng = walk_max_groups(n, c.pp_gather);
     SX_TRY(dalloc(&c.grp, ng));
-    SX_TRY(dalloc(&c.pl_leaf, ng * PL_CAP));
-    SX_TRY(dalloc(&c.pl_mask, ng * PL_CAP));
+    c.pl_entries = near_list_entries(n);
+    SX_TRY(dalloc(&c.pl_leaf, c.pl_entries));
+    SX_TRY(dalloc(&c.pl_mask, c.pl_entries));
     SX_TRY(dalloc(&c.pl_len, ng));
     SX_TRY(dalloc(&c.hlist, (((size_t)n + 255) / 256) * 4 * HL_SLOTS * WAVE));
     // work queue of the gravity walk: one entry per group
@@ -350,7 +351,8 @@ size_t ctx_device_bytes(const Ctx &c) {
     const size_t nn = (size_t)c.cap + 1, cc = (size_t)c.ccap;
     size_t b = 0;
     b += nn * (24 * 8 + 16 * 8 + 12 * 8);                   // SoA, tmpd, hydro/gravity records
-    b += walk_max_groups(c.cap, c.pp_gather) * ((size_t)PL_CAP * 12 + 4 + 8 + 6 * 8);  // walk groups
+    b += walk_max_groups(c.cap, c.pp_gather) * (size_t)(4 + 8 + 6 * 8);  // walk groups
+    b += near_list_entries(c.cap) * 12;                              // near-field lists
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
@@ -1415,7 +1417,7 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
         return SERENX_OK;
     case SERENX_TUNE_NEAR_CAP:
         if (!(value >= 1.0 && value <= (double)PL_CAP) || value != (double)(int)value)
-            return set_error(SERENX_EINVAL, "near-list capacity must be an integer in 1..1024");
+            return set_error(SERENX_EINVAL, "near-list capacity must be an integer in 1..4096");
         c.near_cap = (int)value;
         return SERENX_OK;
     default:

@@ -287,17 +287,18 @@ __device__ __forceinline__ void cell_fp64(const GNode &g, const double dr[3], do
 // (first particle << 4) | particle count, with the mask of the lanes that
 // opened each.  Phase 1 of the walk writes it, phase 2 reads it.
 struct NearList {
-    int *leaf;                    // ngroups x PL_CAP entries
-    unsigned long long *mask;     // ngroups x PL_CAP lane masks
+    int *leaf;                    // ngroups x stride entries
+    unsigned long long *mask;     // ngroups x stride lane masks
     int *len;                     // per group: entries found (> cap: list incomplete)
-    int cap;                      // entries a group may store (<= PL_CAP)
+    int cap;                      // entries a group may store (<= stride)
+    int stride;                   // entries per group in leaf[] and mask[]
 };
 
 __device__ __forceinline__ void near_append(const NearList &L, int g, int &len, int first,
                                             int count, unsigned long long mask) {
     if ((threadIdx.x & 63) == 0 && len < L.cap) {
-        L.leaf[(size_t)g * PL_CAP + len] = (first << 4) | count;
-        L.mask[(size_t)g * PL_CAP + len] = mask;
+        L.leaf[(size_t)g * L.stride + len] = (first << 4) | count;
+        L.mask[(size_t)g * L.stride + len] = mask;
     }
     len++;
 }
@@ -425,8 +426,8 @@ __device__ __forceinline__ void pairs_group_fp64(
     const int len = uniform(near.len[g]);
     if (len <= near.cap) {
         for (int e = 0; e < len; e++) {
-            const int d = uniform(near.leaf[(size_t)g * PL_CAP + e]);
-            const unsigned long long m = near.mask[(size_t)g * PL_CAP + e];
+            const int d = uniform(near.leaf[(size_t)g * near.stride + e]);
+            const unsigned long long m = near.mask[(size_t)g * near.stride + e];
             leaf_pairs(d >> 4, d & 15, valid && ((m >> lane) & 1ull));
         }
     } else {
@@ -773,7 +774,7 @@ __device__ __forceinline__ void pairs_group_mixed(
             const int nb = min(PL_STAGE, len - e0);
             int dl = 0;
             if (lane < nb) {
-                dl = near.leaf[(size_t)g * PL_CAP + e0 + lane];
+                dl = near.leaf[(size_t)g * near.stride + e0 + lane];
                 const int f = dl >> 4, k = dl & 15;
                 float4 *dst = stage + 2 * LEAFMAX * lane;
                 for (int i = 0; i < k; i++) {
@@ -788,7 +789,7 @@ __device__ __forceinline__ void pairs_group_mixed(
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (int e = 0; e < nb; e++) {
                 const int d = __builtin_amdgcn_readlane(dl, e);
-                const unsigned long long m = near.mask[(size_t)g * PL_CAP + e0 + e];
+                const unsigned long long m = near.mask[(size_t)g * near.stride + e0 + e];
                 leaf_pairs_staged(stage + 2 * LEAFMAX * e, d >> 4, d & 15,
                                   valid && ((m >> lane) & 1ull));
             }
@@ -962,7 +963,15 @@ int walk_blocks(Ctx &c, K kernel, int cap) {
     return std::max(1, std::min(per_wave, per_cu * c.ncu));
 }
 
-NearList near_list(Ctx &c) { return NearList{c.pl_leaf, c.pl_mask, c.pl_len, c.near_cap}; }
+// The lists share pl_entries slots: each of the ngrp groups gets an equal
+// share, at most PL_CAP.  One cloud has at most n / 64 + 1 groups and gets
+// PL_CAP entries per group; a forest of many small clouds, whose groups walk
+// small trees, gets fewer.
+NearList near_list(Ctx &c) {
+    const size_t share = c.pl_entries / (size_t)std::max(c.ngrp, 1);
+    const int stride = (int)std::min<size_t>(PL_CAP, share);
+    return NearList{c.pl_leaf, c.pl_mask, c.pl_len, std::min(c.near_cap, stride), stride};
+}
 
 // phase 1 of the walk (traversal and cell terms), persistent over the
 // queue, on stream s with at most cap blocks per CU

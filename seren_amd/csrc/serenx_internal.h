@@ -66,8 +66,10 @@ constexpr int HL_SLOTS = 96;
 // selection of the pp_gather-th neighbour (more are narrowed by bisection)
 constexpr int BAND_MAX = 64;
 // near-field list entries per group of the gravity walk (a longer list makes
-// the group repeat its walk in the pair phase)
-constexpr int PL_CAP = 1024;
+// the group repeat its walk in the pair phase).  Groups deep in a 1M uniform
+// sphere under the relative MAC at 1e-4 open more than 2048 leaves; 4096
+// entries (48 KB per group, 768 B per particle) hold them.
+constexpr int PL_CAP = 4096;
 
 // Counters that every wave of a launch updates (walk statistics, interaction
 // counts, the step's reductions) are spread over CTR_ROWS rows of CTR_SLOTS
@@ -212,10 +214,11 @@ struct Ctx {
     double *grec = nullptr;      // gravity source records, 8 doubles per particle
     double *gax = nullptr, *gay = nullptr, *gaz = nullptr;   // gravity acceleration
     double *gpc = nullptr;       // potential of the walk's cell terms (phase 1)
-    // near-field lists of the walk: per group of 64 particles, PL_CAP leaves
-    // (first << 4 | count) and the masks of the lanes that opened them
+    // near-field lists of the walk: per group of 64 particles, up to PL_CAP
+    // leaves (first << 4 | count) and the masks of the lanes that opened them
     int *pl_leaf = nullptr, *pl_len = nullptr;
     unsigned long long *pl_mask = nullptr;
+    size_t pl_entries = 0;       // slots of pl_leaf and pl_mask
     // groups of the gravity walk: [start, end) of up to 64 consecutive
     // particles of one cloud (each cloud's groups start at its first
     // particle, so a cloud is grouped the same alone and in a forest)
@@ -398,6 +401,9 @@ inline int forest_max_clouds(int n, int pp_gather) { return n / (pp_gather > 1 ?
 inline size_t walk_max_groups(int n, int pp_gather) {
     return ((size_t)n + WAVE - 1) / WAVE + (size_t)forest_max_clouds(n, pp_gather);
 }
+// near-list slots of a context for n particles: PL_CAP for each group of a
+// single cloud (at most n / 64 + 1 groups), shared by all groups otherwise
+inline size_t near_list_entries(int n) { return (((size_t)n + WAVE - 1) / WAVE + 1) * PL_CAP; }
 
 int diag_blocks(int n, int *per_block);
 int diagnostics(Ctx &c, double *partial, double *dout);

@@ -84,7 +84,7 @@ def test_bad_tuning_values_are_einval():
         for key, v in ((TUNE_HSOLVE_SLOTS, 0), (TUNE_HSOLVE_SLOTS, 97), (TUNE_HSOLVE_SLOTS, 2.5),
                        (TUNE_MIXED_ABSERROR, -1.0), (TUNE_HSOLVE_WAVES, 4),
                        (TUNE_WALK_QUEUE, -1), (TUNE_WALK_QUEUE, 65), (TUNE_NEAR_CAP, 0),
-                       (TUNE_NEAR_CAP, 1025), (99, 1.0)):
+                       (TUNE_NEAR_CAP, 4097), (99, 1.0)):
             with pytest.raises(SerenxError) as ei:
                 eng.set_tuning(key, v)
             assert ei.value.code == EINVAL
@@ -92,6 +92,6 @@ def test_bad_tuning_values_are_einval():
         assert eng.get_tuning(TUNE_HSOLVE_SLOTS) == 8
         assert eng.get_tuning(TUNE_MIXED_ABSERROR) == 1e-4
         assert eng.get_tuning(TUNE_WALK_QUEUE) == 64   # as many blocks as fit
-        assert eng.get_tuning(TUNE_NEAR_CAP) == 1024
+        assert eng.get_tuning(TUNE_NEAR_CAP) == 4096
     finally:
         eng.close()
