@@ -78,6 +78,9 @@ def parse():
     ap.add_argument("--walk-queue", type=int, default=-1,
                     help="gravity walk blocks per compute unit (-1: autotuned, or the library's "
                          "default with --no-tune-walk)")
+    ap.add_argument("--pair-blocks", type=int, default=-1,
+                    help="blocks per CU of the walk's pair phase (0: the walk's; -1: the library's "
+                         "default, 0)")
     ap.add_argument("--no-tune-walk", dest="tune_walk", action="store_false", default=True,
                     help="keep the library's walk blocks per CU (no timing of candidates)")
     ap.add_argument("--serve-requests", type=int, default=1000,
@@ -258,6 +261,10 @@ def step_leg(args, cloud, precision, tune=False):
             from seren_amd.engine import TUNE_WALK_QUEUE
             eng.set_tuning(TUNE_WALK_QUEUE, args.walk_queue)
             out["walk_queue"] = args.walk_queue
+        if args.pair_blocks >= 0:
+            from seren_amd.engine import TUNE_PAIR_BLOCKS
+            eng.set_tuning(TUNE_PAIR_BLOCKS, args.pair_blocks)
+            out["pair_blocks"] = args.pair_blocks
         eng.load(pos, vel, mass, h)
         eng.force_eval(hguess=True)                  # setup evaluation (sph_setup)
         eng.sync()

@@ -329,6 +329,14 @@ int serenx_serve_info(serenx_ctx *ctx, int32_t ncloud, double *dt, double *time)
  *    pair phase with the same MAC decisions (the same leaves in the same
  *    order): results do not change, only speed; tests lower it to drive
  *    that path.
+ *  SERENX_TUNE_NEIB_LEAVES: leaves each wave of the neighbour walk records
+ *    (in DFS order) for its particles with more than pp_limit neighbours,
+ *    0..512 (default 512).  Such a particle's complete row is formed from
+ *    that list by the whole wave; when the list did not fit (or at 0) the
+ *    particle walks the tree alone.  The rows are the same either way.
+ *  SERENX_TUNE_PAIR_BLOCKS: blocks per CU of the walk's pair phase, 1..64,
+ *    or 0 (default) for SERENX_TUNE_WALK_QUEUE's value; used when the work
+ *    queue is on.  Results do not change.
  * serenx_get_tuning reads the current value. */
 enum {
     SERENX_TUNE_HSOLVE_SLOTS = 1,
@@ -336,7 +344,9 @@ enum {
     SERENX_TUNE_WALK_QUEUE = 3,
     SERENX_TUNE_NEIB_GROUP = 4,
     SERENX_TUNE_HSOLVE_WAVES = 5,
-    SERENX_TUNE_NEAR_CAP = 6
+    SERENX_TUNE_NEAR_CAP = 6,
+    SERENX_TUNE_NEIB_LEAVES = 7,
+    SERENX_TUNE_PAIR_BLOCKS = 8
 };
 int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value);
 int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value);

@@ -242,6 +242,8 @@ int ensure_capacity(Ctx &c, int n) {
     SX_TRY(dalloc(&c.pplist, nn * (size_t)c.pp_limit));
     SX_TRY(dalloc(&c.ovf_off, nn));
     SX_TRY(dalloc(&c.ovf_idx, nn));
+    SX_TRY(dalloc(&c.nbl, particle_waves(n) * NB_CAP));
+    SX_TRY(dalloc(&c.nbl_len, particle_waves(n)));
     SX_TRY(dalloc(&c.hrec, nn * 16));
     SX_TRY(dalloc(&c.grec, nn * 8));
     SX_TRY(dalloc(&c.gax, nn));
@@ -285,7 +287,7 @@ void free_all(Ctx &c) {
                     c.press, c.sound, c.divv, c.drhodt, c.ax, c.ay, c.az, c.gpot, c.agm,
                     c.u, c.dudt,
                     c.tmpd, c.hrec, c.hlist, c.porig, c.tmpi, c.active, c.pptot, c.pplist, c.ovf_off,
-                    c.ovf_idx,
+                    c.ovf_idx, c.nbl, c.nbl_len,
                     c.ovf_list, c.ovf_count, c.khi, c.klo, c.khi2, c.klo2, c.idx, c.idx2,
                     c.cpl, c.pact, c.flag, c.incl, c.cellof, c.ncs, c.off, c.firstlev,
                     c.root, c.bbpart, c.cloud_off, c.cid, c.chunks, c.chunk_first,
@@ -353,6 +355,7 @@ size_t ctx_device_bytes(const Ctx &c) {
     b += nn * (24 * 8 + 16 * 8 + 12 * 8);                   // SoA, tmpd, hydro/gravity records
     b += walk_max_groups(c.cap, c.pp_gather) * (size_t)(4 + 8 + 6 * 8);  // walk groups
     b += near_list_entries(c.cap) * 12;                              // near-field lists
+    b += particle_waves(c.cap) * (NB_CAP + 1) * 4;                   // neighbour-walk leaves
     b += nn * (5 * 4 + 1) + nn * (size_t)c.pp_limit * 4;    // ints, active, neighbour rows
     b += (((size_t)c.cap + 255) / 256) * 4 * HL_SLOTS * WAVE * 4;   // h-solve lists
     b += nn * (4 * 8 + 2 * 4 + 2 + 5 * 4 + 1);              // tree build scratch
@@ -1420,6 +1423,16 @@ int serenx_set_tuning(serenx_ctx *ctx, int32_t key, double value) {
             return set_error(SERENX_EINVAL, "near-list capacity must be an integer in 1..4096");
         c.near_cap = (int)value;
         return SERENX_OK;
+    case SERENX_TUNE_PAIR_BLOCKS:
+        if (!(value >= 0.0 && value <= 64.0) || value != (double)(int)value)
+            return set_error(SERENX_EINVAL, "pair-phase blocks per CU must be an integer in 0..64");
+        c.pair_blocks = (int)value;
+        return SERENX_OK;
+    case SERENX_TUNE_NEIB_LEAVES:
+        if (!(value >= 0.0 && value <= (double)NB_CAP) || value != (double)(int)value)
+            return set_error(SERENX_EINVAL, "neighbour-walk leaf list length must be an integer in 0..512");
+        c.nb_cap = (int)value;
+        return SERENX_OK;
     default:
         return set_error(SERENX_EINVAL, "unknown tuning key");
     }
@@ -1437,6 +1450,8 @@ int serenx_get_tuning(const serenx_ctx *ctx, int32_t key, double *value) {
     case SERENX_TUNE_NEIB_GROUP: *value = c.neib_group ? 1.0 : 0.0; return SERENX_OK;
     case SERENX_TUNE_HSOLVE_WAVES: *value = c.hsolve_waves; return SERENX_OK;
     case SERENX_TUNE_NEAR_CAP: *value = c.near_cap; return SERENX_OK;
+    case SERENX_TUNE_NEIB_LEAVES: *value = c.nb_cap; return SERENX_OK;
+    case SERENX_TUNE_PAIR_BLOCKS: *value = c.pair_blocks; return SERENX_OK;
     default: return set_error(SERENX_EINVAL, "unknown tuning key");
     }
 }

@@ -948,6 +948,14 @@ __global__ void __launch_bounds__(256) k_direct(KGeom kg, int n, int nt,
 // takes all that fit.
 int walk_cap(const Ctx &c) { return c.walk_alone ? 64 : c.walk_queue; }
 
+// Blocks per CU of the pair phase: its own knob when set.  The pair phase
+// runs beside the neighbour and hydro kernels, which need less of the
+// device than the h solve beside the cell phase.
+int pair_cap(const Ctx &c) {
+    if (c.walk_alone) return 64;
+    return c.pair_blocks > 0 ? c.pair_blocks : c.walk_queue;
+}
+
 // Blocks of the walk launch: with the work queue, at most cap per CU and as
 // many as the device holds at once (each wave then takes group after
 // group); without it, one wave per group.
@@ -1092,9 +1100,9 @@ int launch_grav_pairs(Ctx &c) {
         hipLaunchKernelGGL(k_pack_grav, dim3(nblk(c.n)), dim3(TB), 0, c.gstream, c.n, c.kg.range,
                            c.x, c.y, c.z, c.m, c.h, c.zeta, rec);
     if (c.kg.range == walk_geom<true>().range)
-        launch_pairs<MACT, true>(c, rec, c.gstream, walk_cap(c));
+        launch_pairs<MACT, true>(c, rec, c.gstream, pair_cap(c));
     else
-        launch_pairs<MACT, false>(c, rec, c.gstream, walk_cap(c));
+        launch_pairs<MACT, false>(c, rec, c.gstream, pair_cap(c));
     return SERENX_OK;
 }
 

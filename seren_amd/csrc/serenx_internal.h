@@ -70,6 +70,9 @@ constexpr int BAND_MAX = 64;
 // sphere under the relative MAC at 1e-4 open more than 2048 leaves; 4096
 // entries (48 KB per group, 768 B per particle) hold them.
 constexpr int PL_CAP = 4096;
+// leaves a wave of the neighbour walk records for the rows of its particles
+// above pp_limit (a longer list makes such a particle walk the tree alone)
+constexpr int NB_CAP = 512;
 
 // Counters that every wave of a launch updates (walk statistics, interaction
 // counts, the step's reductions) are spread over CTR_ROWS rows of CTR_SLOTS
@@ -283,6 +286,9 @@ struct Ctx {
     size_t ovf_cap = 0;
     int *ovf_count = nullptr;     // device scalar: number of overflow particles
     int *ovf_idx = nullptr;       // the overflow particles, in tree order
+    // leaves each wave of the neighbour walk opened (NB_CAP per wave of 64
+    // particles, in DFS order) and their number, for the overflow rows
+    int *nbl = nullptr, *nbl_len = nullptr;
 
     // kernel tables [NTAB][KT] (kernel.F90)
     double *ktab = nullptr;
@@ -302,7 +308,9 @@ struct Ctx {
         bool order_ready = false;    // order was sorted, after the last walk, for the next
     } gq;
     int walk_queue = 64;             // SERENX_TUNE_WALK_QUEUE: blocks per CU (0: no queue)
+    int pair_blocks = 0;             // SERENX_TUNE_PAIR_BLOCKS: pair phase's (0: walk_queue's)
     int near_cap = PL_CAP;           // SERENX_TUNE_NEAR_CAP: near-list leaves a group may store
+    int nb_cap = NB_CAP;             // SERENX_TUNE_NEIB_LEAVES: leaves a neighbour-walk wave records
     bool walk_alone = false;         // no SPH kernels run beside the walk (gravity only)
     int ncu = 1;                     // compute units of the device
 
@@ -403,6 +411,8 @@ inline size_t walk_max_groups(int n, int pp_gather) {
 }
 // near-list slots of a context for n particles: PL_CAP for each group of a
 // single cloud (at most n / 64 + 1 groups), shared by all groups otherwise
+// waves of a one-thread-per-particle launch over n particles
+inline size_t particle_waves(int n) { return (((size_t)n + 255) / 256) * 4; }
 inline size_t near_list_entries(int n) { return (((size_t)n + WAVE - 1) / WAVE + 1) * PL_CAP; }
 
 int diag_blocks(int n, int *per_block);

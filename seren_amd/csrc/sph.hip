@@ -408,7 +408,8 @@ __global__ void __launch_bounds__(256) k_neib(KGeom kg, bool box, int n, const i
                                               const int *__restrict__ ovf_off,
                                               int *__restrict__ ovf_list,
                                               int *__restrict__ ovf_count,
-                                              unsigned long long *__restrict__ wstat) {
+                                              unsigned long long *__restrict__ wstat,
+                                              int *__restrict__, int *__restrict__, int) {
     // the overflow pass runs over the list of overflow particles (n of them),
     // OVF_PER_WAVE to a wave: their walks are long, and a wave walks the union
     // of its lanes' walks, so a few lanes per wave keep many waves in flight
@@ -470,6 +471,86 @@ __global__ void __launch_bounds__(256) k_neib(KGeom kg, bool box, int n, const i
     if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
 }
 
+// The cell test of the group neighbour walk for one lane (own: the cell is in
+// the lane's cloud).  With fresh boxes, the box test at the larger of the two
+// radii alone: every neighbour j of p in the cell's subtree lies in the box
+// and within max(KERNRANGE h_p, hrangemax) of p, so the cells it passes are
+// a superset of those holding neighbours.  With stale boxes,
+// BHhydro_walk.F90:36-43 for this lane.
+__device__ __forceinline__ bool neib_pass(const HNode &nd, bool box, bool own, double R,
+                                          double rpx, double rpy, double rpz) {
+    const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
+    if (box) {
+        const double Rb = fmax(R, nd.hrangemax);
+        return own && box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
+    }
+    const double drsqd = dx * dx + dy * dy + dz * dz;
+    const double m1 = (R + nd.rmax) * (R + nd.rmax);
+    const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
+    return own && (drsqd <= m1 || drsqd <= m2);
+}
+
+// The complete row of overflow particle p from the leaves its first-pass
+// wave (particles 64 g .. 64 g + 63) opened, in DFS order: lane l takes
+// leaf e0 + l, applies p's own cell test and the pair test to its
+// particles, and the lanes' matches are placed by a prefix sum of their
+// counts, so the row holds the candidates in leaf order and, within a leaf,
+// in particle order -- the order of p's own walk.  The recorded leaves
+// include every leaf p's own test passes (the test of a leaf implies the
+// test of each ancestor, so the wave reached it through p's own path), and
+// only those pass here: the row is the one the walk would write.
+__device__ void neib_ovf_scan(KGeom kg, bool box, int p, int g, int nl,
+                              const int *__restrict__ nbl, const HNode *__restrict__ hn,
+                              const double4 *__restrict__ rec, const int *__restrict__ cid,
+                              const int2 *__restrict__ root_range, const int *__restrict__ pptot,
+                              const int *__restrict__ ovf_off, int *__restrict__ ovf_list) {
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    const double4 a = rec[p];
+    const double rpx = a.x, rpy = a.y, rpz = a.z, hp = a.w;
+    const double hrangesqd_p = kg.rangesqd * hp * hp;
+    const double R = kg.range * hp;
+    const int2 rr = root_range[cid[p]];
+    int *row = ovf_list + ovf_off[p];
+    const int lim = pptot[p];
+    int base = 0;
+    for (int e0 = 0; e0 < nl; e0 += WAVE) {
+        const int e = e0 + lane;
+        unsigned hit = 0;   // bit i: particle i of the lane's leaf is a neighbour
+        int first = 0, cnt = 0;
+        if (e < nl) {
+            const int c = nbl[(size_t)g * NB_CAP + e];
+            const HNode nd = hn[c];
+            if (nd.leaf > 0 && neib_pass(nd, box, c >= rr.x && c < rr.y, R, rpx, rpy, rpz)) {
+                first = nd.first;
+                for (int i = 0; i < nd.leaf; i++) {
+                    const int j = first + i;
+                    if (j == p) continue;
+                    const double4 aj = rec[j];
+                    const double hrangesqd_pp = kg.rangesqd * aj.w * aj.w;
+                    const double qx = aj.x - rpx, qy = aj.y - rpy, qz = aj.z - rpz;
+                    const double dsq = qx * qx + qy * qy + qz * qz;
+                    if (dsq <= hrangesqd_p || dsq <= hrangesqd_pp) {
+                        hit |= 1u << i;
+                        cnt++;
+                    }
+                }
+            }
+        }
+        int incl = cnt;   // inclusive prefix sum over the lanes
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int u = __shfl_up(incl, d, WAVE);
+            if (lane >= d) incl += u;
+        }
+        int o = base + incl - cnt;
+        for (int i = 0; hit != 0; i++, hit >>= 1)
+            if (hit & 1u) {
+                if (o < lim) row[o] = first + i;
+                o++;
+            }
+        base += __shfl(incl, WAVE - 1, WAVE);
+    }
+}
+
 // get_neib.F90:46-118 with one walk per wave instead of one per particle.
 // Every particle j with r^2 <= KERNRANGESQD h_p^2 or r^2 <= KERNRANGESQD
 // h_j^2 lies in a leaf that the reference's BHhydro_walk opens for p: every
@@ -496,8 +577,24 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
                                                     const int *__restrict__ ovf_off,
                                                     int *__restrict__ ovf_list,
                                                     int *__restrict__ ovf_count,
-                                                    unsigned long long *__restrict__ wstat) {
+                                                    unsigned long long *__restrict__ wstat,
+                                                    int *__restrict__ nbl,
+                                                    int *__restrict__ nbl_len, int nb_cap) {
     const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int wv = t / WAVE;
+    if (OVERFLOW_PASS) {
+        // one overflow particle per wave: from the leaf list of its
+        // first-pass wave when that list is complete, else by its own walk
+        if (wv >= n) return;
+        const int po = uniform(ovf_idx[wv]);
+        const int g = po / WAVE;
+        const int nl = uniform(nbl_len[g]);
+        if (nl <= nb_cap) {
+            neib_ovf_scan(kg, box, po, g, nl, nbl, hn, rec, cid, root_range, pptot, ovf_off,
+                          ovf_list);
+            return;
+        }
+    }
     const int ot = (t / WAVE) * OVF_PER_WAVE + (t % WAVE);
     const bool ovf_lane = (t % WAVE) < OVF_PER_WAVE && ot < n;
     const int p = OVERFLOW_PASS ? (ovf_lane ? ovf_idx[ot] : 0) : t;
@@ -522,26 +619,11 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
     int c = uniform(wave_min_int(rr.x));
     const int cend = uniform(wave_max_int(valid ? rr.y : 0));
     unsigned vis = 0, lvis = 0;
+    int nl = 0;   // leaves opened (wave-uniform), recorded for the overflow pass
     while (c < cend) {
         const HNode nd = hn[c];
         vis++;
-        const bool own = valid && c >= rr.x && c < rr.y;
-        // With fresh boxes, the box test at the larger of the two radii
-        // alone: every neighbour j of p in the cell's subtree lies in the
-        // box and within max(KERNRANGE h_p, hrangemax) of p, so the cells it
-        // passes are a superset of those holding neighbours.  With stale
-        // boxes, BHhydro_walk.F90:36-43 for this lane.
-        const double dx = nd.r[0] - rpx, dy = nd.r[1] - rpy, dz = nd.r[2] - rpz;
-        bool pass;
-        if (box) {
-            const double Rb = fmax(R, nd.hrangemax);
-            pass = own && box_dist2(nd, dx, dy, dz) <= Rb * Rb * (1.0 + 1.0e-12);
-        } else {
-            const double drsqd = dx * dx + dy * dy + dz * dz;
-            const double m1 = (R + nd.rmax) * (R + nd.rmax);
-            const double m2 = (nd.hrangemax + nd.rmax) * (nd.hrangemax + nd.rmax);
-            pass = own && (drsqd <= m1 || drsqd <= m2);
-        }
+        const bool pass = neib_pass(nd, box, valid && c >= rr.x && c < rr.y, R, rpx, rpy, rpz);
         lvis += lanes(pass);
         if (ballot(pass) == 0) {
             c = nd.next;
@@ -550,6 +632,10 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
         if (nd.leaf == 0) {
             c = c + 1;
             continue;
+        }
+        if (!OVERFLOW_PASS) {
+            if ((threadIdx.x & (WAVE - 1)) == 0 && nl < nb_cap) nbl[(size_t)wv * NB_CAP + nl] = c;
+            nl++;
         }
         const int k = nd.leaf, f0 = nd.first;
         for (int i = 0; i < k; i++) {
@@ -570,7 +656,10 @@ __global__ void __launch_bounds__(256) k_neib_group(KGeom kg, bool box, int n,
         pptot[p] = tot;
         if (tot > pp_limit) atomicAdd(ovf_count, 1);
     }
-    if (!OVERFLOW_PASS) wave_stats(wstat, 8, vis, lvis);
+    if (!OVERFLOW_PASS) {
+        if ((threadIdx.x & (WAVE - 1)) == 0) nbl_len[wv] = nl;
+        wave_stats(wstat, 8, vis, lvis);
+    }
 }
 
 __global__ void k_ovf_sizes(int n, int pp_limit, const int *__restrict__ pptot,
@@ -1293,7 +1382,7 @@ int sph_neib_rows(Ctx &c) {
     hipLaunchKernelGGL(c.neib_group ? k_neib_group<false> : k_neib<false>, dim3(nblk(c.n)),
                        dim3(TB), 0, s, c.kg, c.box_ok, c.n, c.ovf_idx, c.nc, c.hn,
                        rec, c.cid, c.root_range, c.active, c.pp_limit, c.pptot,
-                       c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
+                       c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat, c.nbl, c.nbl_len, c.nb_cap);
     SX_HIP(hipGetLastError());
     SX_HIP(hipMemcpyAsync(c.hostpin + 9, c.ovf_count, sizeof(int), hipMemcpyDeviceToHost, s));
     return SERENX_OK;
@@ -1343,7 +1432,8 @@ int sph_neib_finish(Ctx &c) {
         const long ovf_threads = (long)((nov + OVF_PER_WAVE - 1) / OVF_PER_WAVE) * WAVE;
         hipLaunchKernelGGL(c.neib_group ? k_neib_group<true> : k_neib<true>, dim3(nblk(ovf_threads)), dim3(TB), 0, s, c.kg, c.box_ok, nov, c.ovf_idx, c.nc,
                            c.hn, rec, c.cid, c.root_range, c.active, c.pp_limit,
-                           c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat);
+                           c.pptot, c.pplist, c.ovf_off, c.ovf_list, c.ovf_count, c.wstat,
+                           c.nbl, c.nbl_len, c.nb_cap);
         SX_HIP(hipGetLastError());
     }
     return SERENX_OK;

@@ -80,6 +80,8 @@ EINVAL, EHIP, ELMAX, ESTATE, ENOMEM, EHNEG, EFEW = 1, 2, 3, 4, 5, 6, 7
 TUNE_HSOLVE_SLOTS, TUNE_MIXED_ABSERROR, TUNE_WALK_QUEUE, TUNE_NEIB_GROUP = 1, 2, 3, 4
 TUNE_HSOLVE_WAVES = 5
 TUNE_NEAR_CAP = 6
+TUNE_NEIB_LEAVES = 7
+TUNE_PAIR_BLOCKS = 8
 
 
 class Options(C.Structure):

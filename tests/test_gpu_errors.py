@@ -77,14 +77,16 @@ def test_gradh_with_fewer_particles_than_pp_gather_is_efew():
 
 def test_bad_tuning_values_are_einval():
     from seren_amd.engine import (EINVAL, TUNE_HSOLVE_SLOTS, TUNE_HSOLVE_WAVES,
-                                  TUNE_MIXED_ABSERROR, TUNE_NEAR_CAP, TUNE_WALK_QUEUE,
+                                  TUNE_MIXED_ABSERROR, TUNE_NEAR_CAP, TUNE_NEIB_LEAVES,
+                                  TUNE_PAIR_BLOCKS, TUNE_WALK_QUEUE,
                                   SerenxError)
     eng = _engine()
     try:
         for key, v in ((TUNE_HSOLVE_SLOTS, 0), (TUNE_HSOLVE_SLOTS, 97), (TUNE_HSOLVE_SLOTS, 2.5),
                        (TUNE_MIXED_ABSERROR, -1.0), (TUNE_HSOLVE_WAVES, 4),
                        (TUNE_WALK_QUEUE, -1), (TUNE_WALK_QUEUE, 65), (TUNE_NEAR_CAP, 0),
-                       (TUNE_NEAR_CAP, 4097), (99, 1.0)):
+                       (TUNE_NEAR_CAP, 4097), (TUNE_NEIB_LEAVES, -1), (TUNE_NEIB_LEAVES, 513),
+                       (TUNE_PAIR_BLOCKS, -1), (TUNE_PAIR_BLOCKS, 65), (99, 1.0)):
             with pytest.raises(SerenxError) as ei:
                 eng.set_tuning(key, v)
             assert ei.value.code == EINVAL
@@ -93,5 +95,7 @@ def test_bad_tuning_values_are_einval():
         assert eng.get_tuning(TUNE_MIXED_ABSERROR) == 1e-4
         assert eng.get_tuning(TUNE_WALK_QUEUE) == 64   # as many blocks as fit
         assert eng.get_tuning(TUNE_NEAR_CAP) == 4096
+        assert eng.get_tuning(TUNE_NEIB_LEAVES) == 512
+        assert eng.get_tuning(TUNE_PAIR_BLOCKS) == 0
     finally:
         eng.close()

@@ -22,17 +22,24 @@ KEYS = ("porig", "pos", "v", "h", "zeta", "rho", "omega", "div_v", "a", "gpot", 
         "pptot")
 
 
-def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None, near_cap=None):
-    from seren_amd.engine import TUNE_NEAR_CAP, TUNE_WALK_QUEUE, ForceEngine
+def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None, near_cap=None,
+         neib_leaves=None, pair_blocks=None):
+    from seren_amd.engine import (TUNE_NEAR_CAP, TUNE_NEIB_LEAVES, TUNE_PAIR_BLOCKS,
+                                  TUNE_WALK_QUEUE, ForceEngine)
     eng = ForceEngine(multipole="octupole", mac="gadget2", precision=precision,
                       thetamaxsqd=0.25, abserror=1e-4, sound_speed=CS, thermal=thermal,
                       serial=serial)
     try:
         if walk_queue is not None:
             eng.set_tuning(TUNE_WALK_QUEUE, walk_queue)
+        if pair_blocks is not None:
+            eng.set_tuning(TUNE_PAIR_BLOCKS, pair_blocks)
         if near_cap is not None:
             eng.set_tuning(TUNE_NEAR_CAP, near_cap)
             assert eng.get_tuning(TUNE_NEAR_CAP) == near_cap
+        if neib_leaves is not None:
+            eng.set_tuning(TUNE_NEIB_LEAVES, neib_leaves)
+            assert eng.get_tuning(TUNE_NEIB_LEAVES) == neib_leaves
         n = len(mass)
         eng.load(pos, vel, mass, np.zeros(n))
         if thermal == "energy":
@@ -45,6 +52,7 @@ def _run(serial, precision, thermal, pos, vel, mass, walk_queue=None, near_cap=N
         st = eng.fetch()
         out = {"eval_" + k: ev[k] for k in KEYS}
         out.update({"step_" + k: st[k] for k in KEYS})
+        out["pp_limit"] = eng.pp_limit
         if thermal == "energy":
             out["u"], out["du_dt"] = eng.fetch_energy()
         return out
@@ -68,20 +76,21 @@ def test_overlapped_walk_matches_serial(precision, thermal):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_walk_occupancy_and_queue_change_nothing(precision):
-    """The walk's blocks per CU (all that fit by default, 1, 3) and its queue
-    (off: one wave per group in tree order) only change which wave takes
-    which group; the queue order sorted at the end of one walk for the next
-    is used from the second evaluation on."""
+    """The walk's blocks per CU (all that fit by default, 1, 3; the pair
+    phase's own 5 beside 3) and its queue (off: one wave per group in tree
+    order) only change which wave takes which group; the queue order sorted
+    at the end of one walk for the next is used from the second evaluation
+    on."""
     print(f"case walk queue settings {precision}", flush=True)
     pos, vel, mass, h = cloud(30000, "r2", 42)
     ref = _run(False, precision, "isothermal", pos, vel, mass)
-    for wq in (0, 1, 3):
-        b = _run(False, precision, "isothermal", pos, vel, mass, walk_queue=wq)
+    for wq, pb in ((0, None), (1, None), (3, None), (3, 5)):
+        b = _run(False, precision, "isothermal", pos, vel, mass, walk_queue=wq, pair_blocks=pb)
         for k in ref:
             ne = int(np.sum(np.asarray(ref[k]) != np.asarray(b[k])))
             if ne:
-                print(f"  walk queue {wq} {k}: {ne} differ", flush=True)
-            assert ne == 0, (wq, k)
+                print(f"  walk queue {wq} pair blocks {pb} {k}: {ne} differ", flush=True)
+            assert ne == 0, (wq, pb, k)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -98,4 +107,24 @@ def test_near_list_fallback_changes_nothing(precision):
         ne = int(np.sum(np.asarray(ref[k]) != np.asarray(b[k])))
         if ne:
             print(f"  near cap 4 {k}: {ne} differ", flush=True)
+        assert ne == 0, k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_overflow_rows_from_leaf_lists_change_nothing(precision):
+    """Rows above pp_limit are formed from the leaves the particle's
+    neighbour-walk wave recorded, by the whole wave; with no recorded leaves
+    (0) every such particle walks the tree alone.  The rows -- and so every
+    force, evaluation and step -- are the same bit for bit."""
+    print(f"case overflow rows from leaf lists {precision}", flush=True)
+    pos, vel, mass, h = cloud(20000, "plummer", 47)
+    ref = _run(False, precision, "isothermal", pos, vel, mass)
+    over = int(np.sum(ref["eval_pptot"] > ref["pp_limit"]))
+    print(f"  rows above pp_limit: {over}", flush=True)
+    assert over > 0
+    b = _run(False, precision, "isothermal", pos, vel, mass, neib_leaves=0)
+    for k in ref:
+        ne = int(np.sum(np.asarray(ref[k]) != np.asarray(b[k])))
+        if ne:
+            print(f"  leaf lists off {k}: {ne} differ", flush=True)
         assert ne == 0, k
