@@ -1004,10 +1004,10 @@ int run_force_eval(Ctx &c, int flags) {
         SX_TRY(grav_launch(c));
     }
     SX_TRY(run_sph_h(c));
-    // hmax before the pair phase: the persistent pair kernel fills the
-    // device, and the small per-level hmax launches would wait behind it
-    SX_TRY(update_hmax(c));
+    // the pair phase needs the new h only; the per-level hmax launches (for
+    // the neighbour walk) follow it, so the walk's stream does not wait
     if (grav) SX_TRY(grav_pairs(c));
+    SX_TRY(update_hmax(c));
     SX_TRY(run_sph_lists(c));
     SX_TRY(sph_thermal(c));
     SX_HIP(hipEventRecord(c.ev[2], c.stream));

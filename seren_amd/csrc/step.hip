@@ -573,10 +573,11 @@ int step_once(Ctx &c) {
         guard.armed = true;
         SX_TRY(grav_launch(c));   // cell phase beside the h solve
         SX_TRY(run_sph_h(c));
-        // hmax before the pair phase: the persistent pair kernel fills the
-        // device, and the small per-level hmax launches would wait behind it
-        SX_TRY(update_hmax(c));
+        // the pair phase needs the new h only; the per-level hmax launches
+        // (for the neighbour walk) follow it, so the walk's stream, the
+        // longer of the two, does not wait for them
         SX_TRY(grav_pairs(c));    // pair phase beside the lists, thermal and hydro
+        SX_TRY(update_hmax(c));
         SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));
     } else {
