@@ -282,10 +282,13 @@ def step_leg(args, cloud, precision, tune=False):
         phases = np.zeros(4)
         wstats = None
 
-        def unit():
-            """One timed unit: a leapfrog step (two sph_integrate calls) or a
-            bare force evaluation; returns per-phase ms, interaction counts
-            and the walk's (cell phase, pair phase) ms."""
+        def unit(instrument=False):
+            """One unit: a leapfrog step (two sph_integrate calls) or a bare
+            force evaluation; returns per-phase ms and the walk's (cell
+            phase, pair phase) ms, read from the events the library records
+            (no device work).  With instrument also the interaction counts
+            and walk statistics, which copy device counters to the host:
+            they are read on one more unit after the timed ones."""
             nonlocal wstats
             ph, cnt, glaunch = np.zeros(4), np.zeros(4), []
             for _ in range(2 if args.mode == "step" else 1):
@@ -295,12 +298,13 @@ def step_leg(args, cloud, precision, tune=False):
                     eng.force_eval()
                 t = eng.timings()
                 ph += np.array([t["tree"], t["sph"], t["hydro"], t["grav"]])
-                c = np.array(eng.interaction_counts(), np.float64)
-                if c[0] > 0:                          # the call that evaluated forces
-                    wstats = eng.walk_stats()
-                    w = eng.walk_timings()
+                w = eng.walk_timings()
+                if w["cells"] > 0:                    # the call that evaluated forces
                     glaunch.append((w["cells"], w["pairs"]))
-                cnt += c
+                    if instrument:
+                        wstats = eng.walk_stats()
+                if instrument:
+                    cnt += np.array(eng.interaction_counts(), np.float64)
             return ph, cnt, glaunch
 
         if args.mode == "step":
@@ -319,12 +323,15 @@ def step_leg(args, cloud, precision, tune=False):
         print(f"# {precision} warm", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            ph, cnt, gl = unit()
+            ph, _, gl = unit()
             phases += ph
-            counts += cnt
             grav_ms.extend(gl)
         eng.sync()
         dt = time.perf_counter() - t0
+        # interaction counts and walk statistics: one more unit (untimed),
+        # counted for each timed unit (the counts of consecutive steps agree
+        # to a fraction of a percent)
+        counts = unit(instrument=True)[1] * args.steps
         out["step_info"] = eng.step_info() if args.mode == "step" else None
         # accuracy of the tree gravity against the FP64 direct sum on a seeded
         # subsample of targets (one more, gravity-only, evaluation; untimed)

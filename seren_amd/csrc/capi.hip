@@ -1225,6 +1225,9 @@ int serenx_export_tree(const serenx_ctx *ctx, int32_t *leaf, int32_t *plist, int
     Ctx &c = *const_cast<Ctx *>(as_ctx(ctx));
     if (!c.tree_ok) return set_error(SERENX_ESTATE, "no tree built");
     SX_HIP(hipSetDevice(c.device));
+    // hrangemax from the current h: a step with no active particles leaves
+    // it to the next force evaluation, which recomputes it before use
+    SX_TRY(update_hmax(c));
     SX_HIP(hipStreamSynchronize(c.stream));
     const int nc = c.nc;
     std::vector<int> b2d(nc), d2b(nc), cleaf(nc);

@@ -563,8 +563,9 @@ int step_once(Ctx &c) {
     SX_HIP(hipEventRecord(c.ev[1], c.stream));
     // sph_update, thermal, forces for the active particles; with none active
     // (the first step, the half steps of a one-level leapfrog) they change
-    // nothing but the tree's hrangemax, which the next sph_update recomputes
-    // from the same h, and it is kept current here all the same
+    // nothing but the tree's hrangemax, which the next force evaluation
+    // recomputes from the same h before its neighbour walk reads it (and
+    // serenx_export_tree before it exports it), so such a step skips it
     // the gravity walk runs on its own stream beside the neighbour lists,
     // thermal and hydro forces, from the new h on
     c.grav_timed = false;
@@ -580,8 +581,6 @@ int step_once(Ctx &c) {
         SX_TRY(update_hmax(c));
         SX_TRY(run_sph_lists(c));
         SX_TRY(sph_thermal(c));
-    } else {
-        SX_TRY(update_hmax(c));
     }
     SX_HIP(hipEventRecord(c.ev[2], c.stream));
     if (s.nactive > 0) SX_TRY(run_hydro(c));
