@@ -314,7 +314,13 @@ def step_leg(args, cloud, precision, tune=False):
         if args.walk_queue < 0 and args.tune_walk:
             # the walk's blocks per CU, timed on the unit itself (untimed here)
             from seren_amd.autotune import tune_walk_blocks
-            out["walk_blocks"] = tune_walk_blocks(eng, unit)
+            if args.mode == "step":
+                # mean over 4 steps, which hold exactly one tree rebuild (every
+                # 8 sph_integrate calls); all that fit never wins a step
+                out["walk_blocks"] = tune_walk_blocks(eng, unit, blocks=(3, 4, 6), reps=4,
+                                                      mean=True)
+            else:
+                out["walk_blocks"] = tune_walk_blocks(eng, unit)
             print(f"# walk blocks per CU {out['walk_blocks']['blocks_per_cu']}",
                   file=sys.stderr, flush=True)
         for _ in range(args.warmup):
