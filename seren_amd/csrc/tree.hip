@@ -647,22 +647,37 @@ __global__ void k_hguess(int nc, int thresh, double invkernrange, const int *__r
     for (int i = 0; i < k; i++) h[s + i] = hg;
 }
 
-// ---- BHhydro_update_hmax.F90:29-88 --------------------------------------
-__global__ void k_hmax(int nc, int base, int cnt, const int *__restrict__ bfs2dfs, HNode *__restrict__ hn,
-                       const double *__restrict__ h, double kernrange) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cnt) return;
-    const int c = bfs2dfs[base + j];
-    HNode &hh = hn[c];
+
+// BHhydro_update_hmax.F90:29-88 in one pass.  Branches start at zero; one
+// thread per leaf writes the leaf's maximum of KERNRANGE h and raises its
+// ancestors to it with 64-bit atomic maxima (non-negative doubles order as
+// their bit patterns), stopping at the first ancestor that already holds as
+// much: the thread that put that value there carries it further up.  Every
+// cell ends at the maximum over its subtree, as a pass over the levels from
+// the deepest up gives it: a maximum does not depend on the order.
+__global__ void k_hmax_zero(int nc, HNode *__restrict__ hn) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc && hn[c].leaf == 0) hn[c].hrangemax = 0.0;
+}
+
+__global__ void k_hmax_up(int nc, HNode *__restrict__ hn, const int *__restrict__ dfs_parent,
+                          const double *__restrict__ h, double kernrange) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int leaf = hn[c].leaf, first = hn[c].first;
+    if (leaf <= 0) return;
     double hr = 0.0;
-    if (hh.leaf == 1) {
-        hr = kernrange * h[hh.first];
-    } else if (hh.leaf > 0) {
-        for (int i = 0; i < hh.leaf; i++) hr = fmax(hr, kernrange * h[hh.first + i]);
+    if (leaf == 1) {
+        hr = kernrange * h[first];
     } else {
-        for (int cc = c + 1; cc != hh.next && cc < nc; cc = hn[cc].next) hr = fmax(hr, hn[cc].hrangemax);
+        for (int i = 0; i < leaf; i++) hr = fmax(hr, kernrange * h[first + i]);
     }
-    hh.hrangemax = hr;
+    hn[c].hrangemax = hr;
+    const unsigned long long v = (unsigned long long)__double_as_longlong(hr);
+    for (int p = dfs_parent[c]; p >= 0; p = dfs_parent[p]) {
+        unsigned long long *slot = reinterpret_cast<unsigned long long *>(&hn[p].hrangemax);
+        if (atomicMax(slot, v) >= v) break;
+    }
 }
 
 }  // namespace
@@ -987,12 +1002,10 @@ int hguess(Ctx &c) {
 }
 
 int update_hmax(Ctx &c) {
-    for (int L = c.ltot; L >= 0; L--) {
-        const int cnt = c.level_cnt[L];
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_hmax, dim3(nblk(cnt)), dim3(TB), 0, c.stream, c.nc, c.level_off[L], cnt,
-                           c.bfs2dfs, c.hn, c.h, c.kg.range);
-    }
+    if (c.nc <= 0) return SERENX_OK;
+    hipLaunchKernelGGL(k_hmax_zero, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, c.hn);
+    hipLaunchKernelGGL(k_hmax_up, dim3(nblk(c.nc)), dim3(TB), 0, c.stream, c.nc, c.hn,
+                       c.dfs_parent, c.h, c.kg.range);
     SX_HIP(hipGetLastError());
     return SERENX_OK;
 }
